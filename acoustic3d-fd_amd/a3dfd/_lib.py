@@ -1,0 +1,221 @@
+"""ctypes binding of lib/libacoustic3d.so (the C ABI declared in include/acoustic3d.h).
+
+The library is the only compute path: there is no CPU fallback. If it is
+missing, importing the binding raises, naming the build step.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_longlong, c_void_p
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libacoustic3d.so")
+
+OP_2ND, OP_ISO, OP_VTI, OP_TTI = 0, 1, 2, 3
+OP_NAMES = {"2nd": OP_2ND, "iso": OP_ISO, "vti": OP_VTI, "tti": OP_TTI}
+
+A3D_OK = 0
+A3D_EUNSTABLE = -5
+
+# Legacy library file names of the reference (Makefile:10-13); each is a
+# symlink to libacoustic3d.so.
+LEGACY_NAMES = {
+    OP_2ND: "acoustic2order.so",
+    OP_ISO: "acoustic1order.so",
+    OP_VTI: "acoustic1vti.so",
+    OP_TTI: "acoustic1tti.so",
+}
+
+
+class ShotDesc(ctypes.Structure):
+    """Mirror of a3d_shot_desc (include/acoustic3d.h)."""
+
+    _fields_ = [
+        ("op", c_int),
+        ("nx", c_int), ("ny", c_int), ("nz", c_int),
+        ("dx", c_float), ("dy", c_float), ("dz", c_float),
+        ("nb", c_int),
+        ("sx", c_int), ("sy", c_int), ("sz", c_int),
+        ("nt", c_int),
+        ("dt", c_float), ("fm", c_float),
+        ("wtype", c_int),
+        ("pfac", c_float),
+        ("rec_z", c_int),
+        ("sv", c_int),
+        ("snap_interval", c_int),
+        ("snap_skip0", c_int),
+        ("mute", c_int),
+        ("v_mute", c_float),
+        ("mute_tt", c_int),
+        ("storage", c_int),
+    ]
+
+
+# Reference per-operator constants: boundary width (Toa_3dfd_cuda.cu:19; iso:395,
+# vti:414, tti:418), source factor (iso:407, vti:431, tti:437), receiver depth
+# (Toa_3dfd_cuda.cu:159 interior z=1; iso:302 interior z=0), SV taper (vti:567).
+REFERENCE_DEFAULTS = {
+    OP_2ND: dict(nb=50, pfac=1.0, rec_z=1, sv=0, wtype=1),
+    OP_ISO: dict(nb=20, pfac=10000.0, rec_z=0, sv=0, wtype=1),
+    OP_VTI: dict(nb=20, pfac=10.0, rec_z=0, sv=1, wtype=1),
+    OP_TTI: dict(nb=40, pfac=10.0, rec_z=0, sv=1, wtype=1),
+}
+
+
+def make_desc(op, nx, ny, nz, dx, dy, dz, sx, sy, sz, nt, dt, fm, **kw):
+    """Builds a ShotDesc with the reference defaults of `op`, overridden by kw."""
+    op = OP_NAMES.get(op, op)
+    d = dict(REFERENCE_DEFAULTS[op])
+    d.update(dict(snap_interval=0, snap_skip0=0, mute=0, v_mute=0.0, mute_tt=60, storage=0))
+    d.update(kw)
+    return ShotDesc(op, nx, ny, nz, dx, dy, dz, d["nb"], sx, sy, sz, nt, dt, fm, d["wtype"], d["pfac"],
+                    d["rec_z"], d["sv"], d["snap_interval"], d["snap_skip0"], d["mute"], d["v_mute"],
+                    d["mute_tt"], d["storage"])
+
+
+def desc_dict(d):
+    return {name: getattr(d, name) for name, _ in ShotDesc._fields_}
+
+
+_lib = None
+
+
+def load(path=None):
+    """Loads libacoustic3d.so and declares the signatures. Raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()'"
+            " or make -C acoustic3d-fd_amd)")
+    lib = ctypes.CDLL(p)
+    vp = c_void_p
+    fp = POINTER(c_float)
+    lib.a3d_create.argtypes = [POINTER(ShotDesc), c_int, POINTER(vp)]
+    lib.a3d_destroy.argtypes = [vp]
+    lib.a3d_set_model.argtypes = [vp, fp, fp, fp, fp, fp, fp]
+    lib.a3d_run.argtypes = [vp, POINTER(ShotDesc), fp, fp, c_int, POINTER(c_int), POINTER(c_double)]
+    lib.a3d_set_kernel_timing.argtypes = [vp, c_int]
+    lib.a3d_kernel_times.argtypes = [vp, POINTER(c_double), POINTER(c_longlong), c_int]
+    lib.a3d_set_launch.argtypes = [vp, c_int, c_int, c_int]
+    lib.a3d_get_launch.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                   POINTER(c_int)]
+    lib.a3d_grid_info.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_longlong)]
+    lib.a3d_last_error.restype = c_char_p
+    lib.a3d_version.restype = c_char_p
+    for name in ("a3d_create", "a3d_destroy", "a3d_set_model", "a3d_run", "a3d_set_kernel_timing",
+                 "a3d_kernel_times", "a3d_set_launch", "a3d_get_launch", "a3d_grid_info"):
+        getattr(lib, name).restype = c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class A3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"acoustic3d error {code}: {msg}")
+        self.code = code
+
+
+def _check(lib, rc):
+    if rc != A3D_OK:
+        raise A3DError(rc, lib.a3d_last_error().decode())
+    return rc
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+def _model(a, shape):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"model array shape {a.shape} != {shape} (ny, nx, nz)")
+    return a
+
+
+class Modeller:
+    """One device-resident model and its wavefields (an a3d_ctx).
+
+    Arrays use the reference file order: models are (ny, nx, nz) C-ordered,
+    z fastest; shot records are (ny, nx, nt), time fastest.
+    """
+
+    def __init__(self, desc, device=0):
+        self.lib = load()
+        self.desc = desc
+        self._h = c_void_p()
+        _check(self.lib, self.lib.a3d_create(ctypes.byref(desc), device, ctypes.byref(self._h)))
+        self._keep = ()
+
+    @property
+    def shape(self):
+        return (self.desc.ny, self.desc.nx, self.desc.nz)
+
+    def set_model(self, vel, rho=None, eps=None, delta=None, theta=None, phi=None):
+        arrs = [_model(a, self.shape) for a in (vel, rho, eps, delta, theta, phi)]
+        _check(self.lib, self.lib.a3d_set_model(self._h, *[_fptr(a) for a in arrs]))
+
+    def run(self, desc=None, shot=True, nsnap_max=0):
+        """Runs one shot; returns (shot or None, snapshots (k, ny, nx, nz) or None, loop_ms)."""
+        d = desc or self.desc
+        out = np.empty((d.ny, d.nx, d.nt), np.float32) if shot else None
+        snaps = np.empty((nsnap_max, d.ny, d.nx, d.nz), np.float32) if nsnap_max > 0 else None
+        ns = c_int(0)
+        ms = c_double(0.0)
+        rc = self.lib.a3d_run(self._h, ctypes.byref(d), _fptr(out), _fptr(snaps), nsnap_max, ctypes.byref(ns),
+                              ctypes.byref(ms))
+        _check(self.lib, rc)
+        if snaps is not None:
+            snaps = snaps[: min(ns.value, nsnap_max)]
+        return out, snaps, ms.value
+
+    def set_kernel_timing(self, on=True):
+        _check(self.lib, self.lib.a3d_set_kernel_timing(self._h, int(on)))
+
+    def kernel_times(self, reset=True):
+        ms = (c_double * 2)()
+        n = (c_longlong * 2)()
+        _check(self.lib, self.lib.a3d_kernel_times(self._h, ms, n, int(reset)))
+        return [ms[0], ms[1]], [n[0], n[1]]
+
+    def set_launch(self, ty_vel, ty_stress, zseg=0):
+        _check(self.lib, self.lib.a3d_set_launch(self._h, ty_vel, ty_stress, zseg))
+
+    def launch_info(self):
+        v = [c_int() for _ in range(5)]
+        _check(self.lib, self.lib.a3d_get_launch(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(ty_vel=v[0].value, ty_stress=v[1].value, zseg=v[2].value, blocks_vel=v[3].value,
+                    blocks_stress=v[4].value)
+
+    def grid_info(self):
+        a, b, c = c_int(), c_int(), c_int()
+        n = c_longlong()
+        _check(self.lib, self.lib.a3d_grid_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                 ctypes.byref(n)))
+        return dict(nnx=a.value, nny=b.value, nnz=c.value, updated_points=n.value)
+
+    def close(self):
+        if self._h:
+            self.lib.a3d_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,995 @@
+// a3d_host.hip -- host runtime of libacoustic3d.so: contexts, setup tables,
+// the per-shot time loop and the four legacy entry points.
+//
+// The setup tables (wavelet values, PML profiles, d0, sponge factors, SV
+// taper) are time-invariant or per-step scalars; they are computed here on
+// the host with the reference's float/double arithmetic and handed to the
+// kernels, so no transcendental function runs on the GPU except the per-point
+// cos/sin of the TTI rotation.
+#include "a3d_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/acoustic3d.h"
+
+using namespace a3d;
+
+namespace {
+
+constexpr double kPi = 3.141592653;   // reference #define pi 3.141592653
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(A3D_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+enum Vol {
+    V_VP, V_RHO, V_EPS, V_DEL, V_TH, V_PH,
+    V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ,
+    V_VV, V_PA, V_PB, NVOL
+};
+
+std::vector<int> volumes_of(int op) {
+    switch (op) {
+        case A3D_OP_2ND: return {V_VV, V_PA, V_PB};
+        case A3D_OP_ISO: return {V_VP, V_RHO, V_U, V_V, V_W, V_P, V_PX, V_PY, V_PZ};
+        case A3D_OP_VTI:
+            return {V_VP, V_RHO, V_EPS, V_DEL, V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
+        case A3D_OP_TTI:
+            return {V_VP, V_EPS, V_DEL, V_TH, V_PH, V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
+    }
+    return {};
+}
+
+bool is_wavefield(int v) { return (v >= V_U && v <= V_QZ) || v == V_PA || v == V_PB; }
+
+// ---------------------------------------------------------------------------
+// Setup tables (host), reference arithmetic
+// ---------------------------------------------------------------------------
+
+// add_source wavelet value at float-accumulated time t (reference iso:52-89;
+// vti:54-79 and tti:31-56 use types 1..3).
+float wavelet_value(int wtype, float t, float favg) {
+    float x_, xx_, tdelay, ts, source = 0.0f;
+    tdelay = (float)(1.0 / favg);
+    ts = t - tdelay;
+    switch (wtype) {
+        case 1:
+            x_ = favg * ts;
+            xx_ = x_ * x_;
+            source = (float)((1 - 2 * kPi * kPi * (xx_)) * std::exp(-(kPi * kPi * xx_)));
+            break;
+        case 2:
+            x_ = (float)((-4) * favg * favg * kPi * kPi / std::log(0.1));
+            source = (float)((-2) * kPi * kPi * ts * std::exp(-x_ * ts * ts));
+            break;
+        case 3:
+            x_ = (float)((-1) * favg * favg * kPi * kPi / std::log(0.1));
+            source = std::exp(-x_ * ts * ts);
+            break;
+        case 4:
+            x_ = (float)(std::exp(-favg * favg * kPi * kPi * ts * ts) * ts);
+            source = -x_;
+            break;
+        case 5:
+            x_ = (float)(-1.0 / (2.0 * favg * favg * kPi * kPi));
+            x_ = (float)(x_ * std::exp(-favg * favg * kPi * kPi * ts * ts));
+            source = x_;
+            break;
+        case 6:
+            x_ = (float)(-std::erf(favg * kPi * ts) / (4 * favg * favg * favg * kPi * kPi * sqrtf((float)kPi)));
+            source = x_;
+            break;
+        default: break;
+    }
+    return source;
+}
+
+// cuda_ricker_wavelet (reference Toa_3dfd_cuda.cu:34-43).
+float ricker_value(int it, float fm, float dt) {
+    float tmp = (float)(kPi * fm * std::fabs((float)(it * dt - 1.0 / fm)));
+    tmp *= tmp;
+    return (float)((1.0 - 2.0 * tmp) * std::exp(-tmp));
+}
+
+// initial_coffe (reference iso:263-291) for one axis of interior length n.
+void pml_axis(float dt, float d0, int n, int npml, float* c1, float* c2, float* a1, float* a2) {
+    const float dtd0 = dt * d0;
+    for (int id = 0; id < n + 2 * npml; ++id) {
+        double r, ra;
+        if (id < npml) {
+            r = (npml - 0.5 - id) / npml;
+            ra = ((npml - id) * 1.0) / npml;
+        } else if (id < npml + n) {
+            c1[id] = c2[id] = a1[id] = a2[id] = 1.0f;
+            continue;
+        } else {
+            r = (0.5 + id - n - npml) / npml;
+            ra = ((id - n - npml) * 1.0) / npml;
+        }
+        c1[id] = (float)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
+        c2[id] = (float)(c1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
+        a1[id] = (float)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
+        a2[id] = (float)(a1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
+    }
+}
+
+// cuda_absorb_bndr factor (reference Toa_3dfd_cuda.cu:138-151): qp*r^2 + 1
+// inside the npd-wide bands, 1.0 elsewhere.
+double sponge_value(int i, int n, int npd) {
+    const double qp = (double)(-0.25f);
+    if (i < npd) {
+        const double r = (npd - i) / (1.0 * npd);
+        return qp * (r * r) + 1;
+    }
+    if (i >= 2 * MM + npd + n) {
+        const double r = (i - 2 * MM - npd - n) / (1.0 * npd);
+        return qp * (r * r) + 1;
+    }
+    return 1.0;
+}
+
+int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> marks;   // (kernel index, first event)
+    double ms[2] = {0, 0};
+    long long n[2] = {0, 0};
+};
+
+}  // namespace
+
+struct a3d_ctx {
+    a3d_shot_desc d{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    Geo g{};
+    size_t nvox = 0;
+    int off = 0;
+    float* vol[NVOL] = {};
+    float* stage = nullptr;     // 2nd order: interior velocity for (v*dt)^2
+    float vv_dt = 0.0f;
+    float* tab = nullptr;       // 12 PML arrays
+    double* dtab = nullptr;     // sponge fx, fy, fz; SV taper
+    float* rec = nullptr;
+    size_t rec_cap = 0;
+    float* shot_dev = nullptr;
+    size_t shot_cap = 0;
+    float* snap_dev = nullptr;
+    int* flag = nullptr;
+    bool model_set = false;
+    float vel_first = 0.f, eps_first = 0.f, vel_d0 = 0.f, vel_raw1 = 0.f;
+    int ty_vel = 4, ty_stress = 4, zseg = 0;
+    Timing tm;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+Launch make_launch(const Geo& g, int ty, int zseg) {
+    Launch l{};
+    l.ntx = g.ld / TX;
+    l.nty = (g.nny - 2 * MM + ty - 1) / ty;
+    const int nzi = g.nnz - 2 * MM;
+    int nseg = zseg;
+    if (nseg <= 0) {
+        // enough workgroups to keep ~8 per CU resident on 256 CUs, segments >= 24 planes
+        const int per_plane = l.ntx * l.nty;
+        nseg = (2048 + per_plane - 1) / per_plane;
+        nseg = std::max(1, std::min(nseg, nzi / 24));
+    }
+    nseg = std::max(1, std::min(nseg, nzi));
+    l.zlen = (nzi + nseg - 1) / nseg;
+    l.nseg = (nzi + l.zlen - 1) / l.zlen;
+    l.ntiles = l.ntx * l.nty * l.nseg;
+    l.grid = ((l.ntiles + 7) / 8) * 8;
+    return l;
+}
+
+bool same_geometry(const a3d_shot_desc& a, const a3d_shot_desc& b) {
+    return a.op == b.op && a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx &&
+           a.dy == b.dy && a.dz == b.dz && a.nb == b.nb;
+}
+
+int check_desc_geometry(const a3d_shot_desc* d) {
+    if (!d) return fail(A3D_EINVAL, "null shot descriptor");
+    if (d->op < A3D_OP_2ND || d->op > A3D_OP_TTI) return fail(A3D_EINVAL, "unknown operator %d", d->op);
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(A3D_EINVAL, "model size must be positive");
+    if (!(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0)) return fail(A3D_EINVAL, "grid spacing must be positive");
+    if (d->op == A3D_OP_2ND ? d->nb < 1 : d->nb < MM) return fail(A3D_EINVAL, "boundary width %d too small", d->nb);
+    if (d->storage != 0) return fail(A3D_EINVAL, "storage mode %d not available", d->storage);
+    const int ring = d->op == A3D_OP_2ND ? 2 * MM + 2 * d->nb : 2 * d->nb;
+    const double n = (double)(d->nx + ring) * (d->ny + ring) * (d->nz + ring);
+    if (n * 1.1 > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
+    return A3D_OK;
+}
+
+int check_desc_shot(const a3d_ctx* c, const a3d_shot_desc* d) {
+    if (!same_geometry(c->d, *d)) return fail(A3D_EINVAL, "shot geometry differs from the context's");
+    if (d->nt < 1) return fail(A3D_EINVAL, "nt must be positive");
+    if (!(d->dt > 0) || !(d->fm > 0)) return fail(A3D_EINVAL, "dt and fm must be positive");
+    if (d->rec_z < 0 || d->rec_z >= d->nz) return fail(A3D_EINVAL, "rec_z %d outside [0,%d)", d->rec_z, d->nz);
+    if (d->snap_interval < 0) return fail(A3D_EINVAL, "snap_interval must be >= 0");
+    if (d->op == A3D_OP_2ND) {
+        if (d->sx < 0 || d->sx >= d->nx || d->sy < 0 || d->sy >= d->ny || d->sz < 0 || d->sz + 2 >= d->nz + d->nb)
+            return fail(A3D_EINVAL, "source (%d,%d,%d) outside the model", d->sx, d->sy, d->sz);
+    } else {
+        if (d->sx < 1 || d->sx > d->nx || d->sy < 1 || d->sy > d->ny || d->sz < 1 || d->sz > d->nz)
+            return fail(A3D_EINVAL, "source (%d,%d,%d) outside 1..n", d->sx, d->sy, d->sz);
+        if (d->op == A3D_OP_ISO && d->sz + d->nb + 1 >= d->nz + 2 * d->nb - MM)
+            return fail(A3D_EINVAL, "dipole partner of source z=%d outside the grid", d->sz);
+        if (d->op == A3D_OP_ISO && (d->wtype < 1 || d->wtype > 6))
+            return fail(A3D_EINVAL, "wavelet type %d not in 1..6", d->wtype);
+        if (d->op != A3D_OP_ISO && (d->wtype < 1 || d->wtype > 3))
+            return fail(A3D_EINVAL, "wavelet type %d not in 1..3", d->wtype);
+    }
+    return A3D_OK;
+}
+
+int ensure(float** p, size_t* cap, size_t n) {
+    if (*cap >= n) return A3D_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, n * sizeof(float)) != hipSuccess) {
+        *p = nullptr;
+        return fail(A3D_ENOMEM, "device allocation of %zu bytes failed", n * sizeof(float));
+    }
+    *cap = n;
+    return A3D_OK;
+}
+
+hipEvent_t timing_event(Timing& t) {
+    if (t.used == t.pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        t.pool.push_back(e);
+    }
+    return t.pool[t.used++];
+}
+
+void timing_begin(a3d_ctx* c, int k) {
+    if (!c->tm.on) return;
+    hipEvent_t e = timing_event(c->tm);
+    if (!e) return;
+    c->tm.marks.push_back({k, c->tm.used - 1});
+    (void)hipEventRecord(e, c->st);
+}
+
+void timing_end(a3d_ctx* c) {
+    if (!c->tm.on || c->tm.marks.empty()) return;
+    hipEvent_t e = timing_event(c->tm);
+    if (e) (void)hipEventRecord(e, c->st);
+}
+
+void timing_collect(a3d_ctx* c) {
+    Timing& t = c->tm;
+    for (auto& m : t.marks) {
+        if (m.second + 1 >= t.used) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.pool[m.second], t.pool[m.second + 1]) == hipSuccess) {
+            t.ms[m.first] += ms;
+            t.n[m.first] += 1;
+        }
+    }
+    t.marks.clear();
+    t.used = 0;
+}
+
+using SnapFn = std::function<void(int k, int it, const float* host)>;
+
+int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn,
+             int* nsnap_taken, double* loop_ms) {
+    int rc = check_desc_shot(c, d);
+    if (rc) return rc;
+    if (!c->model_set) return fail(A3D_ESTATE, "model not set");
+    HIP_TRY(hipSetDevice(c->device));
+    const Geo g = c->g;
+    const int nx = d->nx, ny = d->ny, nz = d->nz, nt = d->nt, nb = d->nb;
+    const size_t plane_rec = (size_t)nx * ny;
+    if ((rc = ensure(&c->rec, &c->rec_cap, plane_rec * nt))) return rc;
+
+    std::vector<float> hsnap;
+    const bool snaps = d->snap_interval > 0 && snapfn;
+    if (snaps) {
+        if (!c->snap_dev && hipMalloc(&c->snap_dev, (size_t)nx * ny * nz * sizeof(float)) != hipSuccess) {
+            c->snap_dev = nullptr;
+            return fail(A3D_ENOMEM, "snapshot buffer allocation failed");
+        }
+        hsnap.resize((size_t)nx * ny * nz);
+    }
+    auto want_snap = [&](int it) {
+        return d->snap_interval > 0 && it % d->snap_interval == 0 && !(d->snap_skip0 && it == 0);
+    };
+    int nsnap = 0;
+    auto take_snap = [&](int it, const float* src, int s0, float a0, int s1, float a1, bool sponge) -> int {
+        ++nsnap;
+        if (!snaps) return A3D_OK;
+        const double* fx = sponge ? c->dtab : nullptr;
+        const double* fy = sponge ? c->dtab + g.nnx : nullptr;
+        const double* fz = sponge ? c->dtab + g.nnx + g.nny : nullptr;
+        HIP_TRY(launch_snap(src, g, c->off, nx, ny, nz, s0, a0, s1, a1, fx, fy, fz, c->snap_dev, c->st));
+        HIP_TRY(hipMemcpyAsync(hsnap.data(), c->snap_dev, hsnap.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        snapfn(nsnap - 1, it, hsnap.data());
+        return A3D_OK;
+    };
+
+    if (d->op == A3D_OP_2ND) {
+        // -------- 2nd order (reference cuda_3dfd, Toa_3dfd_cuda.cu:366-508)
+        if (c->vv_dt != d->dt) {
+            HIP_TRY(launch_pad(c->stage, nx, ny, nz, c->off, g, c->vol[V_VV], 1, d->dt, c->st));
+            c->vv_dt = d->dt;
+        }
+        HIP_TRY(hipMemsetAsync(c->vol[V_PA], 0, c->nvox * sizeof(float), c->st));
+        HIP_TRY(hipMemsetAsync(c->vol[V_PB], 0, c->nvox * sizeof(float), c->st));
+        std::vector<float> wlt(nt);
+        for (int it = 0; it < nt; ++it) wlt[it] = ricker_value(it, d->fm, d->dt);
+        Step2Args a{};
+        a.g = g;
+        a.l = make_launch(g, c->ty_vel, c->zseg);
+        a.vv = c->vol[V_VV];
+        a.fx = c->dtab;
+        a.fy = c->dtab + g.nnx;
+        a.fz = c->dtab + g.nnx + g.nny;
+        a.dz2 = (float)(1.0 / (d->dz * d->dz));
+        a.dx2 = (float)(1.0 / (d->dx * d->dx));
+        a.dy2 = (float)(1.0 / (d->dy * d->dy));
+        const int off = c->off;
+        a.s0 = (d->sx + off) + g.ld * (d->sy + off) + g.pl * (d->sz + off);
+        a.s1 = a.s0 + 2 * g.pl;
+        a.zrec = off + d->rec_z;
+        a.off = off;
+        a.nx = nx;
+        a.ny = ny;
+        float* X = c->vol[V_PA];
+        float* Y = c->vol[V_PB];
+        HIP_TRY(hipEventRecord(c->ev0, c->st));
+        for (int it = 0; it < nt; ++it) {
+            a.X = X;
+            a.Y = Y;
+            a.ya0 = wlt[it];
+            a.ya1 = -wlt[it];
+            a.xa0 = it > 0 ? wlt[it - 1] : 0.0f;
+            a.xa1 = it > 0 ? -wlt[it - 1] : 0.0f;
+            a.rec = c->rec + plane_rec * it;
+            timing_begin(c, 0);
+            HIP_TRY(launch_step2(c->ty_vel, a, c->st));
+            timing_end(c);
+            if (want_snap(it)) {
+                if ((rc = take_snap(it, Y, a.s0, a.ya0, a.s1, a.ya1, true))) return rc;
+            }
+            std::swap(X, Y);
+        }
+        HIP_TRY(hipEventRecord(c->ev1, c->st));
+    } else {
+        // -------- 1st-order family (reference iso:372-671, vti:389-688, tti:390-676)
+        const int op = d->op;
+        const float dt = d->dt;
+        // get_d0 (reference iso:193-195) from the padded velocity at z-fastest index N/2
+        const float dsum = d->dx + d->dy + d->dz;
+        const float d0 = (float)(10.0 * c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
+        std::vector<float> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
+        float* t = tabh.data();
+        float *cx1 = t, *cx2 = cx1 + g.nnx, *ax1 = cx2 + g.nnx, *ax2 = ax1 + g.nnx;
+        float *cy1 = ax2 + g.nnx, *cy2 = cy1 + g.nny, *ay1 = cy2 + g.nny, *ay2 = ay1 + g.nny;
+        float *cz1 = ay2 + g.nny, *cz2 = cz1 + g.nnz, *az1 = cz2 + g.nnz, *az2 = az1 + g.nnz;
+        pml_axis(dt, d0, nx, nb, cx1, cx2, ax1, ax2);
+        pml_axis(dt, d0, ny, nb, cy1, cy2, ay1, ay2);
+        pml_axis(dt, d0, nz, nb, cz1, cz2, az1, az2);
+        HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(float), hipMemcpyHostToDevice));
+        const float* T = c->tab;
+        Pml pm{T,
+               T + g.nnx,
+               T + 4 * g.nnx,
+               T + 4 * g.nnx + g.nny,
+               T + 4 * g.nnx + 4 * g.nny,
+               T + 4 * g.nnx + 4 * g.nny + g.nnz,
+               T + 2 * g.nnx,
+               T + 3 * g.nnx,
+               T + 4 * g.nnx + 2 * g.nny,
+               T + 4 * g.nnx + 3 * g.nny,
+               T + 4 * g.nnx + 4 * g.nny + 2 * g.nnz,
+               T + 4 * g.nnx + 4 * g.nny + 3 * g.nnz};
+        for (int v : volumes_of(op))
+            if (is_wavefield(v)) HIP_TRY(hipMemsetAsync(c->vol[v], 0, c->nvox * sizeof(float), c->st));
+
+        const float dtx = dt / d->dx, dty = dt / d->dy, dtz = dt / d->dz;
+        const int sxp = d->sx + nb - 1, syp = d->sy + nb - 1, szp = d->sz + nb - 1;
+        const int s0 = sxp + g.ld * syp + g.pl * szp;
+        VelArgs va{};
+        va.g = g;
+        va.l = make_launch(g, c->ty_vel, c->zseg);
+        va.c = pm;
+        va.P = c->vol[V_P];
+        va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
+        va.rho = c->vol[V_RHO];
+        va.th = c->vol[V_TH];
+        va.ph = c->vol[V_PH];
+        va.u = c->vol[V_U];
+        va.v = c->vol[V_V];
+        va.w = c->vol[V_W];
+        va.dtx = dtx;
+        va.dty = dty;
+        va.dtz = dtz;
+        StressArgs sa{};
+        sa.g = g;
+        sa.l = make_launch(g, c->ty_stress, c->zseg);
+        sa.c = pm;
+        sa.u = c->vol[V_U];
+        sa.v = c->vol[V_V];
+        sa.w = c->vol[V_W];
+        sa.px = c->vol[V_PX];
+        sa.py = c->vol[V_PY];
+        sa.pz = c->vol[V_PZ];
+        sa.qx = c->vol[V_QX];
+        sa.qy = c->vol[V_QY];
+        sa.qz = c->vol[V_QZ];
+        sa.vp = c->vol[V_VP];
+        sa.rho = c->vol[V_RHO];
+        sa.eps = c->vol[V_EPS];
+        sa.del = c->vol[V_DEL];
+        sa.th = c->vol[V_TH];
+        sa.ph = c->vol[V_PH];
+        sa.P = c->vol[V_P];
+        sa.Q = c->vol[V_Q];
+        sa.dtx = dtx;
+        sa.dty = dty;
+        sa.dtz = dtz;
+        sa.sv = d->sv && op != A3D_OP_ISO;
+        sa.sxp = sxp;
+        sa.syp = syp;
+        sa.szp = szp;
+        sa.taper = c->dtab + g.nnx + g.nny + g.nnz;
+        sa.zrec = nb + d->rec_z;
+        sa.nb = nb;
+        sa.nx = nx;
+        sa.ny = ny;
+
+        const float favg = d->fm;
+        const float tdelay = (float)(1.0 / favg);
+        const float pfac = d->pfac;
+        HIP_TRY(hipEventRecord(c->ev0, c->st));
+        float tcur = dt;
+        for (int it = 0; it < nt; it++, tcur += dt) {
+            // add_source (reference iso:523-530; vti:557; tti:571), applied on load
+            Src S{-1, -1, 0.f, 0.f, 0.f, 0.f};
+            if (tcur <= 2 * tdelay) {
+                const float src = wavelet_value(d->wtype, tcur, favg);
+                if (op == A3D_OP_ISO) {
+                    const float mp = (float)(-1.0 * pfac);
+                    S = {s0, s0 + 2 * g.pl, mp * src, pfac * src, mp * src, pfac * src};
+                } else {
+                    S = {s0, -1, pfac * src, 0.f, pfac * src, 0.f};
+                }
+            }
+            va.s = S;
+            sa.rec = c->rec + plane_rec * it;
+            timing_begin(c, 0);
+            HIP_TRY(launch_vel(op, c->ty_vel, va, c->st));
+            timing_end(c);
+            timing_begin(c, 1);
+            HIP_TRY(launch_stress(op, c->ty_stress, sa, c->st));
+            timing_end(c);
+            if (want_snap(it)) {
+                if ((rc = take_snap(it, c->vol[V_P], -1, 0.f, -1, 0.f, false))) return rc;
+            }
+        }
+        HIP_TRY(hipEventRecord(c->ev1, c->st));
+    }
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (loop_ms) *loop_ms = ms;
+    if (c->tm.on) timing_collect(c);
+    if (nsnap_taken) *nsnap_taken = nsnap;
+
+    // non-finite sentinel over the whole record
+    HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->st));
+    HIP_TRY(launch_check_finite(c->rec, plane_rec * nt, c->flag, c->st));
+    int hflag = 0;
+    HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+
+    if (shot_out) {
+        if ((rc = ensure(&c->shot_dev, &c->shot_cap, plane_rec * nt))) return rc;
+        MuteArgs m{};
+        m.mute = d->mute;
+        m.dx = d->dx;
+        m.dy = d->dy;
+        m.dz = d->dz;
+        m.dt = d->dt;
+        m.fm = d->fm;
+        m.sx = d->sx;
+        m.sy = d->sy;
+        m.sz = d->sz;
+        m.tt = d->mute_tt;
+        if (d->v_mute > 0) m.vcut = d->v_mute;
+        else if (d->op == A3D_OP_2ND) m.vcut = c->vel_raw1;
+        else if (d->op == A3D_OP_ISO) m.vcut = c->vel_first;
+        else m.vcut = c->vel_first * sqrtf(1 + 2 * c->eps_first);
+        HIP_TRY(launch_shot_transpose(c->rec, nt, nx, ny, m, c->shot_dev, c->st));
+        HIP_TRY(hipMemcpyAsync(shot_out, c->shot_dev, plane_rec * nt * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+    }
+    if (hflag) return fail(A3D_EUNSTABLE, "non-finite values in the shot record (unstable time step?)");
+    return A3D_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Handle API
+// ===========================================================================
+extern "C" {
+
+const char* a3d_last_error(void) { return g_err.c_str(); }
+const char* a3d_version(void) { return "acoustic3d-mi355x 0.1 (gfx950)"; }
+
+int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
+    if (!out) return fail(A3D_EINVAL, "null output pointer");
+    *out = nullptr;
+    int rc = check_desc_geometry(desc);
+    if (rc) return rc;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(A3D_EINVAL, "device %d not present (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    a3d_ctx* c = new a3d_ctx();
+    c->d = *desc;
+    c->device = device;
+    const int op = desc->op;
+    c->off = op == A3D_OP_2ND ? MM + desc->nb : desc->nb;
+    const int ring = op == A3D_OP_2ND ? 2 * MM + 2 * desc->nb : 2 * desc->nb;
+    c->g.nnx = desc->nx + ring;
+    c->g.nny = desc->ny + ring;
+    c->g.nnz = desc->nz + ring;
+    c->g.ld = (c->g.nnx + TX - 1) / TX * TX;
+    c->g.pl = c->g.ld * c->g.nny;
+    c->nvox = (size_t)c->g.pl * c->g.nnz;
+    auto cleanup = [&](int code) {
+        a3d_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        c->st = nullptr;
+        return cleanup(fail(A3D_EHIP, "stream creation failed"));
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return cleanup(fail(A3D_EHIP, "event creation failed"));
+    for (int v : volumes_of(op)) {
+        if (hipMalloc(&c->vol[v], c->nvox * sizeof(float)) != hipSuccess) {
+            c->vol[v] = nullptr;
+            return cleanup(fail(A3D_ENOMEM, "device allocation of %zu MiB failed", c->nvox * 4 >> 20));
+        }
+    }
+    const size_t ntab = 4 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz);
+    const size_t ndtab = (size_t)c->g.nnx + c->g.nny + c->g.nnz + 226;
+    if (hipMalloc(&c->tab, ntab * sizeof(float)) != hipSuccess) { c->tab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
+    if (hipMalloc(&c->dtab, ndtab * sizeof(double)) != hipSuccess) { c->dtab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
+    if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) { c->flag = nullptr; return cleanup(fail(A3D_ENOMEM, "flag allocation failed")); }
+    // double tables: sponge factors (2nd order) and SV taper (VTI/TTI)
+    std::vector<double> dt(ndtab, 1.0);
+    if (op == A3D_OP_2ND) {
+        for (int i = 0; i < c->g.nnx; ++i) dt[i] = sponge_value(i, desc->nx, desc->nb);
+        for (int i = 0; i < c->g.nny; ++i) dt[c->g.nnx + i] = sponge_value(i, desc->ny, desc->nb);
+        for (int i = 0; i < c->g.nnz; ++i) dt[c->g.nnx + c->g.nny + i] = sponge_value(i, desc->nz, desc->nb);
+    }
+    double* taper = dt.data() + c->g.nnx + c->g.nny + c->g.nnz;
+    for (int r2 = 0; r2 < 226; ++r2) {
+        // reference vti:163-166: 0.5*(1-cos(pi*((sqrtf(r2)-r)*4.0/(R*3.0-1)))), R=15, r=4
+        taper[r2] = r2 <= 16 ? 0.0 : 0.5 * (1 - std::cos(kPi * ((sqrtf((float)r2) - 4) * 4.0 / (15 * 3.0 - 1))));
+    }
+    if (hipMemcpy(c->dtab, dt.data(), ndtab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(A3D_EHIP, "table upload failed"));
+    if (const char* e = getenv("A3D_TY_VEL")) c->ty_vel = atoi(e);
+    if (const char* e = getenv("A3D_TY_STRESS")) c->ty_stress = atoi(e);
+    if (const char* e = getenv("A3D_ZSEG")) c->zseg = atoi(e);
+    *out = c;
+    return A3D_OK;
+}
+
+int a3d_destroy(a3d_ctx* c) {
+    if (!c) return A3D_OK;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto& p : c->vol)
+        if (p) (void)hipFree(p);
+    for (float* p : {c->stage, c->tab, c->rec, c->shot_dev, c->snap_dev})
+        if (p) (void)hipFree(p);
+    if (c->dtab) (void)hipFree(c->dtab);
+    if (c->flag) (void)hipFree(c->flag);
+    for (auto e : c->tm.pool) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return A3D_OK;
+}
+
+int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* eps, const float* del,
+                  const float* theta, const float* phi) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    const int op = c->d.op;
+    if (!vel) return fail(A3D_EINVAL, "velocity model required");
+    if ((op == A3D_OP_ISO || op == A3D_OP_VTI) && !rho) return fail(A3D_EINVAL, "density model required");
+    if ((op == A3D_OP_VTI || op == A3D_OP_TTI) && (!eps || !del)) return fail(A3D_EINVAL, "epsilon and delta required");
+    if (op == A3D_OP_TTI && (!theta || !phi)) return fail(A3D_EINVAL, "theta and phi required");
+    HIP_TRY(hipSetDevice(c->device));
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    float* stage = nullptr;
+    if (hipMalloc(&stage, n * sizeof(float)) != hipSuccess) return fail(A3D_ENOMEM, "staging allocation failed");
+    auto up = [&](const float* h, int v) -> int {
+        HIP_TRY(hipMemcpy(stage, h, n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(launch_pad(stage, nx, ny, nz, c->off, c->g, c->vol[v], 0, 0.f, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        return A3D_OK;
+    };
+    int rc = A3D_OK;
+    if (op == A3D_OP_2ND) {
+        HIP_TRY(hipMemcpy(stage, vel, n * sizeof(float), hipMemcpyHostToDevice));
+        if (c->stage) (void)hipFree(c->stage);
+        c->stage = stage;
+        stage = nullptr;
+        c->vv_dt = 0.0f;   // (v*dt)^2 is built at the first run
+    } else {
+        if (!rc) rc = up(vel, V_VP);
+        if (!rc && (op == A3D_OP_ISO || op == A3D_OP_VTI)) rc = up(rho, V_RHO);
+        if (!rc && op != A3D_OP_ISO) rc = up(eps, V_EPS);
+        if (!rc && op != A3D_OP_ISO) rc = up(del, V_DEL);
+        if (!rc && op == A3D_OP_TTI) rc = up(theta, V_TH);
+        if (!rc && op == A3D_OP_TTI) rc = up(phi, V_PH);
+        (void)hipFree(stage);
+    }
+    if (rc) return rc;
+    // model-derived scalars
+    if (op != A3D_OP_2ND) {
+        const int nb = c->d.nb;
+        const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
+        // padded z-fastest linear index N/2 (reference get_d0, iso:194)
+        const long long i = (long long)nny * nnx * nnz / 2;
+        const int iz = (int)(i % nnz), ix = (int)((i / nnz) % nnx), iy = (int)(i / ((long long)nnz * nnx));
+        const size_t k = (size_t)clampi(iz - nb, 0, nz - 1) + (size_t)nz * clampi(ix - nb, 0, nx - 1) +
+                         (size_t)nz * nx * clampi(iy - nb, 0, ny - 1);
+        c->vel_d0 = vel[k];
+        // padded index 1 = (iz=1, ix=0, iy=0) (reference mute_directwave, iso:340, vti:357)
+        const size_t k1 = (size_t)clampi(1 - nb, 0, nz - 1) + (size_t)nz * clampi(-nb, 0, nx - 1) +
+                          (size_t)nz * nx * clampi(-nb, 0, ny - 1);
+        c->vel_first = vel[k1];
+        c->eps_first = eps ? eps[k1] : 0.0f;
+    } else {
+        c->vel_raw1 = n > 1 ? vel[1] : vel[0];   // reference Toa_3dfd_cuda.cu:403
+    }
+    c->model_set = true;
+    return A3D_OK;
+}
+
+int a3d_run(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, float* snaps_out, int nsnap_max,
+            int* nsnap_taken, double* loop_ms) {
+    if (!c || !d) return fail(A3D_EINVAL, "null argument");
+    const size_t vol = (size_t)d->nx * d->ny * d->nz;
+    SnapFn fn;
+    if (snaps_out && nsnap_max > 0)
+        fn = [&](int k, int, const float* h) {
+            if (k < nsnap_max) std::memcpy(snaps_out + (size_t)k * vol, h, vol * sizeof(float));
+        };
+    return run_shot(c, d, shot_out, fn, nsnap_taken, loop_ms);
+}
+
+int a3d_set_kernel_timing(a3d_ctx* c, int enable) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    c->tm.on = enable != 0;
+    return A3D_OK;
+}
+
+int a3d_kernel_times(a3d_ctx* c, double* ms, long long* launches, int reset) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    for (int k = 0; k < 2; ++k) {
+        if (ms) ms[k] = c->tm.ms[k];
+        if (launches) launches[k] = c->tm.n[k];
+    }
+    if (reset) {
+        c->tm.ms[0] = c->tm.ms[1] = 0;
+        c->tm.n[0] = c->tm.n[1] = 0;
+    }
+    return A3D_OK;
+}
+
+int a3d_set_launch(a3d_ctx* c, int ty_vel, int ty_stress, int zseg) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    auto ok = [](int t) { return t == 2 || t == 4 || t == 8; };
+    if (!ok(ty_vel) || !ok(ty_stress)) return fail(A3D_EINVAL, "tile rows must be 2, 4 or 8");
+    if (zseg < 0) return fail(A3D_EINVAL, "zseg must be >= 0 (0: automatic)");
+    c->ty_vel = ty_vel;
+    c->ty_stress = ty_stress;
+    c->zseg = zseg;
+    return A3D_OK;
+}
+
+int a3d_get_launch(a3d_ctx* c, int* ty_vel, int* ty_stress, int* zseg, int* blocks_vel, int* blocks_stress) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    if (ty_vel) *ty_vel = c->ty_vel;
+    if (ty_stress) *ty_stress = c->ty_stress;
+    const Launch lv = make_launch(c->g, c->ty_vel, c->zseg);
+    const Launch ls = make_launch(c->g, c->ty_stress, c->zseg);
+    if (zseg) *zseg = lv.nseg;
+    if (blocks_vel) *blocks_vel = lv.grid;
+    if (blocks_stress) *blocks_stress = ls.grid;
+    return A3D_OK;
+}
+
+int a3d_grid_info(const a3d_ctx* c, int* nnx, int* nny, int* nnz, long long* updated) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    if (nnx) *nnx = c->g.nnx;
+    if (nny) *nny = c->g.nny;
+    if (nnz) *nnz = c->g.nnz;
+    if (updated)
+        *updated = (long long)(c->g.nnx - 2 * MM) * (c->g.nny - 2 * MM) * (c->g.nnz - 2 * MM);
+    return A3D_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Legacy entry points
+// ===========================================================================
+namespace {
+
+int read_model(const char* path, size_t n, std::vector<float>& out) {
+    if (!path) return fail(A3D_EIO, "null model file name");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(A3D_EIO, "error open <%s>!", path);
+    out.resize(n);
+    const size_t got = fread(out.data(), sizeof(float), n, f);
+    fclose(f);
+    if (got != n) return fail(A3D_EIO, "model file <%s> holds %zu of %zu floats", path, got, n);
+    return A3D_OK;
+}
+
+// Decimated shot write (reference iso:591-597): fseek to is*nt*nx*ny floats,
+// then [iy/jy][ix/jx][it/jt], time fastest.
+int write_shot(FILE* f, int is, const float* shot, int nx, int ny, int nt, float dx, float dy,
+               float dt, float dgx, float dgy, float dgt) {
+    const int jx = int(dgx / dx), jy = int(dgy / dy), jt = int(dgt / dt);
+    if (jx < 1 || jy < 1 || jt < 1)
+        return fail(A3D_EINVAL, "receiver/time decimation dgx/dx=%d dgy/dy=%d dgt/dt=%d must be >= 1", jx, jy, jt);
+    std::vector<float> buf;
+    buf.reserve((size_t)((ny + jy - 1) / jy) * ((nx + jx - 1) / jx) * ((nt + jt - 1) / jt));
+    for (int iy = 0; iy < ny; iy += jy)
+        for (int ix = 0; ix < nx; ix += jx)
+            for (int it = 0; it < nt; it += jt) buf.push_back(shot[(size_t)iy * nx * nt + (size_t)ix * nt + it]);
+    if (fseek(f, (long)is * nt * nx * ny * (long)sizeof(float), SEEK_SET) != 0) return fail(A3D_EIO, "fseek failed");
+    if (fwrite(buf.data(), sizeof(float), buf.size(), f) != buf.size()) return fail(A3D_EIO, "short write of shot file");
+    return A3D_OK;
+}
+
+struct Legacy {
+    const char* name;
+    a3d_shot_desc d;
+    const char* snapfmt;    // snapshot name after the FNsnap prefix
+    int snap_is;            // shot index in the snapshot name
+};
+
+int legacy_run(Legacy& L, const char* FNsnap, const char* FNshot, FILE* fshot, int is_first, int ns,
+               int sxbeg, int sybeg, int jsx, int jsy, int shot_is_offset, float dgx, float dgy, float dgt,
+               const float* vel, const float* rho, const float* eps, const float* del, const float* th,
+               const float* ph, int device) {
+    (void)FNshot;
+    a3d_ctx* c = nullptr;
+    int rc = a3d_create(&L.d, device, &c);
+    if (rc) return rc;
+    rc = a3d_set_model(c, vel, rho, eps, del, th, ph);
+    std::vector<float> shot((size_t)L.d.nx * L.d.ny * L.d.nt);
+    for (int is = is_first; !rc && is < is_first + ns; ++is) {
+        a3d_shot_desc d = L.d;
+        if (d.op != A3D_OP_2ND) {
+            d.sx = sxbeg + is % ns * jsx;   // reference vti:65-66 with nsx = ns
+            d.sy = sybeg + is / ns * jsy;
+        }
+        const int snap_is = d.op == A3D_OP_2ND ? L.snap_is : is;
+        SnapFn fn = [&](int, int it, const float* h) {
+            char name[600];
+            char idbuf[64];
+            snprintf(idbuf, sizeof idbuf, L.snapfmt, snap_is, it);
+            snprintf(name, sizeof name, "%s%s", FNsnap ? FNsnap : "", idbuf);
+            FILE* f = fopen(name, "wb");
+            if (!f) {
+                printf("cannot write snapfile\n");
+                return;
+            }
+            fwrite(h, sizeof(float), (size_t)d.nx * d.ny * d.nz, f);
+            fclose(f);
+        };
+        int nsnap = 0;
+        double ms = 0;
+        rc = run_shot(c, &d, shot.data(), fn, &nsnap, &ms);
+        if (rc == A3D_EUNSTABLE) {
+            fprintf(stderr, "%s: %s\n", L.name, g_err.c_str());
+            rc = A3D_OK;   // the shot is still written, as the reference would
+        }
+        if (!rc)
+            rc = write_shot(fshot, shot_is_offset >= 0 ? shot_is_offset : is, shot.data(), d.nx, d.ny, d.nt,
+                            d.dx, d.dy, d.dt, dgx, dgy, dgt);
+    }
+    a3d_destroy(c);
+    return rc;
+}
+
+a3d_shot_desc base_desc(int op, int nx, int ny, int nz, float dx, float dy, float dz, int nb, int nt,
+                        float dt, float fm) {
+    a3d_shot_desc d{};
+    d.op = op;
+    d.nx = nx;
+    d.ny = ny;
+    d.nz = nz;
+    d.dx = dx;
+    d.dy = dy;
+    d.dz = dz;
+    d.nb = nb;
+    d.nt = nt;
+    d.dt = dt;
+    d.fm = fm;
+    d.wtype = 1;
+    d.mute_tt = 60;
+    return d;
+}
+
+void report(const char* who, int rc) {
+    if (rc) fprintf(stderr, "%s: error %d: %s\n", who, rc, g_err.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+void cuda_3dfd(char* FNvel, char* FNsnap, char* FNshot, int is, int ns, int nx, int ny, int nz, float dx,
+               float dy, float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy, int jsz, float dgx,
+               float dgy, float dgt, int nt, int kt, float dt, float fm, bool show_snapshot,
+               bool cut_directwave, int snap_interval, int cudaDevicei) {
+    (void)ns;
+    (void)kt;
+    (void)cut_directwave;   // mute is dead code in the reference (Toa_3dfd_cuda.cu:474-477)
+    (void)snap_interval;    // cadence fixed at 500 (Toa_3dfd_cuda.cu:456)
+    FILE* fshot = fopen(FNshot, "wb");
+    if (!fshot) return report("cuda_3dfd", fail(A3D_EIO, "cannot open shot file <%s>", FNshot));
+    std::vector<float> vel;
+    int rc = read_model(FNvel, (size_t)nx * ny * nz, vel);
+    if (!rc) {
+        Legacy L{"cuda_3dfd", base_desc(A3D_OP_2ND, nx, ny, nz, dx, dy, dz, 50, nt, dt, fm), "ishot_%d_it_%d.bin", is};
+        // cuda_set_s (Toa_3dfd_cuda.cu:51): shot `is` of the (jsx, jsy, jsz) line
+        L.d.sx = sxbeg + is * jsx;
+        L.d.sy = sybeg + is * jsy;
+        L.d.sz = szbeg + is * jsz;
+        L.d.rec_z = 1;
+        L.d.snap_interval = show_snapshot ? 500 : 0;
+        rc = legacy_run(L, FNsnap, FNshot, fshot, 0, 1, L.d.sx, L.d.sy, 0, 0, is, dgx, dgy, dgt, vel.data(),
+                        nullptr, nullptr, nullptr, nullptr, nullptr, cudaDevicei);
+    }
+    fclose(fshot);
+    report("cuda_3dfd", rc);
+}
+
+void cuda_3dfd_1order(char* FNvel, char* FNrho, char* FNsnap, char* FNshot, int is, int ns, int nx, int ny,
+                      int nz, float dx, float dy, float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy,
+                      int jsz, float dgx, float dgy, float dgt, int nt, float dt, int wtype, float fm,
+                      bool show_snapshot, bool cut_directwave, int snap_interval, int cudaDevicei) {
+    (void)is;
+    (void)jsz;
+    FILE* fshot = fopen(FNshot, "wb");
+    if (!fshot) return report("cuda_3dfd_1order", fail(A3D_EIO, "cannot open shot file <%s>", FNshot));
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> vel, rho;
+    int rc = read_model(FNvel, n, vel);
+    if (!rc) rc = read_model(FNrho, n, rho);
+    if (!rc && ns < 1) rc = fail(A3D_EINVAL, "ns must be >= 1");
+    if (!rc) {
+        Legacy L{"cuda_3dfd_1order", base_desc(A3D_OP_ISO, nx, ny, nz, dx, dy, dz, 20, nt, dt, fm), "ishot_%dit_%d.bin", 0};
+        L.d.sz = szbeg;
+        L.d.wtype = wtype;
+        L.d.pfac = 10000.0f;
+        L.d.snap_interval = show_snapshot ? snap_interval : 0;
+        L.d.mute = cut_directwave;
+        rc = legacy_run(L, FNsnap, FNshot, fshot, 0, ns, sxbeg, sybeg, jsx, jsy, -1, dgx, dgy, dgt, vel.data(),
+                        rho.data(), nullptr, nullptr, nullptr, nullptr, cudaDevicei);
+    }
+    fclose(fshot);
+    report("cuda_3dfd_1order", rc);
+}
+
+void cuda_3dfd_vti(char* FNvel, char* FNrho, char* FNepsilon, char* FNdelta, char* FNsnap, char* FNshot, int is,
+                   int ns, int nx, int ny, int nz, float dx, float dy, float dz, int sxbeg, int sybeg, int szbeg,
+                   int jsx, int jsy, int jsz, float dgx, float dgy, float dgt, int nt, float dt, float fm,
+                   bool show_snapshot, bool cut_directwave, int snap_interval, int cudaDevicei) {
+    (void)is;
+    (void)jsz;
+    FILE* fshot = fopen(FNshot, "wb");
+    if (!fshot) return report("cuda_3dfd_vti", fail(A3D_EIO, "cannot open shot file <%s>", FNshot));
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> vel, rho, eps, del;
+    int rc = read_model(FNvel, n, vel);
+    if (!rc) rc = read_model(FNepsilon, n, eps);
+    if (!rc) rc = read_model(FNdelta, n, del);
+    if (!rc) rc = read_model(FNrho, n, rho);
+    if (!rc && ns < 1) rc = fail(A3D_EINVAL, "ns must be >= 1");
+    if (!rc) {
+        Legacy L{"cuda_3dfd_vti", base_desc(A3D_OP_VTI, nx, ny, nz, dx, dy, dz, 20, nt, dt, fm), "ishot_%d_it_%d.bin", 0};
+        L.d.sz = szbeg;
+        L.d.pfac = 10.0f;
+        L.d.sv = 1;
+        L.d.snap_interval = show_snapshot ? snap_interval : 0;
+        L.d.mute = cut_directwave;
+        rc = legacy_run(L, FNsnap, FNshot, fshot, 0, ns, sxbeg, sybeg, jsx, jsy, -1, dgx, dgy, dgt, vel.data(),
+                        rho.data(), eps.data(), del.data(), nullptr, nullptr, cudaDevicei);
+    }
+    fclose(fshot);
+    report("cuda_3dfd_vti", rc);
+}
+
+int cuda_3dfd_tti(char* FNvel, char* FNrho, char* FNepsilon, char* FNdelta, char* FNtheta, char* FNpsi,
+                  char* FNsnap, char* FNshot, int is, int ns, int nx, int ny, int nz, float dx, float dy,
+                  float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy, int jsz, float dgx, float dgy,
+                  float dgt, int nt, float dt, float fm, bool show_snapshot, bool cut_directwave,
+                  int snap_interval, int cudaDevicei) {
+    (void)FNrho;            // TTI has no density (reference tti:461)
+    (void)is;
+    (void)ns;               // forced to 1 (reference tti:439)
+    (void)jsz;
+    (void)show_snapshot;    // snapshots every 100 steps regardless (reference tti:593)
+    (void)snap_interval;
+    FILE* fshot = fopen(FNshot, "wb");
+    if (!fshot) {
+        printf("cannot write shot file");
+        return 1;
+    }
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> vel, eps, del, th, ph;
+    int rc = read_model(FNvel, n, vel);
+    if (!rc) rc = read_model(FNepsilon, n, eps);
+    if (!rc) rc = read_model(FNdelta, n, del);
+    if (!rc) rc = read_model(FNtheta, n, th);
+    if (!rc) rc = read_model(FNpsi, n, ph);
+    if (!rc) {
+        Legacy L{"cuda_3dfd_tti", base_desc(A3D_OP_TTI, nx, ny, nz, dx, dy, dz, 40, nt, dt, fm), "ishot_%d_it_%d.bin", 0};
+        L.d.sz = szbeg;
+        L.d.pfac = 10.0f;
+        L.d.sv = 1;
+        L.d.snap_interval = 100;
+        L.d.snap_skip0 = 1;
+        L.d.mute = cut_directwave;
+        rc = legacy_run(L, FNsnap, FNshot, fshot, 0, 1, sxbeg, sybeg, jsx, jsy, -1, dgx, dgy, dgt, vel.data(),
+                        nullptr, eps.data(), del.data(), th.data(), ph.data(), cudaDevicei);
+    }
+    fclose(fshot);
+    report("cuda_3dfd_tti", rc);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// a3d_internal.h -- shared between the HIP kernels (a3d_kernels.hip) and the
+// host runtime (a3d_host.hip). Not part of the public ABI.
+//
+// Device storage of every padded volume: x fastest, then y, then z
+// (index x + ld*y + ld*nny*z, ld a multiple of 64 floats). The time-stepping
+// kernels stream along z: a workgroup owns an x-y tile (64 x TY points, one
+// wavefront per tile row) and walks a segment of z planes, keeping the
+// z-neighbourhood of its own column in registers and the x-y neighbourhood of
+// the current plane in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace a3d {
+
+constexpr int MM = 4;    // half stencil length (8th order)
+constexpr int TX = 64;   // tile width: one wavefront per tile row
+
+struct Geo {
+    int nnx, nny, nnz;   // padded extents
+    int ld;              // x pitch in elements
+    int pl;              // plane stride ld*nny
+};
+
+// Work decomposition of one sweep: ntx*nty tiles per plane times nseg
+// z segments of zlen planes. grid = 8*ceil(ntiles/8) workgroups.
+struct Launch {
+    int ntx, nty, nseg, zlen, ntiles, grid;
+};
+
+// Per-axis split-PML profiles (1st-order operators), padded index.
+struct Pml {
+    const float *cx1, *cx2, *cy1, *cy2, *cz1, *cz2;   // velocity sweep (half-cell)
+    const float *ax1, *ax2, *ay1, *ay2, *az1, *az2;   // stress sweep
+};
+
+// Transient point source of the 1st-order operators: added to P (and Q) on
+// every load of the velocity sweep, never stored (the stress sweep rewrites
+// P and Q everywhere in the interior). Index -1: no source this step.
+struct Src {
+    int s0, s1;
+    float p0, p1, q0, q1;
+};
+
+struct VelArgs {
+    Geo g;
+    Launch l;
+    Pml c;
+    const float* P;
+    const float* Q;
+    const float* rho;
+    const float* th;
+    const float* ph;
+    float* u;
+    float* v;
+    float* w;
+    float dtx, dty, dtz;
+    Src s;
+};
+
+struct StressArgs {
+    Geo g;
+    Launch l;
+    Pml c;
+    const float* u;
+    const float* v;
+    const float* w;
+    float *px, *py, *pz, *qx, *qy, *qz;
+    const float *vp, *rho, *eps, *del, *th, *ph;
+    float* P;
+    float* Q;
+    float dtx, dty, dtz;
+    int sv, sxp, syp, szp;     // SV taper: on/off and padded source cell
+    const double* taper;       // taper factor by r^2, valid for 16 < r^2 <= 225
+    float* rec;                // receiver plane of this step ([ny][nx]) or nullptr
+    int zrec, nb, nx, ny;
+};
+
+struct Step2Args {
+    Geo g;
+    Launch l;
+    float* X;                  // level n-1 in, level n+1 (damped once) out, in place
+    const float* Y;            // level n (damped once, without this step's dipole)
+    const float* vv;           // (v*dt)^2
+    const double *fx, *fy, *fz;  // sponge factors per padded index (1.0: undamped)
+    float dx2, dy2, dz2;
+    int s0, s1;                // dipole cells
+    float ya0, ya1;            // this step's dipole, added to Y on load
+    float xa0, xa1;            // previous step's dipole, added to X on load
+    float* rec;                // receiver plane of this step or nullptr
+    int zrec, off, nx, ny;
+};
+
+hipError_t launch_vel(int op, int ty, const VelArgs& a, hipStream_t st);
+hipError_t launch_stress(int op, int ty, const StressArgs& a, hipStream_t st);
+hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st);
+
+// Pads an interior z-fastest model (nx,ny,nz) into the device layout by edge
+// replication; the interior starts at padded index `off` on every axis.
+// mode 0: copy; mode 1: (v*dt)^2 with float arithmetic.
+hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, float* dst,
+                      int mode, float dt, hipStream_t st);
+
+// Interior crop of a padded volume into z-fastest order ([ny][nx][nz]).
+// 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied
+// (fx != nullptr); 1st-order: plain copy.
+hipError_t launch_snap(const float* src, const Geo& g, int off, int nx, int ny, int nz, int s0,
+                       float a0, int s1, float a1, const double* fx, const double* fy,
+                       const double* fz, float* dst, hipStream_t st);
+
+// Record [nt][ny][nx] -> shot [ny][nx][nt], zeroing samples before the
+// direct-wave cut when mute != 0.
+struct MuteArgs {
+    int mute;
+    float dx, dy, dz, dt, fm, vcut;
+    int sx, sy, sz, tt;
+};
+hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const MuteArgs& m,
+                                 float* shot, hipStream_t st);
+
+// Sets *flag to 1 if any of the n values is not finite.
+hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st);
+
+}  // namespace a3d

@@ -1,0 +1,629 @@
+// a3d_kernels.hip -- CDNA4 (gfx950) time-stepping kernels of the acoustic
+// forward modeller.
+//
+// One kernel per sweep and operator family:
+//   k_step2   2nd-order isotropic leapfrog with the sponge folded in
+//             (reference cuda_step_fd3d + cuda_absorb_bndr + cuda_add_dipole +
+//             cuda_record, src/Toa_3dfd_cuda.cu:123-298);
+//   k_vel     1st-order velocity sweep, ISO / VTI / TTI
+//             (reference update_vel + add_source: iso:52-142, vti:54-128, tti:31-112);
+//   k_stress  1st-order stress sweep with split PML, SV taper and the
+//             receiver-plane write (reference update_stress + shot_record:
+//             iso:145-190,294-304; vti:131-203; tti:115-201).
+//
+// Every kernel is HBM-bandwidth bound (no dense contraction; ~1-3 flop/B
+// against a 20 flop/B ridge). Design: workgroup = 64 x TY threads owns an
+// x-y tile and streams a z segment. Fields differentiated along z live in a
+// per-thread register window; fields differentiated along x or y are staged
+// per plane in LDS with a 4-cell halo on the needed sides only; pointwise
+// fields are read once. Workgroup ids are remapped so each XCD (ids b with
+// equal b % 8) walks a contiguous band of tiles and its L2 serves the halo
+// rows its neighbours share.
+//
+// Arithmetic is written operation for operation as the reference writes it
+// (evaluation order, float/double mix), compiled with -ffp-contract=off and
+// IEEE division/sqrt, so the FP32 results equal the CPU restatement bit for bit.
+#include "a3d_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace a3d {
+namespace {
+
+// Staggered 8th-order coefficients (reference iso:37, vti:40, tti:17).
+constexpr float kC0 = 1.196289f, kC1 = -0.0797526f, kC2 = 0.009570313f, kC3 = -0.0006975447f;
+// 2nd-derivative coefficients (reference Toa_3dfd_cuda.cu:32, double literals stored to float).
+constexpr float kS0 = (float)(-205.0 / 72.0), kS1 = (float)(8.0 / 5.0), kS2 = (float)(-1.0 / 5.0),
+                kS3 = (float)(8.0 / 315.0), kS4 = (float)(-1.0 / 560.0);
+
+// Staggered difference over an 8-point window w(0..7) whose centre pair is
+// (w(4), w(3)): forward windows hold F[-3..+4], backward windows F[-4..+3].
+// Accumulation order m = 0..3 as the reference loop.
+template <class F>
+__device__ __forceinline__ float dst8(F w) {
+    float r = 0.0f;
+    r += kC0 * (w(4) - w(3));
+    r += kC1 * (w(5) - w(2));
+    r += kC2 * (w(6) - w(1));
+    r += kC3 * (w(7) - w(0));
+    return r;
+}
+
+// 2nd derivative over a 9-point window w(0..8) centred at w(4)
+// (reference Toa_3dfd_cuda.cu:282-289).
+template <class F>
+__device__ __forceinline__ float lap9(F w) {
+    float c = kS0 * w(4);
+    c += kS1 * (w(3) + w(5));
+    c += kS2 * (w(2) + w(6));
+    c += kS3 * (w(1) + w(7));
+    c += kS4 * (w(0) + w(8));
+    return c;
+}
+
+// Workgroup -> tile. The hardware deals workgroup b to XCD b % 8; tile ids
+// are assigned so that XCD k owns the contiguous run [k*per, (k+1)*per).
+// Tile order: x tile fastest, then y tile, then z segment.
+__device__ __forceinline__ bool decode_tile(const Launch& l, int& tix, int& tiy, int& seg) {
+    const int per = l.grid >> 3;
+    const int t = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (t >= l.ntiles) return false;
+    tix = t % l.ntx;
+    const int r = t / l.ntx;
+    tiy = r % l.nty;
+    seg = r / l.nty;
+    return true;
+}
+
+__device__ __forceinline__ float inj(float v, int i, int s0, float a0, int s1, float a1) {
+    if (i == s0) v += a0;
+    if (i == s1) v += a1;
+    return v;
+}
+
+// Sponge of the 2nd-order operator (reference cuda_absorb_bndr, y then x then z,
+// each product in double and stored to float). Factor 1.0 leaves v unchanged.
+__device__ __forceinline__ float damp3(float v, double fy, double fx, double fz) {
+    v = (float)(fy * (double)v);
+    v = (float)(fx * (double)v);
+    v = (float)(fz * (double)v);
+    return v;
+}
+
+// Loads the halo of plane offset zo into an LDS tile s[TY+8][72] whose
+// interior rows (ty+4) were written by the owning threads. HY: 4 rows above
+// and below; HX: 4 columns left and right of the thread's own row.
+template <int TY, bool HX, bool HY, class LD>
+__device__ __forceinline__ void fill_halo(float (*s)[TX + 2 * MM], const Geo& g, int zo, int x0,
+                                          int y0, int tx, int ty, int yl, bool yin, LD ld) {
+    if constexpr (HY) {
+#pragma unroll
+        for (int k = ty * TX + tx; k < 2 * MM * TX; k += TX * TY) {
+            const int r = k >> 6, c = k & 63;
+            const int yy = r < MM ? y0 - MM + r : y0 + TY + (r - MM);
+            const int sr = r < MM ? r : TY + r;
+            if (yy < g.nny) s[sr][c + MM] = ld(zo + yy * g.ld + x0 + c);
+        }
+    }
+    if constexpr (HX) {
+        if (tx < 2 * MM && yin) {
+            const int xx = tx < MM ? x0 - MM + tx : x0 + TX + (tx - MM);
+            const int sc = tx < MM ? tx : TX + tx;
+            if (xx >= 0 && xx < g.nnx) s[ty + MM][sc] = ld(zo + yl * g.ld + xx);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// 1st-order velocity sweep
+// --------------------------------------------------------------------------
+template <int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
+    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
+    constexpr bool P_Z = !VTI;   // P differentiated along z (ISO: all axes; TTI: all axes)
+    constexpr bool HAS_Q = !ISO;
+    constexpr bool Q_XY = TTI;
+    constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
+    __shared__ float sP[H][W];
+    __shared__ float sQ[Q_XY ? H : 1][Q_XY ? W : 1];
+
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    const Geo g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = tix * TX, y0 = MM + tiy * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool yin = y < g.nny;
+    const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
+    const int yl = yin ? y : g.nny - 1;
+    const int col = x + g.ld * yl;
+    const int zb = MM + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const Src S = a.s;
+
+    auto ldP = [&](int i) { return inj(a.P[i], i, S.s0, S.p0, S.s1, S.p1); };
+    auto ldQ = [&](int i) { return inj(a.Q[i], i, S.s0, S.q0, S.s1, S.q1); };
+
+    float wP[8], wQ[8];   // forward windows: F(z-3+k), k = 0..7
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const int zi = (zb - 3 + k) * g.pl + col;
+        if constexpr (P_Z) wP[k] = ldP(zi);
+        if constexpr (HAS_Q) wQ[k] = ldQ(zi);
+    }
+    const float cx1 = a.c.cx1[x < g.nnx ? x : 0], cx2 = a.c.cx2[x < g.nnx ? x : 0];
+    const float cy1 = a.c.cy1[yl], cy2 = a.c.cy2[yl];
+
+    for (int z = zb; z < ze; ++z) {
+        const int zo = z * g.pl;
+        const int id = zo + col;
+        if constexpr (P_Z) wP[7] = ldP(id + 4 * g.pl);
+        if constexpr (HAS_Q) wQ[7] = ldQ(id + 4 * g.pl);
+        float u0 = 0.f, v0 = 0.f, w0 = 0.f, r0 = 1.f, t0 = 0.f, f0 = 0.f;
+        if (valid) {
+            u0 = a.u[id];
+            v0 = a.v[id];
+            w0 = a.w[id];
+            if constexpr (!TTI) r0 = a.rho[id];
+            if constexpr (TTI) {
+                t0 = a.th[id];
+                f0 = a.ph[id];
+            }
+        }
+        float pc;
+        if constexpr (P_Z) pc = wP[3];
+        else pc = ldP(id);
+        __syncthreads();
+        sP[ty + MM][tx + MM] = pc;
+        fill_halo<TY, true, true>(sP, g, zo, x0, y0, tx, ty, yl, yin, ldP);
+        if constexpr (Q_XY) {
+            sQ[ty + MM][tx + MM] = wQ[3];
+            fill_halo<TY, true, true>(sQ, g, zo, x0, y0, tx, ty, yl, yin, ldQ);
+        }
+        __syncthreads();
+        if (valid) {
+            const int r = ty + MM, c = tx + MM;
+            float xx, yy, zz;
+            if constexpr (!TTI) {
+                yy = dst8([&](int k) { return sP[r - 3 + k][c]; });
+                xx = dst8([&](int k) { return sP[r][c - 3 + k]; });
+                if constexpr (ISO) zz = dst8([&](int k) { return wP[k]; });
+                else zz = dst8([&](int k) { return wQ[k]; });
+                xx /= r0;
+                yy /= r0;
+                zz /= r0;
+            } else {
+                const float Py = dst8([&](int k) { return sP[r - 3 + k][c]; });
+                const float Px = dst8([&](int k) { return sP[r][c - 3 + k]; });
+                const float Pz = dst8([&](int k) { return wP[k]; });
+                const float Qy = dst8([&](int k) { return sQ[r - 3 + k][c]; });
+                const float Qx = dst8([&](int k) { return sQ[r][c - 3 + k]; });
+                const float Qz = dst8([&](int k) { return wQ[k]; });
+                const float ct = (float)cos((double)t0), st = (float)sin((double)t0);
+                const float cp = (float)cos((double)f0), sp = (float)sin((double)f0);
+                xx = ct * cp * Px - ct * sp * Py - st * Pz;
+                yy = sp * Px + cp * Py;
+                zz = st * cp * Qx - st * sp * Qy + ct * Qz;
+            }
+            const float cz1 = a.c.cz1[z], cz2 = a.c.cz2[z];
+            a.u[id] = cx2 * u0 - cx1 * a.dtx * xx;
+            a.v[id] = cy2 * v0 - cy1 * a.dty * yy;
+            a.w[id] = cz2 * w0 - cz1 * a.dtz * zz;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if constexpr (P_Z) wP[k] = wP[k + 1];
+            if constexpr (HAS_Q) wQ[k] = wQ[k + 1];
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// 1st-order stress sweep
+// --------------------------------------------------------------------------
+template <int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
+    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
+    constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
+    __shared__ float sU[H][W];
+    __shared__ float sV[H][W];
+    __shared__ float sW[TTI ? H : 1][TTI ? W : 1];
+
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    const Geo g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = tix * TX, y0 = MM + tiy * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool yin = y < g.nny;
+    const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
+    const int yl = yin ? y : g.nny - 1;
+    const int col = x + g.ld * yl;
+    const int zb = MM + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+
+    auto ldU = [&](int i) { return a.u[i]; };
+    auto ldV = [&](int i) { return a.v[i]; };
+    auto ldW = [&](int i) { return a.w[i]; };
+
+    float wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const int zi = (zb - 4 + k) * g.pl + col;
+        wW[k] = a.w[zi];
+        if constexpr (TTI) {
+            wU[k] = a.u[zi];
+            wV[k] = a.v[zi];
+        }
+    }
+    const int xs = x < g.nnx ? x : 0;
+    const float ax1 = a.c.ax1[xs], ax2 = a.c.ax2[xs];
+    const float ay1 = a.c.ay1[yl], ay2 = a.c.ay2[yl];
+    const bool recx = a.rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
+
+    for (int z = zb; z < ze; ++z) {
+        const int zo = z * g.pl;
+        const int id = zo + col;
+        wW[7] = a.w[id + 3 * g.pl];
+        if constexpr (TTI) {
+            wU[7] = a.u[id + 3 * g.pl];
+            wV[7] = a.v[id + 3 * g.pl];
+        }
+        float px0 = 0.f, py0 = 0.f, pz0 = 0.f, qx0 = 0.f, qy0 = 0.f, qz0 = 0.f;
+        float c0 = 0.f, r0 = 0.f, e0 = 0.f, d0 = 0.f, t0 = 0.f, f0 = 0.f;
+        if (valid) {
+            px0 = a.px[id];
+            py0 = a.py[id];
+            pz0 = a.pz[id];
+            c0 = a.vp[id];
+            if constexpr (!TTI) r0 = a.rho[id];
+            if constexpr (!ISO) {
+                qx0 = a.qx[id];
+                qy0 = a.qy[id];
+                qz0 = a.qz[id];
+                e0 = a.eps[id];
+                d0 = a.del[id];
+            }
+            if constexpr (TTI) {
+                t0 = a.th[id];
+                f0 = a.ph[id];
+            }
+        }
+        float uc, vc;
+        if constexpr (TTI) {
+            uc = wU[4];
+            vc = wV[4];
+        } else {
+            uc = a.u[id];
+            vc = a.v[id];
+        }
+        __syncthreads();
+        sU[ty + MM][tx + MM] = uc;
+        sV[ty + MM][tx + MM] = vc;
+        fill_halo<TY, true, TTI>(sU, g, zo, x0, y0, tx, ty, yl, yin, ldU);
+        fill_halo<TY, TTI, true>(sV, g, zo, x0, y0, tx, ty, yl, yin, ldV);
+        if constexpr (TTI) {
+            sW[ty + MM][tx + MM] = wW[4];
+            fill_halo<TY, true, true>(sW, g, zo, x0, y0, tx, ty, yl, yin, ldW);
+        }
+        __syncthreads();
+        if (valid) {
+            const int r = ty + MM, c = tx + MM;
+            const float az1 = a.c.az1[z], az2 = a.c.az2[z];
+            float P, Q = 0.f;
+            if constexpr (ISO) {
+                const float yy = dst8([&](int k) { return sV[r - 4 + k][c]; });
+                const float xx = dst8([&](int k) { return sU[r][c - 4 + k]; });
+                const float zz = dst8([&](int k) { return wW[k]; });
+                const float npx = ax2 * px0 - ax1 * r0 * c0 * c0 * a.dtx * xx;
+                const float npy = ay2 * py0 - ay1 * r0 * c0 * c0 * a.dty * yy;
+                const float npz = az2 * pz0 - az1 * r0 * c0 * c0 * a.dtz * zz;
+                a.px[id] = npx;
+                a.py[id] = npy;
+                a.pz[id] = npz;
+                P = npx + npy + npz;
+            } else {
+                // SV taper (reference vti:154-175, tti:139-160)
+                float ee = e0, dd = d0;
+                if (a.sv) {
+                    const int rx = x - a.sxp, ry = y - a.syp, rz = z - a.szp;
+                    const int r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 <= 225) {
+                        if (r2 <= 16) {
+                            ee = 0.0f;
+                            dd = 0.0f;
+                        } else {
+                            const double f = a.taper[r2];
+                            ee = (float)(f * (double)e0);
+                            dd = (float)(f * (double)d0);
+                        }
+                    }
+                }
+                float xx, yy, zz;
+                if constexpr (VTI) {
+                    yy = dst8([&](int k) { return sV[r - 4 + k][c]; });
+                    xx = dst8([&](int k) { return sU[r][c - 4 + k]; });
+                    zz = dst8([&](int k) { return wW[k]; });
+                } else {
+                    const float uy = dst8([&](int k) { return sU[r - 4 + k][c]; });
+                    const float ux = dst8([&](int k) { return sU[r][c - 4 + k]; });
+                    const float uz = dst8([&](int k) { return wU[k]; });
+                    const float vy = dst8([&](int k) { return sV[r - 4 + k][c]; });
+                    const float vx = dst8([&](int k) { return sV[r][c - 4 + k]; });
+                    const float wy = dst8([&](int k) { return sW[r - 4 + k][c]; });
+                    const float wx = dst8([&](int k) { return sW[r][c - 4 + k]; });
+                    const float wz = dst8([&](int k) { return wW[k]; });
+                    const float ct = (float)cos((double)t0), st = (float)sin((double)t0);
+                    const float cp = (float)cos((double)f0), sp = (float)sin((double)f0);
+                    xx = ct * cp * ux - ct * sp * uy - st * uz;
+                    yy = sp * vx + cp * vy;
+                    zz = st * cp * wx - st * sp * wy + ct * wz;
+                }
+                const float e2 = 1 + 2 * ee;
+                const float s2 = sqrtf(1 + 2 * dd);
+                float npx, npy, npz, nqx, nqy, nqz;
+                if constexpr (VTI) {
+                    npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * a.dtx * xx;
+                    npy = ay2 * py0 - ay1 * r0 * c0 * c0 * e2 * a.dty * yy;
+                    npz = az2 * pz0 - az1 * r0 * c0 * c0 * s2 * a.dtz * zz;
+                    nqx = ax2 * qx0 - ax1 * r0 * c0 * c0 * s2 * a.dtx * xx;
+                    nqy = ay2 * qy0 - ay1 * r0 * c0 * c0 * s2 * a.dty * yy;
+                    nqz = az2 * qz0 - az1 * r0 * c0 * c0 * a.dtz * zz;
+                } else {
+                    npx = ax2 * px0 - ax1 * c0 * c0 * e2 * a.dtx * xx;
+                    npy = ay2 * py0 - ay1 * c0 * c0 * e2 * a.dty * yy;
+                    npz = az2 * pz0 - az1 * c0 * c0 * s2 * a.dtz * zz;
+                    nqx = ax2 * qx0 - ax1 * c0 * c0 * s2 * a.dtx * xx;
+                    nqy = ay2 * qy0 - ay1 * c0 * c0 * s2 * a.dty * yy;
+                    nqz = az2 * qz0 - az1 * c0 * c0 * a.dtz * zz;
+                }
+                a.px[id] = npx;
+                a.py[id] = npy;
+                a.pz[id] = npz;
+                a.qx[id] = nqx;
+                a.qy[id] = nqy;
+                a.qz[id] = nqz;
+                P = npx + npy + npz;
+                Q = nqx + nqy + nqz;
+                a.Q[id] = Q;
+            }
+            a.P[id] = P;
+            if (recx && z == a.zrec) a.rec[(y - a.nb) * a.nx + (x - a.nb)] = P;
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            wW[k] = wW[k + 1];
+            if constexpr (TTI) {
+                wU[k] = wU[k + 1];
+                wV[k] = wV[k + 1];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// 2nd-order step with sponge, dipole and receivers folded in
+// --------------------------------------------------------------------------
+template <int TY>
+__global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
+    constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
+    __shared__ float sY[H][W];
+
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    const Geo g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = tix * TX, y0 = MM + tiy * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool yin = y < g.nny;
+    const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
+    const int yl = yin ? y : g.nny - 1;
+    const int col = x + g.ld * yl;
+    const int zb = MM + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+
+    auto ldY = [&](int i) { return inj(a.Y[i], i, a.s0, a.ya0, a.s1, a.ya1); };
+
+    float wY[9];   // Y(z-4+k), k = 0..8
+#pragma unroll
+    for (int k = 0; k < 8; ++k) wY[k] = ldY((zb - 4 + k) * g.pl + col);
+    const int xs = x < g.nnx ? x : 0;
+    const double fx = a.fx[xs], fy = a.fy[yl];
+    const bool recx = a.rec && valid && x >= a.off && x < a.off + a.nx && y >= a.off && y < a.off + a.ny;
+
+    for (int z = zb; z < ze; ++z) {
+        const int zo = z * g.pl;
+        const int id = zo + col;
+        wY[8] = ldY(id + 4 * g.pl);
+        float xo = 0.f, vv = 0.f;
+        if (valid) {
+            xo = inj(a.X[id], id, a.s0, a.xa0, a.s1, a.xa1);
+            vv = a.vv[id];
+        }
+        __syncthreads();
+        sY[ty + MM][tx + MM] = wY[4];
+        fill_halo<TY, true, true>(sY, g, zo, x0, y0, tx, ty, yl, yin, ldY);
+        __syncthreads();
+        if (valid) {
+            const int r = ty + MM, c = tx + MM;
+            const double fz = a.fz[z];
+            float c1 = lap9([&](int k) { return wY[k]; });              // z
+            float c2 = lap9([&](int k) { return sY[r][c - 4 + k]; });   // x
+            float c3 = lap9([&](int k) { return sY[r - 4 + k][c]; });   // y
+            c1 *= a.dz2;
+            c2 *= a.dx2;
+            c3 *= a.dy2;
+            const float cur = wY[4];
+            const float prev = damp3(xo, fy, fx, fz);
+            const float lap = vv * (c1 + c2 + c3);
+            const float nv = (float)(2.0 * (double)cur - (double)prev + (double)lap);
+            a.X[id] = damp3(nv, fy, fx, fz);
+            if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = damp3(cur, fy, fx, fz);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) wY[k] = wY[k + 1];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Setup and output kernels
+// --------------------------------------------------------------------------
+__global__ void k_pad(const float* __restrict__ src, int nx, int ny, int nz, int off, Geo g,
+                      float* __restrict__ dst, int mode, float dt) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= g.ld) return;
+    float v = 0.0f;
+    if (x < g.nnx) {
+        const int cx = min(max(x - off, 0), nx - 1);
+        const int cy = min(max(y - off, 0), ny - 1);
+        const int cz = min(max(z - off, 0), nz - 1);
+        v = src[cz + (size_t)nz * cx + (size_t)nz * nx * cy];
+        if (mode == 1) {
+            const float t = v * dt;
+            v = t * t;
+        }
+    }
+    dst[x + (size_t)g.ld * y + (size_t)g.pl * z] = v;
+}
+
+// One thread per (ix, iy) column of the interior; loops z.
+__global__ void k_snap(const float* __restrict__ src, Geo g, int off, int nx, int ny, int nz, int s0,
+                       float a0, int s1, float a1, const double* fx, const double* fy,
+                       const double* fz, float* __restrict__ dst) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    if (ix >= nx) return;
+    const int x = ix + off, y = iy + off;
+    for (int iz = 0; iz < nz; ++iz) {
+        const int z = iz + off;
+        const int i = x + g.ld * y + g.pl * z;
+        float v = inj(src[i], i, s0, a0, s1, a1);
+        if (fx) v = damp3(v, fy[y], fx[x], fz[z]);
+        dst[iz + (size_t)nz * ix + (size_t)nz * nx * iy] = v;
+    }
+}
+
+// [nt][ny][nx] -> [ny][nx][nt] through a 64x64 LDS tile, with the mute
+// (reference mute_directwave, iso:322-369).
+__global__ void k_shot_transpose(const float* __restrict__ rec, int nt, int nx, int ny, MuteArgs m,
+                                 float* __restrict__ shot) {
+    __shared__ float t[64][65];
+    const int it0 = blockIdx.x * 64, ix0 = blockIdx.y * 64, iy = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;   // 64 x 4
+    for (int j = ty; j < 64; j += 4) {
+        const int it = it0 + j, ix = ix0 + tx;
+        t[j][tx] = (it < nt && ix < nx) ? rec[(size_t)it * nx * ny + (size_t)iy * nx + ix] : 0.0f;
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        const int ix = ix0 + j, it = it0 + tx;
+        if (ix < nx && it < nt) {
+            float v = t[tx][j];
+            if (m.mute) {
+                const float mx = m.dx * (float)abs(ix - m.sx);
+                const float my = m.dy * (float)abs(iy - m.sy);
+                const float mz = m.dz * (float)m.sz;
+                const float t0 = sqrtf(mx * mx + my * my + mz * mz) / m.vcut;
+                const int mu_t = (int)(2.0 / (double)(m.dt * m.fm));
+                const int mu_nt = (int)(t0 / m.dt) + mu_t + m.tt;
+                if (it < mu_nt) v = 0.0f;
+            }
+            shot[(size_t)iy * nx * nt + (size_t)ix * nt + it] = v;
+        }
+    }
+}
+
+__global__ void k_check_finite(const float* __restrict__ a, size_t n, int* flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    for (; i < n; i += stride) bad |= !isfinite(a[i]);
+    if (bad) *flag = 1;
+}
+
+template <int OP>
+hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
+    const dim3 grid(a.l.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_vel<OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_vel<OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_vel<OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int OP>
+hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
+    const dim3 grid(a.l.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_stress<OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stress<OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_stress<OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_vel(int op, int ty, const VelArgs& a, hipStream_t st) {
+    switch (op) {
+        case 1: return vel_ty<1>(ty, a, st);
+        case 2: return vel_ty<2>(ty, a, st);
+        case 3: return vel_ty<3>(ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stress(int op, int ty, const StressArgs& a, hipStream_t st) {
+    switch (op) {
+        case 1: return stress_ty<1>(ty, a, st);
+        case 2: return stress_ty<2>(ty, a, st);
+        case 3: return stress_ty<3>(ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st) {
+    const dim3 grid(a.l.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_step2<2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_step2<4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_step2<8>), grid, dim3(TX, 8), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, float* dst,
+                      int mode, float dt, hipStream_t st) {
+    const dim3 grid((g.ld + 255) / 256, g.nny, g.nnz);
+    hipLaunchKernelGGL(k_pad, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, dst, mode, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_snap(const float* src, const Geo& g, int off, int nx, int ny, int nz, int s0,
+                       float a0, int s1, float a1, const double* fx, const double* fy,
+                       const double* fz, float* dst, hipStream_t st) {
+    const dim3 grid((nx + 63) / 64, ny);
+    hipLaunchKernelGGL(k_snap, grid, dim3(64), 0, st, src, g, off, nx, ny, nz, s0, a0, s1, a1, fx, fy,
+                       fz, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const MuteArgs& m,
+                                 float* shot, hipStream_t st) {
+    const dim3 grid((nt + 63) / 64, (nx + 63) / 64, ny);
+    hipLaunchKernelGGL(k_shot_transpose, grid, dim3(64, 4), 0, st, rec, nt, nx, ny, m, shot);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st) {
+    hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, a, n, flag);
+    return hipGetLastError();
+}
+
+}  // namespace a3d

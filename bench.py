@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: 1st-order VTI pseudo-acoustic shot, 400 x 400 x 300 cells,
+dx = 12.5 m, dt = 1 ms, nt = 3000, 20-cell split PML, FP32, on one MI355X.
+
+A "step" is one full shot (nt time steps of the velocity and stress sweeps)
+over the synthetic seeded model, with the model and wavefields resident in
+HBM when the timed region starts. value = grid-point updates per second
+(updated points x nt x shots / seconds). The roofline entry is the stress
+sweep (the dominant kernel): algorithmic bytes per launch / its average
+launch time from HIP events recorded around every launch on the context
+stream. cpu_baseline: the FP32 CPU oracle (OpenMP over all host cores) on a
+bounded number of time steps of the same model, scaled to the same unit.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "acoustic3d-fd_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per updated point per time step of the VTI formulation
+# implemented (reference layout: split PML fields stored everywhere, FP32).
+# velocity sweep: R P,Q,rho,u,v,w; W u,v,w            -> 9 x 4 = 36 B
+# stress sweep:   R u,v,w, 6 split, vp,rho,eps,delta;
+#                 W 6 split, P, Q                      -> 21 x 4 = 84 B
+A_VEL, A_STRESS = 36, 84
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, target_s=15.0):
+    """Times the CPU oracle's FP32 time loop on a bounded number of steps."""
+    spec = importlib.util.spec_from_file_location("oracle_binding", os.path.join(ROOT, "oracle", "binding.py"))
+    orc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(orc)
+    if not os.path.exists(orc.LIB_PATH):
+        return None
+    orc.load()
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    shot = orc.make_shot("vti", cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"],
+                         cfg["sy"], cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], nthreads=cores)
+    m = dict(vel=cfg["vel"], rho=cfg["rho"], eps=cfg["eps"], delta=cfg["delta"])
+    t2 = orc.time_steps(shot, 2, **m)
+    nsteps = int(max(2, min(200, target_s / max(t2 / 2, 1e-3))))
+    secs = orc.time_steps(shot, nsteps, **m)
+    npts = (cfg["nx"] + 2 * 20 - 8) * (cfg["ny"] + 2 * 20 - 8) * (cfg["nz"] + 2 * 20 - 8)
+    return dict(value=npts * nsteps / secs / 1e9, unit="Gpt/s", cores=cores, kind="port",
+                sample=f"{nsteps} time steps of the same VTI shot (FP32 oracle, OpenMP), {secs:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", type=str, default="", help="ty_vel,ty_stress,zseg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs every shot on a single MI355X "
+              "(multi-GPU shot sharding is a design note, not part of this build).", file=sys.stderr)
+        sys.exit(2)
+
+    from a3dfd import Modeller, make_desc
+    from a3dfd import models
+
+    t0 = time.time()
+    cfg = models.cfg3()
+    cfg["nt"] = args.nt
+    log(f"bench: model generated in {time.time() - t0:.1f} s")
+    desc = make_desc("vti", cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"],
+                     cfg["sy"], cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"])
+    mdl = Modeller(desc, device=0)
+    mdl.set_model(vel=cfg["vel"], rho=cfg["rho"], eps=cfg["eps"], delta=cfg["delta"])
+    if args.launch:
+        mdl.set_launch(*[int(x) for x in args.launch.split(",")])
+    gi = mdl.grid_info()
+    npts = gi["updated_points"]
+    log(f"bench: padded {gi['nnx']}x{gi['nny']}x{gi['nnz']}, updated points {npts}, launch {mdl.launch_info()}")
+
+    for w in range(args.warmup):
+        _, _, ms = mdl.run(shot=False)
+        log(f"bench: warmup shot {w}: {ms:.1f} ms loop")
+
+    mdl.set_kernel_timing(True)
+    mdl.kernel_times(reset=True)
+    loop_ms = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        _, _, ms = mdl.run(shot=False)
+        loop_ms.append(ms)
+        log(f"bench: shot {k}: {ms:.1f} ms loop, {npts * args.nt / ms / 1e6:.2f} Gpt/s")
+    t_wall = time.perf_counter() - t_start
+    kms, kn = mdl.kernel_times(reset=True)
+    mdl.close()
+
+    total_loop_s = sum(loop_ms) / 1e3
+    gpts = npts * args.nt * args.steps / total_loop_s / 1e9
+    avg_vel = kms[0] / max(kn[0], 1)
+    avg_str = kms[1] / max(kn[1], 1)
+    ach_str = A_STRESS * npts / (avg_str * 1e-3) / 1e9
+    ach_vel = A_VEL * npts / (avg_vel * 1e-3) / 1e9
+    step_gbs = (A_VEL + A_STRESS) * gpts
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_vti_400x400x300.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("stress_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (FP32 oracle) ...")
+        try:
+            cpu = cpu_baseline(cfg)
+        except Exception as e:  # the baseline is informative; never fail the GPU line on it
+            log(f"bench: CPU baseline failed: {e}")
+
+    line = {
+        "metric": "Gpt-updates/s (VTI 400x400x300 FP32 shot) & % HBM roofline",
+        "value": round(gpts, 3),
+        "unit": "Gpt/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic seeded VTI model (layered 1500-5000 m/s + salt body, Gardner density, "
+                "smooth epsilon/delta); inputs resident in HBM",
+        "config": {"workload": "vti_400x400x300_nt%d" % args.nt, "nx": cfg["nx"], "ny": cfg["ny"],
+                   "nz": cfg["nz"], "npml": 20, "nt": args.nt, "dx": cfg["dx"], "dt": cfg["dt"],
+                   "updated_points": npts, "step": "one shot", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_stress<VTI>", "achieved": round(ach_str, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "bytes_per_point": A_STRESS,
+                     "avg_launch_ms": round(avg_str, 4), "launches": kn[1]},
+        "kernels": {"k_vel<VTI>": {"avg_launch_ms": round(avg_vel, 4), "achieved_GBs": round(ach_vel, 1),
+                                   "bytes_per_point": A_VEL},
+                    "whole_step": {"achieved_GBs": round(step_gbs, 1),
+                                   "frac": round(step_gbs / HBM_PEAK_GBS, 4), "bytes_per_point": A_VEL + A_STRESS}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

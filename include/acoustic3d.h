@@ -1,0 +1,168 @@
+/*
+ * acoustic3d.h -- C ABI of libacoustic3d.so, the MI355X (gfx950) 3D acoustic
+ * finite-difference forward modeller.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The four legacy entry points, with the exact parameter lists and
+ *     semantics of the reference library symbols that readmodel.py binds with
+ *     ctypes (one call = one shot: read model files, pad, time-step, record,
+ *     optionally mute and write snapshots, write the shot file).
+ *
+ *  2. A handle API (a3d_*) for in-memory models and outputs, model data that
+ *     stays resident in HBM across shots, per-kernel timing, and the
+ *     parameters the legacy symbols hard-code (boundary width, receiver depth,
+ *     mute velocity, snapshot cadence).
+ *
+ * Every a3d_* function returns A3D_OK (0) or a negative A3D_E* code;
+ * a3d_last_error() returns a message for the calling thread's last failure.
+ * The legacy void symbols print the message and return (the reference calls
+ * exit(0) on a GPU error; this library never terminates the host process).
+ *
+ * Threading: contexts are independent; one context must not be used by two
+ * host threads at once. All GPU work of a context runs on its own HIP stream.
+ */
+#ifndef ACOUSTIC3D_H
+#define ACOUSTIC3D_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------- */
+/* 1. Legacy entry points                                                   */
+/* ----------------------------------------------------------------------- */
+
+/* Replaces cuda_3dfd, reference src/Toa_3dfd_cuda.cu:366-370 (acoustic2order.so;
+ * bound by readmodel.py:324-339). 2nd-order isotropic operator, 50-cell sponge,
+ * receivers at interior z=1, snapshots every 500 steps when show_snapshot;
+ * kt, cut_directwave and snap_interval are accepted and ignored, as in the
+ * reference (readmodel.py passes the last two swapped). */
+void cuda_3dfd(char* FNvel, char* FNsnap, char* FNshot, int is, int ns, int nx, int ny, int nz,
+               float dx, float dy, float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy,
+               int jsz, float dgx, float dgy, float dgt, int nt, int kt, float dt, float fm,
+               bool show_snapshot, bool cut_directwave, int snap_interval, int cudaDevicei);
+
+/* Replaces cuda_3dfd_1order, reference src/Toa_gpu_3diso_fd_1orderfunciton.cu:372-377
+ * (acoustic1order.so; bound by readmodel.py:228-244). Staggered-grid isotropic
+ * operator with density, 20-cell split PML, z-dipole source (pfac 1e4), wavelet
+ * types 1..6, receivers at interior z=0, snapshots when it % snap_interval == 0. */
+void cuda_3dfd_1order(char* FNvel, char* FNrho, char* FNsnap, char* FNshot, int is, int ns,
+                      int nx, int ny, int nz, float dx, float dy, float dz, int sxbeg, int sybeg,
+                      int szbeg, int jsx, int jsy, int jsz, float dgx, float dgy, float dgt, int nt,
+                      float dt, int wtype, float fm, bool show_snapshot, bool cut_directwave,
+                      int snap_interval, int cudaDevicei);
+
+/* Replaces cuda_3dfd_vti, reference src/Toa_gpu_3dvti_fd_1orderfunciton.cu:389-395
+ * (acoustic1vti.so; bound by readmodel.py:278-296). VTI pseudo-acoustic, 20-cell
+ * split PML, Ricker monopole into P and Q (pfac 10), SV taper around the source. */
+void cuda_3dfd_vti(char* FNvel, char* FNrho, char* FNepsilon, char* FNdelta, char* FNsnap,
+                   char* FNshot, int is, int ns, int nx, int ny, int nz, float dx, float dy,
+                   float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy, int jsz, float dgx,
+                   float dgy, float dgt, int nt, float dt, float fm, bool show_snapshot,
+                   bool cut_directwave, int snap_interval, int cudaDevicei);
+
+/* Replaces cuda_3dfd_tti, reference src/Toa_gpu_3dtti_fd_1orderfunciton.cu:390-398
+ * (acoustic1tti.so; bound by readmodel.py:366-386). TTI pseudo-acoustic without
+ * density (FNrho is accepted and not read), 40-cell split PML, one shot (ns is
+ * forced to 1), snapshots every 100 steps for it != 0 regardless of
+ * show_snapshot. Returns 0, or 1 when the shot file cannot be opened; other
+ * failures return a negative A3D_E* code (the reference returns 0 or 1 only). */
+int cuda_3dfd_tti(char* FNvel, char* FNrho, char* FNepsilon, char* FNdelta, char* FNtheta,
+                  char* FNpsi, char* FNsnap, char* FNshot, int is, int ns, int nx, int ny, int nz,
+                  float dx, float dy, float dz, int sxbeg, int sybeg, int szbeg, int jsx, int jsy,
+                  int jsz, float dgx, float dgy, float dgt, int nt, float dt, float fm,
+                  bool show_snapshot, bool cut_directwave, int snap_interval, int cudaDevicei);
+
+/* ----------------------------------------------------------------------- */
+/* 2. Handle API (no counterpart in the reference: in-memory form of the    */
+/*    entry points above, used by the Python driver, the tests and bench)   */
+/* ----------------------------------------------------------------------- */
+
+enum {
+    A3D_OK = 0,
+    A3D_EINVAL = -1,      /* bad argument */
+    A3D_EHIP = -2,        /* HIP runtime error */
+    A3D_ENOMEM = -3,      /* device or host allocation failed */
+    A3D_EIO = -4,         /* file missing, short, or not writable */
+    A3D_EUNSTABLE = -5,   /* non-finite wavefield detected */
+    A3D_ESTATE = -6       /* call out of order (e.g. run before set_model) */
+};
+
+enum { A3D_OP_2ND = 0, A3D_OP_ISO = 1, A3D_OP_VTI = 2, A3D_OP_TTI = 3 };
+
+/* One shot's description. Conventions follow the reference operator chosen:
+ * op 2ND: source 0-based interior index, boundary = sponge width npd
+ *         (plus the 4-cell ring), receivers default rec_z = 1;
+ * op ISO/VTI/TTI: source 1-based interior index, boundary = npml,
+ *         receivers default rec_z = 0. */
+typedef struct a3d_shot_desc {
+    int op;
+    int nx, ny, nz;        /* interior model size */
+    float dx, dy, dz;
+    int nb;                /* boundary cells (reference: npd 50; npml 20, 20, 40) */
+    int sx, sy, sz;        /* source position (see above) */
+    int nt;
+    float dt, fm;
+    int wtype;             /* ISO: 1..6; VTI/TTI reference: 1; 2ND: Ricker table */
+    float pfac;            /* 1st-order source factor (reference: 1e4 ISO, 10 VTI/TTI) */
+    int rec_z;             /* receiver plane, interior z index */
+    int sv;                /* VTI/TTI: SV-suppression taper around the source */
+    int snap_interval;     /* 0: no snapshots */
+    int snap_skip0;        /* 1: no snapshot at it == 0 */
+    int mute;              /* 1: direct-wave mute of the shot record */
+    float v_mute;          /* <= 0: reference velocity taken from the model */
+    int mute_tt;           /* mute pad in samples (reference: 60) */
+    int storage;           /* 0: FP32 wavefields */
+} a3d_shot_desc;
+
+typedef struct a3d_ctx a3d_ctx;
+
+/* Allocates device memory for the geometry and operator of *desc on device
+ * `device` and creates the context's stream. */
+int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out);
+int a3d_destroy(a3d_ctx* ctx);
+
+/* Uploads the model: interior arrays of nx*ny*nz floats, z fastest
+ * (iz + nz*ix + nz*nx*iy) as in the model files; NULL for arrays the operator
+ * does not use (2ND: vel; ISO: vel, rho; VTI: vel, rho, eps, del; TTI: vel, eps,
+ * del, theta, phi in radians). Pads by edge replication and builds the tables. */
+int a3d_set_model(a3d_ctx* ctx, const float* vel, const float* rho, const float* eps,
+                  const float* del, const float* theta, const float* phi);
+
+/* Runs one shot. desc gives the source, time axis and output options; its
+ * geometry fields (op, nx, ny, nz, dx, dy, dz, nb) must equal the context's.
+ * shot_out: host [ny][nx][nt] (time fastest), undecimated; may be NULL (the
+ * record stays on the device). snaps_out: host, up to nsnap_max interior
+ * crops of nz*nx*ny (z fastest); may be NULL. *nsnap_taken receives the
+ * number of snapshot steps. *loop_ms receives the time loop's duration
+ * (HIP events on the context stream), excluding setup and host copies. */
+int a3d_run(a3d_ctx* ctx, const a3d_shot_desc* desc, float* shot_out, float* snaps_out,
+            int nsnap_max, int* nsnap_taken, double* loop_ms);
+
+/* Per-kernel timing: when enabled, every launch of the two hot kernels is
+ * bracketed by HIP events on the context stream. a3d_kernel_times returns
+ * the summed milliseconds and launch counts since the last reset (reset=1
+ * clears after reading). Index 0: the 2nd-order step or 1st-order
+ * velocity sweep; index 1: the 1st-order stress sweep. */
+int a3d_set_kernel_timing(a3d_ctx* ctx, int enable);
+int a3d_kernel_times(a3d_ctx* ctx, double* ms, long long* launches, int reset);
+
+/* Launch geometry in use: tile rows TY and z segments per sweep (for tuning). */
+int a3d_set_launch(a3d_ctx* ctx, int ty_vel, int ty_stress, int zseg);
+int a3d_get_launch(a3d_ctx* ctx, int* ty_vel, int* ty_stress, int* zseg, int* blocks_vel,
+                   int* blocks_stress);
+
+/* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
+int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
+
+const char* a3d_last_error(void);
+const char* a3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,121 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg load this module, and only as the checker or the timed CPU
+baseline. The product (acoustic3d-fd_amd/) never imports it.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+OPS = {"2nd": 0, "iso": 1, "vti": 2, "tti": 3}
+
+
+class OracleShot(ctypes.Structure):
+    _fields_ = [
+        ("op", c_int), ("nx", c_int), ("ny", c_int), ("nz", c_int),
+        ("dx", c_float), ("dy", c_float), ("dz", c_float),
+        ("nb", c_int), ("sx", c_int), ("sy", c_int), ("sz", c_int),
+        ("nt", c_int), ("dt", c_float), ("fm", c_float),
+        ("wtype", c_int), ("pfac", c_float), ("rec_z", c_int), ("sv", c_int),
+        ("snap_interval", c_int), ("snap_skip0", c_int),
+        ("mute", c_int), ("v_mute", c_float), ("mute_tt", c_int),
+        ("nthreads", c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
+    lib = ctypes.CDLL(LIB_PATH)
+    fp = POINTER(c_float)
+    lib.oracle_run.argtypes = [c_int, POINTER(OracleShot), fp, fp, fp, fp, fp, fp, c_void_p, c_void_p, c_int,
+                               POINTER(c_double)]
+    lib.oracle_run.restype = c_int
+    lib.oracle_time_steps.argtypes = [POINTER(OracleShot), fp, fp, fp, fp, fp, fp, c_int]
+    lib.oracle_time_steps.restype = c_double
+    lib.oracle_ricker_table.argtypes = [c_int, c_float, c_float, c_int, c_void_p]
+    lib.oracle_wavelet.argtypes = [c_int, c_int, c_float, c_float, c_int, c_void_p]
+    lib.oracle_pml_tables.argtypes = [c_int, c_int, c_int, c_float, c_float, c_void_p, c_void_p, c_void_p,
+                                      c_void_p]
+    lib.oracle_d0.argtypes = [fp, c_int, c_int, c_int, c_int, c_float, c_float, c_float]
+    lib.oracle_d0.restype = c_double
+    lib.oracle_pad.argtypes = [fp, c_int, c_int, c_int, c_int, fp]
+    lib.oracle_sponge_factor.argtypes = [c_int, c_int, POINTER(c_double)]
+    lib.oracle_taper.argtypes = [c_int, POINTER(c_double)]
+    lib.oracle_staggered_dx.argtypes = [fp, c_int, c_int, fp]
+    lib.oracle_laplacian_z.argtypes = [fp, c_int, fp]
+    _lib = lib
+    return lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_float))
+
+
+def _arr(a, shape):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+# Reference per-operator constants (see the reference files cited in oracle.cpp).
+DEFAULTS = {
+    "2nd": dict(nb=50, pfac=1.0, rec_z=1, sv=0, wtype=1),
+    "iso": dict(nb=20, pfac=10000.0, rec_z=0, sv=0, wtype=1),
+    "vti": dict(nb=20, pfac=10.0, rec_z=0, sv=1, wtype=1),
+    "tti": dict(nb=40, pfac=10.0, rec_z=0, sv=1, wtype=1),
+}
+
+
+def make_shot(op, nx, ny, nz, dx, dy, dz, sx, sy, sz, nt, dt, fm, nthreads=0, **kw):
+    d = dict(DEFAULTS[op])
+    d.update(dict(snap_interval=0, snap_skip0=0, mute=0, v_mute=0.0, mute_tt=60))
+    d.update(kw)
+    return OracleShot(OPS[op], nx, ny, nz, dx, dy, dz, d["nb"], sx, sy, sz, nt, dt, fm, d["wtype"], d["pfac"],
+                      d["rec_z"], d["sv"], d["snap_interval"], d["snap_skip0"], d["mute"], d["v_mute"],
+                      d["mute_tt"], nthreads)
+
+
+def run(shot, prec=32, vel=None, rho=None, eps=None, delta=None, theta=None, phi=None, nsnap_max=0):
+    """Runs the oracle; returns (shot (ny,nx,nt), snaps (k,ny,nx,nz) or None, loop seconds)."""
+    lib = load()
+    shape = (shot.ny, shot.nx, shot.nz)
+    arrs = [_arr(a, shape) for a in (vel, rho, eps, delta, theta, phi)]
+    dt = np.float32 if prec == 32 else np.float64
+    out = np.zeros((shot.ny, shot.nx, shot.nt), dt)
+    snaps = np.zeros((max(nsnap_max, 1), shot.ny, shot.nx, shot.nz), dt) if nsnap_max > 0 else None
+    secs = c_double(0.0)
+    n = lib.oracle_run(prec, ctypes.byref(shot), *[_fp(a) for a in arrs], out.ctypes.data_as(c_void_p),
+                       None if snaps is None else snaps.ctypes.data_as(c_void_p), nsnap_max,
+                       ctypes.byref(secs))
+    if n < 0:
+        raise RuntimeError(f"oracle_run failed with {n}")
+    if snaps is not None:
+        snaps = snaps[: min(n, nsnap_max)]
+    return out, snaps, secs.value
+
+
+def time_steps(shot, nsteps, vel=None, rho=None, eps=None, delta=None, theta=None, phi=None):
+    lib = load()
+    shape = (shot.ny, shot.nx, shot.nz)
+    arrs = [_arr(a, shape) for a in (vel, rho, eps, delta, theta, phi)]
+    return lib.oracle_time_steps(ctypes.byref(shot), *[_fp(a) for a in arrs], nsteps)

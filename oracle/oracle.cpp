@@ -1,0 +1,728 @@
+/*
+ * oracle.cpp -- TEST INFRASTRUCTURE, not product code.
+ *
+ * A CPU restatement of the reference's forward-modelling arithmetic
+ * (sh39o/Acoustic3d-fd), one function per reference kernel, each citing the
+ * reference file:line it follows. The layout is the reference's own padded,
+ * z-fastest volume (id = iz + ix*nnz + iy*nnz*nnx), so an indexing or
+ * layout mistake in the GPU path (which uses x-fastest, z-slowest storage)
+ * cannot cancel against the same mistake here.
+ *
+ * Instances: T = float is the parity oracle; every expression keeps the
+ * reference's float/double mix (C++ promotion of float and double operands
+ * is the same on the host as in CUDA device code). T = double is the FP64
+ * error reference: every value and table in double.
+ *
+ * Where the reference calls a CUDA device math function on the per-shot or
+ * per-step path (exp, expf, cos, log, pow, erf), this file calls glibc; the
+ * CUDA Toolkit's results for those calls are not reproduced (parity with
+ * CUDA's math library at the last ulp is unpinned). pow(x, 2) is restated
+ * as x*x (the correctly rounded square). The per-point float cos/sin of the
+ * TTI rotation is restated as the correctly rounded float value,
+ * (float)cos((double)x).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * use this file, and only as the checker.
+ */
+#include "oracle.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int mm = 4;                     /* half stencil length, all files */
+constexpr double kPi = 3.141592653;       /* Toa_3dfd_cuda.cu:14, iso:28, vti:31, tti:8 */
+
+/* Staggered 8th-order coefficients: iso:37, vti:40, tti:17 (float constants). */
+template <class T> struct Coef;
+template <> struct Coef<float> {
+    static constexpr float c[4] = {1.196289f, -0.0797526f, 0.009570313f, -0.0006975447f};
+    /* Toa_3dfd_cuda.cu:32: double literals rounded to float */
+    static constexpr float s[5] = {(float)(-205.0 / 72.0), (float)(8.0 / 5.0), (float)(-1.0 / 5.0),
+                                   (float)(8.0 / 315.0), (float)(-1.0 / 560.0)};
+};
+template <> struct Coef<double> {
+    static constexpr double c[4] = {1.196289, -0.0797526, 0.009570313, -0.0006975447};
+    static constexpr double s[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+};
+constexpr float Coef<float>::c[4];
+constexpr float Coef<float>::s[5];
+constexpr double Coef<double>::c[4];
+constexpr double Coef<double>::s[5];
+
+/* Forward staggered difference (update_vel: iso:125-127, vti:113-115, tti:94-99):
+ * r += c[m] * (a[id + (m+1)s] - a[id - m s]), m = 0..3 in order. */
+template <class T>
+inline T dfwd(const T* a, long id, long s) {
+    T r = 0;
+    for (int m = 0; m < mm; ++m) r += Coef<T>::c[m] * (a[id + (m + 1) * s] - a[id - m * s]);
+    return r;
+}
+/* Backward staggered difference (update_stress: iso:178-180, vti:183-185, tti:174-182). */
+template <class T>
+inline T dbwd(const T* a, long id, long s) {
+    T r = 0;
+    for (int m = 0; m < mm; ++m) r += Coef<T>::c[m] * (a[id + m * s] - a[id - (m + 1) * s]);
+    return r;
+}
+
+/* 8th-order second derivative along one axis, accumulated as in
+ * cuda_step_fd3d (Toa_3dfd_cuda.cu:281-289): c = s0*cur, then c += s[i]*(a[-i]+a[+i]). */
+template <class T>
+inline T lap1(const T* a, long id, long s) {
+    T c = Coef<T>::s[0] * a[id];
+    for (int i = 1; i <= mm; ++i) c += Coef<T>::s[i] * (a[id - i * s] + a[id + i * s]);
+    return c;
+}
+
+template <class T> inline T fsqrt(T x) { return std::sqrt(x); }   /* sqrtf for float */
+template <class T> inline T fcos(T x) { return (T)std::cos((double)x); }
+template <class T> inline T fsin(T x) { return (T)std::sin((double)x); }
+
+/* pad_vv (iso:198-238, vti:211-251, tti:209-249): edge replication in three
+ * passes, x then y then z, over a padded z-fastest volume whose interior was
+ * filled by read_file (iso:241-260). */
+template <class T>
+void pad_vv(int nnx, int nny, int nnz, int npml, T* ee) {
+    const long pl = (long)nnz * nnx;
+    for (int iy = 0; iy < nny; iy++)
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                long id = iz + (long)ix * nnz + iy * pl;
+                if (ix < npml) ee[id] = ee[iz + (long)npml * nnz + iy * pl];
+                else if (ix >= nnx - npml) ee[id] = ee[iz + (long)(nnx - npml - 1) * nnz + iy * pl];
+            }
+    for (int iy = 0; iy < nny; iy++)
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                long id = iz + (long)ix * nnz + iy * pl;
+                if (iy < npml) ee[id] = ee[iz + (long)ix * nnz + npml * pl];
+                else if (iy >= nny - npml) ee[id] = ee[iz + (long)ix * nnz + (nny - npml - 1) * pl];
+            }
+    for (int iy = 0; iy < nny; iy++)
+        for (int ix = 0; ix < nnx; ix++)
+            for (int iz = 0; iz < nnz; iz++) {
+                long id = iz + (long)ix * nnz + iy * pl;
+                if (iz < npml) ee[id] = ee[npml + (long)ix * nnz + iy * pl];
+                else if (iz >= nnz - npml) ee[id] = ee[nnz - npml - 1 + (long)ix * nnz + iy * pl];
+            }
+}
+
+template <class T>
+void read_interior(const float* src, int nx, int ny, int nz, int nnx, int nnz, int off, T* dst) {
+    /* read_file (iso:249-257): interior at offset npml in every axis, file order z fastest */
+    const long pl = (long)nnz * nnx;
+    for (int iy = 0; iy < ny; iy++)
+        for (int ix = 0; ix < nx; ix++)
+            for (int iz = 0; iz < nz; iz++)
+                dst[(iz + off) + (long)(ix + off) * nnz + (iy + off) * pl] =
+                    (T)src[iz + (long)nz * ix + (long)nz * nx * iy];
+}
+
+/* initial_coffe (iso:263-291; identical vti:282-310, tti:282-310): per-axis
+ * split-PML profiles. dt*d0 is a float product in the reference, then the
+ * expression is evaluated in double and stored to float. */
+template <class T>
+void initial_coffe(T dt, T d0, int nn, int npml, T* coff1, T* coff2, T* acoff1, T* acoff2) {
+    const T dtd0 = dt * d0;
+    for (int id = 0; id < nn + 2 * npml; ++id) {
+        if (id < npml) {
+            double r = (npml - 0.5 - id) / npml;
+            double ra = ((npml - id) * 1.0) / npml;
+            coff1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
+            coff2[id] = (T)(coff1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
+            acoff1[id] = (T)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
+            acoff2[id] = (T)(acoff1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
+        } else if (id < npml + nn) {
+            coff1[id] = coff2[id] = acoff1[id] = acoff2[id] = (T)1.0;
+        } else {
+            double r = (0.5 + id - nn - npml) / npml;
+            double ra = ((id - nn - npml) * 1.0) / npml;
+            coff1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
+            coff2[id] = (T)(coff1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
+            acoff1[id] = (T)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
+            acoff2[id] = (T)(acoff1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
+        }
+    }
+}
+
+/* get_d0 (iso:193-195): uses the padded z-fastest velocity at linear index N/2. */
+template <class T>
+T get_d0(const T* vp, int nnx, int nny, int nnz, int npml, float dx, float dy, float dz) {
+    const T dsum = (T)dx + (T)dy + (T)dz;   /* float adds in the reference */
+    return (T)(10.0 * vp[(long)nny * nnx * nnz / 2] * std::log(100000.0) /
+               (2.0 * npml * (dsum / 3.0)));
+}
+
+/* add_source wavelet (iso:52-96 types 1-6; vti:54-88 and tti:31-65 types 1-3).
+ * t is the float-accumulated time. Returns the value added (before pfac). */
+template <class T>
+T wavelet(int wtype, T t, T favg) {
+    T x_, xx_, tdelay, ts, source = 0;
+    tdelay = (T)(1.0 / favg);
+    ts = t - tdelay;
+    if (wtype == 1) {
+        x_ = favg * ts;
+        xx_ = x_ * x_;
+        source = (T)((1 - 2 * kPi * kPi * (xx_)) * std::exp(-(kPi * kPi * xx_)));
+    } else if (wtype == 2) {
+        x_ = (T)((-4) * favg * favg * kPi * kPi / std::log(0.1));
+        source = (T)((-2) * kPi * kPi * ts * std::exp(-x_ * ts * ts));
+    } else if (wtype == 3) {
+        x_ = (T)((-1) * favg * favg * kPi * kPi / std::log(0.1));
+        source = (T)std::exp(-x_ * ts * ts);
+    } else if (wtype == 4) {
+        x_ = (T)(std::exp(-favg * favg * kPi * kPi * ts * ts) * ts);
+        source = -x_;
+    } else if (wtype == 5) {
+        x_ = (T)(-1.0 / (2.0 * favg * favg * kPi * kPi));
+        x_ = (T)(x_ * std::exp(-favg * favg * kPi * kPi * ts * ts));
+        source = x_;
+    } else if (wtype == 6) {
+        x_ = (T)(-std::erf(favg * kPi * ts) /
+                 (4 * favg * favg * favg * kPi * kPi * fsqrt<T>((T)kPi)));
+        source = x_;
+    }
+    return source;
+}
+
+/* cuda_ricker_wavelet (Toa_3dfd_cuda.cu:34-43). */
+template <class T>
+void ricker_table(T fm, T dt, int nt, T* wlt) {
+    for (int it = 0; it < nt; ++it) {
+        T tmp;
+        if (sizeof(T) == 4)
+            tmp = (T)(kPi * fm * std::fabs((float)(it * dt - 1.0 / fm)));
+        else
+            tmp = (T)(kPi * fm * std::fabs(it * dt - 1.0 / fm));
+        tmp *= tmp;
+        wlt[it] = (T)((1.0 - 2.0 * tmp) * std::exp(-tmp));
+    }
+}
+
+/* Sponge factor of cuda_absorb_bndr (Toa_3dfd_cuda.cu:138-151) for index i of
+ * an axis of interior length n: qp*r^2 + 1 in double, or 1 (no damping). */
+inline double sponge_factor(int i, int n, int npd, bool* damped) {
+    const double qp = (double)(-0.25f);
+    *damped = true;
+    if (i < npd) {
+        double r = (npd - i) / (1.0 * npd);
+        return qp * (r * r) + 1;
+    } else if (i >= 2 * mm + npd + n) {
+        double r = (i - 2 * mm - npd - n) / (1.0 * npd);
+        return qp * (r * r) + 1;
+    }
+    *damped = false;
+    return 1.0;
+}
+
+/* SV-suppression taper factor (vti:157-175, tti:142-160) for r^2 in (16, 225]. */
+inline double taper_factor(int r2) {
+    const int R = 15, r = 4;
+    return 0.5 * (1 - std::cos(kPi * ((std::sqrt((float)r2) - r) * 4.0 / (R * 3.0 - 1))));
+}
+
+template <class T>
+void copy_out(T* dst, const T* src, size_t n) { std::memcpy(dst, src, n * sizeof(T)); }
+
+/* ------------------------------------------------------------------------ */
+/* 2nd-order operator: cuda_3dfd (Toa_3dfd_cuda.cu:366-508)                 */
+/* ------------------------------------------------------------------------ */
+template <class T>
+int run_2nd(const OracleShot& p, const float* vel, T* shot, T* snaps, int nsnap_max,
+            double* secs, int nsteps_override) {
+    const int npd = p.nb, nx = p.nx, ny = p.ny, nz = p.nz;
+    const int nnz = nz + 2 * mm + 2 * npd, nnx = nx + 2 * mm + 2 * npd, nny = ny + 2 * mm + 2 * npd;
+    const long pl = (long)nnz * nnx;
+    const size_t N = (size_t)pl * nny;
+    const int off = mm + npd;
+    const int nt = p.nt;
+    const int nsteps = nsteps_override > 0 ? nsteps_override : nt;
+
+    /* velocity_transform (Toa_3dfd_cuda.cu:300-347): vv = (v*dt)^2, edge replication
+     * z (top/down), then x (left/right), then y (front/back). */
+    std::vector<T> vv(N, (T)0);
+    for (int i3 = 0; i3 < ny; i3++)
+        for (int i2 = 0; i2 < nx; i2++)
+            for (int i1 = 0; i1 < nz; i1++) {
+                T tmp = (T)vel[i1 + (long)nz * i2 + (long)nz * nx * i3] * (T)p.dt;
+                vv[(i1 + off) + (long)nnz * (i2 + off) + pl * (i3 + off)] = tmp * tmp;
+            }
+    for (int i3 = 0; i3 < nny; i3++)
+        for (int i2 = 0; i2 < nnx; i2++)
+            for (int i1 = 0; i1 < off; i1++) {
+                vv[i1 + (long)nnz * i2 + pl * i3] = vv[off + (long)nnz * i2 + pl * i3];
+                vv[(nnz - i1 - 1) + (long)nnz * i2 + pl * i3] = vv[(nnz - off - 1) + (long)nnz * i2 + pl * i3];
+            }
+    for (int i3 = 0; i3 < nny; i3++)
+        for (int i2 = 0; i2 < off; i2++)
+            for (int i1 = 0; i1 < nnz; i1++) {
+                vv[i1 + (long)nnz * i2 + pl * i3] = vv[i1 + (long)nnz * off + pl * i3];
+                vv[i1 + (long)nnz * (nnx - i2 - 1) + pl * i3] = vv[i1 + (long)nnz * (nnx - off - 1) + pl * i3];
+            }
+    for (int i3 = 0; i3 < off; i3++)
+        for (int i2 = 0; i2 < nnx; i2++)
+            for (int i1 = 0; i1 < nnz; i1++) {
+                vv[i1 + (long)nnz * i2 + pl * i3] = vv[i1 + (long)nnz * i2 + pl * off];
+                vv[i1 + (long)nnz * i2 + pl * (nny - 1 - i3)] = vv[i1 + (long)nnz * i2 + pl * (nny - off - 1)];
+            }
+
+    /* Toa_3dfd_cuda.cu:387-389: 1.0/(d*d) with a float product, stored to float */
+    const T dz2 = (T)(1.0 / ((T)p.dz * (T)p.dz));
+    const T dx2 = (T)(1.0 / ((T)p.dx * (T)p.dx));
+    const T dy2 = (T)(1.0 / ((T)p.dy * (T)p.dy));
+
+    std::vector<T> wlt(nt > nsteps ? nt : nsteps);
+    ricker_table<T>((T)p.fm, (T)p.dt, (int)wlt.size(), wlt.data());
+
+    /* cuda_set_s (Toa_3dfd_cuda.cu:45-52): 0-based interior source, dipole partner at z+2 */
+    const long s = (p.sz + off) + (long)nnz * (p.sx + off) + pl * (p.sy + off);
+
+    /* per-axis sponge factors */
+    std::vector<double> fz(nnz), fx(nnx), fy(nny);
+    std::vector<char> dz_(nnz), dx_(nnx), dy_(nny);
+    for (int i = 0; i < nnz; ++i) { bool d; fz[i] = sponge_factor(i, nz, npd, &d); dz_[i] = d; }
+    for (int i = 0; i < nnx; ++i) { bool d; fx[i] = sponge_factor(i, nx, npd, &d); dx_[i] = d; }
+    for (int i = 0; i < nny; ++i) { bool d; fy[i] = sponge_factor(i, ny, npd, &d); dy_[i] = d; }
+
+    std::vector<T> pa(N, (T)0), pb(N, (T)0);
+    T* p0 = pa.data();
+    T* p1 = pb.data();
+    int nsnap = 0;
+    const int zr = off + p.rec_z;
+
+    auto absorb = [&](T* d) {
+        /* cuda_absorb_bndr (Toa_3dfd_cuda.cu:123-154), applied per axis in the order y, x, z;
+         * each multiplication is rounded to the stored type. Bounds-checked (no aliasing). */
+#pragma omp parallel for schedule(static)
+        for (int iy = 0; iy < nny; iy++)
+            for (int ix = 0; ix < nnx; ix++)
+                for (int iz = 0; iz < nnz; iz++) {
+                    long id = iz + (long)ix * nnz + iy * pl;
+                    if (dy_[iy]) d[id] = (T)(fy[iy] * d[id]);
+                    if (dx_[ix]) d[id] = (T)(fx[ix] * d[id]);
+                    if (dz_[iz]) d[id] = (T)(fz[iz] * d[id]);
+                }
+    };
+
+    auto t0 = std::chrono::steady_clock::now();
+    for (int it = 0; it < nsteps; ++it) {
+        /* cuda_add_dipole (Toa_3dfd_cuda.cu:177-187): persistent add into p1 */
+        p1[s] += wlt[it];
+        p1[s + 2] -= wlt[it];
+        /* cuda_step_fd3d (Toa_3dfd_cuda.cu:189-298), in place on p0 */
+#pragma omp parallel for schedule(static)
+        for (int iy = mm; iy < nny - mm; iy++)
+            for (int ix = mm; ix < nnx - mm; ix++)
+                for (int iz = mm; iz < nnz - mm; iz++) {
+                    long id = iz + (long)ix * nnz + iy * pl;
+                    T c1 = lap1<T>(p1, id, 1);
+                    T c2 = lap1<T>(p1, id, nnz);
+                    T c3 = lap1<T>(p1, id, pl);
+                    c1 *= dz2;
+                    c2 *= dx2;
+                    c3 *= dy2;
+                    /* Toa_3dfd_cuda.cu:295: 2.0*p1 - p0 in double, plus the float product */
+                    T lap = vv[id] * (c1 + c2 + c3);
+                    p0[id] = (T)(2.0 * (double)p1[id] - (double)p0[id] + (double)lap);
+                }
+        T* tmp = p0; p0 = p1; p1 = tmp;               /* :445-447 */
+        absorb(p0);                                   /* :449 */
+        absorb(p1);                                   /* :450 */
+        if (shot) {                                   /* cuda_record + trans_xy2txy (:452-453) */
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix)
+                    shot[it + (long)nt * ix + (long)nt * nx * iy] =
+                        p0[zr + (long)nnz * (ix + off) + pl * (iy + off)];
+        }
+        if (p.snap_interval > 0 && it % p.snap_interval == 0 && !(p.snap_skip0 && it == 0)) {
+            if (snaps && nsnap < nsnap_max) {         /* window3d (:350-363) */
+                T* o = snaps + (size_t)nsnap * nz * nx * ny;
+                for (int iy = 0; iy < ny; ++iy)
+                    for (int ix = 0; ix < nx; ++ix)
+                        for (int iz = 0; iz < nz; ++iz)
+                            o[iz + (long)nz * ix + (long)nz * nx * iy] =
+                                p0[(iz + off) + (long)nnz * (ix + off) + pl * (iy + off)];
+            }
+            ++nsnap;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
+
+    if (shot && p.mute) {
+        /* mute_directwave (Toa_3dfd_cuda.cu:73-121) is never called by the reference's
+         * 2nd-order entry; applied here only when asked, with v = v0[1] (:403). */
+        const T v = p.v_mute > 0 ? (T)p.v_mute : (T)vel[1];
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                T mx = (T)p.dx * (T)std::abs(ix - p.sx);
+                T my = (T)p.dy * (T)std::abs(iy - p.sy);
+                T mz = (T)p.dz * (T)p.sz;
+                T t0m = fsqrt<T>(mx * mx + my * my + mz * mz) / v;
+                int mu_t = (int)(2.0 / ((T)p.dt * (T)p.fm));
+                int mu_nt = (int)(t0m / (T)p.dt) + mu_t + p.mute_tt;
+                for (int it = 0; it < nt && it < mu_nt; ++it) shot[it + (long)nt * ix + (long)nt * nx * iy] = 0;
+            }
+    }
+    return nsnap;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1st-order family: cuda_3dfd_1order (iso:372-671), cuda_3dfd_vti          */
+/* (vti:389-688), cuda_3dfd_tti (tti:390-676)                               */
+/* ------------------------------------------------------------------------ */
+template <class T>
+int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const float* eps_in,
+            const float* del_in, const float* th_in, const float* ph_in, T* shot, T* snaps,
+            int nsnap_max, double* secs, int nsteps_override) {
+    const int op = p.op;
+    const int npml = p.nb, nx = p.nx, ny = p.ny, nz = p.nz;
+    const int nnx = nx + 2 * npml, nny = ny + 2 * npml, nnz = nz + 2 * npml;
+    const long pl = (long)nnz * nnx;
+    const size_t N = (size_t)pl * nny;
+    const int nt = p.nt;
+    const int nsteps = nsteps_override > 0 ? nsteps_override : nt;
+    const bool iso = op == ORC_OP_ISO, vti = op == ORC_OP_VTI, tti = op == ORC_OP_TTI;
+
+    std::vector<T> vp(N, 0), rho, eps, del, th, ph;
+    read_interior<T>(vel, nx, ny, nz, nnx, nnz, npml, vp.data());
+    pad_vv<T>(nnx, nny, nnz, npml, vp.data());
+    if (iso || vti) {
+        rho.assign(N, 0);
+        read_interior<T>(rho_in, nx, ny, nz, nnx, nnz, npml, rho.data());
+        pad_vv<T>(nnx, nny, nnz, npml, rho.data());
+    }
+    if (vti || tti) {
+        eps.assign(N, 0);
+        del.assign(N, 0);
+        read_interior<T>(eps_in, nx, ny, nz, nnx, nnz, npml, eps.data());
+        read_interior<T>(del_in, nx, ny, nz, nnx, nnz, npml, del.data());
+        pad_vv<T>(nnx, nny, nnz, npml, eps.data());
+        pad_vv<T>(nnx, nny, nnz, npml, del.data());
+    }
+    if (tti) {
+        th.assign(N, 0);
+        ph.assign(N, 0);
+        read_interior<T>(th_in, nx, ny, nz, nnx, nnz, npml, th.data());
+        read_interior<T>(ph_in, nx, ny, nz, nnx, nnz, npml, ph.data());
+        pad_vv<T>(nnx, nny, nnz, npml, th.data());
+        pad_vv<T>(nnx, nny, nnz, npml, ph.data());
+    }
+
+    const T dt = (T)p.dt;
+    const T d0 = get_d0<T>(vp.data(), nnx, nny, nnz, npml, p.dx, p.dy, p.dz);
+    std::vector<T> cx1(nnx), cx2(nnx), ax1(nnx), ax2(nnx);
+    std::vector<T> cy1(nny), cy2(nny), ay1(nny), ay2(nny);
+    std::vector<T> cz1(nnz), cz2(nnz), az1(nnz), az2(nnz);
+    initial_coffe<T>(dt, d0, nx, npml, cx1.data(), cx2.data(), ax1.data(), ax2.data());
+    initial_coffe<T>(dt, d0, ny, npml, cy1.data(), cy2.data(), ay1.data(), ay2.data());
+    initial_coffe<T>(dt, d0, nz, npml, cz1.data(), cz2.data(), az1.data(), az2.data());
+
+    const T dtx = dt / (T)p.dx, dty = dt / (T)p.dy, dtz = dt / (T)p.dz;   /* iso:109-111 */
+
+    std::vector<T> u(N, 0), v(N, 0), w(N, 0), P(N, 0), Q, px(N, 0), py(N, 0), pz(N, 0), qx, qy, qz;
+    if (!iso) { Q.assign(N, 0); qx.assign(N, 0); qy.assign(N, 0); qz.assign(N, 0); }
+    T* Pp = P.data();
+    T* Qp = iso ? P.data() : Q.data();
+
+    /* add_source: 1-based source, iso:91-94 */
+    const long s = (p.sz + npml - 1) + (long)(p.sx + npml - 1) * nnz + (long)(p.sy + npml - 1) * pl;
+    const T favg = (T)p.fm;
+    const T pfac = (T)p.pfac;
+    int nsnap = 0;
+    const int sxp = p.sx + npml - 1, syp = p.sy + npml - 1, szp = p.sz + npml - 1;
+
+    auto t0 = std::chrono::steady_clock::now();
+    /* The time grid is the reference's float accumulation t = dt, t += dt (iso:521) and
+     * the gate t <= 2*tdelay is taken in float, in both instances: the FP64 instance
+     * differs from FP32 only by the precision of the wavelet value and the propagation. */
+    const float tdelay_f = (float)(1.0 / p.fm);
+    float t = p.dt;
+    for (int it = 0; it < nsteps; it++, t += p.dt) {
+        /* add_source (iso:523-530: -pfac at zs, +pfac at zs+2; vti:557, tti:571: +pfac to P and Q) */
+        T src = wavelet<T>(p.wtype, (T)t, favg);
+        if (t <= 2 * tdelay_f) {
+            if (iso) {
+                T mpf = (T)(-1.0 * pfac);
+                Pp[s] += mpf * src;
+                Pp[s + 2] += pfac * src;
+            } else {
+                Pp[s] += pfac * src;
+                Qp[s] += pfac * src;
+            }
+        }
+        /* update_vel (iso:99-142, vti:91-128, tti:68-112); in place, which equals the
+         * reference's ping-pong because every update reads only its own point of u0/v0/w0. */
+#pragma omp parallel for schedule(static)
+        for (int iy = mm; iy < nny - mm; iy++)
+            for (int ix = mm; ix < nnx - mm; ix++)
+                for (int iz = mm; iz < nnz - mm; iz++) {
+                    long id = iz + (long)ix * nnz + iy * pl;
+                    T xx, yy, zz;
+                    if (!tti) {
+                        yy = dfwd<T>(Pp, id, pl);
+                        xx = dfwd<T>(Pp, id, nnz);
+                        zz = dfwd<T>(Qp, id, 1);
+                        xx /= rho[id];
+                        yy /= rho[id];
+                        zz /= rho[id];
+                    } else {
+                        T Py = dfwd<T>(Pp, id, pl), Px = dfwd<T>(Pp, id, nnz), Pz = dfwd<T>(Pp, id, 1);
+                        T Qy = dfwd<T>(Qp, id, pl), Qx = dfwd<T>(Qp, id, nnz), Qz = dfwd<T>(Qp, id, 1);
+                        T ct = fcos<T>(th[id]), st = fsin<T>(th[id]);
+                        T cp = fcos<T>(ph[id]), sp = fsin<T>(ph[id]);
+                        xx = ct * cp * Px - ct * sp * Py - st * Pz;     /* tti:101 */
+                        yy = sp * Px + cp * Py;                         /* tti:102 */
+                        zz = st * cp * Qx - st * sp * Qy + ct * Qz;     /* tti:103 */
+                    }
+                    u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
+                    v[id] = cy2[iy] * v[id] - cy1[iy] * dty * yy;
+                    w[id] = cz2[iz] * w[id] - cz1[iz] * dtz * zz;
+                }
+        /* update_stress (iso:145-190, vti:131-203, tti:115-201) */
+#pragma omp parallel for schedule(static)
+        for (int iy = mm; iy < nny - mm; iy++)
+            for (int ix = mm; ix < nnx - mm; ix++)
+                for (int iz = mm; iz < nnz - mm; iz++) {
+                    long id = iz + (long)ix * nnz + iy * pl;
+                    if (iso) {
+                        T yy = dbwd<T>(v.data(), id, pl);
+                        T xx = dbwd<T>(u.data(), id, nnz);
+                        T zz = dbwd<T>(w.data(), id, 1);
+                        px[id] = ax2[ix] * px[id] - ax1[ix] * rho[id] * vp[id] * vp[id] * dtx * xx;
+                        py[id] = ay2[iy] * py[id] - ay1[iy] * rho[id] * vp[id] * vp[id] * dty * yy;
+                        pz[id] = az2[iz] * pz[id] - az1[iz] * rho[id] * vp[id] * vp[id] * dtz * zz;
+                        Pp[id] = px[id] + py[id] + pz[id];
+                        continue;
+                    }
+                    /* SV taper (vti:154-175) */
+                    T ee, dd;
+                    int rx = ix - sxp, ry = iy - syp, rz = iz - szp;
+                    int r2 = rx * rx + ry * ry + rz * rz;
+                    if (p.sv && r2 <= 15 * 15) {
+                        if (r2 <= 4 * 4) {
+                            ee = 0;
+                            dd = 0;
+                        } else {
+                            double f = (sizeof(T) == 4) ? taper_factor(r2)
+                                : 0.5 * (1 - std::cos(kPi * ((std::sqrt((double)r2) - 4) * 4.0 / 44.0)));
+                            ee = (T)(f * eps[id]);
+                            dd = (T)(f * del[id]);
+                        }
+                    } else {
+                        ee = eps[id];
+                        dd = del[id];
+                    }
+                    T xx, yy, zz;
+                    if (vti) {
+                        yy = dbwd<T>(v.data(), id, pl);
+                        xx = dbwd<T>(u.data(), id, nnz);
+                        zz = dbwd<T>(w.data(), id, 1);
+                        const T r = rho[id], c = vp[id];
+                        px[id] = ax2[ix] * px[id] - ax1[ix] * r * c * c * (1 + 2 * ee) * dtx * xx;
+                        py[id] = ay2[iy] * py[id] - ay1[iy] * r * c * c * (1 + 2 * ee) * dty * yy;
+                        pz[id] = az2[iz] * pz[id] - az1[iz] * r * c * c * fsqrt<T>(1 + 2 * dd) * dtz * zz;
+                        qx[id] = ax2[ix] * qx[id] - ax1[ix] * r * c * c * fsqrt<T>(1 + 2 * dd) * dtx * xx;
+                        qy[id] = ay2[iy] * qy[id] - ay1[iy] * r * c * c * fsqrt<T>(1 + 2 * dd) * dty * yy;
+                        qz[id] = az2[iz] * qz[id] - az1[iz] * r * c * c * dtz * zz;
+                    } else {
+                        T uy = dbwd<T>(u.data(), id, pl), ux = dbwd<T>(u.data(), id, nnz), uz = dbwd<T>(u.data(), id, 1);
+                        T vy = dbwd<T>(v.data(), id, pl), vx = dbwd<T>(v.data(), id, nnz), vz = dbwd<T>(v.data(), id, 1);
+                        T wy = dbwd<T>(w.data(), id, pl), wx = dbwd<T>(w.data(), id, nnz), wz = dbwd<T>(w.data(), id, 1);
+                        (void)vz;   /* computed but unused by the reference (tti:179, :185) */
+                        T ct = fcos<T>(th[id]), st = fsin<T>(th[id]);
+                        T cp = fcos<T>(ph[id]), sp = fsin<T>(ph[id]);
+                        xx = ct * cp * ux - ct * sp * uy - st * uz;     /* tti:184 */
+                        yy = sp * vx + cp * vy;                         /* tti:185 */
+                        zz = st * cp * wx - st * sp * wy + ct * wz;     /* tti:186 */
+                        const T c = vp[id];
+                        px[id] = ax2[ix] * px[id] - ax1[ix] * c * c * (1 + 2 * ee) * dtx * xx;
+                        py[id] = ay2[iy] * py[id] - ay1[iy] * c * c * (1 + 2 * ee) * dty * yy;
+                        pz[id] = az2[iz] * pz[id] - az1[iz] * c * c * fsqrt<T>(1 + 2 * dd) * dtz * zz;
+                        qx[id] = ax2[ix] * qx[id] - ax1[ix] * c * c * fsqrt<T>(1 + 2 * dd) * dtx * xx;
+                        qy[id] = ay2[iy] * qy[id] - ay1[iy] * c * c * fsqrt<T>(1 + 2 * dd) * dty * yy;
+                        qz[id] = az2[iz] * qz[id] - az1[iz] * c * c * dtz * zz;
+                    }
+                    Pp[id] = px[id] + py[id] + pz[id];
+                    Qp[id] = qx[id] + qy[id] + qz[id];
+                }
+        /* shot_record (iso:294-304): interior z = rec_z (reference: 0) */
+        if (shot) {
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix)
+                    shot[it + (long)nt * ix + (long)nt * nx * iy] =
+                        Pp[(npml + p.rec_z) + (long)nnz * (ix + npml) + pl * (iy + npml)];
+        }
+        /* snapshots (iso:568-584, vti:601-617, tti:593-608) */
+        if (p.snap_interval > 0 && it % p.snap_interval == 0 && !(p.snap_skip0 && it == 0)) {
+            if (snaps && nsnap < nsnap_max) {
+                T* o = snaps + (size_t)nsnap * nz * nx * ny;
+                for (int iy = 0; iy < ny; ++iy)
+                    for (int ix = 0; ix < nx; ++ix)
+                        for (int iz = 0; iz < nz; ++iz)
+                            o[iz + (long)nz * ix + (long)nz * nx * iy] =
+                                Pp[(iz + npml) + (long)nnz * (ix + npml) + pl * (iy + npml)];
+            }
+            ++nsnap;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
+
+    if (shot && p.mute) {
+        /* mute_directwave (iso:322-369, vti:339-386): v = vp[1] (padded index 1, i.e. the
+         * model's first sample), times sqrtf(1+2*epsilon[1]) for VTI/TTI. */
+        T vcut;
+        if (p.v_mute > 0) vcut = (T)p.v_mute;
+        else if (iso) vcut = vp[1];
+        else vcut = vp[1] * fsqrt<T>(1 + 2 * eps[1]);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                T mx = (T)p.dx * (T)std::abs(ix - p.sx);
+                T my = (T)p.dy * (T)std::abs(iy - p.sy);
+                T mz = (T)p.dz * (T)p.sz;
+                T t0m = fsqrt<T>(mx * mx + my * my + mz * mz) / vcut;
+                int mu_t = (int)(2.0 / (dt * favg));
+                int mu_nt = (int)(t0m / dt) + mu_t + p.mute_tt;
+                for (int it = 0; it < nt && it < mu_nt; ++it) shot[it + (long)nt * ix + (long)nt * nx * iy] = 0;
+            }
+    }
+    return nsnap;
+}
+
+template <class T>
+int dispatch(const OracleShot* p, const float* vel, const float* rho, const float* eps,
+             const float* del, const float* th, const float* ph, T* shot, T* snaps, int nsnap_max,
+             double* secs, int nsteps_override) {
+    if (p->op == ORC_OP_2ND) return run_2nd<T>(*p, vel, shot, snaps, nsnap_max, secs, nsteps_override);
+    return run_1st<T>(*p, vel, rho, eps, del, th, ph, shot, snaps, nsnap_max, secs, nsteps_override);
+}
+
+int check(const OracleShot* p, const float* vel, const float* rho, const float* eps,
+          const float* del, const float* th, const float* ph) {
+    if (!p || !vel) return -1;
+    if (p->op < 0 || p->op > 3) return -2;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nt < 1) return -3;
+    if ((p->op == ORC_OP_ISO || p->op == ORC_OP_VTI) && !rho) return -4;
+    if ((p->op == ORC_OP_VTI || p->op == ORC_OP_TTI) && (!eps || !del)) return -4;
+    if (p->op == ORC_OP_TTI && (!th || !ph)) return -4;
+    if (p->op != ORC_OP_2ND && p->nb < 2) return -5;
+    if (p->op != ORC_OP_2ND) {
+        if (p->sx < 1 || p->sx > p->nx || p->sy < 1 || p->sy > p->ny || p->sz < 1 || p->sz > p->nz) return -6;
+        if (p->op == ORC_OP_ISO && (p->sz + p->nb - 1 + 2) >= p->nz + 2 * p->nb - mm) return -6;
+    } else {
+        if (p->sx < 0 || p->sx >= p->nx || p->sy < 0 || p->sy >= p->ny || p->sz < 0 || p->sz + 2 >= p->nz + p->nb) return -6;
+    }
+    if (p->rec_z < 0 || p->rec_z >= p->nz) return -7;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_run(int prec, const OracleShot* p, const float* vel, const float* rho, const float* eps,
+               const float* del, const float* theta, const float* phi, void* shot, void* snaps,
+               int nsnap_max, double* loop_seconds) {
+    int rc = check(p, vel, rho, eps, del, theta, phi);
+    if (rc) return rc;
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    if (prec == 32)
+        return dispatch<float>(p, vel, rho, eps, del, theta, phi, (float*)shot, (float*)snaps, nsnap_max,
+                               loop_seconds, 0);
+    if (prec == 64)
+        return dispatch<double>(p, vel, rho, eps, del, theta, phi, (double*)shot, (double*)snaps, nsnap_max,
+                                loop_seconds, 0);
+    return -8;
+}
+
+double oracle_time_steps(const OracleShot* p, const float* vel, const float* rho, const float* eps,
+                         const float* del, const float* theta, const float* phi, int nsteps) {
+    if (check(p, vel, rho, eps, del, theta, phi)) return -1.0;
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    double secs = -1.0;
+    dispatch<float>(p, vel, rho, eps, del, theta, phi, (float*)nullptr, (float*)nullptr, 0, &secs, nsteps);
+    return secs;
+}
+
+int oracle_ricker_table(int prec, float fm, float dt, int nt, void* out) {
+    if (prec == 32) ricker_table<float>(fm, dt, nt, (float*)out);
+    else ricker_table<double>(fm, dt, nt, (double*)out);
+    return 0;
+}
+
+int oracle_wavelet(int prec, int wtype, float fm, float dt, int nt, void* out) {
+    /* the per-step add_source value at float-accumulated t (iso:521) */
+    if (prec == 32) {
+        float t = dt;
+        for (int it = 0; it < nt; ++it, t += dt) ((float*)out)[it] = wavelet<float>(wtype, t, fm);
+    } else {
+        double t = dt;
+        for (int it = 0; it < nt; ++it, t += (double)dt) ((double*)out)[it] = wavelet<double>(wtype, t, fm);
+    }
+    return 0;
+}
+
+int oracle_pml_tables(int prec, int n, int npml, float dt, float d0, void* c1, void* c2, void* a1,
+                      void* a2) {
+    if (prec == 32)
+        initial_coffe<float>(dt, d0, n, npml, (float*)c1, (float*)c2, (float*)a1, (float*)a2);
+    else
+        initial_coffe<double>(dt, d0, n, npml, (double*)c1, (double*)c2, (double*)a1, (double*)a2);
+    return 0;
+}
+
+double oracle_d0(const float* vp_padded, int nnx, int nny, int nnz, int npml, float dx, float dy,
+                 float dz) {
+    return (double)get_d0<float>(vp_padded, nnx, nny, nnz, npml, dx, dy, dz);
+}
+
+int oracle_pad(const float* in, int nx, int ny, int nz, int nb, float* out) {
+    const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
+    std::memset(out, 0, sizeof(float) * (size_t)nnx * nny * nnz);
+    read_interior<float>(in, nx, ny, nz, nnx, nnz, nb, out);
+    pad_vv<float>(nnx, nny, nnz, nb, out);
+    return 0;
+}
+
+int oracle_sponge_factor(int n, int npd, double* out) {
+    for (int i = 0; i < n + 2 * mm + 2 * npd; ++i) {
+        bool d;
+        out[i] = sponge_factor(i, n, npd, &d);
+    }
+    return 0;
+}
+
+int oracle_taper(int r2, double* out) {
+    *out = taper_factor(r2);
+    return 0;
+}
+
+int oracle_staggered_dx(const float* in, int n, int forward, float* out) {
+    for (int i = 0; i < n; ++i) out[i] = 0.0f;
+    for (int i = mm; i < n - mm; ++i) out[i] = forward ? dfwd<float>(in, i, 1) : dbwd<float>(in, i, 1);
+    return 0;
+}
+
+int oracle_laplacian_z(const float* in, int n, float* out) {
+    for (int i = 0; i < n; ++i) out[i] = 0.0f;
+    for (int i = mm; i < n - mm; ++i) out[i] = lap1<float>(in, i, 1);
+    return 0;
+}
+
+}  // extern "C"
