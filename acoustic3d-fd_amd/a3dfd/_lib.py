@@ -101,14 +101,14 @@ def load(path=None):
     lib.a3d_run.argtypes = [vp, POINTER(ShotDesc), fp, fp, c_int, POINTER(c_int), POINTER(c_double)]
     lib.a3d_set_kernel_timing.argtypes = [vp, c_int]
     lib.a3d_kernel_times.argtypes = [vp, POINTER(c_double), POINTER(c_longlong), c_int]
-    lib.a3d_set_launch.argtypes = [vp, c_int, c_int, c_int]
-    lib.a3d_get_launch.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),
-                                   POINTER(c_int)]
+    lib.a3d_set_launch.argtypes = [vp, c_int, c_int, c_int, c_int]
+    lib.a3d_get_launch.argtypes = [vp, POINTER(c_int)]
+    lib.a3d_autotune.argtypes = [vp, POINTER(ShotDesc), c_int, POINTER(c_double)]
     lib.a3d_grid_info.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_longlong)]
     lib.a3d_last_error.restype = c_char_p
     lib.a3d_version.restype = c_char_p
     for name in ("a3d_create", "a3d_destroy", "a3d_set_model", "a3d_run", "a3d_set_kernel_timing",
-                 "a3d_kernel_times", "a3d_set_launch", "a3d_get_launch", "a3d_grid_info"):
+                 "a3d_kernel_times", "a3d_set_launch", "a3d_get_launch", "a3d_autotune", "a3d_grid_info"):
         getattr(lib, name).restype = c_int
     if path is None:
         _lib = lib
@@ -187,14 +187,19 @@ class Modeller:
         _check(self.lib, self.lib.a3d_kernel_times(self._h, ms, n, int(reset)))
         return [ms[0], ms[1]], [n[0], n[1]]
 
-    def set_launch(self, ty_vel, ty_stress, zseg=0):
-        _check(self.lib, self.lib.a3d_set_launch(self._h, ty_vel, ty_stress, zseg))
+    def set_launch(self, ty_vel, zseg_vel, ty_stress, zseg_stress):
+        _check(self.lib, self.lib.a3d_set_launch(self._h, ty_vel, zseg_vel, ty_stress, zseg_stress))
 
     def launch_info(self):
-        v = [c_int() for _ in range(5)]
-        _check(self.lib, self.lib.a3d_get_launch(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(ty_vel=v[0].value, ty_stress=v[1].value, zseg=v[2].value, blocks_vel=v[3].value,
-                    blocks_stress=v[4].value)
+        v = (c_int * 6)()
+        _check(self.lib, self.lib.a3d_get_launch(self._h, v))
+        return dict(ty_vel=v[0], zseg_vel=v[1], blocks_vel=v[2], ty_stress=v[3], zseg_stress=v[4],
+                    blocks_stress=v[5])
+
+    def autotune(self, desc=None, steps=4):
+        best = (c_double * 2)()
+        _check(self.lib, self.lib.a3d_autotune(self._h, ctypes.byref(desc or self.desc), steps, best))
+        return [best[0], best[1]]
 
     def grid_info(self):
         a, b, c = c_int(), c_int(), c_int()

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -186,7 +187,10 @@ struct a3d_ctx {
     int* flag = nullptr;
     bool model_set = false;
     float vel_first = 0.f, eps_first = 0.f, vel_d0 = 0.f, vel_raw1 = 0.f;
-    int ty_vel = 4, ty_stress = 4, zseg = 0;
+    // launch shape per sweep: [0] 2nd-order step / velocity sweep, [1] stress sweep
+    int ty[2] = {8, 4}, zseg[2] = {0, 0};
+    bool tuned = false;
+    int ncu = 256;
     Timing tm;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -198,19 +202,31 @@ Launch make_launch(const Geo& g, int ty, int zseg) {
     l.ntx = g.ld / TX;
     l.nty = (g.nny - 2 * MM + ty - 1) / ty;
     const int nzi = g.nnz - 2 * MM;
-    int nseg = zseg;
-    if (nseg <= 0) {
-        // enough workgroups to keep ~8 per CU resident on 256 CUs, segments >= 24 planes
-        const int per_plane = l.ntx * l.nty;
-        nseg = (2048 + per_plane - 1) / per_plane;
-        nseg = std::max(1, std::min(nseg, nzi / 24));
-    }
-    nseg = std::max(1, std::min(nseg, nzi));
+    int nseg = std::max(1, std::min(zseg > 0 ? zseg : 1, nzi));
     l.zlen = (nzi + nseg - 1) / nseg;
     l.nseg = (nzi + l.zlen - 1) / l.zlen;
     l.ntiles = l.ntx * l.nty * l.nseg;
     l.grid = ((l.ntiles + 7) / 8) * 8;
     return l;
+}
+
+// Default launch shape: a z segmentation that makes one sweep a single wave of
+// resident workgroups (occupancy from the compiled kernel's registers and LDS).
+int single_wave_zseg(const a3d_ctx* c, int kind, int ty) {
+    const Launch l1 = make_launch(c->g, ty, 1);
+    const int per_plane = l1.ntx * l1.nty;
+    const int cap = std::max(1, max_blocks_per_cu(kind, c->d.op, ty)) * c->ncu;
+    const int nzi = c->g.nnz - 2 * MM;
+    int nseg = std::max(1, cap / std::max(per_plane, 1));
+    nseg = std::min(nseg, std::max(1, nzi / 16));
+    return nseg;
+}
+
+void default_shapes(a3d_ctx* c) {
+    c->ty[0] = 8;
+    c->ty[1] = 4;
+    c->zseg[0] = single_wave_zseg(c, 0, c->ty[0]);
+    c->zseg[1] = single_wave_zseg(c, 1, c->ty[1]);
 }
 
 bool same_geometry(const a3d_shot_desc& a, const a3d_shot_desc& b) {
@@ -354,7 +370,7 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         for (int it = 0; it < nt; ++it) wlt[it] = ricker_value(it, d->fm, d->dt);
         Step2Args a{};
         a.g = g;
-        a.l = make_launch(g, c->ty_vel, c->zseg);
+        a.l = make_launch(g, c->ty[0], c->zseg[0]);
         a.vv = c->vol[V_VV];
         a.fx = c->dtab;
         a.fy = c->dtab + g.nnx;
@@ -381,7 +397,7 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
             a.xa1 = it > 0 ? -wlt[it - 1] : 0.0f;
             a.rec = c->rec + plane_rec * it;
             timing_begin(c, 0);
-            HIP_TRY(launch_step2(c->ty_vel, a, c->st));
+            HIP_TRY(launch_step2(c->ty[0], a, c->st));
             timing_end(c);
             if (want_snap(it)) {
                 if ((rc = take_snap(it, Y, a.s0, a.ya0, a.s1, a.ya1, true))) return rc;
@@ -426,7 +442,7 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         const int s0 = sxp + g.ld * syp + g.pl * szp;
         VelArgs va{};
         va.g = g;
-        va.l = make_launch(g, c->ty_vel, c->zseg);
+        va.l = make_launch(g, c->ty[0], c->zseg[0]);
         va.c = pm;
         va.P = c->vol[V_P];
         va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
@@ -441,7 +457,7 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         va.dtz = dtz;
         StressArgs sa{};
         sa.g = g;
-        sa.l = make_launch(g, c->ty_stress, c->zseg);
+        sa.l = make_launch(g, c->ty[1], c->zseg[1]);
         sa.c = pm;
         sa.u = c->vol[V_U];
         sa.v = c->vol[V_V];
@@ -493,10 +509,10 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
             va.s = S;
             sa.rec = c->rec + plane_rec * it;
             timing_begin(c, 0);
-            HIP_TRY(launch_vel(op, c->ty_vel, va, c->st));
+            HIP_TRY(launch_vel(op, c->ty[0], va, c->st));
             timing_end(c);
             timing_begin(c, 1);
-            HIP_TRY(launch_stress(op, c->ty_stress, sa, c->st));
+            HIP_TRY(launch_stress(op, c->ty[1], sa, c->st));
             timing_end(c);
             if (want_snap(it)) {
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.f, -1, 0.f, false))) return rc;
@@ -541,6 +557,115 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
     }
     if (hflag) return fail(A3D_EUNSTABLE, "non-finite values in the shot record (unstable time step?)");
     return A3D_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Autotuner: per sweep, the tile rows TY and z segmentation with the lowest
+// measured launch time on this grid, from a few real time steps per candidate
+// (every candidate gives bit-identical results). Results are cached per
+// process by (device, operator, padded grid).
+// ---------------------------------------------------------------------------
+struct TuneKey {
+    int dev, op, nnx, nny, nnz;
+    bool operator==(const TuneKey& o) const {
+        return dev == o.dev && op == o.op && nnx == o.nnx && nny == o.nny && nnz == o.nnz;
+    }
+};
+struct TuneVal {
+    int ty[2], zseg[2];
+};
+std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
+std::mutex g_tune_mu;
+
+int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
+    const TuneKey key{c->device, c->d.op, c->g.nnx, c->g.nny, c->g.nnz};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (auto& kv : g_tune_cache)
+            if (kv.first == key) {
+                for (int k = 0; k < 2; ++k) {
+                    c->ty[k] = kv.second.ty[k];
+                    c->zseg[k] = kv.second.zseg[k];
+                }
+                c->tuned = true;
+                return A3D_OK;
+            }
+    }
+    a3d_shot_desc td = *d;
+    td.nt = std::max(2, steps);
+    td.snap_interval = 0;
+    td.mute = 0;
+    const Timing saved = c->tm;
+    const int nzi = c->g.nnz - 2 * MM;
+    double best[2] = {1e30, 1e30};
+    int bty[2] = {c->ty[0], c->ty[1]}, bzs[2] = {c->zseg[0], c->zseg[1]};
+    const int tys[3] = {2, 4, 8};
+    const int segs[9] = {1, 2, 3, 4, 6, 8, 12, 16, 24};
+    int rc = A3D_OK;
+    c->tm = Timing();
+    c->tm.pool = saved.pool;   // reuse the event pool
+    c->tm.on = true;
+    c->tuned = true;           // run_shot below must not re-enter the tuner
+    for (int ty : tys) {
+        for (int ns : segs) {
+            if (ns > 1 && nzi / ns < 8) continue;
+            for (int k = 0; k < 2; ++k) {
+                c->ty[k] = ty;
+                c->zseg[k] = ns;
+            }
+            c->tm.ms[0] = c->tm.ms[1] = 0;
+            c->tm.n[0] = c->tm.n[1] = 0;
+            rc = run_shot(c, &td, nullptr, SnapFn(), nullptr, nullptr);
+            if (rc && rc != A3D_EUNSTABLE) break;
+            rc = A3D_OK;
+            for (int k = 0; k < 2; ++k) {
+                if (c->tm.n[k] == 0) continue;
+                const double avg = c->tm.ms[k] / c->tm.n[k];
+                if (avg < best[k]) {
+                    best[k] = avg;
+                    bty[k] = ty;
+                    bzs[k] = ns;
+                }
+            }
+        }
+        if (rc) break;
+    }
+    std::vector<hipEvent_t> pool = c->tm.pool;
+    c->tm = saved;
+    c->tm.pool = pool;
+    c->tm.used = 0;
+    c->tm.marks.clear();
+    for (int k = 0; k < 2; ++k) {
+        c->ty[k] = bty[k];
+        c->zseg[k] = bzs[k];
+    }
+    if (rc) {
+        c->tuned = false;
+        return rc;
+    }
+    if (best_ms) {
+        best_ms[0] = best[0] < 1e29 ? best[0] : 0.0;
+        best_ms[1] = best[1] < 1e29 ? best[1] : 0.0;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache.push_back({key, TuneVal{{bty[0], bty[1]}, {bzs[0], bzs[1]}}});
+    return A3D_OK;
+}
+
+// Tunes on the first shot of a large grid (>= 4M updated points, nt >= 100)
+// unless A3D_AUTOTUNE=0 or a shape was set explicitly.
+int maybe_autotune(a3d_ctx* c, const a3d_shot_desc* d) {
+    if (c->tuned) return A3D_OK;
+    const char* e = getenv("A3D_AUTOTUNE");
+    if (e && e[0] == '0') return A3D_OK;
+    const double pts = (double)(c->g.nnx - 2 * MM) * (c->g.nny - 2 * MM) * (c->g.nnz - 2 * MM);
+    if (pts < 4.0e6 || d->nt < 100) return A3D_OK;
+    if (check_desc_shot(c, d) || !c->model_set) return A3D_OK;   // run_shot reports the error
+    return autotune(c, d, 4, nullptr);
 }
 
 }  // namespace
@@ -609,9 +734,9 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
     }
     if (hipMemcpy(c->dtab, dt.data(), ndtab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(A3D_EHIP, "table upload failed"));
-    if (const char* e = getenv("A3D_TY_VEL")) c->ty_vel = atoi(e);
-    if (const char* e = getenv("A3D_TY_STRESS")) c->ty_stress = atoi(e);
-    if (const char* e = getenv("A3D_ZSEG")) c->zseg = atoi(e);
+    if (int ncu = 0; hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+        c->ncu = ncu;
+    default_shapes(c);
     *out = c;
     return A3D_OK;
 }
@@ -696,6 +821,7 @@ int a3d_run(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, float* snaps_ou
             int* nsnap_taken, double* loop_ms) {
     if (!c || !d) return fail(A3D_EINVAL, "null argument");
     const size_t vol = (size_t)d->nx * d->ny * d->nz;
+    if (int rc = maybe_autotune(c, d)) return rc;
     SnapFn fn;
     if (snaps_out && nsnap_max > 0)
         fn = [&](int k, int, const float* h) {
@@ -723,27 +849,33 @@ int a3d_kernel_times(a3d_ctx* c, double* ms, long long* launches, int reset) {
     return A3D_OK;
 }
 
-int a3d_set_launch(a3d_ctx* c, int ty_vel, int ty_stress, int zseg) {
+int a3d_set_launch(a3d_ctx* c, int ty_vel, int zseg_vel, int ty_stress, int zseg_stress) {
     if (!c) return fail(A3D_EINVAL, "null context");
     auto ok = [](int t) { return t == 2 || t == 4 || t == 8; };
     if (!ok(ty_vel) || !ok(ty_stress)) return fail(A3D_EINVAL, "tile rows must be 2, 4 or 8");
-    if (zseg < 0) return fail(A3D_EINVAL, "zseg must be >= 0 (0: automatic)");
-    c->ty_vel = ty_vel;
-    c->ty_stress = ty_stress;
-    c->zseg = zseg;
+    if (zseg_vel < 0 || zseg_stress < 0) return fail(A3D_EINVAL, "z segments must be >= 0 (0: automatic)");
+    c->ty[0] = ty_vel;
+    c->ty[1] = ty_stress;
+    c->zseg[0] = zseg_vel > 0 ? zseg_vel : single_wave_zseg(c, 0, ty_vel);
+    c->zseg[1] = zseg_stress > 0 ? zseg_stress : single_wave_zseg(c, 1, ty_stress);
+    c->tuned = true;   // an explicit shape is never overridden by the autotuner
     return A3D_OK;
 }
 
-int a3d_get_launch(a3d_ctx* c, int* ty_vel, int* ty_stress, int* zseg, int* blocks_vel, int* blocks_stress) {
-    if (!c) return fail(A3D_EINVAL, "null context");
-    if (ty_vel) *ty_vel = c->ty_vel;
-    if (ty_stress) *ty_stress = c->ty_stress;
-    const Launch lv = make_launch(c->g, c->ty_vel, c->zseg);
-    const Launch ls = make_launch(c->g, c->ty_stress, c->zseg);
-    if (zseg) *zseg = lv.nseg;
-    if (blocks_vel) *blocks_vel = lv.grid;
-    if (blocks_stress) *blocks_stress = ls.grid;
+int a3d_get_launch(a3d_ctx* c, int* out6) {
+    if (!c || !out6) return fail(A3D_EINVAL, "null argument");
+    for (int k = 0; k < 2; ++k) {
+        const Launch l = make_launch(c->g, c->ty[k], c->zseg[k]);
+        out6[3 * k + 0] = c->ty[k];
+        out6[3 * k + 1] = l.nseg;
+        out6[3 * k + 2] = l.grid;
+    }
     return A3D_OK;
+}
+
+int a3d_autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
+    if (!c || !d) return fail(A3D_EINVAL, "null argument");
+    return autotune(c, d, steps, best_ms);
 }
 
 int a3d_grid_info(const a3d_ctx* c, int* nnx, int* nny, int* nnz, long long* updated) {
@@ -830,7 +962,8 @@ int legacy_run(Legacy& L, const char* FNsnap, const char* FNshot, FILE* fshot, i
         };
         int nsnap = 0;
         double ms = 0;
-        rc = run_shot(c, &d, shot.data(), fn, &nsnap, &ms);
+        rc = maybe_autotune(c, &d);
+        if (!rc) rc = run_shot(c, &d, shot.data(), fn, &nsnap, &ms);
         if (rc == A3D_EUNSTABLE) {
             fprintf(stderr, "%s: %s\n", L.name, g_err.c_str());
             rc = A3D_OK;   // the shot is still written, as the reference would

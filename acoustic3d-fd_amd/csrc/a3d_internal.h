@@ -98,6 +98,10 @@ hipError_t launch_vel(int op, int ty, const VelArgs& a, hipStream_t st);
 hipError_t launch_stress(int op, int ty, const StressArgs& a, hipStream_t st);
 hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st);
 
+// Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
+// the 2nd-order step for op 0), kind 1 = stress sweep.
+int max_blocks_per_cu(int kind, int op, int ty);
+
 // Pads an interior z-fastest model (nx,ny,nz) into the device layout by edge
 // replication; the interior starts at padded index `off` on every axis.
 // mode 0: copy; mode 1: (v*dt)^2 with float arithmetic.

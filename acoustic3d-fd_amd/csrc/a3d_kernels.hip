@@ -598,6 +598,44 @@ hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
+template <class K>
+int occ(K kern, int ty) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, TX * ty, 0) != hipSuccess) return 1;
+    return nb;
+}
+template <int OP>
+int occ_op(int kind, int ty) {
+    if (kind == 0) {
+        switch (ty) {
+            case 2: return occ(k_vel<OP, 2>, 2);
+            case 4: return occ(k_vel<OP, 4>, 4);
+            default: return occ(k_vel<OP, 8>, 8);
+        }
+    }
+    switch (ty) {
+        case 2: return occ(k_stress<OP, 2>, 2);
+        case 4: return occ(k_stress<OP, 4>, 4);
+        default: return occ(k_stress<OP, 8>, 8);
+    }
+}
+}  // namespace
+
+int max_blocks_per_cu(int kind, int op, int ty) {
+    switch (op) {
+        case 0:
+            switch (ty) {
+                case 2: return occ(k_step2<2>, 2);
+                case 4: return occ(k_step2<4>, 4);
+                default: return occ(k_step2<8>, 8);
+            }
+        case 1: return occ_op<1>(kind, ty);
+        case 2: return occ_op<2>(kind, ty);
+        default: return occ_op<3>(kind, ty);
+    }
+}
+
 hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, float* dst,
                       int mode, float dt, hipStream_t st) {
     const dim3 grid((g.ld + 255) / 256, g.nny, g.nnz);

@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--launch", type=str, default="", help="ty_vel,ty_stress,zseg")
+    ap.add_argument("--launch", type=str, default="", help="ty_vel,zseg_vel,ty_stress,zseg_stress (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every shot on a single MI355X "
@@ -95,7 +95,8 @@ def main():
 
     for w in range(args.warmup):
         _, _, ms = mdl.run(shot=False)
-        log(f"bench: warmup shot {w}: {ms:.1f} ms loop")
+        log(f"bench: warmup shot {w}: {ms:.1f} ms loop, launch {mdl.launch_info()}")
+    launch = mdl.launch_info()
 
     mdl.set_kernel_timing(True)
     mdl.kernel_times(reset=True)
@@ -146,7 +147,8 @@ def main():
                 "smooth epsilon/delta); inputs resident in HBM",
         "config": {"workload": "vti_400x400x300_nt%d" % args.nt, "nx": cfg["nx"], "ny": cfg["ny"],
                    "nz": cfg["nz"], "npml": 20, "nt": args.nt, "dx": cfg["dx"], "dt": cfg["dt"],
-                   "updated_points": npts, "step": "one shot", "parallelism": "single GPU"},
+                   "updated_points": npts, "step": "one shot", "parallelism": "single GPU",
+                   "launch": launch},
         "roofline": {"bound": "hbm", "kernel": "k_stress<VTI>", "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_point": A_STRESS,

@@ -151,10 +151,22 @@ int a3d_run(a3d_ctx* ctx, const a3d_shot_desc* desc, float* shot_out, float* sna
 int a3d_set_kernel_timing(a3d_ctx* ctx, int enable);
 int a3d_kernel_times(a3d_ctx* ctx, double* ms, long long* launches, int reset);
 
-/* Launch geometry in use: tile rows TY and z segments per sweep (for tuning). */
-int a3d_set_launch(a3d_ctx* ctx, int ty_vel, int ty_stress, int zseg);
-int a3d_get_launch(a3d_ctx* ctx, int* ty_vel, int* ty_stress, int* zseg, int* blocks_vel,
-                   int* blocks_stress);
+/* Launch shape per sweep: tile rows TY (2, 4 or 8; a tile is 64 x TY points)
+ * and the number of z segments each tile column is split into (0: one wave
+ * of resident workgroups). Sweep 0 is the 1st-order velocity sweep or the
+ * 2nd-order step; sweep 1 the stress sweep. Every shape gives bit-identical
+ * results. An explicit shape disables the autotuner for the context. */
+int a3d_set_launch(a3d_ctx* ctx, int ty_vel, int zseg_vel, int ty_stress, int zseg_stress);
+/* out6 = {ty, z segments, workgroups} of sweep 0, then of sweep 1. */
+int a3d_get_launch(a3d_ctx* ctx, int* out6);
+
+/* Autotuner: times `steps` real time steps of *desc for each candidate
+ * shape (TY in {2,4,8} x z segments in {1..24}) and keeps the fastest per
+ * sweep; best_ms (may be NULL) receives the two best average launch times.
+ * Results are cached per process by (device, operator, padded grid). a3d_run
+ * and the legacy entry points call it on the first shot of a grid with at
+ * least 4M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
+int a3d_autotune(a3d_ctx* ctx, const a3d_shot_desc* desc, int steps, double* best_ms);
 
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
