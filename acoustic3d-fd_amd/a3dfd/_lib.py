@@ -105,6 +105,8 @@ def load(path=None):
     lib.a3d_get_launch.argtypes = [vp, POINTER(c_int)]
     lib.a3d_autotune.argtypes = [vp, POINTER(ShotDesc), c_int, POINTER(c_double)]
     lib.a3d_grid_info.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_longlong)]
+    lib.a3d_probe_bandwidth.argtypes = [c_int, ctypes.c_size_t, c_int, POINTER(c_double), POINTER(c_double)]
+    lib.a3d_probe_bandwidth.restype = c_int
     lib.a3d_last_error.restype = c_char_p
     lib.a3d_version.restype = c_char_p
     for name in ("a3d_create", "a3d_destroy", "a3d_set_model", "a3d_run", "a3d_set_kernel_timing",
@@ -113,6 +115,14 @@ def load(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+def probe_bandwidth(device=0, nbytes=2 << 30, reps=10):
+    """Returns (copy GB/s counting read+write, read-only GB/s) of the calibration streams."""
+    lib = load()
+    c, r = c_double(), c_double()
+    _check(lib, lib.a3d_probe_bandwidth(device, nbytes, reps, ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
 
 
 class A3DError(RuntimeError):

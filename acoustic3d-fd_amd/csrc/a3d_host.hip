@@ -878,6 +878,45 @@ int a3d_autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms)
     return autotune(c, d, steps, best_ms);
 }
 
+int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs) {
+    if (bytes < (1u << 20) || reps < 1) return fail(A3D_EINVAL, "probe needs >= 1 MiB and >= 1 repetition");
+    HIP_TRY(hipSetDevice(device));
+    bytes &= ~(size_t)15;
+    float *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return fail(A3D_ENOMEM, "probe allocation failed");
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(A3D_ENOMEM, "probe allocation failed");
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = A3D_OK;
+    auto run = [&](int ro, int grid, double* gbs) -> int {
+        HIP_TRY(launch_probe(ro, a, b, bytes, grid, st));   // warm-up
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) HIP_TRY(launch_probe(ro, a, b, bytes, grid, st));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        const double moved = (ro ? 1.0 : 2.0) * (double)bytes * reps;
+        if (gbs) *gbs = moved / (ms * 1e-3) / 1e9;
+        return A3D_OK;
+    };
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = fail(A3D_EHIP, "probe stream/event creation failed");
+    if (!rc) rc = hipMemsetAsync(a, 0, bytes, st) == hipSuccess ? A3D_OK : fail(A3D_EHIP, "memset failed");
+    if (!rc) rc = run(0, 1024, copy_gbs);
+    if (!rc) rc = run(1, 1024, read_gbs);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 int a3d_grid_info(const a3d_ctx* c, int* nnx, int* nny, int* nnz, long long* updated) {
     if (!c) return fail(A3D_EINVAL, "null context");
     if (nnx) *nnx = c->g.nnx;

@@ -125,6 +125,9 @@ struct MuteArgs {
 hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const MuteArgs& m,
                                  float* shot, hipStream_t st);
 
+// Calibration streams: float4 copy a -> b, or read-only reduction of a (b: 64 B scratch).
+hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, int grid, hipStream_t st);
+
 // Sets *flag to 1 if any of the n values is not finite.
 hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st);
 

@@ -543,6 +543,25 @@ __global__ void k_check_finite(const float* __restrict__ a, size_t n, int* flag)
     if (bad) *flag = 1;
 }
 
+// Calibration streams for the HBM byte counters and the achievable-bandwidth
+// ceiling: a float4 copy and a float4 read-only reduction, grid-stride.
+__global__ void k_probe_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) b[i] = a[i];
+}
+
+__global__ void k_probe_read(const float4* __restrict__ a, size_t n, float* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    float acc = 0.0f;
+    for (; i < n; i += s) {
+        const float4 v = a[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    if (acc == -1.0f) out[0] = acc;   // keeps the loads live; never true for the zeroed input
+}
+
 template <int OP>
 hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
@@ -656,6 +675,15 @@ hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const
                                  float* shot, hipStream_t st) {
     const dim3 grid((nt + 63) / 64, (nx + 63) / 64, ny);
     hipLaunchKernelGGL(k_shot_transpose, grid, dim3(64, 4), 0, st, rec, nt, nx, ny, m, shot);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, int grid, hipStream_t st) {
+    const size_t n4 = bytes / 16;
+    if (read_only)
+        hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, st, (const float4*)a, n4, b);
+    else
+        hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)b, n4);
     return hipGetLastError();
 }
 

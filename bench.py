@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--launch", type=str, default="", help="ty_vel,zseg_vel,ty_stress,zseg_stress (default: autotuned)")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only run the calibration streams (2 GiB float4 copy and read) and print GB/s")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every shot on a single MI355X "
@@ -78,6 +80,12 @@ def main():
 
     from a3dfd import Modeller, make_desc
     from a3dfd import models
+    if args.probe_copy:
+        from a3dfd._lib import probe_bandwidth
+        c, r = probe_bandwidth(0, 2 << 30, 10)
+        print(json.dumps({"probe": "float4 grid 1024x256, 2 GiB buffers, 10 reps",
+                          "copy_GBs_read_plus_write": round(c, 1), "read_only_GBs": round(r, 1)}), flush=True)
+        return
 
     t0 = time.time()
     cfg = models.cfg3()

@@ -171,6 +171,12 @@ int a3d_autotune(a3d_ctx* ctx, const a3d_shot_desc* desc, int steps, double* bes
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
 
+/* Measurement support: achievable HBM bandwidth of a plain float4 copy
+ * (read+write bytes / time) and of a float4 read-only stream over `bytes`
+ * per buffer, `reps` repetitions, on `device`. Also the calibration kernels
+ * for the FETCH_SIZE / WRITE_SIZE byte counters. */
+int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs);
+
 const char* a3d_last_error(void);
 const char* a3d_version(void);
 
