@@ -50,6 +50,7 @@ class ShotDesc(ctypes.Structure):
         ("v_mute", c_float),
         ("mute_tt", c_int),
         ("storage", c_int),
+        ("scheme", c_int),
     ]
 
 
@@ -68,11 +69,11 @@ def make_desc(op, nx, ny, nz, dx, dy, dz, sx, sy, sz, nt, dt, fm, **kw):
     """Builds a ShotDesc with the reference defaults of `op`, overridden by kw."""
     op = OP_NAMES.get(op, op)
     d = dict(REFERENCE_DEFAULTS[op])
-    d.update(dict(snap_interval=0, snap_skip0=0, mute=0, v_mute=0.0, mute_tt=60, storage=0))
+    d.update(dict(snap_interval=0, snap_skip0=0, mute=0, v_mute=0.0, mute_tt=60, storage=0, scheme=0))
     d.update(kw)
     return ShotDesc(op, nx, ny, nz, dx, dy, dz, d["nb"], sx, sy, sz, nt, dt, fm, d["wtype"], d["pfac"],
                     d["rec_z"], d["sv"], d["snap_interval"], d["snap_skip0"], d["mute"], d["v_mute"],
-                    d["mute_tt"], d["storage"])
+                    d["mute_tt"], d["storage"], d["scheme"])
 
 
 def desc_dict(d):

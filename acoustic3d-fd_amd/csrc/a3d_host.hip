@@ -67,42 +67,46 @@ std::vector<int> volumes_of(int op) {
 }
 
 bool is_wavefield(int v) { return (v >= V_U && v <= V_QZ) || v == V_PA || v == V_PB; }
+bool is_split(int v) { return v >= V_PX && v <= V_QZ; }
 
 // ---------------------------------------------------------------------------
 // Setup tables (host), reference arithmetic
 // ---------------------------------------------------------------------------
 
 // add_source wavelet value at float-accumulated time t (reference iso:52-89;
-// vti:54-79 and tti:31-56 use types 1..3).
-float wavelet_value(int wtype, float t, float favg) {
-    float x_, xx_, tdelay, ts, source = 0.0f;
-    tdelay = (float)(1.0 / favg);
+// vti:54-79 and tti:31-56 use types 1..3). T = float follows the reference's
+// float/double mix; T = double evaluates the same formulas in double.
+template <class T>
+T wavelet_value(int wtype, float tf, float fm) {
+    const T t = (T)tf, favg = (T)fm;
+    T x_, xx_, tdelay, ts, source = 0;
+    tdelay = (T)(1.0 / favg);
     ts = t - tdelay;
     switch (wtype) {
         case 1:
             x_ = favg * ts;
             xx_ = x_ * x_;
-            source = (float)((1 - 2 * kPi * kPi * (xx_)) * std::exp(-(kPi * kPi * xx_)));
+            source = (T)((1 - 2 * kPi * kPi * (xx_)) * std::exp(-(kPi * kPi * xx_)));
             break;
         case 2:
-            x_ = (float)((-4) * favg * favg * kPi * kPi / std::log(0.1));
-            source = (float)((-2) * kPi * kPi * ts * std::exp(-x_ * ts * ts));
+            x_ = (T)((-4) * favg * favg * kPi * kPi / std::log(0.1));
+            source = (T)((-2) * kPi * kPi * ts * std::exp(-x_ * ts * ts));
             break;
         case 3:
-            x_ = (float)((-1) * favg * favg * kPi * kPi / std::log(0.1));
-            source = std::exp(-x_ * ts * ts);
+            x_ = (T)((-1) * favg * favg * kPi * kPi / std::log(0.1));
+            source = (T)std::exp(-x_ * ts * ts);
             break;
         case 4:
-            x_ = (float)(std::exp(-favg * favg * kPi * kPi * ts * ts) * ts);
+            x_ = (T)(std::exp(-favg * favg * kPi * kPi * ts * ts) * ts);
             source = -x_;
             break;
         case 5:
-            x_ = (float)(-1.0 / (2.0 * favg * favg * kPi * kPi));
-            x_ = (float)(x_ * std::exp(-favg * favg * kPi * kPi * ts * ts));
+            x_ = (T)(-1.0 / (2.0 * favg * favg * kPi * kPi));
+            x_ = (T)(x_ * std::exp(-favg * favg * kPi * kPi * ts * ts));
             source = x_;
             break;
         case 6:
-            x_ = (float)(-std::erf(favg * kPi * ts) / (4 * favg * favg * favg * kPi * kPi * sqrtf((float)kPi)));
+            x_ = (T)(-std::erf(favg * kPi * ts) / (4 * favg * favg * favg * kPi * kPi * std::sqrt((T)kPi)));
             source = x_;
             break;
         default: break;
@@ -111,31 +115,36 @@ float wavelet_value(int wtype, float t, float favg) {
 }
 
 // cuda_ricker_wavelet (reference Toa_3dfd_cuda.cu:34-43).
-float ricker_value(int it, float fm, float dt) {
-    float tmp = (float)(kPi * fm * std::fabs((float)(it * dt - 1.0 / fm)));
+template <class T>
+T ricker_value(int it, float fm_, float dt_) {
+    const T fm = (T)fm_, dt = (T)dt_;
+    T tmp;
+    if (sizeof(T) == 4) tmp = (T)(kPi * fm * std::fabs((float)(it * dt - 1.0 / fm)));
+    else tmp = (T)(kPi * fm * std::fabs(it * dt - 1.0 / fm));
     tmp *= tmp;
-    return (float)((1.0 - 2.0 * tmp) * std::exp(-tmp));
+    return (T)((1.0 - 2.0 * tmp) * std::exp(-tmp));
 }
 
 // initial_coffe (reference iso:263-291) for one axis of interior length n.
-void pml_axis(float dt, float d0, int n, int npml, float* c1, float* c2, float* a1, float* a2) {
-    const float dtd0 = dt * d0;
+template <class T>
+void pml_axis(T dt, T d0, int n, int npml, T* c1, T* c2, T* a1, T* a2) {
+    const T dtd0 = dt * d0;
     for (int id = 0; id < n + 2 * npml; ++id) {
         double r, ra;
         if (id < npml) {
             r = (npml - 0.5 - id) / npml;
             ra = ((npml - id) * 1.0) / npml;
         } else if (id < npml + n) {
-            c1[id] = c2[id] = a1[id] = a2[id] = 1.0f;
+            c1[id] = c2[id] = a1[id] = a2[id] = (T)1.0;
             continue;
         } else {
             r = (0.5 + id - n - npml) / npml;
             ra = ((id - n - npml) * 1.0) / npml;
         }
-        c1[id] = (float)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
-        c2[id] = (float)(c1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
-        a1[id] = (float)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
-        a2[id] = (float)(a1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
+        c1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
+        c2[id] = (T)(c1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
+        a1[id] = (T)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
+        a2[id] = (T)(a1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
     }
 }
 
@@ -174,10 +183,12 @@ struct a3d_ctx {
     Geo g{};
     size_t nvox = 0;
     int off = 0;
-    float* vol[NVOL] = {};
+    int prec = PREC_FP32;       // a3d_shot_desc.storage: 0 FP32, 1 FP16 wavefields, 2 FP64
+    size_t wsize = 4, msize = 4, csize = 4;   // bytes per wavefield / model / table element
+    void* vol[NVOL] = {};
     float* stage = nullptr;     // 2nd order: interior velocity for (v*dt)^2
     float vv_dt = 0.0f;
-    float* tab = nullptr;       // 12 PML arrays
+    void* tab = nullptr;        // 12 PML arrays (float, or double for FP64)
     double* dtab = nullptr;     // sponge fx, fy, fz; SV taper
     float* rec = nullptr;
     size_t rec_cap = 0;
@@ -187,6 +198,7 @@ struct a3d_ctx {
     int* flag = nullptr;
     bool model_set = false;
     float vel_first = 0.f, eps_first = 0.f, vel_d0 = 0.f, vel_raw1 = 0.f;
+    float zmin = 1.0f;          // minimum acoustic impedance rho*v (binary16 velocity-field scale)
     // launch shape per sweep: [0] 2nd-order step / velocity sweep, [1] stress sweep
     int ty[2] = {8, 4}, zseg[2] = {0, 0};
     bool tuned = false;
@@ -215,7 +227,7 @@ Launch make_launch(const Geo& g, int ty, int zseg) {
 int single_wave_zseg(const a3d_ctx* c, int kind, int ty) {
     const Launch l1 = make_launch(c->g, ty, 1);
     const int per_plane = l1.ntx * l1.nty;
-    const int cap = std::max(1, max_blocks_per_cu(kind, c->d.op, ty)) * c->ncu;
+    const int cap = std::max(1, max_blocks_per_cu(c->prec, kind, c->d.op, ty)) * c->ncu;
     const int nzi = c->g.nnz - 2 * MM;
     int nseg = std::max(1, cap / std::max(per_plane, 1));
     nseg = std::min(nseg, std::max(1, nzi / 16));
@@ -231,7 +243,7 @@ void default_shapes(a3d_ctx* c) {
 
 bool same_geometry(const a3d_shot_desc& a, const a3d_shot_desc& b) {
     return a.op == b.op && a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx &&
-           a.dy == b.dy && a.dz == b.dz && a.nb == b.nb;
+           a.dy == b.dy && a.dz == b.dz && a.nb == b.nb && a.storage == b.storage;
 }
 
 int check_desc_geometry(const a3d_shot_desc* d) {
@@ -240,7 +252,7 @@ int check_desc_geometry(const a3d_shot_desc* d) {
     if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(A3D_EINVAL, "model size must be positive");
     if (!(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0)) return fail(A3D_EINVAL, "grid spacing must be positive");
     if (d->op == A3D_OP_2ND ? d->nb < 1 : d->nb < MM) return fail(A3D_EINVAL, "boundary width %d too small", d->nb);
-    if (d->storage != 0) return fail(A3D_EINVAL, "storage mode %d not available", d->storage);
+    if (d->storage < 0 || d->storage > 2) return fail(A3D_EINVAL, "storage mode %d not in 0..2", d->storage);
     const int ring = d->op == A3D_OP_2ND ? 2 * MM + 2 * d->nb : 2 * d->nb;
     const double n = (double)(d->nx + ring) * (d->ny + ring) * (d->nz + ring);
     if (n * 1.1 > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
@@ -253,6 +265,7 @@ int check_desc_shot(const a3d_ctx* c, const a3d_shot_desc* d) {
     if (!(d->dt > 0) || !(d->fm > 0)) return fail(A3D_EINVAL, "dt and fm must be positive");
     if (d->rec_z < 0 || d->rec_z >= d->nz) return fail(A3D_EINVAL, "rec_z %d outside [0,%d)", d->rec_z, d->nz);
     if (d->snap_interval < 0) return fail(A3D_EINVAL, "snap_interval must be >= 0");
+    if (d->scheme < 0 || d->scheme > 1) return fail(A3D_EINVAL, "scheme %d not in 0..1", d->scheme);
     if (d->op == A3D_OP_2ND) {
         if (d->sx < 0 || d->sx >= d->nx || d->sy < 0 || d->sy >= d->ny || d->sz < 0 || d->sz + 2 >= d->nz + d->nb)
             return fail(A3D_EINVAL, "source (%d,%d,%d) outside the model", d->sx, d->sy, d->sz);
@@ -321,16 +334,37 @@ void timing_collect(a3d_ctx* c) {
 
 using SnapFn = std::function<void(int k, int it, const float* host)>;
 
-int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn,
-             int* nsnap_taken, double* loop_ms) {
-    int rc = check_desc_shot(c, d);
-    if (rc) return rc;
-    if (!c->model_set) return fail(A3D_ESTATE, "model not set");
-    HIP_TRY(hipSetDevice(c->device));
+// Power-of-two helper for the binary16 field scales.
+float pow2_ceil(double v) { return v > 0 ? (float)std::ldexp(1.0, (int)std::ceil(std::log2(v))) : 1.0f; }
+
+// Scales for binary16 wavefield storage (largest finite half: 65504). Pressures and
+// split stresses: the near-source peak is ~1.2 pfac (measured on the oracle), mapped to
+// <= 2^11. Particle velocities: |u| <= |P| / (rho v) for a plane wave, so their scale is
+// the pressure scale over the model's smallest impedance, leaving >= 16x headroom for
+// the near field. Subnormal halves extend the range down by 2^-24 relative to the scale.
+Scales field_scales(const a3d_ctx* c, const a3d_shot_desc* d) {
+    Scales sc{};
+    for (int f = 0; f < NF; ++f) sc.s[f] = sc.inv[f] = 1.0f;
+    if (c->prec != PREC_FP16) return sc;
+    const double amp = d->op == A3D_OP_2ND ? 1.0 : std::max((double)std::fabs(d->pfac), 1e-30);
+    const float sp = pow2_ceil(amp / 2048.0);
+    const float su = pow2_ceil((double)sp / std::max((double)c->zmin, 1e-30));
+    for (int f : {F_P, F_Q, F_PX, F_PY, F_PZ, F_QX, F_QY, F_QZ}) sc.s[f] = sp;
+    for (int f : {F_U, F_V, F_W}) sc.s[f] = su;
+    for (int f = 0; f < NF; ++f) sc.inv[f] = 1.0f / sc.s[f];
+    return sc;
+}
+
+template <class T>
+int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
+               double* loop_ms) {
     const Geo g = c->g;
+    const int prec = c->prec;
     const int nx = d->nx, ny = d->ny, nz = d->nz, nt = d->nt, nb = d->nb;
     const size_t plane_rec = (size_t)nx * ny;
+    int rc;
     if ((rc = ensure(&c->rec, &c->rec_cap, plane_rec * nt))) return rc;
+    const Scales sc = field_scales(c, d);
 
     std::vector<float> hsnap;
     const bool snaps = d->snap_interval > 0 && snapfn;
@@ -345,29 +379,31 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         return d->snap_interval > 0 && it % d->snap_interval == 0 && !(d->snap_skip0 && it == 0);
     };
     int nsnap = 0;
-    auto take_snap = [&](int it, const float* src, int s0, float a0, int s1, float a1, bool sponge) -> int {
+    auto take_snap = [&](int it, const void* src, int s0, double a0, int s1, double a1, bool sponge) -> int {
         ++nsnap;
         if (!snaps) return A3D_OK;
         const double* fx = sponge ? c->dtab : nullptr;
         const double* fy = sponge ? c->dtab + g.nnx : nullptr;
         const double* fz = sponge ? c->dtab + g.nnx + g.nny : nullptr;
-        HIP_TRY(launch_snap(src, g, c->off, nx, ny, nz, s0, a0, s1, a1, fx, fy, fz, c->snap_dev, c->st));
+        HIP_TRY(launch_snap(prec, src, g, c->off, nx, ny, nz, s0, a0, s1, a1, fx, fy, fz, sc.s[F_P], c->snap_dev,
+                            c->st));
         HIP_TRY(hipMemcpyAsync(hsnap.data(), c->snap_dev, hsnap.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
         snapfn(nsnap - 1, it, hsnap.data());
         return A3D_OK;
     };
+    const size_t wbytes = c->nvox * c->wsize;
 
     if (d->op == A3D_OP_2ND) {
         // -------- 2nd order (reference cuda_3dfd, Toa_3dfd_cuda.cu:366-508)
         if (c->vv_dt != d->dt) {
-            HIP_TRY(launch_pad(c->stage, nx, ny, nz, c->off, g, c->vol[V_VV], 1, d->dt, c->st));
+            HIP_TRY(launch_pad(c->stage, nx, ny, nz, c->off, g, c->vol[V_VV], prec == PREC_FP64, 1, d->dt, c->st));
             c->vv_dt = d->dt;
         }
-        HIP_TRY(hipMemsetAsync(c->vol[V_PA], 0, c->nvox * sizeof(float), c->st));
-        HIP_TRY(hipMemsetAsync(c->vol[V_PB], 0, c->nvox * sizeof(float), c->st));
-        std::vector<float> wlt(nt);
-        for (int it = 0; it < nt; ++it) wlt[it] = ricker_value(it, d->fm, d->dt);
+        HIP_TRY(hipMemsetAsync(c->vol[V_PA], 0, wbytes, c->st));
+        HIP_TRY(hipMemsetAsync(c->vol[V_PB], 0, wbytes, c->st));
+        std::vector<T> wlt(nt);
+        for (int it = 0; it < nt; ++it) wlt[it] = ricker_value<T>(it, d->fm, d->dt);
         Step2Args a{};
         a.g = g;
         a.l = make_launch(g, c->ty[0], c->zseg[0]);
@@ -375,9 +411,10 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         a.fx = c->dtab;
         a.fy = c->dtab + g.nnx;
         a.fz = c->dtab + g.nnx + g.nny;
-        a.dz2 = (float)(1.0 / (d->dz * d->dz));
-        a.dx2 = (float)(1.0 / (d->dx * d->dx));
-        a.dy2 = (float)(1.0 / (d->dy * d->dy));
+        a.dz2 = (T)(1.0 / ((T)d->dz * (T)d->dz));
+        a.dx2 = (T)(1.0 / ((T)d->dx * (T)d->dx));
+        a.dy2 = (T)(1.0 / ((T)d->dy * (T)d->dy));
+        a.sc = sc;
         const int off = c->off;
         a.s0 = (d->sx + off) + g.ld * (d->sy + off) + g.pl * (d->sz + off);
         a.s1 = a.s0 + 2 * g.pl;
@@ -385,19 +422,19 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         a.off = off;
         a.nx = nx;
         a.ny = ny;
-        float* X = c->vol[V_PA];
-        float* Y = c->vol[V_PB];
+        void* X = c->vol[V_PA];
+        void* Y = c->vol[V_PB];
         HIP_TRY(hipEventRecord(c->ev0, c->st));
         for (int it = 0; it < nt; ++it) {
             a.X = X;
             a.Y = Y;
             a.ya0 = wlt[it];
             a.ya1 = -wlt[it];
-            a.xa0 = it > 0 ? wlt[it - 1] : 0.0f;
-            a.xa1 = it > 0 ? -wlt[it - 1] : 0.0f;
+            a.xa0 = it > 0 ? (double)wlt[it - 1] : 0.0;
+            a.xa1 = it > 0 ? -(double)wlt[it - 1] : 0.0;
             a.rec = c->rec + plane_rec * it;
             timing_begin(c, 0);
-            HIP_TRY(launch_step2(c->ty[0], a, c->st));
+            HIP_TRY(launch_step2(prec, c->ty[0], a, c->st));
             timing_end(c);
             if (want_snap(it)) {
                 if ((rc = take_snap(it, Y, a.s0, a.ya0, a.s1, a.ya1, true))) return rc;
@@ -408,42 +445,42 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
     } else {
         // -------- 1st-order family (reference iso:372-671, vti:389-688, tti:390-676)
         const int op = d->op;
-        const float dt = d->dt;
+        const T dt = (T)d->dt;
         // get_d0 (reference iso:193-195) from the padded velocity at z-fastest index N/2
-        const float dsum = d->dx + d->dy + d->dz;
-        const float d0 = (float)(10.0 * c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
-        std::vector<float> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
-        float* t = tabh.data();
-        float *cx1 = t, *cx2 = cx1 + g.nnx, *ax1 = cx2 + g.nnx, *ax2 = ax1 + g.nnx;
-        float *cy1 = ax2 + g.nnx, *cy2 = cy1 + g.nny, *ay1 = cy2 + g.nny, *ay2 = ay1 + g.nny;
-        float *cz1 = ay2 + g.nny, *cz2 = cz1 + g.nnz, *az1 = cz2 + g.nnz, *az2 = az1 + g.nnz;
-        pml_axis(dt, d0, nx, nb, cx1, cx2, ax1, ax2);
-        pml_axis(dt, d0, ny, nb, cy1, cy2, ay1, ay2);
-        pml_axis(dt, d0, nz, nb, cz1, cz2, az1, az2);
-        HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(float), hipMemcpyHostToDevice));
-        const float* T = c->tab;
-        Pml pm{T,
-               T + g.nnx,
-               T + 4 * g.nnx,
-               T + 4 * g.nnx + g.nny,
-               T + 4 * g.nnx + 4 * g.nny,
-               T + 4 * g.nnx + 4 * g.nny + g.nnz,
-               T + 2 * g.nnx,
-               T + 3 * g.nnx,
-               T + 4 * g.nnx + 2 * g.nny,
-               T + 4 * g.nnx + 3 * g.nny,
-               T + 4 * g.nnx + 4 * g.nny + 2 * g.nnz,
-               T + 4 * g.nnx + 4 * g.nny + 3 * g.nnz};
+        const T dsum = (T)d->dx + (T)d->dy + (T)d->dz;
+        const T d0 = (T)(10.0 * (T)c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
+        std::vector<T> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
+        Pml pm{};
+        pm.cx1 = 0;
+        pm.cx2 = g.nnx;
+        pm.ax1 = 2 * g.nnx;
+        pm.ax2 = 3 * g.nnx;
+        pm.cy1 = 4 * g.nnx;
+        pm.cy2 = pm.cy1 + g.nny;
+        pm.ay1 = pm.cy1 + 2 * g.nny;
+        pm.ay2 = pm.cy1 + 3 * g.nny;
+        pm.cz1 = 4 * g.nnx + 4 * g.nny;
+        pm.cz2 = pm.cz1 + g.nnz;
+        pm.az1 = pm.cz1 + 2 * g.nnz;
+        pm.az2 = pm.cz1 + 3 * g.nnz;
+        T* t = tabh.data();
+        pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2);
+        pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2);
+        pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2);
+        HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(T), hipMemcpyHostToDevice));
         for (int v : volumes_of(op))
-            if (is_wavefield(v)) HIP_TRY(hipMemsetAsync(c->vol[v], 0, c->nvox * sizeof(float), c->st));
+            if (is_wavefield(v))
+                HIP_TRY(hipMemsetAsync(c->vol[v], 0, is_split(v) ? c->nvox * std::max(c->wsize, (size_t)4) : wbytes,
+                                       c->st));
 
-        const float dtx = dt / d->dx, dty = dt / d->dy, dtz = dt / d->dz;
+        const T dtx = dt / (T)d->dx, dty = dt / (T)d->dy, dtz = dt / (T)d->dz;
         const int sxp = d->sx + nb - 1, syp = d->sy + nb - 1, szp = d->sz + nb - 1;
         const int s0 = sxp + g.ld * syp + g.pl * szp;
         VelArgs va{};
         va.g = g;
         va.l = make_launch(g, c->ty[0], c->zseg[0]);
         va.c = pm;
+        va.tab = c->tab;
         va.P = c->vol[V_P];
         va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
         va.rho = c->vol[V_RHO];
@@ -455,10 +492,12 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         va.dtx = dtx;
         va.dty = dty;
         va.dtz = dtz;
+        va.sc = sc;
         StressArgs sa{};
         sa.g = g;
         sa.l = make_launch(g, c->ty[1], c->zseg[1]);
         sa.c = pm;
+        sa.tab = c->tab;
         sa.u = c->vol[V_U];
         sa.v = c->vol[V_V];
         sa.w = c->vol[V_W];
@@ -488,34 +527,39 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
         sa.nb = nb;
         sa.nx = nx;
         sa.ny = ny;
+        sa.sc = sc;
+        sa.unsplit = d->scheme != 0 || prec == PREC_FP16;
 
-        const float favg = d->fm;
-        const float tdelay = (float)(1.0 / favg);
-        const float pfac = d->pfac;
+        // The time grid is the reference's float accumulation t = dt, t += dt and the gate
+        // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
+        const float tdelay = (float)(1.0 / d->fm);
+        const T pfac = (T)d->pfac;
         HIP_TRY(hipEventRecord(c->ev0, c->st));
-        float tcur = dt;
-        for (int it = 0; it < nt; it++, tcur += dt) {
+        float tcur = d->dt;
+        for (int it = 0; it < nt; it++, tcur += d->dt) {
             // add_source (reference iso:523-530; vti:557; tti:571), applied on load
-            Src S{-1, -1, 0.f, 0.f, 0.f, 0.f};
+            Src S{-1, -1, 0.0, 0.0, 0.0, 0.0};
             if (tcur <= 2 * tdelay) {
-                const float src = wavelet_value(d->wtype, tcur, favg);
+                const T src = wavelet_value<T>(d->wtype, tcur, d->fm);
                 if (op == A3D_OP_ISO) {
-                    const float mp = (float)(-1.0 * pfac);
-                    S = {s0, s0 + 2 * g.pl, mp * src, pfac * src, mp * src, pfac * src};
+                    const T mp = (T)(-1.0 * pfac);
+                    const T a0 = mp * src, a1 = pfac * src;
+                    S = {s0, s0 + 2 * g.pl, (double)a0, (double)a1, (double)a0, (double)a1};
                 } else {
-                    S = {s0, -1, pfac * src, 0.f, pfac * src, 0.f};
+                    const T a0 = pfac * src;
+                    S = {s0, -1, (double)a0, 0.0, (double)a0, 0.0};
                 }
             }
             va.s = S;
             sa.rec = c->rec + plane_rec * it;
             timing_begin(c, 0);
-            HIP_TRY(launch_vel(op, c->ty[0], va, c->st));
+            HIP_TRY(launch_vel(prec, op, c->ty[0], va, c->st));
             timing_end(c);
             timing_begin(c, 1);
-            HIP_TRY(launch_stress(op, c->ty[1], sa, c->st));
+            HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
             timing_end(c);
             if (want_snap(it)) {
-                if ((rc = take_snap(it, c->vol[V_P], -1, 0.f, -1, 0.f, false))) return rc;
+                if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
             }
         }
         HIP_TRY(hipEventRecord(c->ev1, c->st));
@@ -559,6 +603,16 @@ int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& 
     return A3D_OK;
 }
 
+int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
+             double* loop_ms) {
+    int rc = check_desc_shot(c, d);
+    if (rc) return rc;
+    if (!c->model_set) return fail(A3D_ESTATE, "model not set");
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->prec == PREC_FP64) return run_shot_t<double>(c, d, shot_out, snapfn, nsnap_taken, loop_ms);
+    return run_shot_t<float>(c, d, shot_out, snapfn, nsnap_taken, loop_ms);
+}
+
 }  // namespace
 
 namespace {
@@ -570,9 +624,10 @@ namespace {
 // process by (device, operator, padded grid).
 // ---------------------------------------------------------------------------
 struct TuneKey {
-    int dev, op, nnx, nny, nnz;
+    int dev, op, prec, scheme, nnx, nny, nnz;
     bool operator==(const TuneKey& o) const {
-        return dev == o.dev && op == o.op && nnx == o.nnx && nny == o.nny && nnz == o.nnz;
+        return dev == o.dev && op == o.op && prec == o.prec && scheme == o.scheme && nnx == o.nnx && nny == o.nny &&
+               nnz == o.nnz;
     }
 };
 struct TuneVal {
@@ -582,7 +637,7 @@ std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 std::mutex g_tune_mu;
 
 int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
-    const TuneKey key{c->device, c->d.op, c->g.nnx, c->g.nny, c->g.nnz};
+    const TuneKey key{c->device, c->d.op, c->prec, d->scheme, c->g.nnx, c->g.nny, c->g.nnz};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (auto& kv : g_tune_cache)
@@ -699,6 +754,10 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
     c->g.ld = (c->g.nnx + TX - 1) / TX * TX;
     c->g.pl = c->g.ld * c->g.nny;
     c->nvox = (size_t)c->g.pl * c->g.nnz;
+    c->prec = desc->storage == 1 ? PREC_FP16 : (desc->storage == 2 ? PREC_FP64 : PREC_FP32);
+    c->wsize = c->prec == PREC_FP16 ? 2 : (c->prec == PREC_FP64 ? 8 : 4);
+    c->msize = c->prec == PREC_FP64 ? 8 : 4;
+    c->csize = c->msize;
     auto cleanup = [&](int code) {
         a3d_destroy(c);
         return code;
@@ -710,14 +769,16 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return cleanup(fail(A3D_EHIP, "event creation failed"));
     for (int v : volumes_of(op)) {
-        if (hipMalloc(&c->vol[v], c->nvox * sizeof(float)) != hipSuccess) {
+        const size_t esz = is_split(v) ? std::max(c->wsize, (size_t)4) : (is_wavefield(v) ? c->wsize : c->msize);
+        const size_t bytes = c->nvox * esz;
+        if (hipMalloc(&c->vol[v], bytes) != hipSuccess) {
             c->vol[v] = nullptr;
-            return cleanup(fail(A3D_ENOMEM, "device allocation of %zu MiB failed", c->nvox * 4 >> 20));
+            return cleanup(fail(A3D_ENOMEM, "device allocation of %zu MiB failed", bytes >> 20));
         }
     }
     const size_t ntab = 4 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz);
     const size_t ndtab = (size_t)c->g.nnx + c->g.nny + c->g.nnz + 226;
-    if (hipMalloc(&c->tab, ntab * sizeof(float)) != hipSuccess) { c->tab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
+    if (hipMalloc(&c->tab, ntab * c->csize) != hipSuccess) { c->tab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
     if (hipMalloc(&c->dtab, ndtab * sizeof(double)) != hipSuccess) { c->dtab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
     if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) { c->flag = nullptr; return cleanup(fail(A3D_ENOMEM, "flag allocation failed")); }
     // double tables: sponge factors (2nd order) and SV taper (VTI/TTI)
@@ -729,8 +790,10 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
     }
     double* taper = dt.data() + c->g.nnx + c->g.nny + c->g.nnz;
     for (int r2 = 0; r2 < 226; ++r2) {
-        // reference vti:163-166: 0.5*(1-cos(pi*((sqrtf(r2)-r)*4.0/(R*3.0-1)))), R=15, r=4
-        taper[r2] = r2 <= 16 ? 0.0 : 0.5 * (1 - std::cos(kPi * ((sqrtf((float)r2) - 4) * 4.0 / (15 * 3.0 - 1))));
+        // reference vti:163-166: 0.5*(1-cos(pi*((sqrtf(r2)-r)*4.0/(R*3.0-1)))), R=15, r=4;
+        // the FP64 instance takes the square root in double
+        const double rr = c->prec == PREC_FP64 ? std::sqrt((double)r2) - 4 : (double)(sqrtf((float)r2) - 4);
+        taper[r2] = r2 <= 16 ? 0.0 : 0.5 * (1 - std::cos(kPi * (rr * 4.0 / (15 * 3.0 - 1))));
     }
     if (hipMemcpy(c->dtab, dt.data(), ndtab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(A3D_EHIP, "table upload failed"));
@@ -747,7 +810,7 @@ int a3d_destroy(a3d_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     for (auto& p : c->vol)
         if (p) (void)hipFree(p);
-    for (float* p : {c->stage, c->tab, c->rec, c->shot_dev, c->snap_dev})
+    for (void* p : {(void*)c->stage, c->tab, (void*)c->rec, (void*)c->shot_dev, (void*)c->snap_dev})
         if (p) (void)hipFree(p);
     if (c->dtab) (void)hipFree(c->dtab);
     if (c->flag) (void)hipFree(c->flag);
@@ -774,7 +837,7 @@ int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* e
     if (hipMalloc(&stage, n * sizeof(float)) != hipSuccess) return fail(A3D_ENOMEM, "staging allocation failed");
     auto up = [&](const float* h, int v) -> int {
         HIP_TRY(hipMemcpy(stage, h, n * sizeof(float), hipMemcpyHostToDevice));
-        HIP_TRY(launch_pad(stage, nx, ny, nz, c->off, c->g, c->vol[v], 0, 0.f, c->st));
+        HIP_TRY(launch_pad(stage, nx, ny, nz, c->off, c->g, c->vol[v], c->prec == PREC_FP64, 0, 0.f, c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
         return A3D_OK;
     };
@@ -810,6 +873,10 @@ int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* e
                           (size_t)nz * nx * clampi(-nb, 0, ny - 1);
         c->vel_first = vel[k1];
         c->eps_first = eps ? eps[k1] : 0.0f;
+        // smallest impedance rho*v (TTI has no density: rho = 1)
+        double zmin = 1e300;
+        for (size_t i = 0; i < n; ++i) zmin = std::min(zmin, (double)vel[i] * (rho ? (double)rho[i] : 1.0));
+        c->zmin = (float)std::max(zmin, 1e-30);
     } else {
         c->vel_raw1 = n > 1 ? vel[1] : vel[0];   // reference Toa_3dfd_cuda.cu:403
     }
@@ -1018,6 +1085,9 @@ int legacy_run(Legacy& L, const char* FNsnap, const char* FNshot, FILE* fshot, i
 a3d_shot_desc base_desc(int op, int nx, int ny, int nz, float dx, float dy, float dz, int nb, int nt,
                         float dt, float fm) {
     a3d_shot_desc d{};
+    // A3D_SCHEME=1 selects the unsplit-interior stress update for the legacy entry points
+    // (default 0: the reference's split fields everywhere, bit for bit).
+    if (const char* e = getenv("A3D_SCHEME")) d.scheme = (op != A3D_OP_2ND && e[0] == '1') ? 1 : 0;
     d.op = op;
     d.nx = nx;
     d.ny = ny;

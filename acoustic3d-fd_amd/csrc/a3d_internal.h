@@ -31,10 +31,24 @@ struct Launch {
     int ntx, nty, nseg, zlen, ntiles, grid;
 };
 
-// Per-axis split-PML profiles (1st-order operators), padded index.
+// Precision instances of the sweep kernels.
+enum { PREC_FP32 = 0, PREC_FP16 = 1, PREC_FP64 = 2 };
+
+// Wavefields of the 1st-order family (and the 2nd-order pressure, F_P).
+enum { F_P, F_Q, F_U, F_V, F_W, F_PX, F_PY, F_PZ, F_QX, F_QY, F_QZ, NF };
+
+// Power-of-two scale per wavefield for binary16 storage: value = half * s[f];
+// stored half = value * inv[f]. Unused by the FP32 and FP64 instances.
+struct Scales {
+    float s[NF];
+    float inv[NF];
+};
+
+// Per-axis split-PML profiles (1st-order operators), padded index: offsets
+// into one table of the arithmetic type (float or double).
 struct Pml {
-    const float *cx1, *cx2, *cy1, *cy2, *cz1, *cz2;   // velocity sweep (half-cell)
-    const float *ax1, *ax2, *ay1, *ay2, *az1, *az2;   // stress sweep
+    int cx1, cx2, cy1, cy2, cz1, cz2;   // velocity sweep (half-cell)
+    int ax1, ax2, ay1, ay2, az1, az2;   // stress sweep
 };
 
 // Transient point source of the 1st-order operators: added to P (and Q) on
@@ -42,78 +56,86 @@ struct Pml {
 // P and Q everywhere in the interior). Index -1: no source this step.
 struct Src {
     int s0, s1;
-    float p0, p1, q0, q1;
+    double p0, p1, q0, q1;
 };
 
+// Array pointers are untyped: each precision instance casts them to its
+// storage types (wavefields: float / half / double; model: float / double).
 struct VelArgs {
     Geo g;
     Launch l;
     Pml c;
-    const float* P;
-    const float* Q;
-    const float* rho;
-    const float* th;
-    const float* ph;
-    float* u;
-    float* v;
-    float* w;
-    float dtx, dty, dtz;
+    const void* tab;
+    const void* P;
+    const void* Q;
+    const void* rho;
+    const void* th;
+    const void* ph;
+    void* u;
+    void* v;
+    void* w;
+    double dtx, dty, dtz;
     Src s;
+    Scales sc;
 };
 
 struct StressArgs {
     Geo g;
     Launch l;
     Pml c;
-    const float* u;
-    const float* v;
-    const float* w;
-    float *px, *py, *pz, *qx, *qy, *qz;
-    const float *vp, *rho, *eps, *del, *th, *ph;
-    float* P;
-    float* Q;
-    float dtx, dty, dtz;
+    const void* tab;
+    const void* u;
+    const void* v;
+    const void* w;
+    void *px, *py, *pz, *qx, *qy, *qz;
+    const void *vp, *rho, *eps, *del, *th, *ph;
+    void* P;
+    void* Q;
+    double dtx, dty, dtz;
     int sv, sxp, syp, szp;     // SV taper: on/off and padded source cell
     const double* taper;       // taper factor by r^2, valid for 16 < r^2 <= 225
     float* rec;                // receiver plane of this step ([ny][nx]) or nullptr
     int zrec, nb, nx, ny;
+    int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1
+    Scales sc;
 };
 
 struct Step2Args {
     Geo g;
     Launch l;
-    float* X;                  // level n-1 in, level n+1 (damped once) out, in place
-    const float* Y;            // level n (damped once, without this step's dipole)
-    const float* vv;           // (v*dt)^2
+    void* X;                   // level n-1 in, level n+1 (damped once) out, in place
+    const void* Y;             // level n (damped once, without this step's dipole)
+    const void* vv;            // (v*dt)^2
     const double *fx, *fy, *fz;  // sponge factors per padded index (1.0: undamped)
-    float dx2, dy2, dz2;
+    double dx2, dy2, dz2;
     int s0, s1;                // dipole cells
-    float ya0, ya1;            // this step's dipole, added to Y on load
-    float xa0, xa1;            // previous step's dipole, added to X on load
+    double ya0, ya1;           // this step's dipole, added to Y on load
+    double xa0, xa1;           // previous step's dipole, added to X on load
     float* rec;                // receiver plane of this step or nullptr
     int zrec, off, nx, ny;
+    Scales sc;
 };
 
-hipError_t launch_vel(int op, int ty, const VelArgs& a, hipStream_t st);
-hipError_t launch_stress(int op, int ty, const StressArgs& a, hipStream_t st);
-hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st);
+hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st);
+hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st);
+hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.
-int max_blocks_per_cu(int kind, int op, int ty);
+int max_blocks_per_cu(int prec, int kind, int op, int ty);
 
-// Pads an interior z-fastest model (nx,ny,nz) into the device layout by edge
-// replication; the interior starts at padded index `off` on every axis.
-// mode 0: copy; mode 1: (v*dt)^2 with float arithmetic.
-hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, float* dst,
+// Pads an interior z-fastest float model (nx,ny,nz) into the device layout
+// (float, or double when model_double) by edge replication; the interior
+// starts at padded index `off` on every axis. mode 0: copy; mode 1: (v*dt)^2.
+hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, void* dst, int model_double,
                       int mode, float dt, hipStream_t st);
 
-// Interior crop of a padded volume into z-fastest order ([ny][nx][nz]).
+// Interior crop of a padded wavefield into z-fastest float order ([ny][nx][nz]).
 // 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied
-// (fx != nullptr); 1st-order: plain copy.
-hipError_t launch_snap(const float* src, const Geo& g, int off, int nx, int ny, int nz, int s0,
-                       float a0, int s1, float a1, const double* fx, const double* fy,
-                       const double* fz, float* dst, hipStream_t st);
+// (fx != nullptr); 1st-order: plain copy. scale: binary16 field scale.
+hipError_t launch_snap(int prec, const void* src, const Geo& g, int off, int nx, int ny, int nz, int s0, double a0,
+                       int s1, double a1, const double* fx, const double* fy, const double* fz, float scale,
+                       float* dst, hipStream_t st);
 
 // Record [nt][ny][nx] -> shot [ny][nx][nt], zeroing samples before the
 // direct-wave cut when mute != 0.

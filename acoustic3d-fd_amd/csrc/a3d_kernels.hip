@@ -20,44 +20,91 @@
 // equal b % 8) walks a contiguous band of tiles and its L2 serves the halo
 // rows its neighbours share.
 //
-// Arithmetic is written operation for operation as the reference writes it
-// (evaluation order, float/double mix), compiled with -ffp-contract=off and
-// IEEE division/sqrt, so the FP32 results equal the CPU restatement bit for bit.
+// Precision (template PR): Fp32 (wavefields, model and arithmetic in float,
+// written operation for operation as the reference -- float/double mix,
+// evaluation order -- and compiled with -ffp-contract=off and IEEE division
+// and sqrt, so results equal the CPU restatement bit for bit); Fp16 (wavefields
+// stored as binary16 times a power-of-two scale per field, model and arithmetic
+// in float); Fp64 (everything in double: the GPU error reference).
 #include "a3d_internal.h"
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 namespace a3d {
 namespace {
 
-// Staggered 8th-order coefficients (reference iso:37, vti:40, tti:17).
-constexpr float kC0 = 1.196289f, kC1 = -0.0797526f, kC2 = 0.009570313f, kC3 = -0.0006975447f;
-// 2nd-derivative coefficients (reference Toa_3dfd_cuda.cu:32, double literals stored to float).
-constexpr float kS0 = (float)(-205.0 / 72.0), kS1 = (float)(8.0 / 5.0), kS2 = (float)(-1.0 / 5.0),
-                kS3 = (float)(8.0 / 315.0), kS4 = (float)(-1.0 / 560.0);
+struct Fp32 {
+    using S = float;    // wavefield storage (P, Q, u, v, w; 2nd-order pressure)
+    using SP = float;   // split PML field storage
+    using M = float;    // model storage
+    using C = float;    // arithmetic
+};
+// binary16 wavefields; split PML fields stay FP32: their components with PML factor 1
+// drift secularly under rounding-excited static modes, which binary16 cannot absorb.
+struct Fp16 {
+    using S = __half;
+    using SP = float;
+    using M = float;
+    using C = float;
+};
+struct Fp64 {
+    using S = double;
+    using SP = double;
+    using M = double;
+    using C = double;
+};
+
+// Staggered 8th-order coefficients (reference iso:37, vti:40, tti:17) and the
+// 2nd-derivative coefficients (Toa_3dfd_cuda.cu:32, double literals stored to
+// float in the reference; exact doubles in the Fp64 instance).
+template <class C> struct Coef;
+template <> struct Coef<float> {
+    static constexpr float c0 = 1.196289f, c1 = -0.0797526f, c2 = 0.009570313f, c3 = -0.0006975447f;
+    static constexpr float s0 = (float)(-205.0 / 72.0), s1 = (float)(8.0 / 5.0), s2 = (float)(-1.0 / 5.0),
+                           s3 = (float)(8.0 / 315.0), s4 = (float)(-1.0 / 560.0);
+};
+template <> struct Coef<double> {
+    static constexpr double c0 = 1.196289, c1 = -0.0797526, c2 = 0.009570313, c3 = -0.0006975447;
+    static constexpr double s0 = -205.0 / 72.0, s1 = 8.0 / 5.0, s2 = -1.0 / 5.0, s3 = 8.0 / 315.0,
+                            s4 = -1.0 / 560.0;
+};
+
+__device__ __forceinline__ float csqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double csqrt(double v) { return sqrt(v); }
+
+// Wavefield load / store with the per-field scale of binary16 storage.
+__device__ __forceinline__ float ldw(const float* p, int i, float) { return p[i]; }
+__device__ __forceinline__ double ldw(const double* p, int i, float) { return p[i]; }
+__device__ __forceinline__ float ldw(const __half* p, int i, float sc) { return __half2float(p[i]) * sc; }
+__device__ __forceinline__ void stw(float* p, int i, float v, float) { p[i] = v; }
+__device__ __forceinline__ void stw(double* p, int i, double v, float) { p[i] = v; }
+__device__ __forceinline__ void stw(__half* p, int i, float v, float inv) { p[i] = __float2half_rn(v * inv); }
 
 // Staggered difference over an 8-point window w(0..7) whose centre pair is
 // (w(4), w(3)): forward windows hold F[-3..+4], backward windows F[-4..+3].
 // Accumulation order m = 0..3 as the reference loop.
-template <class F>
-__device__ __forceinline__ float dst8(F w) {
-    float r = 0.0f;
-    r += kC0 * (w(4) - w(3));
-    r += kC1 * (w(5) - w(2));
-    r += kC2 * (w(6) - w(1));
-    r += kC3 * (w(7) - w(0));
+template <class C, class F>
+__device__ __forceinline__ C dst8(F w) {
+    using K = Coef<C>;
+    C r = 0;
+    r += K::c0 * (w(4) - w(3));
+    r += K::c1 * (w(5) - w(2));
+    r += K::c2 * (w(6) - w(1));
+    r += K::c3 * (w(7) - w(0));
     return r;
 }
 
 // 2nd derivative over a 9-point window w(0..8) centred at w(4)
 // (reference Toa_3dfd_cuda.cu:282-289).
-template <class F>
-__device__ __forceinline__ float lap9(F w) {
-    float c = kS0 * w(4);
-    c += kS1 * (w(3) + w(5));
-    c += kS2 * (w(2) + w(6));
-    c += kS3 * (w(1) + w(7));
-    c += kS4 * (w(0) + w(8));
+template <class C, class F>
+__device__ __forceinline__ C lap9(F w) {
+    using K = Coef<C>;
+    C c = K::s0 * w(4);
+    c += K::s1 * (w(3) + w(5));
+    c += K::s2 * (w(2) + w(6));
+    c += K::s3 * (w(1) + w(7));
+    c += K::s4 * (w(0) + w(8));
     return c;
 }
 
@@ -75,27 +122,29 @@ __device__ __forceinline__ bool decode_tile(const Launch& l, int& tix, int& tiy,
     return true;
 }
 
-__device__ __forceinline__ float inj(float v, int i, int s0, float a0, int s1, float a1) {
-    if (i == s0) v += a0;
-    if (i == s1) v += a1;
+template <class C>
+__device__ __forceinline__ C inj(C v, int i, int s0, double a0, int s1, double a1) {
+    if (i == s0) v += (C)a0;
+    if (i == s1) v += (C)a1;
     return v;
 }
 
 // Sponge of the 2nd-order operator (reference cuda_absorb_bndr, y then x then z,
-// each product in double and stored to float). Factor 1.0 leaves v unchanged.
-__device__ __forceinline__ float damp3(float v, double fy, double fx, double fz) {
-    v = (float)(fy * (double)v);
-    v = (float)(fx * (double)v);
-    v = (float)(fz * (double)v);
+// each product in double and stored to the working type). Factor 1.0 leaves v unchanged.
+template <class C>
+__device__ __forceinline__ C damp3(C v, double fy, double fx, double fz) {
+    v = (C)(fy * (double)v);
+    v = (C)(fx * (double)v);
+    v = (C)(fz * (double)v);
     return v;
 }
 
 // Loads the halo of plane offset zo into an LDS tile s[TY+8][72] whose
 // interior rows (ty+4) were written by the owning threads. HY: 4 rows above
 // and below; HX: 4 columns left and right of the thread's own row.
-template <int TY, bool HX, bool HY, class LD>
-__device__ __forceinline__ void fill_halo(float (*s)[TX + 2 * MM], const Geo& g, int zo, int x0,
-                                          int y0, int tx, int ty, int yl, bool yin, LD ld) {
+template <int TY, bool HX, bool HY, class C, class LD>
+__device__ __forceinline__ void fill_halo(C (*s)[TX + 2 * MM], const Geo& g, int zo, int x0, int y0, int tx,
+                                          int ty, int yl, bool yin, LD ld) {
     if constexpr (HY) {
 #pragma unroll
         for (int k = ty * TX + tx; k < 2 * MM * TX; k += TX * TY) {
@@ -117,15 +166,18 @@ __device__ __forceinline__ void fill_halo(float (*s)[TX + 2 * MM], const Geo& g,
 // --------------------------------------------------------------------------
 // 1st-order velocity sweep
 // --------------------------------------------------------------------------
-template <int OP, int TY>
+template <class PR, int OP, int TY>
 __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
+    using S = typename PR::S;
+    using M = typename PR::M;
+    using C = typename PR::C;
     constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
     constexpr bool P_Z = !VTI;   // P differentiated along z (ISO: all axes; TTI: all axes)
     constexpr bool HAS_Q = !ISO;
     constexpr bool Q_XY = TTI;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
-    __shared__ float sP[H][W];
-    __shared__ float sQ[Q_XY ? H : 1][Q_XY ? W : 1];
+    __shared__ C sP[H][W];
+    __shared__ C sQ[Q_XY ? H : 1][Q_XY ? W : 1];
 
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
@@ -139,38 +191,50 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     const int col = x + g.ld * yl;
     const int zb = MM + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, g.nnz - MM);
-    const Src S = a.s;
+    const Src S0 = a.s;
+    const Scales sc = a.sc;
+    const S* P = (const S*)a.P;
+    const S* Q = (const S*)a.Q;
+    S* U = (S*)a.u;
+    S* V = (S*)a.v;
+    S* Wv = (S*)a.w;
+    const M* rho = (const M*)a.rho;
+    const M* th = (const M*)a.th;
+    const M* ph = (const M*)a.ph;
+    const C* T = (const C*)a.tab;
 
-    auto ldP = [&](int i) { return inj(a.P[i], i, S.s0, S.p0, S.s1, S.p1); };
-    auto ldQ = [&](int i) { return inj(a.Q[i], i, S.s0, S.q0, S.s1, S.q1); };
+    auto ldP = [&](int i) { return inj<C>(ldw(P, i, sc.s[F_P]), i, S0.s0, S0.p0, S0.s1, S0.p1); };
+    auto ldQ = [&](int i) { return inj<C>(ldw(Q, i, sc.s[F_Q]), i, S0.s0, S0.q0, S0.s1, S0.q1); };
 
-    float wP[8], wQ[8];   // forward windows: F(z-3+k), k = 0..7
+    C wP[8], wQ[8];   // forward windows: F(z-3+k), k = 0..7
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         const int zi = (zb - 3 + k) * g.pl + col;
         if constexpr (P_Z) wP[k] = ldP(zi);
         if constexpr (HAS_Q) wQ[k] = ldQ(zi);
     }
-    const float cx1 = a.c.cx1[x < g.nnx ? x : 0], cx2 = a.c.cx2[x < g.nnx ? x : 0];
-    const float cy1 = a.c.cy1[yl], cy2 = a.c.cy2[yl];
+    const int xs = x < g.nnx ? x : 0;
+    const C cx1 = T[a.c.cx1 + xs], cx2 = T[a.c.cx2 + xs];
+    const C cy1 = T[a.c.cy1 + yl], cy2 = T[a.c.cy2 + yl];
+    const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
         const int id = zo + col;
         if constexpr (P_Z) wP[7] = ldP(id + 4 * g.pl);
         if constexpr (HAS_Q) wQ[7] = ldQ(id + 4 * g.pl);
-        float u0 = 0.f, v0 = 0.f, w0 = 0.f, r0 = 1.f, t0 = 0.f, f0 = 0.f;
+        C u0 = 0, v0 = 0, w0 = 0, r0 = 1, t0 = 0, f0 = 0;
         if (valid) {
-            u0 = a.u[id];
-            v0 = a.v[id];
-            w0 = a.w[id];
-            if constexpr (!TTI) r0 = a.rho[id];
+            u0 = ldw(U, id, sc.s[F_U]);
+            v0 = ldw(V, id, sc.s[F_V]);
+            w0 = ldw(Wv, id, sc.s[F_W]);
+            if constexpr (!TTI) r0 = (C)rho[id];
             if constexpr (TTI) {
-                t0 = a.th[id];
-                f0 = a.ph[id];
+                t0 = (C)th[id];
+                f0 = (C)ph[id];
             }
         }
-        float pc;
+        C pc;
         if constexpr (P_Z) pc = wP[3];
         else pc = ldP(id);
         __syncthreads();
@@ -183,32 +247,32 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
         __syncthreads();
         if (valid) {
             const int r = ty + MM, c = tx + MM;
-            float xx, yy, zz;
+            C xx, yy, zz;
             if constexpr (!TTI) {
-                yy = dst8([&](int k) { return sP[r - 3 + k][c]; });
-                xx = dst8([&](int k) { return sP[r][c - 3 + k]; });
-                if constexpr (ISO) zz = dst8([&](int k) { return wP[k]; });
-                else zz = dst8([&](int k) { return wQ[k]; });
+                yy = dst8<C>([&](int k) { return sP[r - 3 + k][c]; });
+                xx = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
+                if constexpr (ISO) zz = dst8<C>([&](int k) { return wP[k]; });
+                else zz = dst8<C>([&](int k) { return wQ[k]; });
                 xx /= r0;
                 yy /= r0;
                 zz /= r0;
             } else {
-                const float Py = dst8([&](int k) { return sP[r - 3 + k][c]; });
-                const float Px = dst8([&](int k) { return sP[r][c - 3 + k]; });
-                const float Pz = dst8([&](int k) { return wP[k]; });
-                const float Qy = dst8([&](int k) { return sQ[r - 3 + k][c]; });
-                const float Qx = dst8([&](int k) { return sQ[r][c - 3 + k]; });
-                const float Qz = dst8([&](int k) { return wQ[k]; });
-                const float ct = (float)cos((double)t0), st = (float)sin((double)t0);
-                const float cp = (float)cos((double)f0), sp = (float)sin((double)f0);
+                const C Py = dst8<C>([&](int k) { return sP[r - 3 + k][c]; });
+                const C Px = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
+                const C Pz = dst8<C>([&](int k) { return wP[k]; });
+                const C Qy = dst8<C>([&](int k) { return sQ[r - 3 + k][c]; });
+                const C Qx = dst8<C>([&](int k) { return sQ[r][c - 3 + k]; });
+                const C Qz = dst8<C>([&](int k) { return wQ[k]; });
+                const C ct = (C)cos((double)t0), st = (C)sin((double)t0);
+                const C cp = (C)cos((double)f0), sp = (C)sin((double)f0);
                 xx = ct * cp * Px - ct * sp * Py - st * Pz;
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
             }
-            const float cz1 = a.c.cz1[z], cz2 = a.c.cz2[z];
-            a.u[id] = cx2 * u0 - cx1 * a.dtx * xx;
-            a.v[id] = cy2 * v0 - cy1 * a.dty * yy;
-            a.w[id] = cz2 * w0 - cz1 * a.dtz * zz;
+            const C cz1 = T[a.c.cz1 + z], cz2 = T[a.c.cz2 + z];
+            stw(U, id, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
+            stw(V, id, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
+            stw(Wv, id, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
         }
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -221,13 +285,17 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
 // --------------------------------------------------------------------------
 // 1st-order stress sweep
 // --------------------------------------------------------------------------
-template <int OP, int TY>
+template <class PR, int OP, int TY>
 __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
+    using S = typename PR::S;
+    using SP = typename PR::SP;
+    using M = typename PR::M;
+    using C = typename PR::C;
     constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
-    __shared__ float sU[H][W];
-    __shared__ float sV[H][W];
-    __shared__ float sW[TTI ? H : 1][TTI ? W : 1];
+    __shared__ C sU[H][W];
+    __shared__ C sV[H][W];
+    __shared__ C sW[TTI ? H : 1][TTI ? W : 1];
 
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
@@ -241,61 +309,74 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     const int col = x + g.ld * yl;
     const int zb = MM + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const Scales sc = a.sc;
+    const S* U = (const S*)a.u;
+    const S* V = (const S*)a.v;
+    const S* Wv = (const S*)a.w;
+    SP* px = (SP*)a.px;
+    SP* py = (SP*)a.py;
+    SP* pz = (SP*)a.pz;
+    SP* qx = (SP*)a.qx;
+    SP* qy = (SP*)a.qy;
+    SP* qz = (SP*)a.qz;
+    S* Po = (S*)a.P;
+    S* Qo = (S*)a.Q;
+    const M* vp = (const M*)a.vp;
+    const M* rho = (const M*)a.rho;
+    const M* eps = (const M*)a.eps;
+    const M* del = (const M*)a.del;
+    const M* th = (const M*)a.th;
+    const M* ph = (const M*)a.ph;
+    const C* T = (const C*)a.tab;
 
-    auto ldU = [&](int i) { return a.u[i]; };
-    auto ldV = [&](int i) { return a.v[i]; };
-    auto ldW = [&](int i) { return a.w[i]; };
+    auto ldU = [&](int i) { return ldw(U, i, sc.s[F_U]); };
+    auto ldV = [&](int i) { return ldw(V, i, sc.s[F_V]); };
+    auto ldW = [&](int i) { return ldw(Wv, i, sc.s[F_W]); };
 
-    float wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
+    C wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         const int zi = (zb - 4 + k) * g.pl + col;
-        wW[k] = a.w[zi];
+        wW[k] = ldW(zi);
         if constexpr (TTI) {
-            wU[k] = a.u[zi];
-            wV[k] = a.v[zi];
+            wU[k] = ldU(zi);
+            wV[k] = ldV(zi);
         }
     }
     const int xs = x < g.nnx ? x : 0;
-    const float ax1 = a.c.ax1[xs], ax2 = a.c.ax2[xs];
-    const float ay1 = a.c.ay1[yl], ay2 = a.c.ay2[yl];
+    const C ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
+    const C ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
+    const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
     const bool recx = a.rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
         const int id = zo + col;
-        wW[7] = a.w[id + 3 * g.pl];
+        wW[7] = ldW(id + 3 * g.pl);
         if constexpr (TTI) {
-            wU[7] = a.u[id + 3 * g.pl];
-            wV[7] = a.v[id + 3 * g.pl];
+            wU[7] = ldU(id + 3 * g.pl);
+            wV[7] = ldV(id + 3 * g.pl);
         }
-        float px0 = 0.f, py0 = 0.f, pz0 = 0.f, qx0 = 0.f, qy0 = 0.f, qz0 = 0.f;
-        float c0 = 0.f, r0 = 0.f, e0 = 0.f, d0 = 0.f, t0 = 0.f, f0 = 0.f;
+        C c0 = 0, r0 = 0, e0 = 0, d0 = 0, t0 = 0, f0 = 0;
         if (valid) {
-            px0 = a.px[id];
-            py0 = a.py[id];
-            pz0 = a.pz[id];
-            c0 = a.vp[id];
-            if constexpr (!TTI) r0 = a.rho[id];
+            c0 = (C)vp[id];
+            if constexpr (!TTI) r0 = (C)rho[id];
             if constexpr (!ISO) {
-                qx0 = a.qx[id];
-                qy0 = a.qy[id];
-                qz0 = a.qz[id];
-                e0 = a.eps[id];
-                d0 = a.del[id];
+                e0 = (C)eps[id];
+                d0 = (C)del[id];
             }
             if constexpr (TTI) {
-                t0 = a.th[id];
-                f0 = a.ph[id];
+                t0 = (C)th[id];
+                f0 = (C)ph[id];
             }
         }
-        float uc, vc;
+        C uc, vc;
         if constexpr (TTI) {
             uc = wU[4];
             vc = wV[4];
         } else {
-            uc = a.u[id];
-            vc = a.v[id];
+            uc = ldU(id);
+            vc = ldV(id);
         }
         __syncthreads();
         sU[ty + MM][tx + MM] = uc;
@@ -309,86 +390,108 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
         __syncthreads();
         if (valid) {
             const int r = ty + MM, c = tx + MM;
-            const float az1 = a.c.az1[z], az2 = a.c.az2[z];
-            float P, Q = 0.f;
+            const C az1 = T[a.c.az1 + z], az2 = T[a.c.az2 + z];
+            // Unsplit scheme: outside the PML (all stress factors 1) P and Q are updated
+            // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
+            // updates in exact arithmetic; the split fields are read and written only in
+            // PML cells.
+            const bool split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && az1 == (C)1);
+            C Pn, Qn = 0;
             if constexpr (ISO) {
-                const float yy = dst8([&](int k) { return sV[r - 4 + k][c]; });
-                const float xx = dst8([&](int k) { return sU[r][c - 4 + k]; });
-                const float zz = dst8([&](int k) { return wW[k]; });
-                const float npx = ax2 * px0 - ax1 * r0 * c0 * c0 * a.dtx * xx;
-                const float npy = ay2 * py0 - ay1 * r0 * c0 * c0 * a.dty * yy;
-                const float npz = az2 * pz0 - az1 * r0 * c0 * c0 * a.dtz * zz;
-                a.px[id] = npx;
-                a.py[id] = npy;
-                a.pz[id] = npz;
-                P = npx + npy + npz;
+                const C yy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
+                const C xx = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
+                const C zz = dst8<C>([&](int k) { return wW[k]; });
+                if (split) {
+                    const C px0 = ldw(px, id, sc.s[F_PX]), py0 = ldw(py, id, sc.s[F_PY]);
+                    const C pz0 = ldw(pz, id, sc.s[F_PZ]);
+                    const C npx = ax2 * px0 - ax1 * r0 * c0 * c0 * dtx * xx;
+                    const C npy = ay2 * py0 - ay1 * r0 * c0 * c0 * dty * yy;
+                    const C npz = az2 * pz0 - az1 * r0 * c0 * c0 * dtz * zz;
+                    stw(px, id, npx, sc.inv[F_PX]);
+                    stw(py, id, npy, sc.inv[F_PY]);
+                    stw(pz, id, npz, sc.inv[F_PZ]);
+                    Pn = npx + npy + npz;
+                } else {
+                    const C K = r0 * c0 * c0;
+                    Pn = ldw((const S*)Po, id, sc.s[F_P]) - K * (dtx * xx + dty * yy + dtz * zz);
+                }
             } else {
                 // SV taper (reference vti:154-175, tti:139-160)
-                float ee = e0, dd = d0;
+                C ee = e0, dd = d0;
                 if (a.sv) {
                     const int rx = x - a.sxp, ry = y - a.syp, rz = z - a.szp;
                     const int r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 <= 225) {
                         if (r2 <= 16) {
-                            ee = 0.0f;
-                            dd = 0.0f;
+                            ee = 0;
+                            dd = 0;
                         } else {
                             const double f = a.taper[r2];
-                            ee = (float)(f * (double)e0);
-                            dd = (float)(f * (double)d0);
+                            ee = (C)(f * (double)e0);
+                            dd = (C)(f * (double)d0);
                         }
                     }
                 }
-                float xx, yy, zz;
+                C xx, yy, zz;
                 if constexpr (VTI) {
-                    yy = dst8([&](int k) { return sV[r - 4 + k][c]; });
-                    xx = dst8([&](int k) { return sU[r][c - 4 + k]; });
-                    zz = dst8([&](int k) { return wW[k]; });
+                    yy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
+                    xx = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
+                    zz = dst8<C>([&](int k) { return wW[k]; });
                 } else {
-                    const float uy = dst8([&](int k) { return sU[r - 4 + k][c]; });
-                    const float ux = dst8([&](int k) { return sU[r][c - 4 + k]; });
-                    const float uz = dst8([&](int k) { return wU[k]; });
-                    const float vy = dst8([&](int k) { return sV[r - 4 + k][c]; });
-                    const float vx = dst8([&](int k) { return sV[r][c - 4 + k]; });
-                    const float wy = dst8([&](int k) { return sW[r - 4 + k][c]; });
-                    const float wx = dst8([&](int k) { return sW[r][c - 4 + k]; });
-                    const float wz = dst8([&](int k) { return wW[k]; });
-                    const float ct = (float)cos((double)t0), st = (float)sin((double)t0);
-                    const float cp = (float)cos((double)f0), sp = (float)sin((double)f0);
+                    const C uy = dst8<C>([&](int k) { return sU[r - 4 + k][c]; });
+                    const C ux = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
+                    const C uz = dst8<C>([&](int k) { return wU[k]; });
+                    const C vy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
+                    const C vx = dst8<C>([&](int k) { return sV[r][c - 4 + k]; });
+                    const C wy = dst8<C>([&](int k) { return sW[r - 4 + k][c]; });
+                    const C wx = dst8<C>([&](int k) { return sW[r][c - 4 + k]; });
+                    const C wz = dst8<C>([&](int k) { return wW[k]; });
+                    const C ct = (C)cos((double)t0), st = (C)sin((double)t0);
+                    const C cp = (C)cos((double)f0), sp = (C)sin((double)f0);
                     xx = ct * cp * ux - ct * sp * uy - st * uz;
                     yy = sp * vx + cp * vy;
                     zz = st * cp * wx - st * sp * wy + ct * wz;
                 }
-                const float e2 = 1 + 2 * ee;
-                const float s2 = sqrtf(1 + 2 * dd);
-                float npx, npy, npz, nqx, nqy, nqz;
-                if constexpr (VTI) {
-                    npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * a.dtx * xx;
-                    npy = ay2 * py0 - ay1 * r0 * c0 * c0 * e2 * a.dty * yy;
-                    npz = az2 * pz0 - az1 * r0 * c0 * c0 * s2 * a.dtz * zz;
-                    nqx = ax2 * qx0 - ax1 * r0 * c0 * c0 * s2 * a.dtx * xx;
-                    nqy = ay2 * qy0 - ay1 * r0 * c0 * c0 * s2 * a.dty * yy;
-                    nqz = az2 * qz0 - az1 * r0 * c0 * c0 * a.dtz * zz;
+                const C e2 = 1 + 2 * ee;
+                const C s2 = csqrt(1 + 2 * dd);
+                if (split) {
+                    const C px0 = ldw(px, id, sc.s[F_PX]), py0 = ldw(py, id, sc.s[F_PY]);
+                    const C pz0 = ldw(pz, id, sc.s[F_PZ]), qx0 = ldw(qx, id, sc.s[F_QX]);
+                    const C qy0 = ldw(qy, id, sc.s[F_QY]), qz0 = ldw(qz, id, sc.s[F_QZ]);
+                    C npx, npy, npz, nqx, nqy, nqz;
+                    if constexpr (VTI) {
+                        npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
+                        npy = ay2 * py0 - ay1 * r0 * c0 * c0 * e2 * dty * yy;
+                        npz = az2 * pz0 - az1 * r0 * c0 * c0 * s2 * dtz * zz;
+                        nqx = ax2 * qx0 - ax1 * r0 * c0 * c0 * s2 * dtx * xx;
+                        nqy = ay2 * qy0 - ay1 * r0 * c0 * c0 * s2 * dty * yy;
+                        nqz = az2 * qz0 - az1 * r0 * c0 * c0 * dtz * zz;
+                    } else {
+                        npx = ax2 * px0 - ax1 * c0 * c0 * e2 * dtx * xx;
+                        npy = ay2 * py0 - ay1 * c0 * c0 * e2 * dty * yy;
+                        npz = az2 * pz0 - az1 * c0 * c0 * s2 * dtz * zz;
+                        nqx = ax2 * qx0 - ax1 * c0 * c0 * s2 * dtx * xx;
+                        nqy = ay2 * qy0 - ay1 * c0 * c0 * s2 * dty * yy;
+                        nqz = az2 * qz0 - az1 * c0 * c0 * dtz * zz;
+                    }
+                    stw(px, id, npx, sc.inv[F_PX]);
+                    stw(py, id, npy, sc.inv[F_PY]);
+                    stw(pz, id, npz, sc.inv[F_PZ]);
+                    stw(qx, id, nqx, sc.inv[F_QX]);
+                    stw(qy, id, nqy, sc.inv[F_QY]);
+                    stw(qz, id, nqz, sc.inv[F_QZ]);
+                    Pn = npx + npy + npz;
+                    Qn = nqx + nqy + nqz;
                 } else {
-                    npx = ax2 * px0 - ax1 * c0 * c0 * e2 * a.dtx * xx;
-                    npy = ay2 * py0 - ay1 * c0 * c0 * e2 * a.dty * yy;
-                    npz = az2 * pz0 - az1 * c0 * c0 * s2 * a.dtz * zz;
-                    nqx = ax2 * qx0 - ax1 * c0 * c0 * s2 * a.dtx * xx;
-                    nqy = ay2 * qy0 - ay1 * c0 * c0 * s2 * a.dty * yy;
-                    nqz = az2 * qz0 - az1 * c0 * c0 * a.dtz * zz;
+                    const C K = VTI ? r0 * c0 * c0 : c0 * c0;
+                    const C hxy = dtx * xx + dty * yy, hz = dtz * zz;
+                    Pn = ldw((const S*)Po, id, sc.s[F_P]) - K * (e2 * hxy + s2 * hz);
+                    Qn = ldw((const S*)Qo, id, sc.s[F_Q]) - K * (s2 * hxy + hz);
                 }
-                a.px[id] = npx;
-                a.py[id] = npy;
-                a.pz[id] = npz;
-                a.qx[id] = nqx;
-                a.qy[id] = nqy;
-                a.qz[id] = nqz;
-                P = npx + npy + npz;
-                Q = nqx + nqy + nqz;
-                a.Q[id] = Q;
+                stw(Qo, id, Qn, sc.inv[F_Q]);
             }
-            a.P[id] = P;
-            if (recx && z == a.zrec) a.rec[(y - a.nb) * a.nx + (x - a.nb)] = P;
+            stw(Po, id, Pn, sc.inv[F_P]);
+            if (recx && z == a.zrec) a.rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
         }
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -404,10 +507,13 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
 // --------------------------------------------------------------------------
 // 2nd-order step with sponge, dipole and receivers folded in
 // --------------------------------------------------------------------------
-template <int TY>
+template <class PR, int TY>
 __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
+    using S = typename PR::S;
+    using M = typename PR::M;
+    using C = typename PR::C;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
-    __shared__ float sY[H][W];
+    __shared__ C sY[H][W];
 
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
@@ -421,10 +527,15 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     const int col = x + g.ld * yl;
     const int zb = MM + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const float sp = a.sc.s[F_P], ip = a.sc.inv[F_P];
+    S* X = (S*)a.X;
+    const S* Y = (const S*)a.Y;
+    const M* VV = (const M*)a.vv;
+    const C dx2 = (C)a.dx2, dy2 = (C)a.dy2, dz2 = (C)a.dz2;
 
-    auto ldY = [&](int i) { return inj(a.Y[i], i, a.s0, a.ya0, a.s1, a.ya1); };
+    auto ldY = [&](int i) { return inj<C>(ldw(Y, i, sp), i, a.s0, a.ya0, a.s1, a.ya1); };
 
-    float wY[9];   // Y(z-4+k), k = 0..8
+    C wY[9];   // Y(z-4+k), k = 0..8
 #pragma unroll
     for (int k = 0; k < 8; ++k) wY[k] = ldY((zb - 4 + k) * g.pl + col);
     const int xs = x < g.nnx ? x : 0;
@@ -435,10 +546,10 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
         const int zo = z * g.pl;
         const int id = zo + col;
         wY[8] = ldY(id + 4 * g.pl);
-        float xo = 0.f, vv = 0.f;
+        C xo = 0, vv = 0;
         if (valid) {
-            xo = inj(a.X[id], id, a.s0, a.xa0, a.s1, a.xa1);
-            vv = a.vv[id];
+            xo = inj<C>(ldw((const S*)X, id, sp), id, a.s0, a.xa0, a.s1, a.xa1);
+            vv = (C)VV[id];
         }
         __syncthreads();
         sY[ty + MM][tx + MM] = wY[4];
@@ -447,18 +558,18 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
         if (valid) {
             const int r = ty + MM, c = tx + MM;
             const double fz = a.fz[z];
-            float c1 = lap9([&](int k) { return wY[k]; });              // z
-            float c2 = lap9([&](int k) { return sY[r][c - 4 + k]; });   // x
-            float c3 = lap9([&](int k) { return sY[r - 4 + k][c]; });   // y
-            c1 *= a.dz2;
-            c2 *= a.dx2;
-            c3 *= a.dy2;
-            const float cur = wY[4];
-            const float prev = damp3(xo, fy, fx, fz);
-            const float lap = vv * (c1 + c2 + c3);
-            const float nv = (float)(2.0 * (double)cur - (double)prev + (double)lap);
-            a.X[id] = damp3(nv, fy, fx, fz);
-            if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = damp3(cur, fy, fx, fz);
+            C c1 = lap9<C>([&](int k) { return wY[k]; });              // z
+            C c2 = lap9<C>([&](int k) { return sY[r][c - 4 + k]; });   // x
+            C c3 = lap9<C>([&](int k) { return sY[r - 4 + k][c]; });   // y
+            c1 *= dz2;
+            c2 *= dx2;
+            c3 *= dy2;
+            const C cur = wY[4];
+            const C prev = damp3<C>(xo, fy, fx, fz);
+            const C lap = vv * (c1 + c2 + c3);
+            const C nv = (C)(2.0 * (double)cur - (double)prev + (double)lap);
+            stw(X, id, damp3<C>(nv, fy, fx, fz), ip);
+            if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) wY[k] = wY[k + 1];
@@ -468,20 +579,22 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
 // --------------------------------------------------------------------------
 // Setup and output kernels
 // --------------------------------------------------------------------------
-__global__ void k_pad(const float* __restrict__ src, int nx, int ny, int nz, int off, Geo g,
-                      float* __restrict__ dst, int mode, float dt) {
+// Pads an interior float model into the device layout (float or double), edge replication.
+template <class M>
+__global__ void k_pad(const float* __restrict__ src, int nx, int ny, int nz, int off, Geo g, M* __restrict__ dst,
+                      int mode, float dt) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     const int z = blockIdx.z;
     if (x >= g.ld) return;
-    float v = 0.0f;
+    M v = 0;
     if (x < g.nnx) {
         const int cx = min(max(x - off, 0), nx - 1);
         const int cy = min(max(y - off, 0), ny - 1);
         const int cz = min(max(z - off, 0), nz - 1);
-        v = src[cz + (size_t)nz * cx + (size_t)nz * nx * cy];
+        v = (M)src[cz + (size_t)nz * cx + (size_t)nz * nx * cy];
         if (mode == 1) {
-            const float t = v * dt;
+            const M t = v * (M)dt;
             v = t * t;
         }
     }
@@ -489,9 +602,11 @@ __global__ void k_pad(const float* __restrict__ src, int nx, int ny, int nz, int
 }
 
 // One thread per (ix, iy) column of the interior; loops z.
-__global__ void k_snap(const float* __restrict__ src, Geo g, int off, int nx, int ny, int nz, int s0,
-                       float a0, int s1, float a1, const double* fx, const double* fy,
-                       const double* fz, float* __restrict__ dst) {
+template <class S>
+__global__ void k_snap(const S* __restrict__ src, Geo g, int off, int nx, int ny, int nz, int s0, double a0,
+                       int s1, double a1, const double* fx, const double* fy, const double* fz, float scale,
+                       float* __restrict__ dst) {
+    using C = decltype(ldw(src, 0, 1.0f));
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y;
     if (ix >= nx) return;
@@ -499,9 +614,9 @@ __global__ void k_snap(const float* __restrict__ src, Geo g, int off, int nx, in
     for (int iz = 0; iz < nz; ++iz) {
         const int z = iz + off;
         const int i = x + g.ld * y + g.pl * z;
-        float v = inj(src[i], i, s0, a0, s1, a1);
-        if (fx) v = damp3(v, fy[y], fx[x], fz[z]);
-        dst[iz + (size_t)nz * ix + (size_t)nz * nx * iy] = v;
+        C v = inj<C>(ldw(src, i, scale), i, s0, a0, s1, a1);
+        if (fx) v = damp3<C>(v, fy[y], fx[x], fz[z]);
+        dst[iz + (size_t)nz * ix + (size_t)nz * nx * iy] = (float)v;
     }
 }
 
@@ -562,117 +677,170 @@ __global__ void k_probe_read(const float4* __restrict__ a, size_t n, float* out)
     if (acc == -1.0f) out[0] = acc;   // keeps the loads live; never true for the zeroed input
 }
 
-template <int OP>
+// ---- launch dispatch -------------------------------------------------------
+template <class PR, int OP>
 hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_vel<OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_vel<OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_vel<OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_vel<PR, OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_vel<PR, OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_vel<PR, OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int OP>
+template <class PR, int OP>
 hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress<OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_stress<OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_stress<OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_vel(int op, int ty, const VelArgs& a, hipStream_t st) {
-    switch (op) {
-        case 1: return vel_ty<1>(ty, a, st);
-        case 2: return vel_ty<2>(ty, a, st);
-        case 3: return vel_ty<3>(ty, a, st);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_stress(int op, int ty, const StressArgs& a, hipStream_t st) {
-    switch (op) {
-        case 1: return stress_ty<1>(ty, a, st);
-        case 2: return stress_ty<2>(ty, a, st);
-        case 3: return stress_ty<3>(ty, a, st);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_step2(int ty, const Step2Args& a, hipStream_t st) {
+template <class PR>
+hipError_t step2_ty(int ty, const Step2Args& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_step2<2>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_step2<4>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_step2<8>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_step2<PR, 2>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_step2<PR, 4>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_step2<PR, 8>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-namespace {
+template <class PR>
+hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
+    switch (op) {
+        case 1: return vel_ty<PR, 1>(ty, a, st);
+        case 2: return vel_ty<PR, 2>(ty, a, st);
+        case 3: return vel_ty<PR, 3>(ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class PR>
+hipError_t stress_op(int op, int ty, const StressArgs& a, hipStream_t st) {
+    switch (op) {
+        case 1: return stress_ty<PR, 1>(ty, a, st);
+        case 2: return stress_ty<PR, 2>(ty, a, st);
+        case 3: return stress_ty<PR, 3>(ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <class K>
 int occ(K kern, int ty) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, TX * ty, 0) != hipSuccess) return 1;
     return nb;
 }
-template <int OP>
+
+template <class PR, int OP>
 int occ_op(int kind, int ty) {
     if (kind == 0) {
         switch (ty) {
-            case 2: return occ(k_vel<OP, 2>, 2);
-            case 4: return occ(k_vel<OP, 4>, 4);
-            default: return occ(k_vel<OP, 8>, 8);
+            case 2: return occ(k_vel<PR, OP, 2>, 2);
+            case 4: return occ(k_vel<PR, OP, 4>, 4);
+            default: return occ(k_vel<PR, OP, 8>, 8);
         }
     }
     switch (ty) {
-        case 2: return occ(k_stress<OP, 2>, 2);
-        case 4: return occ(k_stress<OP, 4>, 4);
-        default: return occ(k_stress<OP, 8>, 8);
+        case 2: return occ(k_stress<PR, OP, 2>, 2);
+        case 4: return occ(k_stress<PR, OP, 4>, 4);
+        default: return occ(k_stress<PR, OP, 8>, 8);
     }
 }
-}  // namespace
 
-int max_blocks_per_cu(int kind, int op, int ty) {
+template <class PR>
+int occ_prec(int kind, int op, int ty) {
     switch (op) {
         case 0:
             switch (ty) {
-                case 2: return occ(k_step2<2>, 2);
-                case 4: return occ(k_step2<4>, 4);
-                default: return occ(k_step2<8>, 8);
+                case 2: return occ(k_step2<PR, 2>, 2);
+                case 4: return occ(k_step2<PR, 4>, 4);
+                default: return occ(k_step2<PR, 8>, 8);
             }
-        case 1: return occ_op<1>(kind, ty);
-        case 2: return occ_op<2>(kind, ty);
-        default: return occ_op<3>(kind, ty);
+        case 1: return occ_op<PR, 1>(kind, ty);
+        case 2: return occ_op<PR, 2>(kind, ty);
+        default: return occ_op<PR, 3>(kind, ty);
     }
 }
 
-hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, float* dst,
+}  // namespace
+
+hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP32: return vel_op<Fp32>(op, ty, a, st);
+        case PREC_FP16: return vel_op<Fp16>(op, ty, a, st);
+        case PREC_FP64: return vel_op<Fp64>(op, ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP32: return stress_op<Fp32>(op, ty, a, st);
+        case PREC_FP16: return stress_op<Fp16>(op, ty, a, st);
+        case PREC_FP64: return stress_op<Fp64>(op, ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP32: return step2_ty<Fp32>(ty, a, st);
+        case PREC_FP16: return step2_ty<Fp16>(ty, a, st);
+        case PREC_FP64: return step2_ty<Fp64>(ty, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int max_blocks_per_cu(int prec, int kind, int op, int ty) {
+    switch (prec) {
+        case PREC_FP16: return occ_prec<Fp16>(kind, op, ty);
+        case PREC_FP64: return occ_prec<Fp64>(kind, op, ty);
+        default: return occ_prec<Fp32>(kind, op, ty);
+    }
+}
+
+hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, void* dst, int model_double,
                       int mode, float dt, hipStream_t st) {
     const dim3 grid((g.ld + 255) / 256, g.nny, g.nnz);
-    hipLaunchKernelGGL(k_pad, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, dst, mode, dt);
+    if (model_double)
+        hipLaunchKernelGGL(k_pad<double>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (double*)dst, mode, dt);
+    else
+        hipLaunchKernelGGL(k_pad<float>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (float*)dst, mode, dt);
     return hipGetLastError();
 }
 
-hipError_t launch_snap(const float* src, const Geo& g, int off, int nx, int ny, int nz, int s0,
-                       float a0, int s1, float a1, const double* fx, const double* fy,
-                       const double* fz, float* dst, hipStream_t st) {
+hipError_t launch_snap(int prec, const void* src, const Geo& g, int off, int nx, int ny, int nz, int s0, double a0,
+                       int s1, double a1, const double* fx, const double* fy, const double* fz, float scale,
+                       float* dst, hipStream_t st) {
     const dim3 grid((nx + 63) / 64, ny);
-    hipLaunchKernelGGL(k_snap, grid, dim3(64), 0, st, src, g, off, nx, ny, nz, s0, a0, s1, a1, fx, fy,
-                       fz, dst);
+    switch (prec) {
+        case PREC_FP16:
+            hipLaunchKernelGGL(k_snap<__half>, grid, dim3(64), 0, st, (const __half*)src, g, off, nx, ny, nz, s0, a0,
+                               s1, a1, fx, fy, fz, scale, dst);
+            break;
+        case PREC_FP64:
+            hipLaunchKernelGGL(k_snap<double>, grid, dim3(64), 0, st, (const double*)src, g, off, nx, ny, nz, s0, a0,
+                               s1, a1, fx, fy, fz, scale, dst);
+            break;
+        default:
+            hipLaunchKernelGGL(k_snap<float>, grid, dim3(64), 0, st, (const float*)src, g, off, nx, ny, nz, s0, a0,
+                               s1, a1, fx, fy, fz, scale, dst);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const MuteArgs& m,
-                                 float* shot, hipStream_t st) {
+hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const MuteArgs& m, float* shot,
+                                 hipStream_t st) {
     const dim3 grid((nt + 63) / 64, (nx + 63) / 64, ny);
     hipLaunchKernelGGL(k_shot_transpose, grid, dim3(64, 4), 0, st, rec, nt, nx, ny, m, shot);
     return hipGetLastError();

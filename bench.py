@@ -34,6 +34,25 @@ HBM_PEAK_GBS = 8000.0
 A_VEL, A_STRESS = 36, 84
 
 
+def algorithmic_bytes(cfg, storage, scheme, npts, npml=20):
+    """Algorithmic bytes per updated point of the VTI velocity and stress sweeps for the
+    formulation run: wavefields of ws bytes (4 FP32, 2 binary16, 8 FP64), split PML fields
+    of ss bytes (FP32 under binary16), model of ms bytes.
+    velocity: R P,Q,rho,u,v,w; W u,v,w. stress, split cell: R u,v,w, 6 split, vp,rho,eps,delta;
+    W 6 split, P, Q. stress, unsplit cell (scheme 1 outside the PML): R u,v,w, P,Q, 4 model; W P,Q."""
+    ws = {0: 4, 1: 2, 2: 8}[storage]
+    ms = 8 if storage == 2 else 4
+    ss = max(ws, 4)                     # split PML fields are FP32 under binary16 storage
+    a_vel = 8 * ws + ms
+    split = 5 * ws + 12 * ss + 4 * ms
+    unsplit = 7 * ws + 4 * ms
+    if scheme == 0 and storage != 1:
+        return a_vel, float(split)
+    inner = (cfg["nx"] + 1) * (cfg["ny"] + 1) * (cfg["nz"] + 1)   # cells whose stress PML factors are all 1
+    f = inner / npts
+    return a_vel, f * unsplit + (1 - f) * split
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -62,6 +81,38 @@ def cpu_baseline(cfg, target_s=15.0):
                 sample=f"{nsteps} time steps of the same VTI shot (FP32 oracle, OpenMP), {secs:.1f} s")
 
 
+def precision_study(cfg):
+    """The 400x400x300 VTI shot in FP32, FP16 wavefield storage and FP64; errors vs FP64."""
+    from a3dfd import Modeller, make_desc
+    nz = cfg["nz"]
+    out = {}
+    for name, storage, scheme in (("fp64", 2, 0), ("fp32", 0, 0), ("fp32_unsplit", 0, 1), ("fp16", 1, 1)):
+        d = make_desc("vti", cfg["nx"], cfg["ny"], nz, cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"], cfg["sy"],
+                      cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], storage=storage, scheme=scheme,
+                      snap_interval=1000, snap_skip0=1)
+        with Modeller(d) as mdl:
+            mdl.set_model(vel=cfg["vel"], rho=cfg["rho"], eps=cfg["eps"], delta=cfg["delta"])
+            mdl.run(shot=False)                                  # warm-up (and autotune)
+            shot, snaps, ms = mdl.run(nsnap_max=3)               # it = 1000, 2000 (and timing)
+        npts = (cfg["nx"] + 32) * (cfg["ny"] + 32) * (nz + 32)
+        out[name] = dict(shot=shot, snap=snaps[0], snap2=snaps[1], ms=ms, gpts=npts * cfg["nt"] / ms / 1e6)
+        log(f"bench: {name}: {ms:.1f} ms, {out[name]['gpts']:.2f} Gpt/s")
+    ref = out["fp64"]
+    res = {"workload": "vti_400x400x300_nt%d" % cfg["nt"], "reference": "GPU FP64 instance"}
+    rl = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))  # noqa
+    for name in ("fp32", "fp32_unsplit", "fp16"):
+        o = out[name]
+        res[name] = {"shot_rel_l2_vs_fp64": rl(o["shot"], ref["shot"]),
+                     "snapshot_it1000_rel_l2_vs_fp64": rl(o["snap"], ref["snap"]),
+                     "snapshot_it2000_rel_l2_vs_fp64": rl(o["snap2"], ref["snap2"]),
+                     "loop_ms": round(o["ms"], 1), "Gpt_s": round(o["gpts"], 2)}
+    a, b = out["fp32_unsplit"], out["fp32"]
+    res["fp32_unsplit_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
+                                   "snapshot_it2000": rl(a["snap2"], b["snap2"])}
+    res["fp64"] = {"loop_ms": round(ref["ms"], 1), "Gpt_s": round(ref["gpts"], 2)}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +121,11 @@ def main():
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--launch", type=str, default="", help="ty_vel,zseg_vel,ty_stress,zseg_stress (default: autotuned)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="run the shot in FP32, FP16-storage and FP64 and print the errors vs FP64")
+    ap.add_argument("--storage", type=int, default=0, help="0 FP32 (default), 1 FP16 wavefield storage, 2 FP64")
+    ap.add_argument("--scheme", type=int, default=0,
+                    help="0: split PML fields everywhere (bitwise reference arithmetic); 1: split only in PML cells")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run the calibration streams (2 GiB float4 copy and read) and print GB/s")
     args = ap.parse_args()
@@ -91,8 +147,11 @@ def main():
     cfg = models.cfg3()
     cfg["nt"] = args.nt
     log(f"bench: model generated in {time.time() - t0:.1f} s")
+    if args.precision_study:
+        return precision_study(cfg)
     desc = make_desc("vti", cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"],
-                     cfg["sy"], cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"])
+                     cfg["sy"], cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], storage=args.storage,
+                     scheme=args.scheme)
     mdl = Modeller(desc, device=0)
     mdl.set_model(vel=cfg["vel"], rho=cfg["rho"], eps=cfg["eps"], delta=cfg["delta"])
     if args.launch:
@@ -122,12 +181,13 @@ def main():
     gpts = npts * args.nt * args.steps / total_loop_s / 1e9
     avg_vel = kms[0] / max(kn[0], 1)
     avg_str = kms[1] / max(kn[1], 1)
-    ach_str = A_STRESS * npts / (avg_str * 1e-3) / 1e9
-    ach_vel = A_VEL * npts / (avg_vel * 1e-3) / 1e9
-    step_gbs = (A_VEL + A_STRESS) * gpts
+    a_vel, a_str = algorithmic_bytes(cfg, args.storage, args.scheme, npts)
+    ach_str = a_str * npts / (avg_str * 1e-3) / 1e9
+    ach_vel = a_vel * npts / (avg_vel * 1e-3) / 1e9
+    step_gbs = (a_vel + a_str) * gpts
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_vti_400x400x300.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.storage == 0 and args.scheme == 0:
         with open(tpath) as f:
             traffic = json.load(f).get("stress_bytes_per_launch")
 
@@ -150,7 +210,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {0: "f32", 1: "f32 (f16 wavefield storage)", 2: "f64"}[args.storage],
         "data": "synthetic seeded VTI model (layered 1500-5000 m/s + salt body, Gardner density, "
                 "smooth epsilon/delta); inputs resident in HBM",
         "config": {"workload": "vti_400x400x300_nt%d" % args.nt, "nx": cfg["nx"], "ny": cfg["ny"],
@@ -159,12 +219,12 @@ def main():
                    "launch": launch},
         "roofline": {"bound": "hbm", "kernel": "k_stress<VTI>", "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "bytes_per_point": A_STRESS,
+                     "traffic": traffic, "bytes_per_point": round(a_str, 2),
                      "avg_launch_ms": round(avg_str, 4), "launches": kn[1]},
         "kernels": {"k_vel<VTI>": {"avg_launch_ms": round(avg_vel, 4), "achieved_GBs": round(ach_vel, 1),
-                                   "bytes_per_point": A_VEL},
+                                   "bytes_per_point": a_vel},
                     "whole_step": {"achieved_GBs": round(step_gbs, 1),
-                                   "frac": round(step_gbs / HBM_PEAK_GBS, 4), "bytes_per_point": A_VEL + A_STRESS}},
+                                   "frac": round(step_gbs / HBM_PEAK_GBS, 4), "bytes_per_point": round(a_vel + a_str, 2)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

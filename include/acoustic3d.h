@@ -116,7 +116,17 @@ typedef struct a3d_shot_desc {
     int mute;              /* 1: direct-wave mute of the shot record */
     float v_mute;          /* <= 0: reference velocity taken from the model */
     int mute_tt;           /* mute pad in samples (reference: 60) */
-    int storage;           /* 0: FP32 wavefields */
+    int storage;           /* 0: FP32 (bitwise equal to the FP32 oracle); 1: wavefields stored as
+                              binary16 times a power-of-two scale per field (split PML fields,
+                              model and arithmetic FP32); 2: FP64 storage and arithmetic (error
+                              reference). Outputs
+                              (shot records, snapshots) are FP32 in every mode. Fixed per context. */
+    int scheme;            /* 1st-order stress update. 0: split PML fields stored and updated
+                              everywhere, the reference's arithmetic (FP32 bitwise equal to it);
+                              1: split fields only in PML cells, P and Q updated directly where all
+                              stress PML factors are 1 (same equations, different rounding: within
+                              1e-5 relative L2 of scheme 0, 80 instead of 120 B/point for VTI).
+                              Storage 1 (binary16) always uses scheme 1. */
 } a3d_shot_desc;
 
 typedef struct a3d_ctx a3d_ctx;
