@@ -5,10 +5,10 @@
   stability (CFL) check.
 - `formats`: reference model / shot / snapshot file formats.
 """
-from ._lib import (A3DError, LEGACY_NAMES, LIB_DIR, LIB_PATH, Modeller, OP_2ND, OP_ISO, OP_NAMES, OP_TTI,
-                   OP_VTI, REFERENCE_DEFAULTS, ShotDesc, desc_dict, load, make_desc)
+from ._lib import (A3DError, Batch, LEGACY_NAMES, LIB_DIR, LIB_PATH, Modeller, OP_2ND, OP_ISO, OP_NAMES, OP_TTI,
+                   OP_VTI, REFERENCE_DEFAULTS, ShotDesc, Window, desc_dict, load, make_desc)
 
 __all__ = [
-    "A3DError", "LEGACY_NAMES", "LIB_DIR", "LIB_PATH", "Modeller", "OP_2ND", "OP_ISO", "OP_NAMES", "OP_TTI",
-    "OP_VTI", "REFERENCE_DEFAULTS", "ShotDesc", "desc_dict", "load", "make_desc",
+    "A3DError", "Batch", "LEGACY_NAMES", "LIB_DIR", "LIB_PATH", "Modeller", "OP_2ND", "OP_ISO", "OP_NAMES", "OP_TTI",
+    "OP_VTI", "REFERENCE_DEFAULTS", "ShotDesc", "Window", "desc_dict", "load", "make_desc",
 ]

@@ -54,6 +54,14 @@ class ShotDesc(ctypes.Structure):
     ]
 
 
+class Window(ctypes.Structure):
+    """Mirror of a3d_window: origin and size of an aperture window in the full model,
+    and the source inside it (1-based, operator convention)."""
+
+    _fields_ = [("ox", c_int), ("oy", c_int), ("oz", c_int), ("nx", c_int), ("ny", c_int), ("nz", c_int),
+                ("sx", c_int), ("sy", c_int), ("sz", c_int)]
+
+
 # Reference per-operator constants: boundary width (Toa_3dfd_cuda.cu:19; iso:395,
 # vti:414, tti:418), source factor (iso:407, vti:431, tti:437), receiver depth
 # (Toa_3dfd_cuda.cu:159 interior z=1; iso:302 interior z=0), SV taper (vti:567).
@@ -106,6 +114,15 @@ def load(path=None):
     lib.a3d_get_launch.argtypes = [vp, POINTER(c_int)]
     lib.a3d_autotune.argtypes = [vp, POINTER(ShotDesc), c_int, POINTER(c_double)]
     lib.a3d_grid_info.argtypes = [vp, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_longlong)]
+    lib.a3d_batch_create.argtypes = [POINTER(ShotDesc), c_int, POINTER(Window), c_int, POINTER(vp)]
+    lib.a3d_batch_destroy.argtypes = [vp]
+    lib.a3d_batch_set_model.argtypes = [vp, fp, fp, fp, fp, fp, fp]
+    lib.a3d_batch_tune.argtypes = [vp, c_int, POINTER(c_int)]
+    lib.a3d_batch_run.argtypes = [vp, c_int, POINTER(fp), POINTER(c_double)]
+    lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
+    for name in ("a3d_batch_create", "a3d_batch_destroy", "a3d_batch_set_model", "a3d_batch_tune", "a3d_batch_run",
+                 "a3d_batch_info"):
+        getattr(lib, name).restype = c_int
     lib.a3d_probe_bandwidth.argtypes = [c_int, ctypes.c_size_t, c_int, POINTER(c_double), POINTER(c_double)]
     lib.a3d_probe_bandwidth.restype = c_int
     lib.a3d_last_error.restype = c_char_p
@@ -222,6 +239,65 @@ class Modeller:
     def close(self):
         if self._h:
             self.lib.a3d_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many shots, each an aperture window of one device-resident model (a3d_batch)."""
+
+    def __init__(self, desc, windows, device=0):
+        self.lib = load()
+        self.desc = desc
+        self.windows = (Window * len(windows))(*windows)
+        self._h = c_void_p()
+        _check(self.lib, self.lib.a3d_batch_create(ctypes.byref(desc), len(windows), self.windows, device,
+                                                   ctypes.byref(self._h)))
+
+    def set_model(self, vel, rho=None, eps=None, delta=None, theta=None, phi=None):
+        shape = (self.desc.ny, self.desc.nx, self.desc.nz)
+        arrs = [_model(a, shape) for a in (vel, rho, eps, delta, theta, phi)]
+        _check(self.lib, self.lib.a3d_batch_set_model(self._h, *[_fptr(a) for a in arrs]))
+
+    def tune(self, steps=3):
+        n = c_int()
+        _check(self.lib, self.lib.a3d_batch_tune(self._h, steps, ctypes.byref(n)))
+        return n.value
+
+    def run(self, mode=2, shots=True, which=None):
+        """mode 0: sequential plain runs; 1: batched; 2: batched, one hipGraph. Returns (shots, loop_ms)."""
+        nw = len(self.windows)
+        outs = [None] * nw
+        ptrs = (POINTER(c_float) * nw)()
+        for w, win in enumerate(self.windows):
+            if shots and (which is None or w in which):
+                outs[w] = np.empty((win.ny, win.nx, self.desc.nt), np.float32)
+                ptrs[w] = outs[w].ctypes.data_as(POINTER(c_float))
+        ms = c_double()
+        _check(self.lib, self.lib.a3d_batch_run(self._h, mode, ptrs, ctypes.byref(ms)))
+        return outs, ms.value
+
+    def info(self):
+        nw = len(self.windows)
+        pts, n = c_longlong(), c_int()
+        tr = (c_int * (2 * nw))()
+        _check(self.lib, self.lib.a3d_batch_info(self._h, ctypes.byref(pts), ctypes.byref(n), tr))
+        return dict(updated_points=pts.value, nwin=n.value, tile_rows=[(tr[2 * w], tr[2 * w + 1]) for w in range(nw)])
+
+    def close(self):
+        if self._h:
+            self.lib.a3d_batch_destroy(self._h)
             self._h = c_void_p()
 
     def __enter__(self):

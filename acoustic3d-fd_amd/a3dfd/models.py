@@ -148,3 +148,22 @@ def small_model(op, nx=24, ny=20, nz=22, seed=0, d=10.0):
         m["theta"] = (0.6 * gaussian_filter(rng.random(shape), 2.0, mode="reflect")).astype(np.float32)
         m["phi"] = (2 * np.pi * gaussian_filter(rng.random(shape), 2.0, mode="reflect")).astype(np.float32)
     return m
+
+
+def uneven_windows(nshot, nx, ny, nz, seed=4, lo=(160, 160, 200), hi=(320, 320, 300)):
+    """Seeded shots on the surface, each with an aperture window of uneven size drawn
+    uniformly in [lo, hi] cells (x, y, z), centred on its shot and clipped to the model.
+    Returns a3d_window tuples (ox, oy, oz, wx, wy, wz, sx, sy, sz), sources 1-based
+    inside the window at depth 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nshot):
+        wx = int(rng.integers(lo[0], hi[0] + 1))
+        wy = int(rng.integers(lo[1], hi[1] + 1))
+        wz = int(min(rng.integers(lo[2], hi[2] + 1), nz))
+        gx = int(rng.integers(0, nx))
+        gy = int(rng.integers(0, ny))
+        ox = int(np.clip(gx - wx // 2, 0, nx - wx))
+        oy = int(np.clip(gy - wy // 2, 0, ny - wy))
+        out.append((ox, oy, 0, wx, wy, wz, gx - ox + 1, gy - oy + 1, 1))
+    return out

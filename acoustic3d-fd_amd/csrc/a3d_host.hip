@@ -355,12 +355,131 @@ Scales field_scales(const a3d_ctx* c, const a3d_shot_desc* d) {
     return sc;
 }
 
+// Tables, zeroed wavefields, kernel arguments and the per-step sources of one
+// 1st-order shot (reference iso:372-671, vti:389-688, tti:390-676).
+template <class T>
+int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs& va, StressArgs& sa,
+                  std::vector<Src>& srcs) {
+    const Geo g = c->g;
+    const int prec = c->prec;
+    const int nx = d->nx, ny = d->ny, nz = d->nz, nb = d->nb, nt = d->nt;
+    const size_t wbytes = c->nvox * c->wsize;
+    const int op = d->op;
+    const T dt = (T)d->dt;
+    // get_d0 (reference iso:193-195) from the padded velocity at z-fastest index N/2
+    const T dsum = (T)d->dx + (T)d->dy + (T)d->dz;
+    const T d0 = (T)(10.0 * (T)c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
+    std::vector<T> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
+    Pml pm{};
+    pm.cx1 = 0;
+    pm.cx2 = g.nnx;
+    pm.ax1 = 2 * g.nnx;
+    pm.ax2 = 3 * g.nnx;
+    pm.cy1 = 4 * g.nnx;
+    pm.cy2 = pm.cy1 + g.nny;
+    pm.ay1 = pm.cy1 + 2 * g.nny;
+    pm.ay2 = pm.cy1 + 3 * g.nny;
+    pm.cz1 = 4 * g.nnx + 4 * g.nny;
+    pm.cz2 = pm.cz1 + g.nnz;
+    pm.az1 = pm.cz1 + 2 * g.nnz;
+    pm.az2 = pm.cz1 + 3 * g.nnz;
+    T* t = tabh.data();
+    pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2);
+    pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2);
+    pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2);
+    HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(T), hipMemcpyHostToDevice));
+    for (int v : volumes_of(op))
+        if (is_wavefield(v))
+            HIP_TRY(hipMemsetAsync(c->vol[v], 0, is_split(v) ? c->nvox * std::max(c->wsize, (size_t)4) : wbytes,
+                                   c->st));
+
+    const T dtx = dt / (T)d->dx, dty = dt / (T)d->dy, dtz = dt / (T)d->dz;
+    const int sxp = d->sx + nb - 1, syp = d->sy + nb - 1, szp = d->sz + nb - 1;
+    const int s0 = sxp + g.ld * syp + g.pl * szp;
+    va = VelArgs{};
+    va.g = g;
+    va.l = make_launch(g, c->ty[0], c->zseg[0]);
+    va.c = pm;
+    va.tab = c->tab;
+    va.P = c->vol[V_P];
+    va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
+    va.rho = c->vol[V_RHO];
+    va.th = c->vol[V_TH];
+    va.ph = c->vol[V_PH];
+    va.u = c->vol[V_U];
+    va.v = c->vol[V_V];
+    va.w = c->vol[V_W];
+    va.dtx = dtx;
+    va.dty = dty;
+    va.dtz = dtz;
+    va.sc = sc;
+    sa = StressArgs{};
+    sa.g = g;
+    sa.l = make_launch(g, c->ty[1], c->zseg[1]);
+    sa.c = pm;
+    sa.tab = c->tab;
+    sa.u = c->vol[V_U];
+    sa.v = c->vol[V_V];
+    sa.w = c->vol[V_W];
+    sa.px = c->vol[V_PX];
+    sa.py = c->vol[V_PY];
+    sa.pz = c->vol[V_PZ];
+    sa.qx = c->vol[V_QX];
+    sa.qy = c->vol[V_QY];
+    sa.qz = c->vol[V_QZ];
+    sa.vp = c->vol[V_VP];
+    sa.rho = c->vol[V_RHO];
+    sa.eps = c->vol[V_EPS];
+    sa.del = c->vol[V_DEL];
+    sa.th = c->vol[V_TH];
+    sa.ph = c->vol[V_PH];
+    sa.P = c->vol[V_P];
+    sa.Q = c->vol[V_Q];
+    sa.dtx = dtx;
+    sa.dty = dty;
+    sa.dtz = dtz;
+    sa.sv = d->sv && op != A3D_OP_ISO;
+    sa.sxp = sxp;
+    sa.syp = syp;
+    sa.szp = szp;
+    sa.taper = c->dtab + g.nnx + g.nny + g.nnz;
+    sa.zrec = nb + d->rec_z;
+    sa.nb = nb;
+    sa.nx = nx;
+    sa.ny = ny;
+    sa.sc = sc;
+    sa.unsplit = d->scheme != 0 || prec == PREC_FP16;
+    va.s.s0 = s0;
+    // The time grid is the reference's float accumulation t = dt, t += dt and the gate
+    // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
+    const float tdelay = (float)(1.0 / d->fm);
+    const T pfac = (T)d->pfac;
+    srcs.assign(nt, Src{-1, -1, 0.0, 0.0, 0.0, 0.0});
+    float tcur = d->dt;
+    for (int it = 0; it < nt; it++, tcur += d->dt) {
+        // add_source (reference iso:523-530; vti:557; tti:571), applied on load
+        if (tcur <= 2 * tdelay) {
+            const T src = wavelet_value<T>(d->wtype, tcur, d->fm);
+            if (d->op == A3D_OP_ISO) {
+                const T mp = (T)(-1.0 * pfac);
+                const T a0 = mp * src, a1 = pfac * src;
+                srcs[it] = {s0, s0 + 2 * g.pl, (double)a0, (double)a1, (double)a0, (double)a1};
+            } else {
+                const T a0 = pfac * src;
+                srcs[it] = {s0, -1, (double)a0, 0.0, (double)a0, 0.0};
+            }
+        }
+    }
+    va.s = Src{-1, -1, 0.0, 0.0, 0.0, 0.0};
+    return A3D_OK;
+}
+
 template <class T>
 int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
                double* loop_ms) {
     const Geo g = c->g;
     const int prec = c->prec;
-    const int nx = d->nx, ny = d->ny, nz = d->nz, nt = d->nt, nb = d->nb;
+    const int nx = d->nx, ny = d->ny, nz = d->nz, nt = d->nt;
     const size_t plane_rec = (size_t)nx * ny;
     int rc;
     if ((rc = ensure(&c->rec, &c->rec_cap, plane_rec * nt))) return rc;
@@ -443,114 +562,15 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         }
         HIP_TRY(hipEventRecord(c->ev1, c->st));
     } else {
-        // -------- 1st-order family (reference iso:372-671, vti:389-688, tti:390-676)
+        // -------- 1st-order family
+        VelArgs va;
+        StressArgs sa;
+        std::vector<Src> srcs;
+        if ((rc = prepare_first<T>(c, d, sc, va, sa, srcs))) return rc;
         const int op = d->op;
-        const T dt = (T)d->dt;
-        // get_d0 (reference iso:193-195) from the padded velocity at z-fastest index N/2
-        const T dsum = (T)d->dx + (T)d->dy + (T)d->dz;
-        const T d0 = (T)(10.0 * (T)c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
-        std::vector<T> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
-        Pml pm{};
-        pm.cx1 = 0;
-        pm.cx2 = g.nnx;
-        pm.ax1 = 2 * g.nnx;
-        pm.ax2 = 3 * g.nnx;
-        pm.cy1 = 4 * g.nnx;
-        pm.cy2 = pm.cy1 + g.nny;
-        pm.ay1 = pm.cy1 + 2 * g.nny;
-        pm.ay2 = pm.cy1 + 3 * g.nny;
-        pm.cz1 = 4 * g.nnx + 4 * g.nny;
-        pm.cz2 = pm.cz1 + g.nnz;
-        pm.az1 = pm.cz1 + 2 * g.nnz;
-        pm.az2 = pm.cz1 + 3 * g.nnz;
-        T* t = tabh.data();
-        pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2);
-        pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2);
-        pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2);
-        HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(T), hipMemcpyHostToDevice));
-        for (int v : volumes_of(op))
-            if (is_wavefield(v))
-                HIP_TRY(hipMemsetAsync(c->vol[v], 0, is_split(v) ? c->nvox * std::max(c->wsize, (size_t)4) : wbytes,
-                                       c->st));
-
-        const T dtx = dt / (T)d->dx, dty = dt / (T)d->dy, dtz = dt / (T)d->dz;
-        const int sxp = d->sx + nb - 1, syp = d->sy + nb - 1, szp = d->sz + nb - 1;
-        const int s0 = sxp + g.ld * syp + g.pl * szp;
-        VelArgs va{};
-        va.g = g;
-        va.l = make_launch(g, c->ty[0], c->zseg[0]);
-        va.c = pm;
-        va.tab = c->tab;
-        va.P = c->vol[V_P];
-        va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
-        va.rho = c->vol[V_RHO];
-        va.th = c->vol[V_TH];
-        va.ph = c->vol[V_PH];
-        va.u = c->vol[V_U];
-        va.v = c->vol[V_V];
-        va.w = c->vol[V_W];
-        va.dtx = dtx;
-        va.dty = dty;
-        va.dtz = dtz;
-        va.sc = sc;
-        StressArgs sa{};
-        sa.g = g;
-        sa.l = make_launch(g, c->ty[1], c->zseg[1]);
-        sa.c = pm;
-        sa.tab = c->tab;
-        sa.u = c->vol[V_U];
-        sa.v = c->vol[V_V];
-        sa.w = c->vol[V_W];
-        sa.px = c->vol[V_PX];
-        sa.py = c->vol[V_PY];
-        sa.pz = c->vol[V_PZ];
-        sa.qx = c->vol[V_QX];
-        sa.qy = c->vol[V_QY];
-        sa.qz = c->vol[V_QZ];
-        sa.vp = c->vol[V_VP];
-        sa.rho = c->vol[V_RHO];
-        sa.eps = c->vol[V_EPS];
-        sa.del = c->vol[V_DEL];
-        sa.th = c->vol[V_TH];
-        sa.ph = c->vol[V_PH];
-        sa.P = c->vol[V_P];
-        sa.Q = c->vol[V_Q];
-        sa.dtx = dtx;
-        sa.dty = dty;
-        sa.dtz = dtz;
-        sa.sv = d->sv && op != A3D_OP_ISO;
-        sa.sxp = sxp;
-        sa.syp = syp;
-        sa.szp = szp;
-        sa.taper = c->dtab + g.nnx + g.nny + g.nnz;
-        sa.zrec = nb + d->rec_z;
-        sa.nb = nb;
-        sa.nx = nx;
-        sa.ny = ny;
-        sa.sc = sc;
-        sa.unsplit = d->scheme != 0 || prec == PREC_FP16;
-
-        // The time grid is the reference's float accumulation t = dt, t += dt and the gate
-        // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
-        const float tdelay = (float)(1.0 / d->fm);
-        const T pfac = (T)d->pfac;
         HIP_TRY(hipEventRecord(c->ev0, c->st));
-        float tcur = d->dt;
-        for (int it = 0; it < nt; it++, tcur += d->dt) {
-            // add_source (reference iso:523-530; vti:557; tti:571), applied on load
-            Src S{-1, -1, 0.0, 0.0, 0.0, 0.0};
-            if (tcur <= 2 * tdelay) {
-                const T src = wavelet_value<T>(d->wtype, tcur, d->fm);
-                if (op == A3D_OP_ISO) {
-                    const T mp = (T)(-1.0 * pfac);
-                    const T a0 = mp * src, a1 = pfac * src;
-                    S = {s0, s0 + 2 * g.pl, (double)a0, (double)a1, (double)a0, (double)a1};
-                } else {
-                    const T a0 = pfac * src;
-                    S = {s0, -1, (double)a0, 0.0, (double)a0, 0.0};
-                }
-            }
-            va.s = S;
+        for (int it = 0; it < nt; it++) {
+            va.s = srcs[it];
             sa.rec = c->rec + plane_rec * it;
             timing_begin(c, 0);
             HIP_TRY(launch_vel(prec, op, c->ty[0], va, c->st));
@@ -728,6 +748,42 @@ int maybe_autotune(a3d_ctx* c, const a3d_shot_desc* d) {
 // ===========================================================================
 // Handle API
 // ===========================================================================
+namespace {
+
+// Model-derived scalars of a context whose interior is the window (w.o*, c->d.n*) of a
+// z-fastest host model of size w.n*: the d0 sample, the mute velocity samples, the
+// smallest impedance (binary16 scales).
+void model_scalars(a3d_ctx* c, const float* vel, const float* rho, const float* eps, const Window& w) {
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    auto at = [&](const float* a, int iz, int ix, int iy) {
+        return a[(size_t)(iz + w.oz) + (size_t)w.nz * (ix + w.ox) + (size_t)w.nz * w.nx * (iy + w.oy)];
+    };
+    if (c->d.op != A3D_OP_2ND) {
+        const int nb = c->d.nb;
+        const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
+        // padded z-fastest linear index N/2 (reference get_d0, iso:194)
+        const long long i = (long long)nny * nnx * nnz / 2;
+        const int iz = (int)(i % nnz), ix = (int)((i / nnz) % nnx), iy = (int)(i / ((long long)nnz * nnx));
+        c->vel_d0 = at(vel, clampi(iz - nb, 0, nz - 1), clampi(ix - nb, 0, nx - 1), clampi(iy - nb, 0, ny - 1));
+        // padded index 1 = (iz=1, ix=0, iy=0) (reference mute_directwave, iso:340, vti:357)
+        const int z1 = clampi(1 - nb, 0, nz - 1), x1 = clampi(-nb, 0, nx - 1), y1 = clampi(-nb, 0, ny - 1);
+        c->vel_first = at(vel, z1, x1, y1);
+        c->eps_first = eps ? at(eps, z1, x1, y1) : 0.0f;
+        // smallest impedance rho*v (TTI has no density: rho = 1)
+        double zmin = 1e300;
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iz = 0; iz < nz; ++iz)
+                    zmin = std::min(zmin, (double)at(vel, iz, ix, iy) * (rho ? (double)at(rho, iz, ix, iy) : 1.0));
+        c->zmin = (float)std::max(zmin, 1e-30);
+    } else {
+        // reference Toa_3dfd_cuda.cu:403: v0[1], the interior sample (iz=1, ix=0, iy=0)
+        c->vel_raw1 = at(vel, nz > 1 ? 1 : 0, 0, 0);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* a3d_last_error(void) { return g_err.c_str(); }
@@ -858,28 +914,7 @@ int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* e
         (void)hipFree(stage);
     }
     if (rc) return rc;
-    // model-derived scalars
-    if (op != A3D_OP_2ND) {
-        const int nb = c->d.nb;
-        const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
-        // padded z-fastest linear index N/2 (reference get_d0, iso:194)
-        const long long i = (long long)nny * nnx * nnz / 2;
-        const int iz = (int)(i % nnz), ix = (int)((i / nnz) % nnx), iy = (int)(i / ((long long)nnz * nnx));
-        const size_t k = (size_t)clampi(iz - nb, 0, nz - 1) + (size_t)nz * clampi(ix - nb, 0, nx - 1) +
-                         (size_t)nz * nx * clampi(iy - nb, 0, ny - 1);
-        c->vel_d0 = vel[k];
-        // padded index 1 = (iz=1, ix=0, iy=0) (reference mute_directwave, iso:340, vti:357)
-        const size_t k1 = (size_t)clampi(1 - nb, 0, nz - 1) + (size_t)nz * clampi(-nb, 0, nx - 1) +
-                          (size_t)nz * nx * clampi(-nb, 0, ny - 1);
-        c->vel_first = vel[k1];
-        c->eps_first = eps ? eps[k1] : 0.0f;
-        // smallest impedance rho*v (TTI has no density: rho = 1)
-        double zmin = 1e300;
-        for (size_t i = 0; i < n; ++i) zmin = std::min(zmin, (double)vel[i] * (rho ? (double)rho[i] : 1.0));
-        c->zmin = (float)std::max(zmin, 1e-30);
-    } else {
-        c->vel_raw1 = n > 1 ? vel[1] : vel[0];   // reference Toa_3dfd_cuda.cu:403
-    }
+    model_scalars(c, vel, rho, eps, Window{0, 0, 0, nx, ny, nz});
     c->model_set = true;
     return A3D_OK;
 }
@@ -1232,6 +1267,465 @@ int cuda_3dfd_tti(char* FNvel, char* FNrho, char* FNepsilon, char* FNdelta, char
     fclose(fshot);
     report("cuda_3dfd_tti", rc);
     return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched windows: many shots, each an aperture window cut from one model
+// resident on the device, advanced together (one batched launch per sweep and
+// tile-shape group per time step), optionally captured into one hipGraph.
+// ===========================================================================
+namespace {
+
+struct BatchGroup {
+    int ty = 4;
+    std::vector<int> wins;          // window indices in this group
+    void* args = nullptr;           // device array of VelArgs or StressArgs
+    int* base = nullptr;            // device prefix sums of tile counts (nwin+1)
+    Src* src = nullptr;             // velocity groups: per-step sources [nt][group windows]
+    Launch lb{};
+};
+
+}  // namespace
+
+struct a3d_batch {
+    a3d_shot_desc d{};              // full-model geometry and the shared shot parameters
+    int device = 0;
+    std::vector<a3d_window> win;
+    std::vector<a3d_ctx*> ctx;      // one context (buffers, tables) per window
+    std::vector<a3d_shot_desc> wd;  // per-window shot descriptors
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool model_set = false;
+    std::vector<int> ty_vel, ty_str; // tile rows per window and sweep
+    bool tuned = false;
+    // state of the captured time loop, kept so later runs replay the same graph
+    std::vector<BatchGroup> gv, gs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long long graph_launches = 0;
+};
+
+namespace {
+
+void free_groups(std::vector<BatchGroup>& gs) {
+    for (auto& g : gs) {
+        if (g.args) (void)hipFree(g.args);
+        if (g.base) (void)hipFree(g.base);
+        if (g.src) (void)hipFree(g.src);
+        g.args = nullptr;
+        g.base = nullptr;
+        g.src = nullptr;
+    }
+}
+
+// Per-step source table of a velocity group: src[it * n + k] for its k-th window.
+int group_sources(BatchGroup& g, const std::vector<Src>& all, int nt, int nw) {
+    const int n = (int)g.wins.size();
+    std::vector<Src> t((size_t)nt * n);
+    for (int it = 0; it < nt; ++it)
+        for (int k = 0; k < n; ++k) t[(size_t)it * n + k] = all[(size_t)it * nw + g.wins[k]];
+    if (hipMalloc(&g.src, t.size() * sizeof(Src)) != hipSuccess) {
+        g.src = nullptr;
+        return fail(A3D_ENOMEM, "batch source table allocation failed");
+    }
+    HIP_TRY(hipMemcpy(g.src, t.data(), t.size() * sizeof(Src), hipMemcpyHostToDevice));
+    return A3D_OK;
+}
+
+// Groups windows by tile rows and uploads their arguments and tile prefix sums.
+template <class A>
+int build_groups(a3d_batch* b, const std::vector<A>& args, const std::vector<int>& tys, std::vector<BatchGroup>& out) {
+    for (int ty : {2, 4, 8}) {
+        BatchGroup g;
+        g.ty = ty;
+        for (size_t w = 0; w < args.size(); ++w)
+            if (tys[w] == ty) g.wins.push_back((int)w);
+        if (g.wins.empty()) continue;
+        std::vector<A> ga;
+        std::vector<int> base(1, 0);
+        for (int w : g.wins) {
+            A a = args[w];
+            a.l = make_launch(a.g, ty, 1);
+            ga.push_back(a);
+            base.push_back(base.back() + a.l.ntiles);
+        }
+        g.lb.ntiles = base.back();
+        g.lb.grid = ((g.lb.ntiles + 7) / 8) * 8;
+        if (hipMalloc(&g.args, ga.size() * sizeof(A)) != hipSuccess ||
+            hipMalloc(&g.base, base.size() * sizeof(int)) != hipSuccess) {
+            out.push_back(g);
+            return fail(A3D_ENOMEM, "batch argument allocation failed");
+        }
+        HIP_TRY(hipMemcpy(g.args, ga.data(), ga.size() * sizeof(A), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+        out.push_back(g);
+    }
+    return A3D_OK;
+}
+
+void drop_graph(a3d_batch* b) {
+    free_groups(b->gv);
+    free_groups(b->gs);
+    b->gv.clear();
+    b->gs.clear();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+template <class T>
+int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* group_ms) {
+    const int nw = (int)b->ctx.size();
+    const int nt = b->d.nt;
+    const int op = b->d.op;
+    int rc = A3D_OK;
+    std::vector<VelArgs> va(nw);
+    std::vector<StressArgs> sa(nw);
+    std::vector<Src> src_all((size_t)nt * nw);
+    for (int w = 0; w < nw; ++w) {
+        a3d_ctx* c = b->ctx[w];
+        const a3d_shot_desc* d = &b->wd[w];
+        if ((rc = ensure(&c->rec, &c->rec_cap, (size_t)d->nx * d->ny * nt))) return rc;
+        std::vector<Src> srcs;
+        if ((rc = prepare_first<T>(c, d, field_scales(c, d), va[w], sa[w], srcs))) return rc;
+        sa[w].rec = c->rec;
+        for (int it = 0; it < nt; ++it) src_all[(size_t)it * nw + w] = srcs[it];
+    }
+    // the wavefield memsets of prepare_first ran on the windows' streams
+    for (auto* c : b->ctx) HIP_TRY(hipStreamSynchronize(c->st));
+    const bool replay = mode == 2 && b->exec != nullptr;
+    std::vector<BatchGroup> gv_local, gs_local;
+    std::vector<BatchGroup>& gv = mode == 2 ? b->gv : gv_local;
+    std::vector<BatchGroup>& gs = mode == 2 ? b->gs : gs_local;
+    auto cleanup = [&](int code) {
+        if (mode != 2) {
+            free_groups(gv_local);
+            free_groups(gs_local);
+        } else if (code) {
+            drop_graph(b);
+        }
+        return code;
+    };
+    if (!replay) {
+        if (mode == 2) drop_graph(b);
+        if ((rc = build_groups(b, va, b->ty_vel, gv)) || (rc = build_groups(b, sa, b->ty_str, gs)))
+            return cleanup(rc);
+        for (auto& g : gv)
+            if ((rc = group_sources(g, src_all, nt, nw))) return cleanup(rc);
+    }
+    // per-group timing (used by the tuner): events around each group's launches
+    std::vector<hipEvent_t> gev;
+    const bool timed = group_ms != nullptr && mode != 2;
+    auto step = [&](int it) -> int {
+        for (size_t k = 0; k < gv.size(); ++k) {
+            const BatchGroup& g = gv[k];
+            if (timed) HIP_TRY(hipEventRecord(gev[2 * k], b->st));
+            HIP_TRY(launch_vel_batch(b->ctx[0]->prec, op, g.ty, (const VelArgs*)g.args, g.base, (int)g.wins.size(),
+                                     g.lb, g.src, it, b->st));
+            if (timed) HIP_TRY(hipEventRecord(gev[2 * k + 1], b->st));
+        }
+        for (size_t k = 0; k < gs.size(); ++k) {
+            const BatchGroup& g = gs[k];
+            const size_t e = 2 * (gv.size() + k);
+            if (timed) HIP_TRY(hipEventRecord(gev[e], b->st));
+            HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
+                                        (int)g.wins.size(), g.lb, it, b->st));
+            if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
+        }
+        return A3D_OK;
+    };
+    if (timed) {
+        gev.resize(2 * (gv.size() + gs.size()));
+        for (auto& e : gev) (void)hipEventCreate(&e);
+        for (size_t k = 0; k < gv.size() + gs.size(); ++k) group_ms[k] = 0.0;
+    }
+    if (mode == 2) {
+        if (!replay) {
+            // the whole time loop as one graph; kept and replayed by later runs
+            HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            for (int it = 0; it < nt && !rc; ++it) rc = step(it);
+            hipError_t e = hipStreamEndCapture(b->st, &b->graph);
+            if (rc) return cleanup(rc);
+            if (e != hipSuccess) return cleanup(fail(A3D_EHIP, "graph capture failed: %s", hipGetErrorString(e)));
+            if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+                return cleanup(fail(A3D_EHIP, "graph instantiation failed"));
+        }
+        HIP_TRY(hipEventRecord(b->ev0, b->st));
+        HIP_TRY(hipGraphLaunch(b->exec, b->st));
+        ++b->graph_launches;
+    } else {
+        HIP_TRY(hipEventRecord(b->ev0, b->st));
+        for (int it = 0; it < nt; ++it) {
+            if ((rc = step(it))) return cleanup(rc);
+            if (timed) {
+                HIP_TRY(hipEventSynchronize(gev.back()));
+                for (size_t k = 0; k < gv.size() + gs.size(); ++k) {
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, gev[2 * k], gev[2 * k + 1]) == hipSuccess) group_ms[k] += ms;
+                }
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(b->ev1, b->st));
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    for (auto& e : gev) (void)hipEventDestroy(e);
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (loop_ms) *loop_ms = ms;
+    rc = cleanup(A3D_OK);
+    // outputs: per-window sentinel, transpose (+ mute) and copy
+    int unstable = 0;
+    for (int w = 0; w < nw && !rc; ++w) {
+        a3d_ctx* c = b->ctx[w];
+        const a3d_shot_desc* d = &b->wd[w];
+        const size_t n = (size_t)d->nx * d->ny * nt;
+        HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->st));
+        HIP_TRY(launch_check_finite(c->rec, n, c->flag, c->st));
+        int hflag = 0;
+        HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        unstable |= hflag;
+        if (shots && shots[w]) {
+            if ((rc = ensure(&c->shot_dev, &c->shot_cap, n))) return rc;
+            MuteArgs m{};
+            m.mute = d->mute;
+            m.dx = d->dx;
+            m.dy = d->dy;
+            m.dz = d->dz;
+            m.dt = d->dt;
+            m.fm = d->fm;
+            m.sx = d->sx;
+            m.sy = d->sy;
+            m.sz = d->sz;
+            m.tt = d->mute_tt;
+            m.vcut = d->v_mute > 0 ? d->v_mute
+                                   : (op == A3D_OP_ISO ? c->vel_first : c->vel_first * sqrtf(1 + 2 * c->eps_first));
+            HIP_TRY(launch_shot_transpose(c->rec, nt, d->nx, d->ny, m, c->shot_dev, c->st));
+            HIP_TRY(hipMemcpyAsync(shots[w], c->shot_dev, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+            HIP_TRY(hipStreamSynchronize(c->st));
+        }
+    }
+    if (!rc && unstable) return fail(A3D_EUNSTABLE, "non-finite values in a window's shot record");
+    return rc;
+}
+
+int batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms, double* group_ms) {
+    if (b->ctx[0]->prec == PREC_FP64) return batch_run_t<double>(b, mode, shots, loop_ms, group_ms);
+    return batch_run_t<float>(b, mode, shots, loop_ms, group_ms);
+}
+
+}  // namespace
+
+extern "C" {
+
+int a3d_batch_create(const a3d_shot_desc* desc, int nwin, const a3d_window* wins, int device, a3d_batch** out) {
+    if (!out || !desc || !wins || nwin < 1) return fail(A3D_EINVAL, "null argument or no windows");
+    *out = nullptr;
+    if (desc->op == A3D_OP_2ND) return fail(A3D_EINVAL, "batched windows cover the 1st-order operators");
+    a3d_batch* b = new a3d_batch();
+    b->d = *desc;
+    b->device = device;
+    auto cleanup = [&](int code) {
+        a3d_batch_destroy(b);
+        return code;
+    };
+    for (int w = 0; w < nwin; ++w) {
+        const a3d_window& W = wins[w];
+        if (W.ox < 0 || W.oy < 0 || W.oz < 0 || W.nx < 1 || W.ny < 1 || W.nz < 1 || W.ox + W.nx > desc->nx ||
+            W.oy + W.ny > desc->ny || W.oz + W.nz > desc->nz)
+            return cleanup(fail(A3D_EINVAL, "window %d outside the model", w));
+        a3d_shot_desc d = *desc;
+        d.nx = W.nx;
+        d.ny = W.ny;
+        d.nz = W.nz;
+        d.sx = W.sx;
+        d.sy = W.sy;
+        d.sz = W.sz;
+        d.snap_interval = 0;
+        a3d_ctx* c = nullptr;
+        int rc = a3d_create(&d, device, &c);
+        if (rc) return cleanup(rc);
+        b->ctx.push_back(c);
+        b->wd.push_back(d);
+        b->win.push_back(W);
+        if ((rc = check_desc_shot(c, &d))) return cleanup(rc);
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+        return cleanup(fail(A3D_EHIP, "batch stream/event creation failed"));
+    b->ty_vel.assign(nwin, 8);
+    b->ty_str.assign(nwin, 4);
+    *out = b;
+    return A3D_OK;
+}
+
+int a3d_batch_destroy(a3d_batch* b) {
+    if (!b) return A3D_OK;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    drop_graph(b);
+    for (auto* c : b->ctx) a3d_destroy(c);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+    return A3D_OK;
+}
+
+int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const float* eps, const float* del,
+                        const float* theta, const float* phi) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    const int op = b->d.op;
+    if (!vel || ((op == A3D_OP_ISO || op == A3D_OP_VTI) && !rho) || (op != A3D_OP_ISO && (!eps || !del)) ||
+        (op == A3D_OP_TTI && (!theta || !phi)))
+        return fail(A3D_EINVAL, "model arrays missing for the operator");
+    HIP_TRY(hipSetDevice(b->device));
+    drop_graph(b);
+    const size_t n = (size_t)b->d.nx * b->d.ny * b->d.nz;
+    float* stage = nullptr;
+    if (hipMalloc(&stage, n * sizeof(float)) != hipSuccess) return fail(A3D_ENOMEM, "model staging allocation failed");
+    int rc = A3D_OK;
+    const Window full{0, 0, 0, b->d.nx, b->d.ny, b->d.nz};
+    auto up = [&](const float* h, int v) -> int {
+        HIP_TRY(hipMemcpy(stage, h, n * sizeof(float), hipMemcpyHostToDevice));
+        for (size_t w = 0; w < b->ctx.size(); ++w) {
+            a3d_ctx* c = b->ctx[w];
+            const Window win{b->win[w].ox, b->win[w].oy, b->win[w].oz, full.nx, full.ny, full.nz};
+            HIP_TRY(launch_pad(stage, c->d.nx, c->d.ny, c->d.nz, c->off, c->g, c->vol[v], c->prec == PREC_FP64, 0, 0.f,
+                               b->st, &win));
+        }
+        HIP_TRY(hipStreamSynchronize(b->st));
+        return A3D_OK;
+    };
+    if (!rc) rc = up(vel, V_VP);
+    if (!rc && (op == A3D_OP_ISO || op == A3D_OP_VTI)) rc = up(rho, V_RHO);
+    if (!rc && op != A3D_OP_ISO) rc = up(eps, V_EPS);
+    if (!rc && op != A3D_OP_ISO) rc = up(del, V_DEL);
+    if (!rc && op == A3D_OP_TTI) rc = up(theta, V_TH);
+    if (!rc && op == A3D_OP_TTI) rc = up(phi, V_PH);
+    (void)hipFree(stage);
+    if (rc) return rc;
+    for (size_t w = 0; w < b->ctx.size(); ++w) {
+        const Window win{b->win[w].ox, b->win[w].oy, b->win[w].oz, full.nx, full.ny, full.nz};
+        model_scalars(b->ctx[w], vel, rho, eps, win);
+        b->ctx[w]->model_set = true;
+    }
+    b->model_set = true;
+    return A3D_OK;
+}
+
+int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    if (!b->model_set) return fail(A3D_ESTATE, "model not set");
+    if (mode < 0 || mode > 2) return fail(A3D_EINVAL, "mode must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(b->device));
+    if (mode == 0) {
+        // baseline: the windows one after the other as plain single-shot runs
+        double total = 0.0;
+        for (size_t w = 0; w < b->ctx.size(); ++w) {
+            double ms = 0.0;
+            int rc = maybe_autotune(b->ctx[w], &b->wd[w]);
+            if (!rc) rc = run_shot(b->ctx[w], &b->wd[w], shots ? shots[w] : nullptr, SnapFn(), nullptr, &ms);
+            if (rc) return rc;
+            total += ms;
+        }
+        if (loop_ms) *loop_ms = total;
+        return A3D_OK;
+    }
+    return batch_run(b, mode, shots, loop_ms, nullptr);
+}
+
+int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    if (!b->model_set) return fail(A3D_ESTATE, "model not set");
+    HIP_TRY(hipSetDevice(b->device));
+    const int nw = (int)b->ctx.size();
+    // size classes: windows with the same x-tile count and similar plane size
+    std::vector<int> cls(nw);
+    std::vector<long long> keys;
+    for (int w = 0; w < nw; ++w) {
+        const Geo& g = b->ctx[w]->g;
+        const long long key = (long long)(g.ld / TX) * 1000 + (g.nny + 63) / 64;
+        auto it = std::find(keys.begin(), keys.end(), key);
+        if (it == keys.end()) {
+            keys.push_back(key);
+            cls[w] = (int)keys.size() - 1;
+        } else {
+            cls[w] = (int)(it - keys.begin());
+        }
+    }
+    if (nclasses) *nclasses = (int)keys.size();
+    drop_graph(b);
+    const int saved_nt = b->d.nt;
+    const int tsteps = std::max(2, steps);
+    for (auto& d : b->wd) d.nt = tsteps;
+    b->d.nt = tsteps;
+    // time every class with each tile-row choice (all windows of the class in one group)
+    std::vector<double> best_v(keys.size(), 1e30), best_s(keys.size(), 1e30);
+    std::vector<int> tv(keys.size(), 8), ts(keys.size(), 4);
+    int rc = A3D_OK;
+    for (int ty : {2, 4, 8}) {
+        b->ty_vel.assign(nw, ty);
+        b->ty_str.assign(nw, ty);
+        // one run per class so each group holds one class
+        for (size_t k = 0; k < keys.size() && !rc; ++k) {
+            std::vector<int> saved_v = b->ty_vel;
+            // windows outside the class are moved to another group (a different tile-row value)
+            const int other = ty == 2 ? 4 : 2;
+            for (int w = 0; w < nw; ++w)
+                if (cls[w] != (int)k) b->ty_vel[w] = b->ty_str[w] = other;
+            double gms[6] = {0, 0, 0, 0, 0, 0};
+            double ms = 0.0;
+            rc = batch_run(b, 1, nullptr, &ms, gms);
+            if (rc == A3D_EUNSTABLE) rc = A3D_OK;
+            // group order: vel groups by ty (2,4,8 present), then stress groups likewise
+            std::vector<int> present;
+            for (int t : {2, 4, 8}) {
+                bool any = false;
+                for (int w = 0; w < nw; ++w) any |= b->ty_vel[w] == t;
+                if (any) present.push_back(t);
+            }
+            const int ng = (int)present.size();
+            for (int gi = 0; gi < ng; ++gi) {
+                if (present[gi] != ty) continue;
+                if (gms[gi] < best_v[k]) {
+                    best_v[k] = gms[gi];
+                    tv[k] = ty;
+                }
+                if (gms[ng + gi] < best_s[k]) {
+                    best_s[k] = gms[ng + gi];
+                    ts[k] = ty;
+                }
+            }
+            b->ty_vel = saved_v;
+            b->ty_str = saved_v;
+        }
+    }
+    for (int w = 0; w < nw; ++w) {
+        b->ty_vel[w] = tv[cls[w]];
+        b->ty_str[w] = ts[cls[w]];
+    }
+    for (auto& d : b->wd) d.nt = saved_nt;
+    b->d.nt = saved_nt;
+    b->tuned = rc == A3D_OK;
+    return rc;
+}
+
+int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    (void)b->graph_launches;
+    long long pts = 0;
+    for (auto* c : b->ctx) pts += (long long)(c->g.nnx - 2 * MM) * (c->g.nny - 2 * MM) * (c->g.nnz - 2 * MM);
+    if (updated_points) *updated_points = pts;
+    if (nwin) *nwin = (int)b->ctx.size();
+    if (tile_rows_out)
+        for (size_t w = 0; w < b->ctx.size(); ++w) {
+            tile_rows_out[2 * w] = b->ty_vel[w];
+            tile_rows_out[2 * w + 1] = b->ty_str[w];
+        }
+    return A3D_OK;
 }
 
 }  // extern "C"

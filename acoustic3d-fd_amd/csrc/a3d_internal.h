@@ -120,15 +120,29 @@ hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st);
 hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 
+// Batched sweeps over several windows (each an independent padded grid with its own
+// arguments in device memory). base: nwin+1 prefix sums of the windows' tile counts;
+// lb: the combined decomposition (ntiles = base[nwin], grid = 8*ceil(ntiles/8)).
+hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
+                            const Src* srcs, int it, hipStream_t st);
+hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
+                               const Launch& lb, int it, hipStream_t st);
+
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.
 int max_blocks_per_cu(int prec, int kind, int op, int ty);
 
+// A sub-volume of a larger z-fastest model: origin (ox, oy, oz), full model size (nx, ny, nz).
+struct Window {
+    int ox, oy, oz, nx, ny, nz;
+};
+
 // Pads an interior z-fastest float model (nx,ny,nz) into the device layout
 // (float, or double when model_double) by edge replication; the interior
 // starts at padded index `off` on every axis. mode 0: copy; mode 1: (v*dt)^2.
+// win (may be null): the interior is cut at origin win->o* from a model of size win->n*.
 hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, void* dst, int model_double,
-                      int mode, float dt, hipStream_t st);
+                      int mode, float dt, hipStream_t st, const Window* win = nullptr);
 
 // Interior crop of a padded wavefield into z-fastest float order ([ny][nx][nz]).
 // 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied

@@ -111,15 +111,34 @@ __device__ __forceinline__ C lap9(F w) {
 // Workgroup -> tile. The hardware deals workgroup b to XCD b % 8; tile ids
 // are assigned so that XCD k owns the contiguous run [k*per, (k+1)*per).
 // Tile order: x tile fastest, then y tile, then z segment.
-__device__ __forceinline__ bool decode_tile(const Launch& l, int& tix, int& tiy, int& seg) {
+__device__ __forceinline__ int remap_tile(const Launch& l) {
     const int per = l.grid >> 3;
-    const int t = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
-    if (t >= l.ntiles) return false;
+    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ void split_tile(const Launch& l, int t, int& tix, int& tiy, int& seg) {
     tix = t % l.ntx;
     const int r = t / l.ntx;
     tiy = r % l.nty;
     seg = r / l.nty;
+}
+
+__device__ __forceinline__ bool decode_tile(const Launch& l, int& tix, int& tiy, int& seg) {
+    const int t = remap_tile(l);
+    if (t >= l.ntiles) return false;
+    split_tile(l, t, tix, tiy, seg);
     return true;
+}
+
+// Batched launches: global tile t belongs to window w with base[w] <= t < base[w+1].
+__device__ __forceinline__ int find_window(const int* __restrict__ base, int nwin, int t) {
+    int lo = 0, hi = nwin - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 template <class C>
@@ -167,7 +186,7 @@ __device__ __forceinline__ void fill_halo(C (*s)[TX + 2 * MM], const Geo& g, int
 // 1st-order velocity sweep
 // --------------------------------------------------------------------------
 template <class PR, int OP, int TY>
-__global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
+__device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using M = typename PR::M;
     using C = typename PR::C;
@@ -179,8 +198,6 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     __shared__ C sP[H][W];
     __shared__ C sQ[Q_XY ? H : 1][Q_XY ? W : 1];
 
-    int tix, tiy, seg;
-    if (!decode_tile(a.l, tix, tiy, seg)) return;
     const Geo g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = tix * TX, y0 = MM + tiy * TY;
@@ -191,7 +208,6 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     const int col = x + g.ld * yl;
     const int zb = MM + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, g.nnz - MM);
-    const Src S0 = a.s;
     const Scales sc = a.sc;
     const S* P = (const S*)a.P;
     const S* Q = (const S*)a.Q;
@@ -282,11 +298,32 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     }
 }
 
+template <class PR, int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    vel_body<PR, OP, TY>(a, a.s, tix, tiy, seg);
+}
+
+// Batched velocity sweep: the tiles of all windows of a batch in one launch; per-window
+// arguments in device memory, per-step sources srcs[it * nwin + w].
+template <class PR, int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict__ wins, const int* __restrict__ base,
+                                                      int nwin, Launch lb, const Src* __restrict__ srcs, int it) {
+    const int t = remap_tile(lb);
+    if (t >= lb.ntiles) return;
+    const int w = find_window(base, nwin, t);
+    const VelArgs a = wins[w];
+    int tix, tiy, seg;
+    split_tile(a.l, t - base[w], tix, tiy, seg);
+    vel_body<PR, OP, TY>(a, srcs[(size_t)it * nwin + w], tix, tiy, seg);
+}
+
 // --------------------------------------------------------------------------
 // 1st-order stress sweep
 // --------------------------------------------------------------------------
 template <class PR, int OP, int TY>
-__global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
+__device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using SP = typename PR::SP;
     using M = typename PR::M;
@@ -297,8 +334,6 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     __shared__ C sV[H][W];
     __shared__ C sW[TTI ? H : 1][TTI ? W : 1];
 
-    int tix, tiy, seg;
-    if (!decode_tile(a.l, tix, tiy, seg)) return;
     const Geo g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = tix * TX, y0 = MM + tiy * TY;
@@ -347,7 +382,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     const C ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
     const C ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
     const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
-    const bool recx = a.rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
+    const bool recx = rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
@@ -491,7 +526,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
                 stw(Qo, id, Qn, sc.inv[F_Q]);
             }
             stw(Po, id, Pn, sc.inv[F_P]);
-            if (recx && z == a.zrec) a.rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
+            if (recx && z == a.zrec) rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
         }
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -502,6 +537,27 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
             }
         }
     }
+}
+
+template <class PR, int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    stress_body<PR, OP, TY>(a, a.rec, tix, tiy, seg);
+}
+
+// Batched stress sweep; a window's record plane of step it is rec + it * nx * ny.
+template <class PR, int OP, int TY>
+__global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __restrict__ wins,
+                                                         const int* __restrict__ base, int nwin, Launch lb, int it) {
+    const int t = remap_tile(lb);
+    if (t >= lb.ntiles) return;
+    const int w = find_window(base, nwin, t);
+    const StressArgs a = wins[w];
+    int tix, tiy, seg;
+    split_tile(a.l, t - base[w], tix, tiy, seg);
+    float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
+    stress_body<PR, OP, TY>(a, rec, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -582,17 +638,17 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
 // Pads an interior float model into the device layout (float or double), edge replication.
 template <class M>
 __global__ void k_pad(const float* __restrict__ src, int nx, int ny, int nz, int off, Geo g, M* __restrict__ dst,
-                      int mode, float dt) {
+                      int mode, float dt, Window win) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     const int z = blockIdx.z;
     if (x >= g.ld) return;
     M v = 0;
     if (x < g.nnx) {
-        const int cx = min(max(x - off, 0), nx - 1);
-        const int cy = min(max(y - off, 0), ny - 1);
-        const int cz = min(max(z - off, 0), nz - 1);
-        v = (M)src[cz + (size_t)nz * cx + (size_t)nz * nx * cy];
+        const int cx = min(max(x - off, 0), nx - 1) + win.ox;
+        const int cy = min(max(y - off, 0), ny - 1) + win.oy;
+        const int cz = min(max(z - off, 0), nz - 1) + win.oz;
+        v = (M)src[cz + (size_t)win.nz * cx + (size_t)win.nz * win.nx * cy];
         if (mode == 1) {
             const M t = v * (M)dt;
             v = t * t;
@@ -714,6 +770,54 @@ hipError_t step2_ty(int ty, const Step2Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <class PR, int OP>
+hipError_t vel_batch_ty(int ty, const VelArgs* w, const int* base, int nwin, const Launch& lb, const Src* srcs, int it,
+                        hipStream_t st) {
+    const dim3 grid(lb.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_vel_batch<PR, OP, 2>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 4: hipLaunchKernelGGL((k_vel_batch<PR, OP, 4>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 8: hipLaunchKernelGGL((k_vel_batch<PR, OP, 8>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, srcs, it); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class PR, int OP>
+hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
+                           hipStream_t st) {
+    const dim3 grid(lb.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it); break;
+        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it); break;
+        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class PR>
+hipError_t vel_batch_op(int op, int ty, const VelArgs* w, const int* base, int nwin, const Launch& lb, const Src* srcs,
+                        int it, hipStream_t st) {
+    switch (op) {
+        case 1: return vel_batch_ty<PR, 1>(ty, w, base, nwin, lb, srcs, it, st);
+        case 2: return vel_batch_ty<PR, 2>(ty, w, base, nwin, lb, srcs, it, st);
+        case 3: return vel_batch_ty<PR, 3>(ty, w, base, nwin, lb, srcs, it, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class PR>
+hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
+                           hipStream_t st) {
+    switch (op) {
+        case 1: return stress_batch_ty<PR, 1>(ty, w, base, nwin, lb, it, st);
+        case 2: return stress_batch_ty<PR, 2>(ty, w, base, nwin, lb, it, st);
+        case 3: return stress_batch_ty<PR, 3>(ty, w, base, nwin, lb, it, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <class PR>
 hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
     switch (op) {
@@ -792,6 +896,26 @@ hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStrea
     }
 }
 
+hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
+                            const Src* srcs, int it, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP32: return vel_batch_op<Fp32>(op, ty, wins, base, nwin, lb, srcs, it, st);
+        case PREC_FP16: return vel_batch_op<Fp16>(op, ty, wins, base, nwin, lb, srcs, it, st);
+        case PREC_FP64: return vel_batch_op<Fp64>(op, ty, wins, base, nwin, lb, srcs, it, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
+                               const Launch& lb, int it, hipStream_t st) {
+    switch (prec) {
+        case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, st);
+        case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, st);
+        case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st) {
     switch (prec) {
         case PREC_FP32: return step2_ty<Fp32>(ty, a, st);
@@ -810,12 +934,13 @@ int max_blocks_per_cu(int prec, int kind, int op, int ty) {
 }
 
 hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, void* dst, int model_double,
-                      int mode, float dt, hipStream_t st) {
+                      int mode, float dt, hipStream_t st, const Window* win) {
     const dim3 grid((g.ld + 255) / 256, g.nny, g.nnz);
+    const Window w = win ? *win : Window{0, 0, 0, nx, ny, nz};
     if (model_double)
-        hipLaunchKernelGGL(k_pad<double>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (double*)dst, mode, dt);
+        hipLaunchKernelGGL(k_pad<double>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (double*)dst, mode, dt, w);
     else
-        hipLaunchKernelGGL(k_pad<float>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (float*)dst, mode, dt);
+        hipLaunchKernelGGL(k_pad<float>, grid, dim3(256), 0, st, src, nx, ny, nz, off, g, (float*)dst, mode, dt, w);
     return hipGetLastError();
 }
 

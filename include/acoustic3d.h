@@ -178,6 +178,42 @@ int a3d_get_launch(a3d_ctx* ctx, int* out6);
  * least 4M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
 int a3d_autotune(a3d_ctx* ctx, const a3d_shot_desc* desc, int steps, double* best_ms);
 
+/* ----------------------------------------------------------------------- */
+/* Batched windows (no counterpart in the reference): many shots, each an   */
+/* aperture window cut from one model resident on the device.               */
+/* ----------------------------------------------------------------------- */
+typedef struct a3d_window {
+    int ox, oy, oz;        /* window origin in the full model (0-based interior cells) */
+    int nx, ny, nz;        /* window size */
+    int sx, sy, sz;        /* source inside the window (operator convention: 1-based) */
+} a3d_window;
+
+typedef struct a3d_batch a3d_batch;
+
+/* desc: the full model's geometry (op ISO/VTI/TTI, nx, ny, nz, spacing, nb) and
+ * the shot parameters shared by all windows (nt, dt, fm, pfac, rec_z, sv, mute,
+ * storage, scheme). Each window gets its own padded grid and receivers on its
+ * full surface. */
+int a3d_batch_create(const a3d_shot_desc* desc, int nwin, const a3d_window* wins, int device, a3d_batch** out);
+int a3d_batch_destroy(a3d_batch* b);
+/* Full-model arrays (nx*ny*nz floats, z fastest), uploaded once; every window's
+ * padded model is cut from the device copy. */
+int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const float* eps, const float* del,
+                        const float* theta, const float* phi);
+/* Tile rows per window-size class (windows with equal x-tile and y-block counts),
+ * from `steps` timed batched steps per candidate; *nclasses receives the class count. */
+int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses);
+/* mode 0: the windows one after another as plain single-shot runs (baseline);
+ * mode 1: all windows advanced together, one batched launch per sweep and
+ * tile-row group per step; mode 2: mode 1 with the whole time loop captured in
+ * one hipGraph and launched once. shots: nwin host pointers ([ny][nx][nt] of each
+ * window) or NULL; entries may be NULL. *loop_ms: time-loop duration. Results
+ * are bitwise identical in all modes. */
+int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms);
+/* Updated points per step summed over windows; window count; tile rows per
+ * window (2 ints each: velocity, stress), may be NULL. */
+int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out);
+
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
 
