@@ -120,8 +120,24 @@ def load(path=None):
     lib.a3d_batch_tune.argtypes = [vp, c_int, POINTER(c_int)]
     lib.a3d_batch_run.argtypes = [vp, c_int, POINTER(fp), POINTER(c_double)]
     lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
+    lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
     for name in ("a3d_batch_create", "a3d_batch_destroy", "a3d_batch_set_model", "a3d_batch_tune", "a3d_batch_run",
-                 "a3d_batch_info"):
+                 "a3d_batch_info", "a3d_batch_set_graph_chunk"):
+        getattr(lib, name).restype = c_int
+    lib.a3d_set_strip_net.argtypes = [vp, fp, c_int]
+    lib.a3d_strip_info.argtypes = [vp, POINTER(c_int)]
+    lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_float, ctypes.c_uint, fp, fp, fp,
+                                      c_int, POINTER(c_int)]
+    lib.a3d_mlp_create.argtypes = [c_int, POINTER(vp)]
+    lib.a3d_mlp_destroy.argtypes = [vp]
+    lib.a3d_mlp_set_params.argtypes = [vp, fp, c_int]
+    lib.a3d_mlp_get_params.argtypes = [vp, fp, c_int]
+    lib.a3d_mlp_predict.argtypes = [vp, fp, c_int, fp]
+    lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
+    lib.a3d_mlp_train.argtypes = [vp, fp, fp, c_int, c_int, c_int, c_float, c_float, ctypes.c_uint, POINTER(c_double)]
+    for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
+                 "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
+                 "a3d_mlp_loss_grad", "a3d_mlp_train"):
         getattr(lib, name).restype = c_int
     lib.a3d_probe_bandwidth.argtypes = [c_int, ctypes.c_size_t, c_int, POINTER(c_double), POINTER(c_double)]
     lib.a3d_probe_bandwidth.restype = c_int
@@ -229,6 +245,20 @@ class Modeller:
         _check(self.lib, self.lib.a3d_autotune(self._h, ctypes.byref(desc or self.desc), steps, best))
         return [best[0], best[1]]
 
+    def set_strip_net(self, params):
+        """Learned absorbing strip (VTI, FP32, nb = STRIP_NB): network parameters, or None
+        for the plain PML of width nb."""
+        if params is None:
+            _check(self.lib, self.lib.a3d_set_strip_net(self._h, None, 0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        _check(self.lib, self.lib.a3d_set_strip_net(self._h, _fptr(p), p.size))
+
+    def strip_cells(self):
+        n = c_int()
+        _check(self.lib, self.lib.a3d_strip_info(self._h, ctypes.byref(n)))
+        return n.value
+
     def grid_info(self):
         a, b, c = c_int(), c_int(), c_int()
         n = c_longlong()
@@ -239,6 +269,82 @@ class Modeller:
     def close(self):
         if self._h:
             self.lib.a3d_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+STRIP_NB = 10
+MLP_NIN, MLP_NOUT = 16, 2
+
+
+def strip_collect(teacher, student, desc, every=4, start=0, keep=0.05, seed=0, cap=1 << 20):
+    """Teacher-forced strip samples (a3d_strip_collect): returns X (n, 16), T (n, 2), S (n,)."""
+    lib = load()
+    X = np.empty((cap, MLP_NIN), np.float32)
+    T = np.empty((cap, MLP_NOUT), np.float32)
+    S = np.empty(cap, np.float32)
+    n = c_int()
+    _check(lib, lib.a3d_strip_collect(teacher._h, student._h, ctypes.byref(desc), every, start, keep, seed,
+                                      _fptr(X), _fptr(T), _fptr(S), cap, ctypes.byref(n)))
+    k = n.value
+    return X[:k].copy(), T[:k].copy(), S[:k].copy()
+
+
+class Mlp:
+    """The strip network on the GPU (a3d_mlp): 16 -> 64 -> 64 -> 2, tanh, FP32."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.nparams = self.lib.a3d_mlp_nparams()
+        self._h = c_void_p()
+        _check(self.lib, self.lib.a3d_mlp_create(device, ctypes.byref(self._h)))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        _check(self.lib, self.lib.a3d_mlp_set_params(self._h, _fptr(p), p.size))
+
+    def params(self):
+        p = np.empty(self.nparams, np.float32)
+        _check(self.lib, self.lib.a3d_mlp_get_params(self._h, _fptr(p), p.size))
+        return p
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], MLP_NOUT), np.float32)
+        _check(self.lib, self.lib.a3d_mlp_predict(self._h, _fptr(X), X.shape[0], _fptr(Y)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.nparams, np.float32)
+        loss = c_double()
+        _check(self.lib, self.lib.a3d_mlp_loss_grad(self._h, _fptr(X), _fptr(T), X.shape[0], ctypes.byref(loss),
+                                                    _fptr(g)))
+        return loss.value, g
+
+    def train(self, X, T, epochs, batch=4096, lr=2e-3, lr_end=1e-5, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = (c_double * epochs)()
+        _check(self.lib, self.lib.a3d_mlp_train(self._h, _fptr(X), _fptr(T), X.shape[0], epochs, batch, lr, lr_end,
+                                                seed, hist))
+        return list(hist)
+
+    def close(self):
+        if self._h:
+            self.lib.a3d_mlp_destroy(self._h)
             self._h = c_void_p()
 
     def __enter__(self):
@@ -287,6 +393,9 @@ class Batch:
         ms = c_double()
         _check(self.lib, self.lib.a3d_batch_run(self._h, mode, ptrs, ctypes.byref(ms)))
         return outs, ms.value
+
+    def set_graph_chunk(self, steps):
+        _check(self.lib, self.lib.a3d_batch_set_graph_chunk(self._h, steps))
 
     def info(self):
         nw = len(self.windows)

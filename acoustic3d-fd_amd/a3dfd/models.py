@@ -167,3 +167,29 @@ def uneven_windows(nshot, nx, ny, nz, seed=4, lo=(160, 160, 200), hi=(320, 320, 
         oy = int(np.clip(gy - wy // 2, 0, ny - wy))
         out.append((ox, oy, 0, wx, wy, wz, gx - ox + 1, gy - oy + 1, 1))
     return out
+
+
+def strip_model(seed, n=128, d=10.0, dt=8e-4, vmin=1500.0, vmax=4500.0):
+    """Random VTI model of the learned-strip study (n^3, d = 10 m, dt = 0.8 ms): 4-10 dipping
+    layers in [vmin, vmax] (dips 0-30 deg) times smooth +-5 % noise, Gardner density,
+    smooth epsilon in [0, 0.25] and delta in [0, 0.1] with delta <= epsilon, epsilon capped
+    for stability."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    nl = int(rng.integers(4, 11))
+    vel = dipping_layers(shape, d, seed, nl, vmin, vmax, 30.0)
+    vel = np.clip(vel * (1.0 + 0.1 * (smooth_field(shape, seed + 50) - 0.5)), vmin, vmax).astype(np.float32)
+    eps = 0.25 * smooth_field(shape, seed + 100)
+    dl = np.minimum(0.1 * smooth_field(shape, seed + 200), eps)
+    eps = cap_epsilon(vel, eps, dt, d, margin=0.95)
+    dl = np.minimum(dl, eps).astype(np.float32)
+    return dict(op="vti", nx=n, ny=n, nz=n, dx=d, dy=d, dz=d, dt=dt, vel=vel, rho=gardner(vel),
+                eps=eps.astype(np.float32), delta=dl)
+
+
+def strip_source(seed, n):
+    """Seeded source of the learned-strip study: 1-based interior position anywhere in the
+    volume (so waves reach the faces at all angles) and a Ricker peak frequency in 10-30 Hz."""
+    rng = np.random.default_rng(seed + 7)
+    sx, sy, sz = (int(v) for v in rng.integers(1, n + 1, 3))
+    return dict(sx=sx, sy=sy, sz=sz, fm=float(rng.uniform(10.0, 30.0)))

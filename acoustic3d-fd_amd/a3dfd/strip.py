@@ -1,0 +1,184 @@
+"""Learned absorbing strip of the VTI operator: training data, training, judge and fallback.
+
+The strip replaces the 20-cell PML by the 6 outer updated cells of every face
+(grid padding nb = STRIP_NB = 10 instead of 20), run with the reference's
+split-PML update at that width plus a correction from a small network after
+every stress sweep (the library's a3d_set_strip_net). This module:
+
+- `collect`: teacher-forced samples. A teacher shot with a wide PML (nb = 40)
+  runs on seeded random models; at sampled steps its state is copied onto the
+  strip grid, the strip grid's own stress sweep is applied, and the network's
+  target is what the teacher's next P, Q differ from it by, over the strip
+  cells (a3d_strip_collect).
+- `train`: fits the network with the library's GPU training kernels (a3d_mlp).
+- `reflection`: the judge. Reflected energy of a boundary = FP64 norm of
+  (P_run - P_extended) over the interior and all snapshot steps, over the norm
+  of P_extended, in dB; the extended run has the model padded by `margin`
+  cells per face and a 40-cell PML, computed in FP64 on the GPU.
+- `choose`: the fallback rule: the learned strip is used only when its
+  reflected energy on the held-out models is at most -40 dB, otherwise the
+  plain 20-cell PML.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import models
+from ._lib import STRIP_NB, Mlp, Modeller, make_desc, strip_collect
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NET_PATH = os.path.join(HERE, "strip_net.npz")
+TEACHER_NB = 40
+PML_NB = 20
+THRESHOLD_DB = -40.0
+NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 2 * 64 + 2   # W1 b1 W2 b2 W3 b3
+
+
+def _desc(m, src, nt, nb, storage=0, **kw):
+    return make_desc("vti", m["nx"], m["ny"], m["nz"], m["dx"], m["dy"], m["dz"], src["sx"], src["sy"], src["sz"], nt,
+                     m["dt"], src["fm"], nb=nb, storage=storage, **kw)
+
+
+def _set(mdl, m):
+    mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+
+
+def collect(seeds, n=128, nt=600, every=3, keep=0.02, device=0, log=None):
+    """Samples over seeded models (one source each): X (N, 16), T (N, 2), S (N,). Samples whose
+    local amplitude s is below 1e-4 of the largest s of their shot are dropped."""
+    Xs, Ts, Ss = [], [], []
+    teacher = student = None
+    try:
+        for k, seed in enumerate(seeds):
+            m = models.strip_model(seed, n)
+            src = models.strip_source(seed, n)
+            dt_ = _desc(m, src, nt, TEACHER_NB)
+            if teacher is None:
+                teacher = Modeller(dt_, device)
+                student = Modeller(_desc(m, src, nt, STRIP_NB), device)
+                student.set_strip_net(np.zeros(NPARAMS, np.float32))   # builds the strip cell list
+            _set(teacher, m)
+            _set(student, m)
+            X, T, S = strip_collect(teacher, student, dt_, every=every, start=0, keep=keep, seed=seed)
+            ok = S > 1e-4 * S.max() if S.size else S.astype(bool)
+            Xs.append(X[ok])
+            Ts.append(T[ok])
+            Ss.append(S[ok])
+            if log:
+                log(f"strip collect: model {k + 1}/{len(seeds)} seed {seed}: {int(ok.sum())} samples")
+    finally:
+        for mdl in (teacher, student):
+            if mdl is not None:
+                mdl.close()
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ss)
+
+
+def train(X, T, epochs=30, batch=4096, lr=2e-3, lr_end=2e-5, seed=0, init=None, device=0):
+    """Trains the network on the GPU; returns (params, loss per epoch)."""
+    with Mlp(device) as net:
+        net.set_params(init if init is not None else init_params(seed))
+        hist = net.train(X, T, epochs, batch=batch, lr=lr, lr_end=lr_end, seed=seed)
+        return net.params(), hist
+
+
+def init_params(seed):
+    """Glorot-uniform weights, zero biases, output layer scaled by 0.1 (same rule as the FP64 checker)."""
+    rng = np.random.default_rng(seed)
+    shapes = [(64, 16), (64,), (64, 64), (64,), (2, 64), (2,)]
+    parts = []
+    for i, s in enumerate(shapes):
+        if len(s) == 2:
+            lim = np.sqrt(6.0 / (s[0] + s[1])) * (0.1 if i == 4 else 1.0)
+            parts.append(rng.uniform(-lim, lim, size=s).ravel())
+        else:
+            parts.append(np.zeros(s))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def save(params, path=NET_PATH, **meta):
+    np.savez(path, params=np.asarray(params, np.float32), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_params(path=NET_PATH):
+    """The committed trained network, or None when absent."""
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as f:
+        return f["params"].astype(np.float32)
+
+
+def extended(m, src, margin):
+    """Model and source of the extended-domain run: every face padded by `margin` cells
+    (edge replication, as the boundary padding itself does)."""
+    pad = lambda a: np.ascontiguousarray(np.pad(a, margin, mode="edge"))   # noqa: E731
+    me = dict(m, nx=m["nx"] + 2 * margin, ny=m["ny"] + 2 * margin, nz=m["nz"] + 2 * margin,
+              vel=pad(m["vel"]), rho=pad(m["rho"]), eps=pad(m["eps"]), delta=pad(m["delta"]))
+    se = dict(src, sx=src["sx"] + margin, sy=src["sy"] + margin, sz=src["sz"] + margin)
+    return me, se
+
+
+def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, device=0):
+    """One shot under a boundary: 'strip' (learned, needs params), 'strip0' (the 6-cell strip
+    without the network), 'pml' (20-cell PML), 'extended' (FP64, model padded by margin, 40-cell
+    PML). Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original
+    interior."""
+    if boundary == "extended":
+        me, se = extended(m, src, margin)
+        d = _desc(me, se, nt, TEACHER_NB, storage=2, rec_z=margin, snap_interval=snap_every, snap_skip0=1)
+        with Modeller(d, device) as mdl:
+            _set(mdl, me)
+            shot, snaps, ms = mdl.run(nsnap_max=nt // snap_every)
+        c = slice(margin, margin + m["nx"])
+        return shot[margin:margin + m["ny"], c], snaps[:, margin:margin + m["ny"], c, margin:margin + m["nz"]], ms
+    nb = PML_NB if boundary == "pml" else STRIP_NB
+    d = _desc(m, src, nt, nb, snap_interval=snap_every, snap_skip0=1)
+    with Modeller(d, device) as mdl:
+        _set(mdl, m)
+        if boundary == "strip":
+            mdl.set_strip_net(params)
+        return mdl.run(nsnap_max=nt // snap_every)
+
+
+def energy_db(snaps, ref):
+    """10 log10 of sum (snaps - ref)^2 / sum ref^2, accumulated in FP64."""
+    num = den = 0.0
+    for a, b in zip(snaps, ref):
+        b64 = b.astype(np.float64)
+        num += float(np.sum((a.astype(np.float64) - b64) ** 2))
+        den += float(np.sum(b64 ** 2))
+    return 10.0 * np.log10(max(num, 1e-300) / den)
+
+
+def rel_l2(a, b):
+    b = b.astype(np.float64)
+    return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+
+
+def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, device=0, log=None):
+    """Reflected energy (dB) and shot-record rel. L2 against the extended FP64 run for the
+    learned strip, the strip without network and the 20-cell PML, on held-out models."""
+    rows = []
+    for seed in seeds:
+        m = models.strip_model(seed, n)
+        src = models.strip_source(seed, n)
+        t0 = time.time()
+        ref_shot, ref_snaps, _ = run_case(m, src, nt, snap_every, "extended", margin=margin, device=device)
+        row = {"seed": seed, "fm": round(src["fm"], 2), "src": [src["sx"], src["sy"], src["sz"]]}
+        for b in ("pml", "strip0", "strip"):
+            if b == "strip" and params is None:
+                continue
+            shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, device=device)
+            row[b] = {"reflected_db": round(energy_db(snaps, ref_snaps), 2),
+                      "shot_rel_l2": rel_l2(shot, ref_shot), "loop_ms": round(ms, 2)}
+        rows.append(row)
+        if log:
+            log(f"strip judge: seed {seed} ({time.time() - t0:.1f} s): {row}")
+    return rows
+
+
+def choose(rows, threshold_db=THRESHOLD_DB):
+    """'strip' when every held-out model's learned-strip reflection is <= threshold, else 'pml'."""
+    if not rows or any("strip" not in r for r in rows):
+        return "pml"
+    return "strip" if max(r["strip"]["reflected_db"] for r in rows) <= threshold_db else "pml"

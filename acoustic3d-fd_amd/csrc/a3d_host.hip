@@ -7,6 +7,7 @@
 // kernels, so no transcendental function runs on the GPU except the per-point
 // cos/sin of the TTI rotation.
 #include "a3d_internal.h"
+#include "a3d_mlp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -47,6 +48,12 @@ int fail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess)                                                           \
             return fail(A3D_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
     } while (0)
+
+}  // namespace
+
+int a3d::report_error(int code, const char* what, const char* detail) { return fail(code, "%s: %s", what, detail); }
+
+namespace {
 
 enum Vol {
     V_VP, V_RHO, V_EPS, V_DEL, V_TH, V_PH,
@@ -205,6 +212,12 @@ struct a3d_ctx {
     int ncu = 256;
     Timing tm;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // learned absorbing strip (VTI, FP32, nb = STRIP_NB): network parameters and strip cells
+    float* strip_net = nullptr;
+    int* strip_ids = nullptr;
+    int* strip_meta = nullptr;
+    int strip_n = 0;
+    float2* strip_delta = nullptr;
 };
 
 namespace {
@@ -474,6 +487,36 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     return A3D_OK;
 }
 
+StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
+    StripArgs a{};
+    a.g = c->g;
+    a.ids = c->strip_ids;
+    a.meta = c->strip_meta;
+    a.n = c->strip_n;
+    a.P = (const float*)c->vol[V_P];
+    a.Q = (const float*)c->vol[V_Q];
+    a.u = (const float*)c->vol[V_U];
+    a.v = (const float*)c->vol[V_V];
+    a.w = (const float*)c->vol[V_W];
+    a.vp = (const float*)c->vol[V_VP];
+    a.rho = (const float*)c->vol[V_RHO];
+    a.eps = (const float*)c->vol[V_EPS];
+    a.del = (const float*)c->vol[V_DEL];
+    a.cdt[0] = d->dt / d->dx;
+    a.cdt[1] = d->dt / d->dy;
+    a.cdt[2] = d->dt / d->dz;
+    return a;
+}
+
+// Strip correction after a stress sweep (learned absorbing strip).
+int strip_step(a3d_ctx* c, const StripArgs& a) {
+    HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->st));
+    HIP_TRY(launch_strip_apply(a, c->strip_delta, (float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_PX],
+                               (float*)c->vol[V_PY], (float*)c->vol[V_PZ], (float*)c->vol[V_QX], (float*)c->vol[V_QY],
+                               (float*)c->vol[V_QZ], c->st));
+    return A3D_OK;
+}
+
 template <class T>
 int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
                double* loop_ms) {
@@ -568,6 +611,8 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         std::vector<Src> srcs;
         if ((rc = prepare_first<T>(c, d, sc, va, sa, srcs))) return rc;
         const int op = d->op;
+        const bool strip = c->strip_net != nullptr;
+        const StripArgs stp = strip_args(c, d);
         HIP_TRY(hipEventRecord(c->ev0, c->st));
         for (int it = 0; it < nt; it++) {
             va.s = srcs[it];
@@ -578,6 +623,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             timing_begin(c, 1);
             HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
             timing_end(c);
+            if (strip && (rc = strip_step(c, stp))) return rc;
             if (want_snap(it)) {
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
             }
@@ -870,6 +916,8 @@ int a3d_destroy(a3d_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->dtab) (void)hipFree(c->dtab);
     if (c->flag) (void)hipFree(c->flag);
+    for (void* p : {(void*)c->strip_net, (void*)c->strip_ids, (void*)c->strip_meta, (void*)c->strip_delta})
+        if (p) (void)hipFree(p);
     for (auto e : c->tm.pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1017,6 +1065,131 @@ int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, do
     (void)hipFree(a);
     (void)hipFree(b);
     return rc;
+}
+
+int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (!params) {
+        if (c->strip_net) (void)hipFree(c->strip_net);
+        c->strip_net = nullptr;
+        return A3D_OK;
+    }
+    if (n != LB_NPAR) return fail(A3D_EINVAL, "strip network: %d parameters expected, got %d", LB_NPAR, n);
+    if (c->d.op != A3D_OP_VTI || c->prec != PREC_FP32)
+        return fail(A3D_EINVAL, "the learned strip is built for the VTI operator with FP32 storage");
+    if (c->d.nb != STRIP_NB) return fail(A3D_EINVAL, "the learned strip needs nb = %d (got %d)", STRIP_NB, c->d.nb);
+    if (!c->strip_ids) {
+        // strip cells: updated cells within STRIP_CELLS of the updated region's faces
+        const Geo g = c->g;
+        std::vector<int> ids, meta;
+        const int n3[3] = {g.nnx, g.nny, g.nnz};
+        for (int z = MM; z < g.nnz - MM; ++z)
+            for (int y = MM; y < g.nny - MM; ++y)
+                for (int x = MM; x < g.nnx - MM; ++x) {
+                    const int p[3] = {x, y, z};
+                    int best = 1 << 30, axis = 0, side = 0;
+                    for (int a = 2; a >= 0; --a) {   // ties go to z, then y, then x
+                        const int lo = p[a] - MM + 1, hi = n3[a] - MM - p[a];
+                        if (lo < best) best = lo, axis = a, side = 0;
+                        if (hi < best) best = hi, axis = a, side = 1;
+                    }
+                    if (best > STRIP_CELLS) continue;
+                    ids.push_back(x + g.ld * y + g.pl * z);
+                    meta.push_back(axis | side << 2 | best << 3);
+                }
+        const size_t nb = ids.size() * sizeof(int);
+        if (hipMalloc(&c->strip_ids, nb) != hipSuccess || hipMalloc(&c->strip_meta, nb) != hipSuccess ||
+            hipMalloc(&c->strip_delta, ids.size() * sizeof(float2)) != hipSuccess) {
+            for (void** q : {(void**)&c->strip_ids, (void**)&c->strip_meta, (void**)&c->strip_delta}) {
+                if (*q) (void)hipFree(*q);
+                *q = nullptr;
+            }
+            return fail(A3D_ENOMEM, "strip cell list allocation failed");
+        }
+        HIP_TRY(hipMemcpy(c->strip_ids, ids.data(), nb, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->strip_meta, meta.data(), nb, hipMemcpyHostToDevice));
+        c->strip_n = (int)ids.size();
+    }
+    if (!c->strip_net && hipMalloc(&c->strip_net, LB_NPAR * sizeof(float)) != hipSuccess) {
+        c->strip_net = nullptr;
+        return fail(A3D_ENOMEM, "strip network allocation failed");
+    }
+    HIP_TRY(hipMemcpy(c->strip_net, params, LB_NPAR * sizeof(float), hipMemcpyHostToDevice));
+    return A3D_OK;
+}
+
+int a3d_strip_info(const a3d_ctx* c, int* ncells) {
+    if (!c || !ncells) return fail(A3D_EINVAL, "null argument");
+    *ncells = c->strip_n;
+    return A3D_OK;
+}
+
+int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every, int start, float keep,
+                      unsigned seed, float* X, float* T, float* S, int cap, int* count) {
+    if (!t || !s || !d || !X || !T || !S || !count || cap <= 0 || every <= 0 || !(keep > 0))
+        return fail(A3D_EINVAL, "bad argument");
+    if (t->d.op != A3D_OP_VTI || s->d.op != A3D_OP_VTI || t->prec != PREC_FP32 || s->prec != PREC_FP32)
+        return fail(A3D_EINVAL, "strip sampling: VTI contexts with FP32 storage");
+    if (s->d.nx != t->d.nx || s->d.ny != t->d.ny || s->d.nz != t->d.nz || s->d.dx != t->d.dx ||
+        s->d.dy != t->d.dy || s->d.dz != t->d.dz || t->d.nb <= s->d.nb)
+        return fail(A3D_EINVAL, "strip sampling: same interior, teacher boundary wider than the student's");
+    if (!t->model_set || !s->model_set) return fail(A3D_ESTATE, "model not set");
+    if (!s->strip_ids) return fail(A3D_ESTATE, "student strip cells not built (set a strip network first)");
+    int rc = check_desc_shot(t, d);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(t->device));
+    a3d_shot_desc ds = *d;
+    ds.nb = s->d.nb;
+    VelArgs vt, vs;
+    StressArgs stt, sts;
+    std::vector<Src> srcs, srcs_s;
+    if ((rc = prepare_first<float>(t, d, field_scales(t, d), vt, stt, srcs))) return rc;
+    if ((rc = prepare_first<float>(s, &ds, field_scales(s, &ds), vs, sts, srcs_s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    stt.rec = nullptr;
+    sts.rec = nullptr;
+    const StripArgs a = strip_args(s, &ds);
+    float *dX = nullptr, *dT = nullptr, *dS = nullptr;
+    int* dn = nullptr;
+    auto cleanup = [&](int r) {
+        (void)hipStreamSynchronize(t->st);
+        for (void* q : {(void*)dX, (void*)dT, (void*)dS, (void*)dn})
+            if (q) (void)hipFree(q);
+        return r;
+    };
+    if (hipMalloc(&dX, (size_t)cap * LB_NIN * sizeof(float)) != hipSuccess ||
+        hipMalloc(&dT, (size_t)cap * LB_NOUT * sizeof(float)) != hipSuccess ||
+        hipMalloc(&dS, (size_t)cap * sizeof(float)) != hipSuccess || hipMalloc(&dn, sizeof(int)) != hipSuccess)
+        return cleanup(fail(A3D_ENOMEM, "sample buffer allocation failed"));
+    HIP_TRY(hipMemsetAsync(dn, 0, sizeof(int), t->st));
+    const int off = t->d.nb - s->d.nb;
+    const unsigned keep_u = (unsigned)std::min(4294967295.0, (double)keep * 4294967296.0);
+    const int fields[] = {V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
+    for (int it = 0; it < d->nt; ++it) {
+        vt.s = srcs[it];
+        HIP_TRY(launch_vel(PREC_FP32, A3D_OP_VTI, t->ty[0], vt, t->st));
+        const bool samp = it >= start && (it - start) % every == 0;
+        if (samp) {
+            for (int v : fields)
+                HIP_TRY(launch_strip_copy((const float*)t->vol[v], t->g, (float*)s->vol[v], s->g, off, t->st));
+            HIP_TRY(launch_stress(PREC_FP32, A3D_OP_VTI, s->ty[1], sts, t->st));
+        }
+        HIP_TRY(launch_stress(PREC_FP32, A3D_OP_VTI, t->ty[1], stt, t->st));
+        if (samp)
+            HIP_TRY(launch_strip_sample(a, (const float*)t->vol[V_P], (const float*)t->vol[V_Q], t->g, off, keep_u, seed,
+                                        (unsigned)it, cap, dX, dT, dS, dn, t->st));
+    }
+    int hn = 0;
+    HIP_TRY(hipMemcpyAsync(&hn, dn, sizeof(int), hipMemcpyDeviceToHost, t->st));
+    HIP_TRY(hipStreamSynchronize(t->st));
+    const int n = std::min(hn, cap);
+    HIP_TRY(hipMemcpy(X, dX, (size_t)n * LB_NIN * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(T, dT, (size_t)n * LB_NOUT * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(S, dS, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    *count = n;
+    return cleanup(A3D_OK);
 }
 
 int a3d_grid_info(const a3d_ctx* c, int* nnx, int* nny, int* nnz, long long* updated) {
@@ -1302,8 +1475,9 @@ struct a3d_batch {
     bool tuned = false;
     // state of the captured time loop, kept so later runs replay the same graph
     std::vector<BatchGroup> gv, gs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraph_t> graph;        // the time loop in chunks of graph_chunk steps
+    std::vector<hipGraphExec_t> exec;
+    int graph_chunk = 0;                  // 0: one graph for the whole loop
     long long graph_launches = 0;
 };
 
@@ -1370,10 +1544,10 @@ void drop_graph(a3d_batch* b) {
     free_groups(b->gs);
     b->gv.clear();
     b->gs.clear();
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (auto e : b->exec) (void)hipGraphExecDestroy(e);
+    for (auto g : b->graph) (void)hipGraphDestroy(g);
+    b->exec.clear();
+    b->graph.clear();
 }
 
 template <class T>
@@ -1396,7 +1570,8 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     }
     // the wavefield memsets of prepare_first ran on the windows' streams
     for (auto* c : b->ctx) HIP_TRY(hipStreamSynchronize(c->st));
-    const bool replay = mode == 2 && b->exec != nullptr;
+    const bool replay = mode == 2 && !b->exec.empty();
+    const bool progress = getenv("A3D_PROGRESS") != nullptr;
     std::vector<BatchGroup> gv_local, gs_local;
     std::vector<BatchGroup>& gv = mode == 2 ? b->gv : gv_local;
     std::vector<BatchGroup>& gs = mode == 2 ? b->gs : gs_local;
@@ -1443,19 +1618,33 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
         for (size_t k = 0; k < gv.size() + gs.size(); ++k) group_ms[k] = 0.0;
     }
     if (mode == 2) {
+        const int chunk = b->graph_chunk > 0 ? std::min(b->graph_chunk, nt) : nt;
         if (!replay) {
-            // the whole time loop as one graph; kept and replayed by later runs
-            HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-            for (int it = 0; it < nt && !rc; ++it) rc = step(it);
-            hipError_t e = hipStreamEndCapture(b->st, &b->graph);
-            if (rc) return cleanup(rc);
-            if (e != hipSuccess) return cleanup(fail(A3D_EHIP, "graph capture failed: %s", hipGetErrorString(e)));
-            if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
-                return cleanup(fail(A3D_EHIP, "graph instantiation failed"));
+            // the time loop captured as graphs of `chunk` steps, kept and replayed by later runs
+            for (int i0 = 0; i0 < nt && !rc; i0 += chunk) {
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ex = nullptr;
+                HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                for (int it = i0; it < std::min(nt, i0 + chunk) && !rc; ++it) rc = step(it);
+                hipError_t e = hipStreamEndCapture(b->st, &gr);
+                if (gr) b->graph.push_back(gr);
+                if (rc) return cleanup(rc);
+                if (e != hipSuccess) return cleanup(fail(A3D_EHIP, "graph capture failed: %s", hipGetErrorString(e)));
+                if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess)
+                    return cleanup(fail(A3D_EHIP, "graph instantiation failed"));
+                b->exec.push_back(ex);
+            }
         }
         HIP_TRY(hipEventRecord(b->ev0, b->st));
-        HIP_TRY(hipGraphLaunch(b->exec, b->st));
-        ++b->graph_launches;
+        for (size_t k = 0; k < b->exec.size(); ++k) {
+            HIP_TRY(hipGraphLaunch(b->exec[k], b->st));
+            ++b->graph_launches;
+            if (progress) {
+                HIP_TRY(hipStreamSynchronize(b->st));
+                fprintf(stderr, "a3d: batch graph %zu/%zu done (%d steps)\n", k + 1, b->exec.size(),
+                        (int)std::min((size_t)nt, (k + 1) * (size_t)chunk));
+            }
+        }
     } else {
         HIP_TRY(hipEventRecord(b->ev0, b->st));
         for (int it = 0; it < nt; ++it) {
@@ -1466,6 +1655,10 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
                     float ms = 0.f;
                     if (hipEventElapsedTime(&ms, gev[2 * k], gev[2 * k + 1]) == hipSuccess) group_ms[k] += ms;
                 }
+            }
+            if (progress && (it + 1) % 250 == 0) {
+                HIP_TRY(hipStreamSynchronize(b->st));
+                fprintf(stderr, "a3d: batch step %d/%d done\n", it + 1, nt);
             }
         }
     }
@@ -1630,6 +1823,7 @@ int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms) {
             if (!rc) rc = run_shot(b->ctx[w], &b->wd[w], shots ? shots[w] : nullptr, SnapFn(), nullptr, &ms);
             if (rc) return rc;
             total += ms;
+            if (getenv("A3D_PROGRESS")) fprintf(stderr, "a3d: window %zu/%zu done (%.1f ms)\n", w + 1, b->ctx.size(), ms);
         }
         if (loop_ms) *loop_ms = total;
         return A3D_OK;
@@ -1711,6 +1905,13 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
     b->d.nt = saved_nt;
     b->tuned = rc == A3D_OK;
     return rc;
+}
+
+int a3d_batch_set_graph_chunk(a3d_batch* b, int steps) {
+    if (!b || steps < 0) return fail(A3D_EINVAL, "bad argument");
+    if (steps != b->graph_chunk) drop_graph(b);
+    b->graph_chunk = steps;
+    return A3D_OK;
 }
 
 int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out) {

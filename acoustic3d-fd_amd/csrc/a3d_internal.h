@@ -17,6 +17,10 @@
 namespace a3d {
 
 constexpr int MM = 4;    // half stencil length (8th order)
+
+// Records "what: detail" as the library's last error message and returns code
+// (defined with the rest of the error handling in a3d_host.hip).
+int report_error(int code, const char* what, const char* detail);
 constexpr int TX = 64;   // tile width: one wavefront per tile row
 
 struct Geo {
@@ -163,6 +167,34 @@ hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const
 
 // Calibration streams: float4 copy a -> b, or read-only reduction of a (b: 64 B scratch).
 hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, int grid, hipStream_t st);
+
+// Learned absorbing strip (VTI, FP32): the outer STRIP_CELLS updated cells of
+// every face; the grid's boundary padding is STRIP_CELLS + MM.
+constexpr int STRIP_CELLS = 6;
+constexpr int STRIP_NB = STRIP_CELLS + MM;
+
+// Strip cells of one grid: padded linear index and meta = axis | side << 2 | depth << 3
+// (axis of the nearest face, side 1 for the high face, depth 1 = outermost updated cell).
+struct StripArgs {
+    Geo g;
+    const int* ids;
+    const int* meta;
+    int n;
+    const float *P, *Q, *u, *v, *w, *vp, *rho, *eps, *del;
+    float cdt[3];   // dt/dx, dt/dy, dt/dz
+};
+
+// delta[k] = s * net(features) for every strip cell (after the stress sweep).
+hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, hipStream_t st);
+// Adds delta to P, Q and to the strip cell's normal-axis split components.
+hipError_t launch_strip_apply(const StripArgs& a, const float2* delta, float* P, float* Q, float* px, float* py,
+                              float* pz, float* qx, float* qy, float* qz, hipStream_t st);
+// Appends teacher-forced samples (features, (teacher - strip state) / s, s).
+hipError_t launch_strip_sample(const StripArgs& a, const float* Pt, const float* Qt, const Geo& gt, int off,
+                               unsigned keep, unsigned seed, unsigned tag, int cap, float* X, float* T, float* S,
+                               int* count, hipStream_t st);
+// Copies a teacher field onto the student grid's updated region (index shift off).
+hipError_t launch_strip_copy(const float* src, const Geo& gs, float* dst, const Geo& gd, int off, hipStream_t st);
 
 // Sets *flag to 1 if any of the n values is not finite.
 hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st);

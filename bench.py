@@ -113,6 +113,48 @@ def precision_study(cfg):
     print(json.dumps(res), flush=True)
 
 
+def batched(args):
+    """64 TTI shots with uneven aperture windows cut from one 600x600x300 model, nt=2500,
+    40-cell PML: 64 sequential per-window runs (mode 0) against the batched sweeps with the
+    time loop captured in hipGraphs (mode 2). Prints shots/hour for both."""
+    from a3dfd import Batch, Window, make_desc, models
+    os.environ.setdefault("A3D_PROGRESS", "1")
+    t0 = time.time()
+    cfg = models.cfg4_model()
+    nt = args.nt if args.nt != 3000 else cfg["nt"]
+    wins = models.uneven_windows(args.nwin, cfg["nx"], cfg["ny"], cfg["nz"])
+    log(f"bench: TTI model and {len(wins)} windows generated in {time.time() - t0:.1f} s")
+    desc = make_desc("tti", cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dy"], cfg["dz"], 1, 1, 1, nt,
+                     cfg["dt"], cfg["fm"])
+    res = {}
+    with Batch(desc, [Window(*w) for w in wins]) as b:
+        b.set_model(vel=cfg["vel"], eps=cfg["eps"], delta=cfg["delta"], theta=cfg["theta"], phi=cfg["phi"])
+        t1 = time.time()
+        ncls = b.tune()
+        info = b.info()
+        log(f"bench: tuned {ncls} size classes in {time.time() - t1:.1f} s; updated points/step {info['updated_points']}")
+        b.set_graph_chunk(250)
+        for mode in (0, 2):
+            t1 = time.perf_counter()
+            _, ms = b.run(mode=mode, shots=False)
+            wall = time.perf_counter() - t1
+            res[mode] = dict(loop_s=round(ms / 1e3, 2), wall_s=round(wall, 2),
+                             shots_per_hour=round(len(wins) * 3600e3 / ms, 1),
+                             Gpt_s=round(info["updated_points"] * nt / ms / 1e6, 2))
+            log(f"bench: mode {mode}: {res[mode]}")
+    line = {
+        "metric": "shots/hour, 64 uneven-window TTI shots (batched hipGraph vs sequential per-window runs)",
+        "value": res[2]["shots_per_hour"], "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic seeded TTI model (layered + noise, smooth eps/delta/tilt/azimuth)",
+        "config": {"workload": f"tti_600x600x300_{len(wins)}win_nt{nt}", "npml": 40, "nt": nt,
+                   "windows": "uniform in [160..320]^2 x [200..300] cells, seed 4",
+                   "updated_points_per_step": info["updated_points"], "size_classes": ncls},
+        "sequential": res[0], "batched_graph": res[2],
+        "speedup": round(res[0]["loop_s"] / res[2]["loop_s"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +170,9 @@ def main():
                     help="0: split PML fields everywhere (bitwise reference arithmetic); 1: split only in PML cells")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run the calibration streams (2 GiB float4 copy and read) and print GB/s")
+    ap.add_argument("--batched", action="store_true",
+                    help="64 uneven-window TTI shots: sequential per-window runs vs the batched hipGraph form")
+    ap.add_argument("--nwin", type=int, default=64, help="windows of the batched run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every shot on a single MI355X "
@@ -143,6 +188,8 @@ def main():
                           "copy_GBs_read_plus_write": round(c, 1), "read_only_GBs": round(r, 1)}), flush=True)
         return
 
+    if args.batched:
+        return batched(args)
     t0 = time.time()
     cfg = models.cfg3()
     cfg["nt"] = args.nt

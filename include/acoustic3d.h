@@ -210,9 +210,49 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses);
  * window) or NULL; entries may be NULL. *loop_ms: time-loop duration. Results
  * are bitwise identical in all modes. */
 int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms);
+/* Mode 2 captures the time loop in graphs of `steps` time steps each (0, the
+ * default: one graph for the whole loop); with the environment variable
+ * A3D_PROGRESS set, a line is printed to stderr after each graph completes. */
+int a3d_batch_set_graph_chunk(a3d_batch* b, int steps);
 /* Updated points per step summed over windows; window count; tile rows per
  * window (2 ints each: velocity, stress), may be NULL. */
 int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out);
+
+/* ----------------------------------------------------------------------- */
+/* Learned absorbing strip (no counterpart in the reference): VTI operator,  */
+/* FP32 storage, nb = 10 (6 updated strip cells per face plus the 4-cell     */
+/* zero ring). After every stress sweep a network (16 inputs, two hidden    */
+/* tanh layers of 64, 2 outputs; 5378 FP32 parameters) corrects P, Q and    */
+/* the normal-axis split components of each strip cell.                     */
+/* ----------------------------------------------------------------------- */
+#define A3D_STRIP_NB 10
+/* params: a3d_mlp_nparams() floats, or NULL to run the plain PML of width nb. */
+int a3d_set_strip_net(a3d_ctx* ctx, const float* params, int n);
+/* Number of strip cells of the context's grid (0 before a network was set). */
+int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
+/* Teacher-forced training samples: runs shot *desc on `teacher` (same interior,
+ * wider PML); at steps start, start+every, ... the teacher's state is copied onto
+ * `student` (nb = A3D_STRIP_NB, network set), the student's stress sweep is applied,
+ * and a fraction `keep` of the strip cells yields (features X[16], target
+ * T[2] = (teacher P, Q - student P, Q) / s, scale S = s). *count: samples written (<= cap). */
+int a3d_strip_collect(a3d_ctx* teacher, a3d_ctx* student, const a3d_shot_desc* desc, int every, int start,
+                      float keep, unsigned seed, float* X, float* T, float* S, int cap, int* count);
+
+/* The network on the GPU: forward, gradient and Adam training kernels. */
+typedef struct a3d_mlp a3d_mlp;
+int a3d_mlp_nparams(void);
+int a3d_mlp_create(int device, a3d_mlp** out);
+int a3d_mlp_destroy(a3d_mlp* m);
+int a3d_mlp_set_params(a3d_mlp* m, const float* params, int n);
+int a3d_mlp_get_params(a3d_mlp* m, float* params, int n);
+/* Y[n][2] = net(X[n][16]). */
+int a3d_mlp_predict(a3d_mlp* m, const float* X, int n, float* Y);
+/* Mean over samples of the summed squared error and its gradient (FP32). */
+int a3d_mlp_loss_grad(a3d_mlp* m, const float* X, const float* T, int n, double* loss, float* grad);
+/* Adam over shuffled minibatches, learning rate cosine-decayed from lr to lr_end;
+ * loss_per_epoch (may be NULL): mean training loss of each epoch. */
+int a3d_mlp_train(a3d_mlp* m, const float* X, const float* T, int n, int epochs, int batch, float lr, float lr_end,
+                  unsigned seed, double* loss_per_epoch);
 
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);

@@ -61,6 +61,10 @@ def load():
     lib.oracle_taper.argtypes = [c_int, POINTER(c_double)]
     lib.oracle_staggered_dx.argtypes = [fp, c_int, c_int, fp]
     lib.oracle_laplacian_z.argtypes = [fp, c_int, fp]
+    lib.oracle_set_strip_net.argtypes = [fp, c_int]
+    lib.oracle_set_strip_net.restype = c_int
+    lib.oracle_strip_apply.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float, c_float] + [fp] * 15
+    lib.oracle_strip_apply.restype = c_int
     _lib = lib
     return lib
 
@@ -112,6 +116,30 @@ def run(shot, prec=32, vel=None, rho=None, eps=None, delta=None, theta=None, phi
     if snaps is not None:
         snaps = snaps[: min(n, nsnap_max)]
     return out, snaps, secs.value
+
+
+def set_strip_net(params):
+    """Learned absorbing strip of the FP32 VTI instance (nb = 10), or None to switch it off."""
+    lib = load()
+    if params is None:
+        lib.oracle_set_strip_net(None, 0)
+        return
+    p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+    if lib.oracle_set_strip_net(_fp(p), p.size) != 0:
+        raise ValueError(f"strip network: wrong parameter count {p.size}")
+
+
+def strip_apply(dt, dx, dy, dz, f):
+    """One strip correction in place on the padded float32 arrays of dict f (shape (nny, nnx, nnz),
+    z fastest): P, Q, u, v, w, px, py, pz, qx, qy, qz, vp, rho, eps, del."""
+    lib = load()
+    names = ("P", "Q", "u", "v", "w", "px", "py", "pz", "qx", "qy", "qz", "vp", "rho", "eps", "del")
+    for k in names:
+        if f[k].dtype != np.float32 or not f[k].flags.c_contiguous:
+            raise ValueError(f"{k}: contiguous float32 expected")
+    nny, nnx, nnz = f["P"].shape
+    if lib.oracle_strip_apply(nnx, nny, nnz, dt, dx, dy, dz, *[_fp(f[k]) for k in names]) != 0:
+        raise RuntimeError("no strip network set")
 
 
 def time_steps(shot, nsteps, vel=None, rho=None, eps=None, delta=None, theta=None, phi=None):

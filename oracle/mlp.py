@@ -1,0 +1,72 @@
+"""TEST INFRASTRUCTURE, not product code: FP64 restatement of the absorbing-strip
+network, used only by tests/ as the checker of the GPU training kernels.
+
+The network has no counterpart in the reference. Definition: x in R^16,
+h1 = tanh(W1 x + b1), h2 = tanh(W2 h1 + b2), y = W3 h2 + b3 (hidden 64, y in R^2),
+one flat FP32 parameter vector W1[64][16] b1[64] W2[64][64] b2[64] W3[2][64] b3[2];
+loss = mean over samples of sum over outputs of (y - t)^2.
+
+The gradient below is hand-written backpropagation; tests pin it against central
+finite differences of `loss` (which only needs the forward pass).
+"""
+import numpy as np
+
+NIN, NH, NOUT = 16, 64, 2
+SHAPES = [("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)), ("W3", (NOUT, NH)), ("b3", (NOUT,))]
+NPAR = sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def unpack(p):
+    p = np.asarray(p, dtype=np.float64).ravel()
+    if p.size != NPAR:
+        raise ValueError(f"{p.size} parameters, expected {NPAR}")
+    out, k = {}, 0
+    for name, shape in SHAPES:
+        n = int(np.prod(shape))
+        out[name] = p[k:k + n].reshape(shape)
+        k += n
+    return out
+
+
+def init(seed, scale=1.0):
+    """Seeded initial parameters: Glorot-uniform weights, zero biases, output layer times 0.1."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in SHAPES:
+        if name.startswith("W"):
+            lim = np.sqrt(6.0 / (shape[0] + shape[1])) * scale * (0.1 if name == "W3" else 1.0)
+            parts.append(rng.uniform(-lim, lim, size=shape).ravel())
+        else:
+            parts.append(np.zeros(shape).ravel())
+    return np.concatenate(parts).astype(np.float32)
+
+
+def forward(p, X):
+    w = unpack(p)
+    X = np.asarray(X, dtype=np.float64)
+    h1 = np.tanh(X @ w["W1"].T + w["b1"])
+    h2 = np.tanh(h1 @ w["W2"].T + w["b2"])
+    y = h2 @ w["W3"].T + w["b3"]
+    return y, (X, h1, h2)
+
+
+def loss(p, X, T):
+    y, _ = forward(p, X)
+    return float(np.sum((y - np.asarray(T, np.float64)) ** 2) / len(y))
+
+
+def loss_grad(p, X, T):
+    """Loss and its gradient (flat, FP64) by backpropagation."""
+    w = unpack(p)
+    y, (X, h1, h2) = forward(p, X)
+    n = len(y)
+    e = y - np.asarray(T, np.float64)
+    dy = 2.0 * e / n
+    g = {"W3": dy.T @ h2, "b3": dy.sum(0)}
+    d2 = (dy @ w["W3"]) * (1.0 - h2 ** 2)
+    g["W2"] = d2.T @ h1
+    g["b2"] = d2.sum(0)
+    d1 = (d2 @ w["W2"]) * (1.0 - h1 ** 2)
+    g["W1"] = d1.T @ X
+    g["b1"] = d1.sum(0)
+    return float(np.sum(e ** 2) / n), np.concatenate([g[name].ravel() for name, _ in SHAPES])

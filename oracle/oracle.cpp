@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -382,6 +383,95 @@ int run_2nd(const OracleShot& p, const float* vel, T* shot, T* snaps, int nsnap_
 /* 1st-order family: cuda_3dfd_1order (iso:372-671), cuda_3dfd_vti          */
 /* (vti:389-688), cuda_3dfd_tti (tti:390-676)                               */
 /* ------------------------------------------------------------------------ */
+/* Learned absorbing strip (no counterpart in the reference; restated from the
+ * strip's definition, FP32 only). Grid padding nb = 10: the 6 outer updated
+ * cells of every face are strip cells, each owned by its nearest face (ties:
+ * z, then y, then x). After every stress sweep each strip cell gets
+ *   (dP, dQ) = s * net(f),  f = local state / s,  s = max(1e-30, |local state|),
+ * added to P, Q and to the split components of the owning face's axis. Local
+ * state: P at the centre, one cell inward, one outward and the two tangential
+ * pairs; Q at the centre, inward, outward; the outward normal velocity (mean of
+ * the two staggered samples around the cell) times rho*vp. Then vp*dt/d_axis,
+ * epsilon, delta, depth/6 (1 = outermost), 0. Network: 16 -> 64 -> 64 -> 2,
+ * tanh, parameters W1[64][16] b1 W2[64][64] b2 W3[2][64] b3, sums in input
+ * order with fused multiply-adds. */
+constexpr int kStripCells = 6, kNin = 16, kNh = 64, kNout = 2;
+constexpr int kW1 = 0, kB1 = kW1 + kNh * kNin, kW2 = kB1 + kNh, kB2 = kW2 + kNh * kNh, kW3 = kB2 + kNh,
+              kB3 = kW3 + kNout * kNh, kNpar = kB3 + kNout;
+std::vector<float> g_strip;   /* empty: plain PML */
+
+void strip_net(const float* p, const float* x, float* y) {
+    float h1[kNh], h2[kNh];
+    for (int j = 0; j < kNh; ++j) {
+        float a = p[kB1 + j];
+        for (int i = 0; i < kNin; ++i) a = std::fma(p[kW1 + j * kNin + i], x[i], a);
+        h1[j] = std::tanh(a);
+    }
+    for (int j = 0; j < kNh; ++j) {
+        float a = p[kB2 + j];
+        for (int i = 0; i < kNh; ++i) a = std::fma(p[kW2 + j * kNh + i], h1[i], a);
+        h2[j] = std::tanh(a);
+    }
+    for (int o = 0; o < kNout; ++o) {
+        float a = p[kB3 + o];
+        for (int i = 0; i < kNh; ++i) a = std::fma(p[kW3 + o * kNh + i], h2[i], a);
+        y[o] = a;
+    }
+}
+
+/* Oracle layout: z fastest, id = iz + nnz*ix + pl*iy. */
+void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
+                   const float* u, const float* v, const float* w, float* px, float* py, float* pz, float* qx,
+                   float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del) {
+    const long pl = (long)nnz * nnx;
+    const long stride[3] = {nnz, pl, 1};   /* x, y, z */
+    const int n3[3] = {nnx, nny, nnz};
+    const float cdt[3] = {dt / dx, dt / dy, dt / dz};
+    struct Cell { long id; int axis; float dp, dq; };
+    std::vector<std::vector<Cell>> rows(nny);
+#pragma omp parallel for schedule(static)
+    for (int iy = mm; iy < nny - mm; ++iy)
+        for (int ix = mm; ix < nnx - mm; ++ix)
+            for (int iz = mm; iz < nnz - mm; ++iz) {
+                const int pp[3] = {ix, iy, iz};
+                int best = 1 << 30, axis = 0, side = 0;
+                for (int a = 2; a >= 0; --a) {
+                    const int lo = pp[a] - mm + 1, hi = n3[a] - mm - pp[a];
+                    if (lo < best) { best = lo; axis = a; side = 0; }
+                    if (hi < best) { best = hi; axis = a; side = 1; }
+                }
+                if (best > kStripCells) continue;
+                const long id = iz + (long)nnz * ix + pl * iy;
+                const long sa = stride[axis], sn = side ? sa : -sa;
+                const long s1 = axis == 0 ? stride[1] : stride[0];
+                const long s2 = axis == 2 ? stride[1] : stride[2];
+                const float* vel = axis == 0 ? u : (axis == 1 ? v : w);
+                float f[11] = {P[id], P[id - sn], P[id + sn], P[id - s1], P[id + s1], P[id - s2], P[id + s2],
+                               Q[id], Q[id - sn], Q[id + sn], 0.f};
+                const float un = 0.5f * (vel[id] + vel[id - sa]);
+                f[10] = (side ? un : -un) * (rho[id] * vp[id]);
+                float s = 1e-30f;
+                for (int k = 0; k < 11; ++k) s = std::max(s, std::fabs(f[k]));
+                const float inv = 1.0f / s;
+                float x[kNin], y[kNout];
+                for (int k = 0; k < 11; ++k) x[k] = f[k] * inv;
+                x[11] = vp[id] * cdt[axis];
+                x[12] = eps[id];
+                x[13] = del[id];
+                x[14] = (float)best * (1.0f / kStripCells);
+                x[15] = 0.f;
+                strip_net(g_strip.data(), x, y);
+                rows[iy].push_back(Cell{id, axis, s * y[0], s * y[1]});
+            }
+    for (const auto& r : rows)
+        for (const Cell& c : r) {
+            P[c.id] += c.dp;
+            Q[c.id] += c.dq;
+            (c.axis == 0 ? px : (c.axis == 1 ? py : pz))[c.id] += c.dp;
+            (c.axis == 0 ? qx : (c.axis == 1 ? qy : qz))[c.id] += c.dq;
+        }
+}
+
 template <class T>
 int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const float* eps_in,
             const float* del_in, const float* th_in, const float* ph_in, T* shot, T* snaps,
@@ -557,6 +647,12 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                     Pp[id] = px[id] + py[id] + pz[id];
                     Qp[id] = qx[id] + qy[id] + qz[id];
                 }
+        if constexpr (std::is_same<T, float>::value) {
+            if (vti && !g_strip.empty())
+                strip_correct(nnx, nny, nnz, p.dt, p.dx, p.dy, p.dz, Pp, Qp, u.data(), v.data(), w.data(),
+                              px.data(), py.data(), pz.data(), qx.data(), qy.data(), qz.data(), vp.data(),
+                              rho.data(), eps.data(), del.data());
+        }
         /* shot_record (iso:294-304): interior z = rec_z (reference: 0) */
         if (shot) {
             for (int iy = 0; iy < ny; ++iy)
@@ -647,6 +743,24 @@ int oracle_run(int prec, const OracleShot* p, const float* vel, const float* rho
         return dispatch<double>(p, vel, rho, eps, del, theta, phi, (double*)shot, (double*)snaps, nsnap_max,
                                 loop_seconds, 0);
     return -8;
+}
+
+int oracle_set_strip_net(const float* params, int n) {
+    if (!params) {
+        g_strip.clear();
+        return 0;
+    }
+    if (n != kNpar) return -1;
+    g_strip.assign(params, params + n);
+    return 0;
+}
+
+int oracle_strip_apply(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
+                       const float* u, const float* v, const float* w, float* px, float* py, float* pz, float* qx,
+                       float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del) {
+    if (g_strip.empty()) return -1;
+    strip_correct(nnx, nny, nnz, dt, dx, dy, dz, P, Q, u, v, w, px, py, pz, qx, qy, qz, vp, rho, eps, del);
+    return 0;
 }
 
 double oracle_time_steps(const OracleShot* p, const float* vel, const float* rho, const float* eps,

@@ -51,6 +51,18 @@ int oracle_run(int prec, const OracleShot* p,
                const float* del, const float* theta, const float* phi,
                void* shot, void* snaps, int nsnap_max, double* loop_seconds);
 
+/* Learned absorbing strip of the FP32 VTI instance (grid padding nb = 10):
+ * the network's 5378 parameters, or NULL to switch it off. Returns 0, or -1
+ * for a wrong parameter count. */
+int oracle_set_strip_net(const float* params, int n);
+
+/* One strip correction on padded arrays (oracle layout: z fastest, then x, then y;
+ * extents nnz, nnx, nny) with the network set above; -1 if none is set. */
+int oracle_strip_apply(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
+                       const float* u, const float* v, const float* w, float* px, float* py, float* pz,
+                       float* qx, float* qy, float* qz, const float* vp, const float* rho, const float* eps,
+                       const float* del);
+
 /* Time loop only, for the CPU baseline: runs `nsteps` steps and returns seconds. */
 double oracle_time_steps(const OracleShot* p, const float* vel, const float* rho,
                          const float* eps, const float* del, const float* theta,

@@ -47,3 +47,16 @@ def test_batch_modes_equal_single_runs_and_oracle(oracle, op):
         assert np.abs(ref).max() > 0
         assert rel_l2(seq[w], ref) <= 1e-5
         assert np.array_equal(seq[w], ref)
+
+
+def test_chunked_graphs_equal_single_graph():
+    from a3dfd import Batch, Window
+    desc, wins, m = _setup("tti", seed=33)
+    with Batch(desc, [Window(*w) for w in wins]) as b:
+        b.set_model(**m)
+        one, _ = b.run(mode=2)
+        b.set_graph_chunk(16)            # 70 steps -> 5 graphs
+        chunked, _ = b.run(mode=2)
+        again, _ = b.run(mode=2)
+    for w in range(len(wins)):
+        assert np.array_equal(chunked[w], one[w]) and np.array_equal(again[w], one[w])

@@ -1,0 +1,174 @@
+"""GPU tests of the learned absorbing strip: the training kernels against the FP64
+checker, the fused strip correction against the CPU oracle, teacher-forced sampling."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mlp():
+    spec = importlib.util.spec_from_file_location("oracle_mlp", os.path.join(ROOT, "oracle", "mlp.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+mlp = _mlp()
+
+
+def _params(seed, out_scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = mlp.init(seed).astype(np.float64)
+    p[mlp.NPAR - 130:] += rng.normal(0, 0.2, 130) * out_scale
+    p[64 * 16:64 * 16 + 64] += rng.normal(0, 0.1, 64)   # b1
+    return p.astype(np.float32)
+
+
+def test_forward_matches_fp64():
+    from a3dfd import Mlp
+    rng = np.random.default_rng(0)
+    X = rng.normal(0, 1, (5000, 16)).astype(np.float32)
+    p = _params(1)
+    with Mlp() as net:
+        net.set_params(p)
+        y = net.predict(X)
+    ref, _ = mlp.forward(p, X)
+    print(f"[gpu-strip] forward rel L2 vs FP64: {rel_l2(y, ref):.2e}")
+    assert rel_l2(y, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("n", [1000, 8192])
+def test_gradient_matches_fp64(n):
+    from a3dfd import Mlp
+    rng = np.random.default_rng(n)
+    X = rng.normal(0, 1, (n, 16)).astype(np.float32)
+    T = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    p = _params(2)
+    with Mlp() as net:
+        net.set_params(p)
+        loss, g = net.loss_grad(X, T)
+    rloss, rg = mlp.loss_grad(p, X, T)
+    print(f"[gpu-strip] n={n}: loss {loss:.6f} vs {rloss:.6f}, gradient rel L2 {rel_l2(g, rg):.2e}")
+    assert abs(loss - rloss) <= 1e-5 * abs(rloss)
+    assert rel_l2(g, rg) <= 1e-4
+    # every layer's block, not only the largest one
+    k = 0
+    for name, shape in mlp.SHAPES:
+        m = int(np.prod(shape))
+        assert rel_l2(g[k:k + m], rg[k:k + m]) <= 1e-4, name
+        k += m
+
+
+def test_training_fits_a_teacher_network():
+    from a3dfd import Mlp
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (65536, 16)).astype(np.float32)
+    T, _ = mlp.forward(_params(4), X)
+    T = T.astype(np.float32)
+    with Mlp() as net:
+        net.set_params(mlp.init(5))
+        l0 = net.loss_grad(X[:8192], T[:8192])[0]
+        hist = net.train(X, T, epochs=20, batch=2048, lr=3e-3, lr_end=1e-5, seed=1)
+        l1 = net.loss_grad(X[:8192], T[:8192])[0]
+    print(f"[gpu-strip] training: loss {l0:.4f} -> {l1:.5f} (epochs {hist[0]:.4f} .. {hist[-1]:.5f})")
+    assert hist[-1] < hist[0] and l1 < 0.05 * l0
+
+
+def _small(seed=11, nx=30, ny=28, nz=26):
+    from a3dfd import models
+    return models.small_model("vti", nx, ny, nz, seed=seed)
+
+
+def _desc(nx, ny, nz, nt, nb, **kw):
+    from a3dfd import make_desc
+    return make_desc("vti", nx, ny, nz, 10.0, 10.0, 10.0, 4, nx - 5, 3, nt, 1e-3, 25.0, nb=nb, mute_tt=0, **kw)
+
+
+def _run(m, d, net=None):
+    from a3dfd import Modeller
+    with Modeller(d) as mdl:
+        mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+        if net is not None:
+            mdl.set_strip_net(net)
+        return mdl.run(nsnap_max=8)
+
+
+def _damping_net(seed, gamma=0.1):
+    """A random network with a tiny output layer plus one clean path y ~ -gamma * 0.46 * (P, Q)
+    at the cell: a stable, damping correction of a few per cent per step."""
+    w = mlp.unpack(mlp.init(seed))
+    w = {k: v.copy() for k, v in w.items()}
+    w["W3"] *= 0.02
+    w["W1"][3, :] = 0.0
+    w["W1"][3, 0] = 0.5
+    w["W2"][5, :] = 0.0
+    w["W2"][5, 3] = 1.0
+    w["W3"][:, 5] = -gamma
+    w["W1"][4, :] = 0.0
+    w["W1"][4, 7] = 0.5
+    w["W2"][6, :] = 0.0
+    w["W2"][6, 4] = 1.0
+    w["W3"][:, 6] = 0.0
+    w["W3"][1, 6] = -gamma
+    w["W3"][1, 5] = 0.0
+    return np.concatenate([w[n].ravel() for n, _ in mlp.SHAPES]).astype(np.float32)
+
+
+def test_zero_network_equals_plain_pml():
+    m = _small()
+    d = _desc(30, 28, 26, 200, 10, snap_interval=50)
+    a, sa, _ = _run(m, d)
+    b, sb, _ = _run(m, d, np.zeros(mlp.NPAR, np.float32))
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_strip_matches_oracle(oracle, scheme):
+    m = _small()
+    nt = 220
+    d = _desc(30, 28, 26, nt, 10, snap_interval=55, scheme=scheme)
+    net = _damping_net(6)
+    shot, snaps, _ = _run(m, d, net)
+    plain, _, _ = _run(m, d)
+    os_ = oracle.make_shot("vti", 30, 28, 26, 10.0, 10.0, 10.0, 4, 25, 3, nt, 1e-3, 25.0, nb=10, snap_interval=55,
+                           mute_tt=0)
+    oracle.set_strip_net(net)
+    try:
+        rshot, rsnaps, _ = oracle.run(os_, 32, vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"], nsnap_max=8)
+    finally:
+        oracle.set_strip_net(None)
+    e_shot, e_snap = rel_l2(shot, rshot), rel_l2(snaps, rsnaps)
+    effect = rel_l2(plain, rshot)
+    print(f"[gpu-strip] scheme {scheme}: shot rel L2 {e_shot:.2e}, snapshots {e_snap:.2e}; "
+          f"network effect on the record {effect:.2e}")
+    assert effect > 1e-3          # the correction matters
+    assert e_shot <= 1e-5 and e_snap <= 1e-5
+
+
+def test_collect_samples():
+    from a3dfd import Modeller, strip_collect
+    m = _small(seed=12, nx=24, ny=22, nz=20)
+    dt_ = _desc(24, 22, 20, 150, 40)
+    with Modeller(dt_) as t, Modeller(_desc(24, 22, 20, 150, 10)) as s:
+        for mdl in (t, s):
+            mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+        s.set_strip_net(np.zeros(mlp.NPAR, np.float32))
+        ncell = s.strip_cells()
+        X, T, S = strip_collect(t, s, dt_, every=5, start=0, keep=0.5, seed=3, cap=1 << 20)
+        X2, T2, S2 = strip_collect(t, s, dt_, every=5, start=0, keep=0.5, seed=3, cap=1 << 20)
+    nsteps = len(range(0, 150, 5))
+    print(f"[gpu-strip] {ncell} strip cells, {len(X)} samples")
+    assert ncell == 36 * 34 * 32 - 24 * 22 * 20
+    assert 0.4 * ncell * nsteps < len(X) < 0.6 * ncell * nsteps
+    assert np.all(np.isfinite(X)) and np.all(np.isfinite(T)) and np.all(S > 0)
+    assert np.abs(X[:, :11]).max() <= 1.0 and np.allclose(np.abs(X[:, :11]).max(axis=1)[S > 1e-20], 1.0)
+    # the same selection and values on a second run (order may differ)
+    key = lambda A: np.lexsort(A.T[::-1])   # noqa: E731
+    assert len(X2) == len(X)
+    assert np.array_equal(X[key(X)], X2[key(X2)])
