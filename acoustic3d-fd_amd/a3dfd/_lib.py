@@ -126,8 +126,10 @@ def load(path=None):
         getattr(lib, name).restype = c_int
     lib.a3d_set_strip_net.argtypes = [vp, fp, c_int]
     lib.a3d_strip_info.argtypes = [vp, POINTER(c_int)]
-    lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_float, ctypes.c_uint, fp, fp, fp,
-                                      c_int, POINTER(c_int)]
+    lib.a3d_set_strip_profile.argtypes = [vp, c_float]
+    lib.a3d_set_strip_precision.argtypes = [vp, c_int]
+    lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_int, c_float, ctypes.c_uint, fp, fp,
+                                      fp, c_int, POINTER(c_int)]
     lib.a3d_mlp_create.argtypes = [c_int, POINTER(vp)]
     lib.a3d_mlp_destroy.argtypes = [vp]
     lib.a3d_mlp_set_params.argtypes = [vp, fp, c_int]
@@ -135,7 +137,7 @@ def load(path=None):
     lib.a3d_mlp_predict.argtypes = [vp, fp, c_int, fp]
     lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
     lib.a3d_mlp_train.argtypes = [vp, fp, fp, c_int, c_int, c_int, c_float, c_float, ctypes.c_uint, POINTER(c_double)]
-    for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
+    for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_set_strip_profile", "a3d_set_strip_precision", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
                  "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
                  "a3d_mlp_loss_grad", "a3d_mlp_train"):
         getattr(lib, name).restype = c_int
@@ -254,6 +256,14 @@ class Modeller:
         p = np.ascontiguousarray(params, dtype=np.float32).ravel()
         _check(self.lib, self.lib.a3d_set_strip_net(self._h, _fptr(p), p.size))
 
+    def set_strip_profile(self, amp):
+        """PML amplitude of the learned strip's 6-cell ramp (default 10)."""
+        _check(self.lib, self.lib.a3d_set_strip_profile(self._h, amp))
+
+    def set_strip_precision(self, mfma):
+        """1: strip network on the matrix cores (binary16 operands, FP32 sums); 0: FP32."""
+        _check(self.lib, self.lib.a3d_set_strip_precision(self._h, int(mfma)))
+
     def strip_cells(self):
         n = c_int()
         _check(self.lib, self.lib.a3d_strip_info(self._h, ctypes.byref(n)))
@@ -288,14 +298,15 @@ STRIP_NB = 10
 MLP_NIN, MLP_NOUT = 16, 2
 
 
-def strip_collect(teacher, student, desc, every=4, start=0, keep=0.05, seed=0, cap=1 << 20):
-    """Teacher-forced strip samples (a3d_strip_collect): returns X (n, 16), T (n, 2), S (n,)."""
+def strip_collect(teacher, student, desc, every=4, start=0, keep=0.05, seed=0, cap=1 << 20, horizon=1):
+    """Teacher-forced strip samples (a3d_strip_collect): returns X (n, 16), T (n, 2), S (n,).
+    horizon > 1: the student runs freely (with its network) between synchronisations."""
     lib = load()
     X = np.empty((cap, MLP_NIN), np.float32)
     T = np.empty((cap, MLP_NOUT), np.float32)
     S = np.empty(cap, np.float32)
     n = c_int()
-    _check(lib, lib.a3d_strip_collect(teacher._h, student._h, ctypes.byref(desc), every, start, keep, seed,
+    _check(lib, lib.a3d_strip_collect(teacher._h, student._h, ctypes.byref(desc), every, start, horizon, keep, seed,
                                       _fptr(X), _fptr(T), _fptr(S), cap, ctypes.byref(n)))
     k = n.value
     return X[:k].copy(), T[:k].copy(), S[:k].copy()

@@ -11,7 +11,7 @@ every stress sweep (the library's a3d_set_strip_net). This module:
   target is what the teacher's next P, Q differ from it by, over the strip
   cells (a3d_strip_collect).
 - `train`: fits the network with the library's GPU training kernels (a3d_mlp).
-- `reflection`: the judge. Reflected energy of a boundary = FP64 norm of
+- `judge`: reflected energy. Reflected energy of a boundary = FP64 norm of
   (P_run - P_extended) over the interior and all snapshot steps, over the norm
   of P_extended, in dB; the extended run has the model padded by `margin`
   cells per face and a 40-cell PML, computed in FP64 on the GPU.
@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import models
-from ._lib import STRIP_NB, Mlp, Modeller, make_desc, strip_collect
+from ._lib import STRIP_NB, A3DError, Mlp, Modeller, make_desc, strip_collect
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NET_PATH = os.path.join(HERE, "strip_net.npz")
@@ -44,9 +44,12 @@ def _set(mdl, m):
     mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
 
 
-def collect(seeds, n=128, nt=600, every=3, keep=0.02, device=0, log=None):
+def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, amp=None, device=0, log=None):
     """Samples over seeded models (one source each): X (N, 16), T (N, 2), S (N,). Samples whose
-    local amplitude s is below 1e-4 of the largest s of their shot are dropped."""
+    local amplitude s is below 1e-4 of the largest s of their shot are dropped. horizon > 1:
+    the strip grid runs `horizon` steps on its own (with network `params`) between
+    synchronisations with the teacher, so the samples cover the states the network itself
+    produces."""
     Xs, Ts, Ss = [], [], []
     teacher = student = None
     try:
@@ -57,10 +60,14 @@ def collect(seeds, n=128, nt=600, every=3, keep=0.02, device=0, log=None):
             if teacher is None:
                 teacher = Modeller(dt_, device)
                 student = Modeller(_desc(m, src, nt, STRIP_NB), device)
-                student.set_strip_net(np.zeros(NPARAMS, np.float32))   # builds the strip cell list
+                # builds the strip cell list; the network runs between synchronisations
+                if amp is not None:
+                    student.set_strip_profile(amp)
+                student.set_strip_net(closed_params() if params is None else params)
             _set(teacher, m)
             _set(student, m)
-            X, T, S = strip_collect(teacher, student, dt_, every=every, start=0, keep=keep, seed=seed)
+            X, T, S = strip_collect(teacher, student, dt_, every=every, start=0, keep=keep, seed=seed,
+                                    horizon=horizon)
             ok = S > 1e-4 * S.max() if S.size else S.astype(bool)
             Xs.append(X[ok])
             Ts.append(T[ok])
@@ -82,8 +89,9 @@ def train(X, T, epochs=30, batch=4096, lr=2e-3, lr_end=2e-5, seed=0, init=None, 
         return net.params(), hist
 
 
-def init_params(seed):
-    """Glorot-uniform weights, zero biases, output layer scaled by 0.1 (same rule as the FP64 checker)."""
+def init_params(seed, out_bias=-4.0):
+    """Glorot-uniform weights, zero biases, output layer scaled by 0.1 and output bias out_bias
+    (damping head nearly closed: g = 0.25 * sigmoid(-4) = 0.0045)."""
     rng = np.random.default_rng(seed)
     shapes = [(64, 16), (64,), (64, 64), (64,), (2, 64), (2,)]
     parts = []
@@ -92,8 +100,16 @@ def init_params(seed):
             lim = np.sqrt(6.0 / (s[0] + s[1])) * (0.1 if i == 4 else 1.0)
             parts.append(rng.uniform(-lim, lim, size=s).ravel())
         else:
-            parts.append(np.zeros(s))
+            parts.append(np.full(s, out_bias if i == 5 else 0.0))
     return np.concatenate(parts).astype(np.float32)
+
+
+def closed_params():
+    """A network whose damping head is shut (output bias -30: g = 0.25 * sigmoid(-30) ~ 2e-14,
+    1 - g rounds to 1 in FP32): the strip runs as the plain 6-cell PML."""
+    p = np.zeros(NPARAMS, np.float32)
+    p[-2:] = -30.0
+    return p
 
 
 def save(params, path=NET_PATH, **meta):
@@ -101,11 +117,11 @@ def save(params, path=NET_PATH, **meta):
 
 
 def load_params(path=NET_PATH):
-    """The committed trained network, or None when absent."""
+    """The committed trained network and its strip PML amplitude, or (None, None) when absent."""
     if not os.path.exists(path):
-        return None
+        return None, None
     with np.load(path, allow_pickle=False) as f:
-        return f["params"].astype(np.float32)
+        return f["params"].astype(np.float32), float(f["amp"]) if "amp" in f else None
 
 
 def extended(m, src, margin):
@@ -118,11 +134,12 @@ def extended(m, src, margin):
     return me, se
 
 
-def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, device=0):
+def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None, scheme=0, device=0):
     """One shot under a boundary: 'strip' (learned, needs params), 'strip0' (the 6-cell strip
-    without the network), 'pml' (20-cell PML), 'extended' (FP64, model padded by margin, 40-cell
-    PML). Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original
-    interior."""
+    with its damping head shut: its own PML only), 'pml' (20-cell PML), 'extended' (FP64, model
+    padded by margin, 40-cell PML). amp: the strip's PML amplitude (None: the library default).
+    scheme: the stress update of the non-extended runs (0 reference, 1 split fields in PML cells only).
+    Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original interior."""
     if boundary == "extended":
         me, se = extended(m, src, margin)
         d = _desc(me, se, nt, TEACHER_NB, storage=2, rec_z=margin, snap_interval=snap_every, snap_skip0=1)
@@ -132,11 +149,13 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, device=0)
         c = slice(margin, margin + m["nx"])
         return shot[margin:margin + m["ny"], c], snaps[:, margin:margin + m["ny"], c, margin:margin + m["nz"]], ms
     nb = PML_NB if boundary == "pml" else STRIP_NB
-    d = _desc(m, src, nt, nb, snap_interval=snap_every, snap_skip0=1)
+    d = _desc(m, src, nt, nb, snap_interval=snap_every, snap_skip0=1, scheme=scheme)
     with Modeller(d, device) as mdl:
         _set(mdl, m)
-        if boundary == "strip":
-            mdl.set_strip_net(params)
+        if boundary in ("strip", "strip0"):
+            if amp is not None:
+                mdl.set_strip_profile(amp)
+            mdl.set_strip_net(params if boundary == "strip" else closed_params())
         return mdl.run(nsnap_max=nt // snap_every)
 
 
@@ -155,7 +174,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
 
 
-def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, device=0, log=None):
+def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, device=0, log=None):
     """Reflected energy (dB) and shot-record rel. L2 against the extended FP64 run for the
     learned strip, the strip without network and the 20-cell PML, on held-out models."""
     rows = []
@@ -168,13 +187,35 @@ def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, device=0, log=
         for b in ("pml", "strip0", "strip"):
             if b == "strip" and params is None:
                 continue
-            shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, device=device)
-            row[b] = {"reflected_db": round(energy_db(snaps, ref_snaps), 2),
+            try:
+                shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, amp=amp, device=device)
+            except A3DError as e:   # an unstable boundary is judged as failed
+                row[b] = {"reflected_db": float("inf"), "shot_rel_l2": float("inf"), "loop_ms": None, "error": str(e)}
+                continue
+            row[b] = {"reflected_db": round(float(energy_db(snaps, ref_snaps)), 2),
                       "shot_rel_l2": rel_l2(shot, ref_shot), "loop_ms": round(ms, 2)}
         rows.append(row)
         if log:
             log(f"strip judge: seed {seed} ({time.time() - t0:.1f} s): {row}")
     return rows
+
+
+def tune_amp(seeds, amps=(4.0, 5.0, 6.0, 7.0, 8.5, 10.0), n=128, nt=800, snap_every=20, device=0, log=None):
+    """The strip's PML amplitude with the lowest worst-case reflected energy (damping head shut)
+    over the given training models (the judge asks every model to pass); returns
+    (amp, {amp: worst dB})."""
+    db = {a: [] for a in amps}
+    for seed in seeds:
+        m = models.strip_model(seed, n)
+        src = models.strip_source(seed, n)
+        _, ref, _ = run_case(m, src, nt, snap_every, "extended", device=device)
+        for a in amps:
+            _, snaps, _ = run_case(m, src, nt, snap_every, "strip0", amp=a, device=device)
+            db[a].append(energy_db(snaps, ref))
+    worst = {a: round(float(np.max(v)), 2) for a, v in db.items()}
+    if log:
+        log(f"strip profile: worst reflected dB by amplitude {worst}")
+    return min(worst, key=worst.get), worst
 
 
 def choose(rows, threshold_db=THRESHOLD_DB):

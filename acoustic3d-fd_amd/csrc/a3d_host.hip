@@ -56,7 +56,7 @@ int a3d::report_error(int code, const char* what, const char* detail) { return f
 namespace {
 
 enum Vol {
-    V_VP, V_RHO, V_EPS, V_DEL, V_TH, V_PH,
+    V_VP, V_RHO, V_EPS, V_DEL, V_CT, V_ST, V_CP, V_SP,   // TTI: cos/sin of tilt and azimuth
     V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ,
     V_VV, V_PA, V_PB, NVOL
 };
@@ -68,7 +68,7 @@ std::vector<int> volumes_of(int op) {
         case A3D_OP_VTI:
             return {V_VP, V_RHO, V_EPS, V_DEL, V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
         case A3D_OP_TTI:
-            return {V_VP, V_EPS, V_DEL, V_TH, V_PH, V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
+            return {V_VP, V_EPS, V_DEL, V_CT, V_ST, V_CP, V_SP, V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
     }
     return {};
 }
@@ -132,21 +132,22 @@ T ricker_value(int it, float fm_, float dt_) {
     return (T)((1.0 - 2.0 * tmp) * std::exp(-tmp));
 }
 
-// initial_coffe (reference iso:263-291) for one axis of interior length n.
+// initial_coffe (reference iso:263-291) for one axis of interior length n; the
+// profile's ramp runs over `width` cells (the reference: width = npml).
 template <class T>
-void pml_axis(T dt, T d0, int n, int npml, T* c1, T* c2, T* a1, T* a2) {
+void pml_axis(T dt, T d0, int n, int npml, T* c1, T* c2, T* a1, T* a2, int width) {
     const T dtd0 = dt * d0;
     for (int id = 0; id < n + 2 * npml; ++id) {
         double r, ra;
         if (id < npml) {
-            r = (npml - 0.5 - id) / npml;
-            ra = ((npml - id) * 1.0) / npml;
+            r = (npml - 0.5 - id) / width;
+            ra = ((npml - id) * 1.0) / width;
         } else if (id < npml + n) {
             c1[id] = c2[id] = a1[id] = a2[id] = (T)1.0;
             continue;
         } else {
-            r = (0.5 + id - n - npml) / npml;
-            ra = ((id - n - npml) * 1.0) / npml;
+            r = (0.5 + id - n - npml) / width;
+            ra = ((id - n - npml) * 1.0) / width;
         }
         c1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
         c2[id] = (T)(c1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
@@ -218,6 +219,8 @@ struct a3d_ctx {
     int* strip_meta = nullptr;
     int strip_n = 0;
     float2* strip_delta = nullptr;
+    float strip_amp = 10.0f;    // PML amplitude of the strip's 6-cell ramp
+    int strip_mfma = 1;         // network on the matrix cores (binary16 operands) or FP32 per lane
 };
 
 namespace {
@@ -381,7 +384,12 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     const T dt = (T)d->dt;
     // get_d0 (reference iso:193-195) from the padded velocity at z-fastest index N/2
     const T dsum = (T)d->dx + (T)d->dy + (T)d->dz;
-    const T d0 = (T)(10.0 * (T)c->vel_d0 * std::log(100000.0) / (2.0 * nb * (dsum / 3.0)));
+    // The learned strip ramps its PML over its STRIP_CELLS updated cells with amplitude
+    // strip_amp in place of the reference's 10 over npml cells.
+    const bool strip = c->strip_net != nullptr;
+    const int width = strip ? STRIP_CELLS : nb;
+    const double amp = strip ? (double)c->strip_amp : 10.0;
+    const T d0 = (T)(amp * (T)c->vel_d0 * std::log(100000.0) / (2.0 * width * (dsum / 3.0)));
     std::vector<T> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
     Pml pm{};
     pm.cx1 = 0;
@@ -397,9 +405,9 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     pm.az1 = pm.cz1 + 2 * g.nnz;
     pm.az2 = pm.cz1 + 3 * g.nnz;
     T* t = tabh.data();
-    pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2);
-    pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2);
-    pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2);
+    pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2, width);
+    pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2, width);
+    pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2, width);
     HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(T), hipMemcpyHostToDevice));
     for (int v : volumes_of(op))
         if (is_wavefield(v))
@@ -417,8 +425,10 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     va.P = c->vol[V_P];
     va.Q = op == A3D_OP_ISO ? c->vol[V_P] : c->vol[V_Q];
     va.rho = c->vol[V_RHO];
-    va.th = c->vol[V_TH];
-    va.ph = c->vol[V_PH];
+    va.ct = c->vol[V_CT];
+    va.st = c->vol[V_ST];
+    va.cp = c->vol[V_CP];
+    va.sp = c->vol[V_SP];
     va.u = c->vol[V_U];
     va.v = c->vol[V_V];
     va.w = c->vol[V_W];
@@ -444,8 +454,10 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.rho = c->vol[V_RHO];
     sa.eps = c->vol[V_EPS];
     sa.del = c->vol[V_DEL];
-    sa.th = c->vol[V_TH];
-    sa.ph = c->vol[V_PH];
+    sa.ct = c->vol[V_CT];
+    sa.st = c->vol[V_ST];
+    sa.cp = c->vol[V_CP];
+    sa.sp = c->vol[V_SP];
     sa.P = c->vol[V_P];
     sa.Q = c->vol[V_Q];
     sa.dtx = dtx;
@@ -510,10 +522,11 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
 
 // Strip correction after a stress sweep (learned absorbing strip).
 int strip_step(a3d_ctx* c, const StripArgs& a) {
-    HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->st));
+    HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, c->st));
     HIP_TRY(launch_strip_apply(a, c->strip_delta, (float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_PX],
                                (float*)c->vol[V_PY], (float*)c->vol[V_PZ], (float*)c->vol[V_QX], (float*)c->vol[V_QY],
-                               (float*)c->vol[V_QZ], c->st));
+                               (float*)c->vol[V_QZ], (float*)c->vol[V_U], (float*)c->vol[V_V], (float*)c->vol[V_W],
+                               c->st));
     return A3D_OK;
 }
 
@@ -957,8 +970,14 @@ int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* e
         if (!rc && (op == A3D_OP_ISO || op == A3D_OP_VTI)) rc = up(rho, V_RHO);
         if (!rc && op != A3D_OP_ISO) rc = up(eps, V_EPS);
         if (!rc && op != A3D_OP_ISO) rc = up(del, V_DEL);
-        if (!rc && op == A3D_OP_TTI) rc = up(theta, V_TH);
-        if (!rc && op == A3D_OP_TTI) rc = up(phi, V_PH);
+        if (!rc && op == A3D_OP_TTI) rc = up(theta, V_CT);
+        if (!rc && op == A3D_OP_TTI) rc = up(phi, V_CP);
+        if (!rc && op == A3D_OP_TTI) {
+            if (hipSuccess != launch_trig(c->vol[V_CT], c->vol[V_ST], c->vol[V_CP], c->vol[V_SP], c->nvox,
+                                          c->prec == PREC_FP64, c->st) ||
+                hipSuccess != hipStreamSynchronize(c->st))
+                rc = fail(A3D_EHIP, "TTI rotation terms failed");
+        }
         (void)hipFree(stage);
     }
     if (rc) return rc;
@@ -1120,13 +1139,25 @@ int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
     return A3D_OK;
 }
 
+int a3d_set_strip_precision(a3d_ctx* c, int mfma) {
+    if (!c || mfma < 0 || mfma > 1) return fail(A3D_EINVAL, "strip precision must be 0 or 1");
+    c->strip_mfma = mfma;
+    return A3D_OK;
+}
+
+int a3d_set_strip_profile(a3d_ctx* c, float amp) {
+    if (!c || !(amp >= 0.0f) || !(amp <= 1000.0f)) return fail(A3D_EINVAL, "strip profile amplitude out of range");
+    c->strip_amp = amp;
+    return A3D_OK;
+}
+
 int a3d_strip_info(const a3d_ctx* c, int* ncells) {
     if (!c || !ncells) return fail(A3D_EINVAL, "null argument");
     *ncells = c->strip_n;
     return A3D_OK;
 }
 
-int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every, int start, float keep,
+int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every, int start, int horizon, float keep,
                       unsigned seed, float* X, float* T, float* S, int cap, int* count) {
     if (!t || !s || !d || !X || !T || !S || !count || cap <= 0 || every <= 0 || !(keep > 0))
         return fail(A3D_EINVAL, "bad argument");
@@ -1167,19 +1198,34 @@ int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every,
     const int off = t->d.nb - s->d.nb;
     const unsigned keep_u = (unsigned)std::min(4294967295.0, (double)keep * 4294967296.0);
     const int fields[] = {V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ};
+    const bool free_run = horizon > 1;
     for (int it = 0; it < d->nt; ++it) {
         vt.s = srcs[it];
         HIP_TRY(launch_vel(PREC_FP32, A3D_OP_VTI, t->ty[0], vt, t->st));
-        const bool samp = it >= start && (it - start) % every == 0;
-        if (samp) {
+        const bool live = it >= start;
+        const bool samp = live && (it - start) % every == 0;
+        const bool sync = live && (free_run ? (it - start) % horizon == 0 : samp);
+        if (sync) {
+            // the teacher's state (velocities at n+1/2, stresses at n) onto the strip grid
             for (int v : fields)
                 HIP_TRY(launch_strip_copy((const float*)t->vol[v], t->g, (float*)s->vol[v], s->g, off, t->st));
-            HIP_TRY(launch_stress(PREC_FP32, A3D_OP_VTI, s->ty[1], sts, t->st));
+        } else if (free_run && live) {
+            vs.s = srcs_s[it];
+            HIP_TRY(launch_vel(PREC_FP32, A3D_OP_VTI, s->ty[0], vs, t->st));
         }
+        if (sync || (free_run && live)) HIP_TRY(launch_stress(PREC_FP32, A3D_OP_VTI, s->ty[1], sts, t->st));
         HIP_TRY(launch_stress(PREC_FP32, A3D_OP_VTI, t->ty[1], stt, t->st));
         if (samp)
             HIP_TRY(launch_strip_sample(a, (const float*)t->vol[V_P], (const float*)t->vol[V_Q], t->g, off, keep_u, seed,
                                         (unsigned)it, cap, dX, dT, dS, dn, t->st));
+        if (free_run && live && s->strip_net) {
+            // the strip grid runs on with its own network between synchronisations
+            HIP_TRY(launch_strip_infer(a, s->strip_net, s->strip_delta, s->strip_mfma, t->st));
+            HIP_TRY(launch_strip_apply(a, s->strip_delta, (float*)s->vol[V_P], (float*)s->vol[V_Q], (float*)s->vol[V_PX],
+                                       (float*)s->vol[V_PY], (float*)s->vol[V_PZ], (float*)s->vol[V_QX],
+                                       (float*)s->vol[V_QY], (float*)s->vol[V_QZ], (float*)s->vol[V_U],
+                                       (float*)s->vol[V_V], (float*)s->vol[V_W], t->st));
+        }
     }
     int hn = 0;
     HIP_TRY(hipMemcpyAsync(&hn, dn, sizeof(int), hipMemcpyDeviceToHost, t->st));
@@ -1796,8 +1842,15 @@ int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const 
     if (!rc && (op == A3D_OP_ISO || op == A3D_OP_VTI)) rc = up(rho, V_RHO);
     if (!rc && op != A3D_OP_ISO) rc = up(eps, V_EPS);
     if (!rc && op != A3D_OP_ISO) rc = up(del, V_DEL);
-    if (!rc && op == A3D_OP_TTI) rc = up(theta, V_TH);
-    if (!rc && op == A3D_OP_TTI) rc = up(phi, V_PH);
+    if (!rc && op == A3D_OP_TTI) rc = up(theta, V_CT);
+    if (!rc && op == A3D_OP_TTI) rc = up(phi, V_CP);
+    for (size_t w = 0; w < b->ctx.size() && !rc && op == A3D_OP_TTI; ++w) {
+        a3d_ctx* c = b->ctx[w];
+        if (hipSuccess != launch_trig(c->vol[V_CT], c->vol[V_ST], c->vol[V_CP], c->vol[V_SP], c->nvox,
+                                      c->prec == PREC_FP64, b->st) ||
+            hipSuccess != hipStreamSynchronize(b->st))
+            rc = fail(A3D_EHIP, "TTI rotation terms failed");
+    }
     (void)hipFree(stage);
     if (rc) return rc;
     for (size_t w = 0; w < b->ctx.size(); ++w) {

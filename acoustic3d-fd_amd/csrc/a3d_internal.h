@@ -73,8 +73,7 @@ struct VelArgs {
     const void* P;
     const void* Q;
     const void* rho;
-    const void* th;
-    const void* ph;
+    const void *ct, *st, *cp, *sp;   // TTI: cos/sin of tilt and azimuth, per point
     void* u;
     void* v;
     void* w;
@@ -92,7 +91,8 @@ struct StressArgs {
     const void* v;
     const void* w;
     void *px, *py, *pz, *qx, *qy, *qz;
-    const void *vp, *rho, *eps, *del, *th, *ph;
+    const void *vp, *rho, *eps, *del;
+    const void *ct, *st, *cp, *sp;   // TTI: cos/sin of tilt and azimuth, per point
     void* P;
     void* Q;
     double dtx, dty, dtz;
@@ -148,6 +148,11 @@ struct Window {
 hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const Geo& g, void* dst, int model_double,
                       int mode, float dt, hipStream_t st, const Window* win = nullptr);
 
+// TTI angles to their cosines and sines, in place over n padded points:
+// th -> cos(th) (in th), sin(th) (in st); ph -> cos(ph) (in ph), sin(ph) (in sp),
+// each (M)cos((double)angle) with M float or double (model_double).
+hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int model_double, hipStream_t s);
+
 // Interior crop of a padded wavefield into z-fastest float order ([ny][nx][nz]).
 // 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied
 // (fx != nullptr); 1st-order: plain copy. scale: binary16 field scale.
@@ -184,11 +189,13 @@ struct StripArgs {
     float cdt[3];   // dt/dx, dt/dy, dt/dz
 };
 
-// delta[k] = s * net(features) for every strip cell (after the stress sweep).
-hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, hipStream_t st);
-// Adds delta to P, Q and to the strip cell's normal-axis split components.
+// delta[k] = the damping factors (g_P, g_Q) of every strip cell (after the stress sweep);
+// mfma 1: the network on the matrix cores with binary16 operands, 0: FP32 per lane.
+hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, int mfma, hipStream_t st);
+// Scales P, its split components, u and v by 1 - g_P; Q, its split components and w by 1 - g_Q.
 hipError_t launch_strip_apply(const StripArgs& a, const float2* delta, float* P, float* Q, float* px, float* py,
-                              float* pz, float* qx, float* qy, float* qz, hipStream_t st);
+                              float* pz, float* qx, float* qy, float* qz, float* u, float* v, float* w,
+                              hipStream_t st);
 // Appends teacher-forced samples (features, (teacher - strip state) / s, s).
 hipError_t launch_strip_sample(const StripArgs& a, const float* Pt, const float* Qt, const Geo& gt, int off,
                                unsigned keep, unsigned seed, unsigned tag, int cap, float* X, float* T, float* S,

@@ -28,6 +28,8 @@
 // in float); Fp64 (everything in double: the GPU error reference).
 #include "a3d_internal.h"
 
+#include <algorithm>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -215,8 +217,10 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     S* V = (S*)a.v;
     S* Wv = (S*)a.w;
     const M* rho = (const M*)a.rho;
-    const M* th = (const M*)a.th;
-    const M* ph = (const M*)a.ph;
+    const M* Ct = (const M*)a.ct;
+    const M* St = (const M*)a.st;
+    const M* Cp = (const M*)a.cp;
+    const M* Sp = (const M*)a.sp;
     const C* T = (const C*)a.tab;
 
     auto ldP = [&](int i) { return inj<C>(ldw(P, i, sc.s[F_P]), i, S0.s0, S0.p0, S0.s1, S0.p1); };
@@ -239,15 +243,17 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         const int id = zo + col;
         if constexpr (P_Z) wP[7] = ldP(id + 4 * g.pl);
         if constexpr (HAS_Q) wQ[7] = ldQ(id + 4 * g.pl);
-        C u0 = 0, v0 = 0, w0 = 0, r0 = 1, t0 = 0, f0 = 0;
+        C u0 = 0, v0 = 0, w0 = 0, r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
         if (valid) {
             u0 = ldw(U, id, sc.s[F_U]);
             v0 = ldw(V, id, sc.s[F_V]);
             w0 = ldw(Wv, id, sc.s[F_W]);
             if constexpr (!TTI) r0 = (C)rho[id];
             if constexpr (TTI) {
-                t0 = (C)th[id];
-                f0 = (C)ph[id];
+                ct = (C)Ct[id];
+                st = (C)St[id];
+                cp = (C)Cp[id];
+                sp = (C)Sp[id];
             }
         }
         C pc;
@@ -279,8 +285,6 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
                 const C Qy = dst8<C>([&](int k) { return sQ[r - 3 + k][c]; });
                 const C Qx = dst8<C>([&](int k) { return sQ[r][c - 3 + k]; });
                 const C Qz = dst8<C>([&](int k) { return wQ[k]; });
-                const C ct = (C)cos((double)t0), st = (C)sin((double)t0);
-                const C cp = (C)cos((double)f0), sp = (C)sin((double)f0);
                 xx = ct * cp * Px - ct * sp * Py - st * Pz;
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
@@ -360,8 +364,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const M* rho = (const M*)a.rho;
     const M* eps = (const M*)a.eps;
     const M* del = (const M*)a.del;
-    const M* th = (const M*)a.th;
-    const M* ph = (const M*)a.ph;
+    const M* Ct = (const M*)a.ct;
+    const M* St = (const M*)a.st;
+    const M* Cp = (const M*)a.cp;
+    const M* Sp = (const M*)a.sp;
     const C* T = (const C*)a.tab;
 
     auto ldU = [&](int i) { return ldw(U, i, sc.s[F_U]); };
@@ -392,7 +398,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             wU[7] = ldU(id + 3 * g.pl);
             wV[7] = ldV(id + 3 * g.pl);
         }
-        C c0 = 0, r0 = 0, e0 = 0, d0 = 0, t0 = 0, f0 = 0;
+        C c0 = 0, r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
         if (valid) {
             c0 = (C)vp[id];
             if constexpr (!TTI) r0 = (C)rho[id];
@@ -401,8 +407,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 d0 = (C)del[id];
             }
             if constexpr (TTI) {
-                t0 = (C)th[id];
-                f0 = (C)ph[id];
+                ct = (C)Ct[id];
+                st = (C)St[id];
+                cp = (C)Cp[id];
+                sp = (C)Sp[id];
             }
         }
         C uc, vc;
@@ -481,8 +489,6 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     const C wy = dst8<C>([&](int k) { return sW[r - 4 + k][c]; });
                     const C wx = dst8<C>([&](int k) { return sW[r][c - 4 + k]; });
                     const C wz = dst8<C>([&](int k) { return wW[k]; });
-                    const C ct = (C)cos((double)t0), st = (C)sin((double)t0);
-                    const C cp = (C)cos((double)f0), sp = (C)sin((double)f0);
                     xx = ct * cp * ux - ct * sp * uy - st * uz;
                     yy = sp * vx + cp * vy;
                     zz = st * cp * wx - st * sp * wy + ct * wz;
@@ -703,6 +709,19 @@ __global__ void k_shot_transpose(const float* __restrict__ rec, int nt, int nx, 
             }
             shot[(size_t)iy * nx * nt + (size_t)ix * nt + it] = v;
         }
+    }
+}
+
+// TTI rotation terms, once per model: the same (M)cos((double)angle) values the
+// reference's kernels compute per point and per sweep (tti:101-103, :184-186).
+template <class M>
+__global__ void k_trig(M* __restrict__ th, M* __restrict__ st, M* __restrict__ ph, M* __restrict__ sp, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double t = (double)th[i], f = (double)ph[i];
+        th[i] = (M)cos(t);
+        st[i] = (M)sin(t);
+        ph[i] = (M)cos(f);
+        sp[i] = (M)sin(f);
     }
 }
 
@@ -977,6 +996,15 @@ hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, i
         hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, st, (const float4*)a, n4, b);
     else
         hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)b, n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int model_double, hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (model_double)
+        k_trig<double><<<grid, 256, 0, s>>>((double*)th, (double*)st, (double*)ph, (double*)sp, n);
+    else
+        k_trig<float><<<grid, 256, 0, s>>>((float*)th, (float*)st, (float*)ph, (float*)sp, n);
     return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@
 // in LDS, read as broadcasts); the weight gradients sum over the samples of a
 // minibatch in a fixed order (per-thread partial sums over a strided sample
 // range, then a wave-shuffle and LDS reduction), so a gradient is the same on
-// every run. Loss: mean over samples of the squared error summed over outputs.
+// every run. Loss: mean over samples of the squared error of the damping head's
+// prediction summed over the two outputs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,8 +36,9 @@ __device__ __forceinline__ void stage_params(float* sp, const float* __restrict_
     __syncthreads();
 }
 
-// Forward pass of the minibatch idx[0..nb): activations, output gradient
-// dy = 2 (y - t) / n_total and per-block sums of the squared error.
+// Forward pass of the minibatch idx[0..nb): activations, the gradient of the loss
+// with respect to the raw outputs y (through the damping head) and per-block sums
+// of the squared error.
 __global__ __launch_bounds__(MB) void k_mlp_fwd(const float* __restrict__ p, const float* __restrict__ X,
                                                 const float* __restrict__ T, const int* __restrict__ idx, int nb,
                                                 float inv_n, float* __restrict__ H1, float* __restrict__ H2,
@@ -52,6 +54,8 @@ __global__ __launch_bounds__(MB) void k_mlp_fwd(const float* __restrict__ p, con
 #pragma unroll
         for (int i = 0; i < LB_NIN; ++i) x[i] = X[(size_t)r * LB_NIN + i];
         mlp_forward(sp, x, h1, h2, y);
+        float g[LB_NOUT], pred[LB_NOUT];
+        strip_head(y, x, g, pred);
 #pragma unroll
         for (int i = 0; i < LB_NH; ++i) {
             H1[(size_t)s * LB_NH + i] = h1[i];
@@ -59,9 +63,10 @@ __global__ __launch_bounds__(MB) void k_mlp_fwd(const float* __restrict__ p, con
         }
 #pragma unroll
         for (int o = 0; o < LB_NOUT; ++o) {
-            const float e = y[o] - T[(size_t)r * LB_NOUT + o];
+            const float e = pred[o] - T[(size_t)r * LB_NOUT + o];
             err += e * e;
-            DY[(size_t)s * LB_NOUT + o] = 2.f * e * inv_n;
+            // d pred / d y = -x_k * g * (1 - g / GMAX)
+            DY[(size_t)s * LB_NOUT + o] = 2.f * e * inv_n * (-x[LB_HEAD_IN[o]] * g[o] * (1.f - g[o] * (1.f / STRIP_GMAX)));
         }
     }
     for (int o = 32; o > 0; o >>= 1) err += __shfl_xor(err, o);
@@ -178,8 +183,10 @@ __global__ __launch_bounds__(MB) void k_mlp_predict(const float* __restrict__ p,
 #pragma unroll
     for (int i = 0; i < LB_NIN; ++i) x[i] = X[(size_t)s * LB_NIN + i];
     mlp_forward(sp, x, h1, h2, y);
+    float g[LB_NOUT], pred[LB_NOUT];
+    strip_head(y, x, g, pred);
 #pragma unroll
-    for (int o = 0; o < LB_NOUT; ++o) Y[(size_t)s * LB_NOUT + o] = y[o];
+    for (int o = 0; o < LB_NOUT; ++o) Y[(size_t)s * LB_NOUT + o] = pred[o];
 }
 
 }  // namespace

@@ -3,17 +3,22 @@
 // The strip is the outer STRIP_CELLS updated cells of every face of the
 // padded grid (padding nb = STRIP_CELLS + 4, the 4-cell ring staying zero as
 // in the reference's PML), run with the reference split-PML update at that
-// width. After each stress sweep a network corrects the normal-axis split
-// pressure components (and so P and Q) of every strip cell:
+// width. After each stress sweep a network damps each strip cell's P and Q:
 //   features  f = local state / s, s = max |local state| (amplitude-free)
-//   delta     (dP, dQ) = s * net(f)
-// so the correction is positively homogeneous of degree 1 in the wavefield.
+//   factors   (g_P, g_Q) = STRIP_GMAX * sigmoid(net(f)) in (0, STRIP_GMAX)
+//   update    P, px, py, pz, u, v *= 1 - g_P;  Q, qx, qy, qz, w *= 1 - g_Q
+// (the velocities with their stresses, as a sponge damps every field: damping
+// the stresses alone would leave a net inflow into the strip and a drifting
+// static field). A factor in (0, 1) only removes energy, so the strip stays
+// stable for any parameters; the network decides where and how strongly to damp.
 // Local state of a cell with outward normal axis a (the axis of its nearest
 // face): P at the 7-point star (centre, inward, outward, tangential pairs),
 // Q at centre, inward, outward, and the outward normal velocity times the
 // impedance rho*vp; model features vp*dt/d_a, epsilon, delta and the depth in
 // the strip.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "a3d_internal.h"
 #include "a3d_mlp.h"
@@ -64,27 +69,143 @@ __global__ __launch_bounds__(SB) void k_strip_infer(const StripArgs a, const flo
     __syncthreads();
     const int k = blockIdx.x * SB + threadIdx.x;
     if (k >= a.n) return;
-    float x[LB_NIN], h1[LB_NH], h2[LB_NH], y[LB_NOUT];
-    const float s = strip_features(a, a.ids[k], a.meta[k], x);
+    float x[LB_NIN], h1[LB_NH], h2[LB_NH], y[LB_NOUT], g[LB_NOUT], pred[LB_NOUT];
+    (void)strip_features(a, a.ids[k], a.meta[k], x);
     mlp_forward(sp, x, h1, h2, y);
-    delta[k] = make_float2(s * y[0], s * y[1]);
+    strip_head(y, x, g, pred);
+    delta[k] = make_float2(g[0], g[1]);
+}
+
+// The same network on the matrix cores: one wavefront evaluates 64 strip cells as
+// a 64 x 16 by 16 x 64 and a 64 x 64 by 64 x 64 product with v_mfma_f32_32x32x8_f16
+// (operands rounded to binary16, products exact, sums in FP32), the 2-output last
+// layer per lane in FP32. Weights, features and hidden activations pass through LDS
+// in binary16. MFMA 32x32x8 operand layout (lane l, r = 0..15): A[i][k] with
+// i = l % 32, k = 4 (l / 32) + 0..3; B[k][j] with j = l % 32, same k; D[i][j] with
+// j = l % 32, i = 8 (r / 4) + 4 (l / 32) + r % 4.
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+constexpr int XP = LB_NIN + 4;   // padded LDS rows (halves), 8-byte aligned
+constexpr int HP = LB_NH + 8;
+
+__global__ __launch_bounds__(SB) void k_strip_infer_mfma(const StripArgs a, const float* __restrict__ net,
+                                                         float2* __restrict__ delta) {
+    __shared__ _Float16 sW1[LB_NH][XP];
+    __shared__ _Float16 sW2[LB_NH][HP];
+    __shared__ float sB1[LB_NH], sB2[LB_NH], sW3[LB_NOUT][LB_NH], sB3[LB_NOUT];
+    __shared__ _Float16 sX[SB / 64][64][XP];
+    __shared__ _Float16 sH[SB / 64][64][HP];
+    for (int i = threadIdx.x; i < LB_NH * LB_NIN; i += SB) sW1[i / LB_NIN][i % LB_NIN] = (_Float16)net[LB_W1 + i];
+    for (int i = threadIdx.x; i < LB_NH * LB_NH; i += SB) sW2[i / LB_NH][i % LB_NH] = (_Float16)net[LB_W2 + i];
+    for (int i = threadIdx.x; i < LB_NH; i += SB) {
+        sB1[i] = net[LB_B1 + i];
+        sB2[i] = net[LB_B2 + i];
+        sW3[0][i] = net[LB_W3 + i];
+        sW3[1][i] = net[LB_W3 + LB_NH + i];
+    }
+    if (threadIdx.x < LB_NOUT) sB3[threadIdx.x] = net[LB_B3 + threadIdx.x];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int lr = l & 31, lk = (l >> 5) * 4;
+    // the block stages the weights once and walks the strip cells 256 at a time
+    for (int base = blockIdx.x * SB; base < a.n; base += gridDim.x * SB) {
+        const int k = base + threadIdx.x;
+        float x[LB_NIN];
+#pragma unroll
+        for (int i = 0; i < LB_NIN; ++i) x[i] = 0.f;
+        if (k < a.n) (void)strip_features(a, a.ids[k], a.meta[k], x);
+        __syncthreads();   // the previous group's last reads of sX / sH are done
+#pragma unroll
+        for (int i = 0; i < LB_NIN; ++i) sX[wv][l][i] = (_Float16)x[i];
+        __syncthreads();
+        float16v acc[2][2];
+        // layer 1: H1 = tanh(X W1^T + b1)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                float16v c = {};
+#pragma unroll
+                for (int ks = 0; ks < LB_NIN / 8; ++ks) {
+                    const half4v av = *(const half4v*)&sX[wv][32 * rb + lr][8 * ks + lk];
+                    const half4v bv = *(const half4v*)&sW1[32 * cb + lr][8 * ks + lk];
+                    c = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+                }
+                acc[rb][cb] = c;
+            }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = 32 * rb + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), j = 32 * cb + lr;
+                    sH[wv][i][j] = (_Float16)act_tanh(acc[rb][cb][r] + sB1[j]);
+                }
+        __syncthreads();
+        // layer 2: H2 = tanh(H1 W2^T + b2)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                float16v c = {};
+#pragma unroll
+                for (int ks = 0; ks < LB_NH / 8; ++ks) {
+                    const half4v av = *(const half4v*)&sH[wv][32 * rb + lr][8 * ks + lk];
+                    const half4v bv = *(const half4v*)&sW2[32 * cb + lr][8 * ks + lk];
+                    c = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+                }
+                acc[rb][cb] = c;
+            }
+        __syncthreads();
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = 32 * rb + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), j = 32 * cb + lr;
+                    sH[wv][i][j] = (_Float16)act_tanh(acc[rb][cb][r] + sB2[j]);
+                }
+        __syncthreads();
+        if (k < a.n) {
+            // layer 3 (2 outputs) and the damping head, one cell per lane
+            float y[LB_NOUT], g[LB_NOUT], pred[LB_NOUT];
+#pragma unroll
+            for (int o = 0; o < LB_NOUT; ++o) {
+                float t = 0.f;
+#pragma unroll
+                for (int j = 0; j < LB_NH; ++j) t = fmaf(sW3[o][j], (float)sH[wv][l][j], t);
+                y[o] = t + sB3[o];
+            }
+            strip_head(y, x, g, pred);
+            delta[k] = make_float2(g[0], g[1]);
+        }
+    }
 }
 
 __global__ __launch_bounds__(SB) void k_strip_apply(const StripArgs a, const float2* __restrict__ delta,
                                                     float* __restrict__ P, float* __restrict__ Q,
                                                     float* __restrict__ px, float* __restrict__ py,
                                                     float* __restrict__ pz, float* __restrict__ qx,
-                                                    float* __restrict__ qy, float* __restrict__ qz) {
+                                                    float* __restrict__ qy, float* __restrict__ qz,
+                                                    float* __restrict__ u, float* __restrict__ v,
+                                                    float* __restrict__ w) {
     const int k = blockIdx.x * SB + threadIdx.x;
     if (k >= a.n) return;
-    const int id = a.ids[k], axis = a.meta[k] & 3;
+    const int id = a.ids[k];
     const float2 d = delta[k];
-    P[id] += d.x;
-    Q[id] += d.y;
-    float* sp = axis == 0 ? px : (axis == 1 ? py : pz);
-    float* sq = axis == 0 ? qx : (axis == 1 ? qy : qz);
-    sp[id] += d.x;
-    sq[id] += d.y;
+    const float fp = 1.0f - d.x, fq = 1.0f - d.y;
+    P[id] *= fp;
+    px[id] *= fp;
+    py[id] *= fp;
+    pz[id] *= fp;
+    Q[id] *= fq;
+    qx[id] *= fq;
+    qy[id] *= fq;
+    qz[id] *= fq;
+    u[id] *= fp;
+    v[id] *= fp;
+    w[id] *= fq;
 }
 
 // Teacher-forced samples: the strip state a (the student's stress sweep applied to
@@ -131,16 +252,20 @@ __global__ void k_strip_copy(const float* __restrict__ src, Geo gs, float* __res
 
 }  // namespace
 
-hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, hipStream_t st) {
+hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, int mfma, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-    k_strip_infer<<<(a.n + SB - 1) / SB, SB, 0, st>>>(a, net, delta);
+    if (mfma)   // two resident blocks per CU (LDS), each walking its share of the cells
+        k_strip_infer_mfma<<<std::min((a.n + SB - 1) / SB, 512), SB, 0, st>>>(a, net, delta);
+    else
+        k_strip_infer<<<(a.n + SB - 1) / SB, SB, 0, st>>>(a, net, delta);
     return hipGetLastError();
 }
 
 hipError_t launch_strip_apply(const StripArgs& a, const float2* delta, float* P, float* Q, float* px, float* py,
-                              float* pz, float* qx, float* qy, float* qz, hipStream_t st) {
+                              float* pz, float* qx, float* qy, float* qz, float* u, float* v, float* w,
+                              hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-    k_strip_apply<<<(a.n + SB - 1) / SB, SB, 0, st>>>(a, delta, P, Q, px, py, pz, qx, qy, qz);
+    k_strip_apply<<<(a.n + SB - 1) / SB, SB, 0, st>>>(a, delta, P, Q, px, py, pz, qx, qy, qz, u, v, w);
     return hipGetLastError();
 }
 

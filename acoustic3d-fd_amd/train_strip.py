@@ -38,27 +38,53 @@ def main():
     ap.add_argument("--every", type=int, default=3, help="sample every k-th step")
     ap.add_argument("--keep", type=float, default=0.003, help="fraction of strip cells sampled per sampled step")
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--rounds", type=int, default=3,
+                    help="data-aggregation rounds: round r >= 1 re-samples with the strip grid running the "
+                         "current network for --horizon steps between synchronisations")
+    ap.add_argument("--horizon", type=int, default=24)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=3, help="held-out judge models (seeds 90000+k)")
     ap.add_argument("--judge-nt", type=int, default=800)
+    ap.add_argument("--amp", type=float, default=10.0,
+                    help="PML amplitude of the strip's 6-cell ramp (10: the reference's constant)")
+    ap.add_argument("--tune-amp", action="store_true",
+                    help="pick the amplitude with the best worst-case reflection over 8 training models")
     ap.add_argument("--out", default=strip.NET_PATH)
     args = ap.parse_args()
 
+    seeds = [10000 + k for k in range(args.models)]
     t0 = time.time()
-    X, T, S = strip.collect([10000 + k for k in range(args.models)], n=args.n, nt=args.nt, every=args.every,
-                            keep=args.keep, log=log)
+    amp, amp_db = args.amp, {}
+    if args.tune_amp:
+        amp, amp_db = strip.tune_amp(seeds[:8], n=args.n, log=log)
+    log(f"train_strip: strip PML amplitude {amp} ({time.time() - t0:.1f} s)")
+    t0 = time.time()
+    X, T, S = strip.collect(seeds, n=args.n, nt=args.nt, every=args.every, keep=args.keep, amp=amp, log=log)
     log(f"train_strip: {len(X)} samples in {time.time() - t0:.1f} s; target rms {np.sqrt((T ** 2).mean(0))}")
-    t1 = time.time()
     params, hist = strip.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr)
-    log(f"train_strip: trained in {time.time() - t1:.1f} s; loss {hist[0]:.5f} -> {hist[-1]:.5f} "
-        f"(target variance {float((T ** 2).mean() * 2):.5f})")
-    rows = strip.judge([90000 + k for k in range(args.heldout)], params, n=args.n, nt=args.judge_nt, log=log)
+    log(f"train_strip: round 0: loss {hist[0]:.5f} -> {hist[-1]:.5f} (target mean square "
+        f"{float((T ** 2).sum(1).mean()):.5f})")
+    losses = [hist]
+    for r in range(1, args.rounds):
+        t0 = time.time()
+        Xr, Tr, Sr = strip.collect(seeds, n=args.n, nt=args.nt, every=args.every, keep=args.keep,
+                                   horizon=args.horizon, params=params, amp=amp, log=None)
+        X, T = np.concatenate([X, Xr]), np.concatenate([T, Tr])
+        params, hist = strip.train(X, T, epochs=max(5, args.epochs // 2), batch=args.batch, lr=args.lr / 2,
+                                   init=params, seed=r)
+        losses.append(hist)
+        log(f"train_strip: round {r}: +{len(Xr)} samples ({time.time() - t0:.1f} s), loss {hist[0]:.5f} -> "
+            f"{hist[-1]:.5f}")
+    rows = strip.judge([90000 + k for k in range(args.heldout)], params, n=args.n, nt=args.judge_nt, amp=amp,
+                       log=log)
     choice = strip.choose(rows)
-    summary = {"samples": int(len(X)), "models": args.models, "n": args.n, "nt": args.nt,
-               "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+    summary = {"samples": int(len(X)), "models": args.models, "n": args.n, "nt": args.nt, "amp": amp,
+               "amp_worst_db": {str(k): v for k, v in amp_db.items()},
+               "rounds": args.rounds, "horizon": args.horizon,
+               "loss_first_epoch": losses[0][0], "loss_last_epoch": losses[-1][-1],
                "target_mean_square": float((T ** 2).sum(1).mean()), "judge": rows, "choice": choice}
-    strip.save(params, args.out, loss=np.array(hist), summary=json.dumps(summary))
+    strip.save(params, args.out, amp=np.float32(amp), loss=np.concatenate(losses), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
 
 

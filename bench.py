@@ -155,6 +155,47 @@ def batched(args):
     print(json.dumps(line), flush=True)
 
 
+def strip_bench(args):
+    """Learned absorbing strip on the 256^3 VTI study model (d = 10 m, dt = 0.8 ms, nt = 2000):
+    shots/hour of per-shot runs with the 20-cell PML (reference formulation) against the 6-cell
+    learned strip, and each boundary's reflected energy and shot-record error against an
+    extended-domain FP64 run."""
+    from a3dfd import models, strip
+    t0 = time.time()
+    n, nt, every = args.strip_n, args.strip_nt, 50
+    m = models.strip_model(95000, n)
+    src = models.strip_source(95000, n)
+    params, amp = strip.load_params()
+    if params is None:
+        log("bench: no trained strip network found; the strip runs with its damping head shut")
+        params = strip.closed_params()
+    log(f"bench: strip study model {n}^3 generated in {time.time() - t0:.1f} s; strip amplitude {amp}")
+    ref_shot, ref_snaps, ext_ms = strip.run_case(m, src, nt, every, "extended", margin=64)
+    log(f"bench: extended FP64 reference run {ext_ms:.0f} ms")
+    out = {}
+    for name, b, sch in (("pml20", "pml", 0), ("strip_closed", "strip0", 0), ("strip_learned", "strip", 0),
+                         ("pml20_scheme1", "pml", 1), ("strip_learned_scheme1", "strip", 1)):
+        shot, snaps, ms = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch)
+        _, _, ms2 = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch)   # second run: steady timing
+        out[name] = {"loop_ms": round(min(ms, ms2), 1), "shots_per_hour": round(3600e3 / min(ms, ms2), 1),
+                     "reflected_db": round(float(strip.energy_db(snaps, ref_snaps)), 2),
+                     "shot_rel_l2_vs_fp64": strip.rel_l2(shot, ref_shot)}
+        log(f"bench: {name}: {out[name]}")
+    base = out["pml20"]["shots_per_hour"]
+    choice = strip.choose([{"strip": out["strip_learned"]}])
+    line = {
+        "metric": "shots/hour, learned 6-cell absorbing strip vs 20-cell PML (VTI 256^3, per-shot runs)",
+        "value": out["strip_learned"]["shots_per_hour"], "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic seeded VTI model (dipping layers, smooth eps/delta)",
+        "config": {"workload": f"vti_{n}^3_nt{nt}_strip", "d": 10.0, "dt": 8e-4, "strip_amp": amp,
+                   "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"},
+        "runs": out, "speedup_vs_pml20": round(out["strip_learned"]["shots_per_hour"] / base, 3),
+        "speedup_scheme1_vs_pml20": round(out["strip_learned_scheme1"]["shots_per_hour"] / base, 3),
+        "choice": choice,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +214,10 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven-window TTI shots: sequential per-window runs vs the batched hipGraph form")
     ap.add_argument("--nwin", type=int, default=64, help="windows of the batched run")
+    ap.add_argument("--strip", action="store_true",
+                    help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
+    ap.add_argument("--strip-n", type=int, default=256)
+    ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs every shot on a single MI355X "
@@ -190,6 +235,8 @@ def main():
 
     if args.batched:
         return batched(args)
+    if args.strip:
+        return strip_bench(args)
     t0 = time.time()
     cfg = models.cfg3()
     cfg["nt"] = args.nt

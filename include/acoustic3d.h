@@ -219,24 +219,38 @@ int a3d_batch_set_graph_chunk(a3d_batch* b, int steps);
 int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out);
 
 /* ----------------------------------------------------------------------- */
-/* Learned absorbing strip (no counterpart in the reference): VTI operator,  */
+/* Learned absorbing strip (no counterpart in the reference): VTI operator,   */
 /* FP32 storage, nb = 10 (6 updated strip cells per face plus the 4-cell     */
-/* zero ring). After every stress sweep a network (16 inputs, two hidden    */
-/* tanh layers of 64, 2 outputs; 5378 FP32 parameters) corrects P, Q and    */
-/* the normal-axis split components of each strip cell.                     */
+/* zero ring, run with the PML update of that width). After every stress    */
+/* sweep a network (16 inputs, two hidden tanh layers of 64, 2 outputs; 5378 */
+/* FP32 parameters) gives each strip cell two damping factors g in          */
+/* (0, 0.25): P, its split components, u and v are scaled by 1 - g_P; Q, its */
+/* split components and w by 1 - g_Q.                                        */
 /* ----------------------------------------------------------------------- */
 #define A3D_STRIP_NB 10
-/* params: a3d_mlp_nparams() floats, or NULL to run the plain PML of width nb. */
+/* params: a3d_mlp_nparams() floats, or NULL to run the plain PML of width nb.
+ * Bit-identical to the plain PML when the output biases are <= -30 (1 - g rounds to 1). */
 int a3d_set_strip_net(a3d_ctx* ctx, const float* params, int n);
+/* Network evaluation: 1 (default) on the matrix cores, inputs, weights of the two
+ * hidden layers and hidden activations rounded to binary16, sums in FP32; 0: FP32
+ * per lane. */
+int a3d_set_strip_precision(a3d_ctx* ctx, int mfma);
+/* While a strip network is set, the strip's PML ramps over its 6 updated cells with
+ * d0 = amp * v * ln(1e5) / (2 * 6 * d) (the reference's formula with 6 cells in place
+ * of npml and amp in place of 10; default amp = 10). */
+int a3d_set_strip_profile(a3d_ctx* ctx, float amp);
 /* Number of strip cells of the context's grid (0 before a network was set). */
 int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
 /* Teacher-forced training samples: runs shot *desc on `teacher` (same interior,
- * wider PML); at steps start, start+every, ... the teacher's state is copied onto
- * `student` (nb = A3D_STRIP_NB, network set), the student's stress sweep is applied,
- * and a fraction `keep` of the strip cells yields (features X[16], target
- * T[2] = (teacher P, Q - student P, Q) / s, scale S = s). *count: samples written (<= cap). */
+ * wider PML) and copies the teacher's state onto `student` (nb = A3D_STRIP_NB,
+ * network set) at steps start, start+h, start+2h, ... (h = horizon; horizon <= 1:
+ * h = every). With horizon > 1 the student runs its own steps in between, its
+ * network's strip corrections included. At steps start, start+every, ... a fraction
+ * `keep` of the strip cells yields, after the student's stress sweep, features
+ * X[16], target T[2] = (teacher P, Q - student P, Q) / s and scale S = s.
+ * *count: samples written (<= cap). */
 int a3d_strip_collect(a3d_ctx* teacher, a3d_ctx* student, const a3d_shot_desc* desc, int every, int start,
-                      float keep, unsigned seed, float* X, float* T, float* S, int cap, int* count);
+                      int horizon, float keep, unsigned seed, float* X, float* T, float* S, int cap, int* count);
 
 /* The network on the GPU: forward, gradient and Adam training kernels. */
 typedef struct a3d_mlp a3d_mlp;
@@ -245,9 +259,10 @@ int a3d_mlp_create(int device, a3d_mlp** out);
 int a3d_mlp_destroy(a3d_mlp* m);
 int a3d_mlp_set_params(a3d_mlp* m, const float* params, int n);
 int a3d_mlp_get_params(a3d_mlp* m, float* params, int n);
-/* Y[n][2] = net(X[n][16]). */
+/* Y[n][2]: the damping head's change of the cell's P/s and Q/s,
+ * (-X[.][0] g_P, -X[.][7] g_Q), for features X[n][16]. */
 int a3d_mlp_predict(a3d_mlp* m, const float* X, int n, float* Y);
-/* Mean over samples of the summed squared error and its gradient (FP32). */
+/* Mean over samples of the summed squared error of Y against T and its gradient (FP32). */
 int a3d_mlp_loss_grad(a3d_mlp* m, const float* X, const float* T, int n, double* loss, float* grad);
 /* Adam over shuffled minibatches, learning rate cosine-decayed from lr to lr_end;
  * loss_per_epoch (may be NULL): mean training loss of each epoch. */

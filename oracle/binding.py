@@ -65,6 +65,12 @@ def load():
     lib.oracle_set_strip_net.restype = c_int
     lib.oracle_strip_apply.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float, c_float] + [fp] * 15
     lib.oracle_strip_apply.restype = c_int
+    lib.oracle_set_strip_profile.argtypes = [c_float]
+    lib.oracle_set_strip_half.argtypes = [c_int]
+    lib.oracle_set_strip_half.restype = c_int
+    lib.oracle_round_half.argtypes = [c_float]
+    lib.oracle_round_half.restype = c_float
+    lib.oracle_set_strip_profile.restype = c_int
     _lib = lib
     return lib
 
@@ -127,6 +133,19 @@ def set_strip_net(params):
     p = np.ascontiguousarray(params, dtype=np.float32).ravel()
     if lib.oracle_set_strip_net(_fp(p), p.size) != 0:
         raise ValueError(f"strip network: wrong parameter count {p.size}")
+
+
+def set_strip_half(on):
+    load().oracle_set_strip_half(int(on))
+
+
+def round_half(f):
+    return load().oracle_round_half(float(f))
+
+
+def set_strip_profile(amp):
+    if load().oracle_set_strip_profile(amp) != 0:
+        raise ValueError("bad strip profile amplitude")
 
 
 def strip_apply(dt, dx, dy, dz, f):

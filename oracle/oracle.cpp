@@ -134,12 +134,12 @@ void read_interior(const float* src, int nx, int ny, int nz, int nnx, int nnz, i
  * split-PML profiles. dt*d0 is a float product in the reference, then the
  * expression is evaluated in double and stored to float. */
 template <class T>
-void initial_coffe(T dt, T d0, int nn, int npml, T* coff1, T* coff2, T* acoff1, T* acoff2) {
+void initial_coffe(T dt, T d0, int nn, int npml, T* coff1, T* coff2, T* acoff1, T* acoff2, int width) {
     const T dtd0 = dt * d0;
     for (int id = 0; id < nn + 2 * npml; ++id) {
         if (id < npml) {
-            double r = (npml - 0.5 - id) / npml;
-            double ra = ((npml - id) * 1.0) / npml;
+            double r = (npml - 0.5 - id) / width;
+            double ra = ((npml - id) * 1.0) / width;
             coff1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
             coff2[id] = (T)(coff1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
             acoff1[id] = (T)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
@@ -147,14 +147,21 @@ void initial_coffe(T dt, T d0, int nn, int npml, T* coff1, T* coff2, T* acoff1, 
         } else if (id < npml + nn) {
             coff1[id] = coff2[id] = acoff1[id] = acoff2[id] = (T)1.0;
         } else {
-            double r = (0.5 + id - nn - npml) / npml;
-            double ra = ((id - nn - npml) * 1.0) / npml;
+            double r = (0.5 + id - nn - npml) / width;
+            double ra = ((id - nn - npml) * 1.0) / width;
             coff1[id] = (T)(1.0 / (1.0 + (dtd0 * (r * r)) / 2.0));
             coff2[id] = (T)(coff1[id] * (1.0 - (dtd0 * (r * r)) / 2.0));
             acoff1[id] = (T)(1.0 / (1.0 + (dtd0 * (ra * ra)) / 2.0));
             acoff2[id] = (T)(acoff1[id] * (1.0 - (dtd0 * (ra * ra)) / 2.0));
         }
     }
+}
+
+/* get_d0 with amplitude amp in place of 10 (the learned strip's profile). */
+template <class T>
+T get_d0_amp(const T* vp, int nnx, int nny, int nnz, int width, float dx, float dy, float dz, double amp) {
+    const T dsum = (T)dx + (T)dy + (T)dz;
+    return (T)(amp * vp[(long)nny * nnx * nnz / 2] * std::log(100000.0) / (2.0 * width * (dsum / 3.0)));
 }
 
 /* get_d0 (iso:193-195): uses the padded z-fastest velocity at linear index N/2. */
@@ -386,48 +393,89 @@ int run_2nd(const OracleShot& p, const float* vel, T* shot, T* snaps, int nsnap_
 /* Learned absorbing strip (no counterpart in the reference; restated from the
  * strip's definition, FP32 only). Grid padding nb = 10: the 6 outer updated
  * cells of every face are strip cells, each owned by its nearest face (ties:
- * z, then y, then x). After every stress sweep each strip cell gets
- *   (dP, dQ) = s * net(f),  f = local state / s,  s = max(1e-30, |local state|),
- * added to P, Q and to the split components of the owning face's axis. Local
- * state: P at the centre, one cell inward, one outward and the two tangential
- * pairs; Q at the centre, inward, outward; the outward normal velocity (mean of
- * the two staggered samples around the cell) times rho*vp. Then vp*dt/d_axis,
- * epsilon, delta, depth/6 (1 = outermost), 0. Network: 16 -> 64 -> 64 -> 2,
- * tanh, parameters W1[64][16] b1 W2[64][64] b2 W3[2][64] b3, sums in input
- * order with fused multiply-adds. */
+ * z, then y, then x). After every stress sweep each strip cell is damped:
+ *   g = 0.25 * sigmoid(net(f)),  f = local state / s,  s = max(1e-30, |local state|),
+ *   P, px, py, pz, u, v *= 1 - g_0;  Q, qx, qy, qz, w *= 1 - g_1.
+ * Local state: P at the centre, one cell inward, one outward and the two
+ * tangential pairs of the owning face's axis; Q at the centre, inward, outward;
+ * the outward normal velocity (mean of the two staggered samples around the
+ * cell) times rho*vp. Then vp*dt/d_axis, epsilon, delta, depth/6 (1 =
+ * outermost), 0. Network: 16 -> 64 -> 64 -> 2, tanh (as 1 - 2/(exp(2a) + 1)), parameters W1[64][16] b1
+ * W2[64][64] b2 W3[2][64] b3, sums in input order with fused multiply-adds. */
 constexpr int kStripCells = 6, kNin = 16, kNh = 64, kNout = 2;
+constexpr float kGmax = 0.25f;
 constexpr int kW1 = 0, kB1 = kW1 + kNh * kNin, kW2 = kB1 + kNh, kB2 = kW2 + kNh * kNh, kW3 = kB2 + kNh,
               kB3 = kW3 + kNout * kNh, kNpar = kB3 + kNout;
 std::vector<float> g_strip;   /* empty: plain PML */
+double g_strip_amp = 10.0;    /* PML amplitude of the strip's 6-cell ramp */
+bool g_strip_half = true;     /* network operands rounded to binary16 (matrix-core evaluation) */
+
+/* Nearest binary16 value (ties to even), as a float; finite inputs below 65504 in magnitude. */
+float round_half(float f) {
+    const float a = std::fabs(f);
+    if (a < 6.103515625e-05f) return std::nearbyint(f * 16777216.0f) / 16777216.0f;   /* subnormal step 2^-24 */
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x0FFFu + ((u >> 13) & 1u);
+    u &= 0xFFFFE000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+/* tanh(a) = 1 - 2 / (exp(2a) + 1), the network's activation */
+inline float act_tanh(float a) { return 1.0f - 2.0f / (std::exp(2.0f * a) + 1.0f); }
 
 void strip_net(const float* p, const float* x, float* y) {
     float h1[kNh], h2[kNh];
+    if (!g_strip_half) {
+        for (int j = 0; j < kNh; ++j) {
+            float a = p[kB1 + j];
+            for (int i = 0; i < kNin; ++i) a = std::fma(p[kW1 + j * kNin + i], x[i], a);
+            h1[j] = act_tanh(a);
+        }
+        for (int j = 0; j < kNh; ++j) {
+            float a = p[kB2 + j];
+            for (int i = 0; i < kNh; ++i) a = std::fma(p[kW2 + j * kNh + i], h1[i], a);
+            h2[j] = act_tanh(a);
+        }
+        for (int o = 0; o < kNout; ++o) {
+            float a = p[kB3 + o];
+            for (int i = 0; i < kNh; ++i) a = std::fma(p[kW3 + o * kNh + i], h2[i], a);
+            y[o] = a;
+        }
+        return;
+    }
+    /* binary16 operands of the two hidden layers (products exact in float), sums from 0,
+     * bias added after; the last layer in float over the binary16 activations */
+    float xh[kNin];
+    for (int i = 0; i < kNin; ++i) xh[i] = round_half(x[i]);
     for (int j = 0; j < kNh; ++j) {
-        float a = p[kB1 + j];
-        for (int i = 0; i < kNin; ++i) a = std::fma(p[kW1 + j * kNin + i], x[i], a);
-        h1[j] = std::tanh(a);
+        float a = 0.f;
+        for (int i = 0; i < kNin; ++i) a += round_half(p[kW1 + j * kNin + i]) * xh[i];
+        h1[j] = round_half(act_tanh(a + p[kB1 + j]));
     }
     for (int j = 0; j < kNh; ++j) {
-        float a = p[kB2 + j];
-        for (int i = 0; i < kNh; ++i) a = std::fma(p[kW2 + j * kNh + i], h1[i], a);
-        h2[j] = std::tanh(a);
+        float a = 0.f;
+        for (int i = 0; i < kNh; ++i) a += round_half(p[kW2 + j * kNh + i]) * h1[i];
+        h2[j] = round_half(act_tanh(a + p[kB2 + j]));
     }
     for (int o = 0; o < kNout; ++o) {
-        float a = p[kB3 + o];
+        float a = 0.f;
         for (int i = 0; i < kNh; ++i) a = std::fma(p[kW3 + o * kNh + i], h2[i], a);
-        y[o] = a;
+        y[o] = a + p[kB3 + o];
     }
 }
 
 /* Oracle layout: z fastest, id = iz + nnz*ix + pl*iy. */
 void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
-                   const float* u, const float* v, const float* w, float* px, float* py, float* pz, float* qx,
+                   float* u, float* v, float* w, float* px, float* py, float* pz, float* qx,
                    float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del) {
     const long pl = (long)nnz * nnx;
     const long stride[3] = {nnz, pl, 1};   /* x, y, z */
     const int n3[3] = {nnx, nny, nnz};
     const float cdt[3] = {dt / dx, dt / dy, dt / dz};
-    struct Cell { long id; int axis; float dp, dq; };
+    struct Cell { long id; float gp, gq; };
     std::vector<std::vector<Cell>> rows(nny);
 #pragma omp parallel for schedule(static)
     for (int iy = mm; iy < nny - mm; ++iy)
@@ -461,14 +509,22 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
                 x[14] = (float)best * (1.0f / kStripCells);
                 x[15] = 0.f;
                 strip_net(g_strip.data(), x, y);
-                rows[iy].push_back(Cell{id, axis, s * y[0], s * y[1]});
+                rows[iy].push_back(Cell{id, kGmax / (1.0f + std::exp(-y[0])), kGmax / (1.0f + std::exp(-y[1]))});
             }
     for (const auto& r : rows)
         for (const Cell& c : r) {
-            P[c.id] += c.dp;
-            Q[c.id] += c.dq;
-            (c.axis == 0 ? px : (c.axis == 1 ? py : pz))[c.id] += c.dp;
-            (c.axis == 0 ? qx : (c.axis == 1 ? qy : qz))[c.id] += c.dq;
+            const float fp = 1.0f - c.gp, fq = 1.0f - c.gq;
+            P[c.id] *= fp;
+            px[c.id] *= fp;
+            py[c.id] *= fp;
+            pz[c.id] *= fp;
+            Q[c.id] *= fq;
+            qx[c.id] *= fq;
+            qy[c.id] *= fq;
+            qz[c.id] *= fq;
+            u[c.id] *= fp;
+            v[c.id] *= fp;
+            w[c.id] *= fq;
         }
 }
 
@@ -511,13 +567,17 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
     }
 
     const T dt = (T)p.dt;
-    const T d0 = get_d0<T>(vp.data(), nnx, nny, nnz, npml, p.dx, p.dy, p.dz);
+    /* the learned strip (FP32 VTI, network set) ramps its PML over its 6 cells */
+    const bool strip = std::is_same<T, float>::value && vti && !g_strip.empty();
+    const int width = strip ? kStripCells : npml;
+    const T d0 = strip ? get_d0_amp<T>(vp.data(), nnx, nny, nnz, width, p.dx, p.dy, p.dz, g_strip_amp)
+                       : get_d0<T>(vp.data(), nnx, nny, nnz, npml, p.dx, p.dy, p.dz);
     std::vector<T> cx1(nnx), cx2(nnx), ax1(nnx), ax2(nnx);
     std::vector<T> cy1(nny), cy2(nny), ay1(nny), ay2(nny);
     std::vector<T> cz1(nnz), cz2(nnz), az1(nnz), az2(nnz);
-    initial_coffe<T>(dt, d0, nx, npml, cx1.data(), cx2.data(), ax1.data(), ax2.data());
-    initial_coffe<T>(dt, d0, ny, npml, cy1.data(), cy2.data(), ay1.data(), ay2.data());
-    initial_coffe<T>(dt, d0, nz, npml, cz1.data(), cz2.data(), az1.data(), az2.data());
+    initial_coffe<T>(dt, d0, nx, npml, cx1.data(), cx2.data(), ax1.data(), ax2.data(), width);
+    initial_coffe<T>(dt, d0, ny, npml, cy1.data(), cy2.data(), ay1.data(), ay2.data(), width);
+    initial_coffe<T>(dt, d0, nz, npml, cz1.data(), cz2.data(), az1.data(), az2.data(), width);
 
     const T dtx = dt / (T)p.dx, dty = dt / (T)p.dy, dtz = dt / (T)p.dz;   /* iso:109-111 */
 
@@ -755,8 +815,21 @@ int oracle_set_strip_net(const float* params, int n) {
     return 0;
 }
 
+int oracle_set_strip_half(int on) {
+    g_strip_half = on != 0;
+    return 0;
+}
+
+float oracle_round_half(float f) { return round_half(f); }
+
+int oracle_set_strip_profile(float amp) {
+    if (!(amp >= 0.0f)) return -1;
+    g_strip_amp = amp;
+    return 0;
+}
+
 int oracle_strip_apply(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
-                       const float* u, const float* v, const float* w, float* px, float* py, float* pz, float* qx,
+                       float* u, float* v, float* w, float* px, float* py, float* pz, float* qx,
                        float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del) {
     if (g_strip.empty()) return -1;
     strip_correct(nnx, nny, nnz, dt, dx, dy, dz, P, Q, u, v, w, px, py, pz, qx, qy, qz, vp, rho, eps, del);
@@ -795,9 +868,9 @@ int oracle_wavelet(int prec, int wtype, float fm, float dt, int nt, void* out) {
 int oracle_pml_tables(int prec, int n, int npml, float dt, float d0, void* c1, void* c2, void* a1,
                       void* a2) {
     if (prec == 32)
-        initial_coffe<float>(dt, d0, n, npml, (float*)c1, (float*)c2, (float*)a1, (float*)a2);
+        initial_coffe<float>(dt, d0, n, npml, (float*)c1, (float*)c2, (float*)a1, (float*)a2, npml);
     else
-        initial_coffe<double>(dt, d0, n, npml, (double*)c1, (double*)c2, (double*)a1, (double*)a2);
+        initial_coffe<double>(dt, d0, n, npml, (double*)c1, (double*)c2, (double*)a1, (double*)a2, npml);
     return 0;
 }
 

@@ -56,10 +56,19 @@ int oracle_run(int prec, const OracleShot* p,
  * for a wrong parameter count. */
 int oracle_set_strip_net(const float* params, int n);
 
+/* Strip network evaluation: 1 (default) binary16 operands of the hidden layers
+ * (the library's matrix-core evaluation), 0 plain FP32 with fused multiply-adds. */
+int oracle_set_strip_half(int on);
+/* Nearest binary16 value of f (ties to even), as a float. */
+float oracle_round_half(float f);
+
+/* PML amplitude of the strip's 6-cell ramp while a strip network is set (default 10). */
+int oracle_set_strip_profile(float amp);
+
 /* One strip correction on padded arrays (oracle layout: z fastest, then x, then y;
  * extents nnz, nnx, nny) with the network set above; -1 if none is set. */
 int oracle_strip_apply(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
-                       const float* u, const float* v, const float* w, float* px, float* py, float* pz,
+                       float* u, float* v, float* w, float* px, float* py, float* pz,
                        float* qx, float* qy, float* qz, const float* vp, const float* rho, const float* eps,
                        const float* del);
 

@@ -23,8 +23,9 @@ mlp = _mlp()
 
 
 def _params(seed, out_scale=1.0):
+    """Random parameters with an open damping head (output bias about 0: g about 0.125)."""
     rng = np.random.default_rng(seed)
-    p = mlp.init(seed).astype(np.float64)
+    p = mlp.init(seed, out_bias=0.0).astype(np.float64)
     p[mlp.NPAR - 130:] += rng.normal(0, 0.2, 130) * out_scale
     p[64 * 16:64 * 16 + 64] += rng.normal(0, 0.1, 64)   # b1
     return p.astype(np.float32)
@@ -72,7 +73,7 @@ def test_training_fits_a_teacher_network():
     T, _ = mlp.forward(_params(4), X)
     T = T.astype(np.float32)
     with Mlp() as net:
-        net.set_params(mlp.init(5))
+        net.set_params(mlp.init(5, out_bias=0.0))
         l0 = net.loss_grad(X[:8192], T[:8192])[0]
         hist = net.train(X, T, epochs=20, batch=2048, lr=3e-3, lr_end=1e-5, seed=1)
         l1 = net.loss_grad(X[:8192], T[:8192])[0]
@@ -99,53 +100,50 @@ def _run(m, d, net=None):
         return mdl.run(nsnap_max=8)
 
 
-def _damping_net(seed, gamma=0.1):
-    """A random network with a tiny output layer plus one clean path y ~ -gamma * 0.46 * (P, Q)
-    at the cell: a stable, damping correction of a few per cent per step."""
-    w = mlp.unpack(mlp.init(seed))
-    w = {k: v.copy() for k, v in w.items()}
-    w["W3"] *= 0.02
-    w["W1"][3, :] = 0.0
-    w["W1"][3, 0] = 0.5
-    w["W2"][5, :] = 0.0
-    w["W2"][5, 3] = 1.0
-    w["W3"][:, 5] = -gamma
-    w["W1"][4, :] = 0.0
-    w["W1"][4, 7] = 0.5
-    w["W2"][6, :] = 0.0
-    w["W2"][6, 4] = 1.0
-    w["W3"][:, 6] = 0.0
-    w["W3"][1, 6] = -gamma
-    w["W3"][1, 5] = 0.0
-    return np.concatenate([w[n].ravel() for n, _ in mlp.SHAPES]).astype(np.float32)
-
-
-def test_zero_network_equals_plain_pml():
+def test_closed_network_equals_oracle_bitwise(oracle):
+    """With the damping head shut the strip is its own 6-cell PML: bitwise equal to the oracle."""
+    from a3dfd import strip
     m = _small()
     d = _desc(30, 28, 26, 200, 10, snap_interval=50)
-    a, sa, _ = _run(m, d)
-    b, sb, _ = _run(m, d, np.zeros(mlp.NPAR, np.float32))
+    a, sa, _ = _run(m, d, strip.closed_params())
+    os_ = oracle.make_shot("vti", 30, 28, 26, 10.0, 10.0, 10.0, 4, 25, 3, 200, 1e-3, 25.0, nb=10, snap_interval=50,
+                           mute_tt=0)
+    oracle.set_strip_net(strip.closed_params())
+    try:
+        b, sb, _ = oracle.run(os_, 32, vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"], nsnap_max=8)
+    finally:
+        oracle.set_strip_net(None)
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("scheme", [0, 1])
-def test_strip_matches_oracle(oracle, scheme):
+@pytest.mark.parametrize("scheme,mfma", [(0, 1), (1, 1), (0, 0)])
+def test_strip_matches_oracle(oracle, scheme, mfma):
+    """The strip correction fused into the time loop against the oracle's restatement: the
+    matrix-core (binary16 operand) network against the oracle's binary16 form, the FP32
+    per-lane network against its FP32 form."""
+    from a3dfd import Modeller
     m = _small()
     nt = 220
     d = _desc(30, 28, 26, nt, 10, snap_interval=55, scheme=scheme)
-    net = _damping_net(6)
-    shot, snaps, _ = _run(m, d, net)
+    net = _params(6)
+    with Modeller(d) as mdl:
+        mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+        mdl.set_strip_precision(mfma)
+        mdl.set_strip_net(net)
+        shot, snaps, _ = mdl.run(nsnap_max=8)
     plain, _, _ = _run(m, d)
     os_ = oracle.make_shot("vti", 30, 28, 26, 10.0, 10.0, 10.0, 4, 25, 3, nt, 1e-3, 25.0, nb=10, snap_interval=55,
                            mute_tt=0)
     oracle.set_strip_net(net)
+    oracle.set_strip_half(mfma)
     try:
         rshot, rsnaps, _ = oracle.run(os_, 32, vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"], nsnap_max=8)
     finally:
         oracle.set_strip_net(None)
+        oracle.set_strip_half(1)
     e_shot, e_snap = rel_l2(shot, rshot), rel_l2(snaps, rsnaps)
     effect = rel_l2(plain, rshot)
-    print(f"[gpu-strip] scheme {scheme}: shot rel L2 {e_shot:.2e}, snapshots {e_snap:.2e}; "
+    print(f"[gpu-strip] scheme {scheme} mfma {mfma}: shot rel L2 {e_shot:.2e}, snapshots {e_snap:.2e}; "
           f"network effect on the record {effect:.2e}")
     assert effect > 1e-3          # the correction matters
     assert e_shot <= 1e-5 and e_snap <= 1e-5
@@ -158,7 +156,7 @@ def test_collect_samples():
     with Modeller(dt_) as t, Modeller(_desc(24, 22, 20, 150, 10)) as s:
         for mdl in (t, s):
             mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
-        s.set_strip_net(np.zeros(mlp.NPAR, np.float32))
+        s.set_strip_net(mlp.init(0))
         ncell = s.strip_cells()
         X, T, S = strip_collect(t, s, dt_, every=5, start=0, keep=0.5, seed=3, cap=1 << 20)
         X2, T2, S2 = strip_collect(t, s, dt_, every=5, start=0, keep=0.5, seed=3, cap=1 << 20)
