@@ -58,7 +58,9 @@ namespace {
 enum Vol {
     V_VP, V_RHO, V_EPS, V_DEL, V_CT, V_ST, V_CP, V_SP,   // TTI: cos/sin of tilt and azimuth
     V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ,
-    V_VV, V_PA, V_PB, NVOL
+    V_VV, V_PA, V_PB,
+    V_P2, V_Q2, V_U2, V_V2, V_W2,   // second copies for the one-pass VTI step (scheme 2), allocated on first use
+    NVOL
 };
 
 std::vector<int> volumes_of(int op) {
@@ -73,7 +75,7 @@ std::vector<int> volumes_of(int op) {
     return {};
 }
 
-bool is_wavefield(int v) { return (v >= V_U && v <= V_QZ) || v == V_PA || v == V_PB; }
+bool is_wavefield(int v) { return (v >= V_U && v <= V_QZ) || v == V_PA || v == V_PB || (v >= V_P2 && v <= V_W2); }
 bool is_split(int v) { return v >= V_PX && v <= V_QZ; }
 
 // ---------------------------------------------------------------------------
@@ -281,7 +283,9 @@ int check_desc_shot(const a3d_ctx* c, const a3d_shot_desc* d) {
     if (!(d->dt > 0) || !(d->fm > 0)) return fail(A3D_EINVAL, "dt and fm must be positive");
     if (d->rec_z < 0 || d->rec_z >= d->nz) return fail(A3D_EINVAL, "rec_z %d outside [0,%d)", d->rec_z, d->nz);
     if (d->snap_interval < 0) return fail(A3D_EINVAL, "snap_interval must be >= 0");
-    if (d->scheme < 0 || d->scheme > 1) return fail(A3D_EINVAL, "scheme %d not in 0..1", d->scheme);
+    if (d->scheme < 0 || d->scheme > 2) return fail(A3D_EINVAL, "scheme %d not in 0..2", d->scheme);
+    if (d->scheme == 2 && (d->op != A3D_OP_VTI || c->prec != PREC_FP32 || c->strip_net))
+        return fail(A3D_EINVAL, "scheme 2 (one-pass step) is built for the VTI operator, FP32 storage, PML boundary");
     if (d->op == A3D_OP_2ND) {
         if (d->sx < 0 || d->sx >= d->nx || d->sy < 0 || d->sy >= d->ny || d->sz < 0 || d->sz + 2 >= d->nz + d->nb)
             return fail(A3D_EINVAL, "source (%d,%d,%d) outside the model", d->sx, d->sy, d->sz);
@@ -626,6 +630,70 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         const int op = d->op;
         const bool strip = c->strip_net != nullptr;
         const StripArgs stp = strip_args(c, d);
+        if (d->scheme == 2) {
+            // one-pass step: old and new stresses/velocities in separate buffers, swapped every step
+            for (int v : {V_P2, V_Q2, V_U2, V_V2, V_W2}) {
+                if (!c->vol[v] && hipMalloc(&c->vol[v], wbytes) != hipSuccess) {
+                    c->vol[v] = nullptr;
+                    return fail(A3D_ENOMEM, "one-pass step buffers: allocation failed");
+                }
+                HIP_TRY(hipMemsetAsync(c->vol[v], 0, wbytes, c->st));
+            }
+            FusedArgs fa{};
+            fa.g = g;
+            fa.l = make_launch(g, c->ty[1], c->zseg[1]);
+            fa.c = sa.c;
+            fa.tab = c->tab;
+            fa.px = (float*)c->vol[V_PX];
+            fa.py = (float*)c->vol[V_PY];
+            fa.pz = (float*)c->vol[V_PZ];
+            fa.qx = (float*)c->vol[V_QX];
+            fa.qy = (float*)c->vol[V_QY];
+            fa.qz = (float*)c->vol[V_QZ];
+            fa.vp = (const float*)c->vol[V_VP];
+            fa.rho = (const float*)c->vol[V_RHO];
+            fa.eps = (const float*)c->vol[V_EPS];
+            fa.del = (const float*)c->vol[V_DEL];
+            fa.dtx = sa.dtx;
+            fa.dty = sa.dty;
+            fa.dtz = sa.dtz;
+            fa.sv = sa.sv;
+            fa.sxp = sa.sxp;
+            fa.syp = sa.syp;
+            fa.szp = sa.szp;
+            fa.taper = sa.taper;
+            fa.zrec = sa.zrec;
+            fa.nb = sa.nb;
+            fa.nx = sa.nx;
+            fa.ny = sa.ny;
+            float* cur[5] = {(float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_U], (float*)c->vol[V_V],
+                             (float*)c->vol[V_W]};
+            float* nxt[5] = {(float*)c->vol[V_P2], (float*)c->vol[V_Q2], (float*)c->vol[V_U2], (float*)c->vol[V_V2],
+                             (float*)c->vol[V_W2]};
+            HIP_TRY(hipEventRecord(c->ev0, c->st));
+            for (int it = 0; it < nt; it++) {
+                fa.P0 = cur[0];
+                fa.Q0 = cur[1];
+                fa.u0 = cur[2];
+                fa.v0 = cur[3];
+                fa.w0 = cur[4];
+                fa.P1 = nxt[0];
+                fa.Q1 = nxt[1];
+                fa.u1 = nxt[2];
+                fa.v1 = nxt[3];
+                fa.w1 = nxt[4];
+                fa.s = srcs[it];
+                fa.rec = c->rec + plane_rec * it;
+                timing_begin(c, 1);
+                HIP_TRY(launch_fused_vti(c->ty[1], fa, c->st));
+                timing_end(c);
+                for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
+                if (want_snap(it)) {
+                    if ((rc = take_snap(it, cur[0], -1, 0.0, -1, 0.0, false))) return rc;
+                }
+            }
+            HIP_TRY(hipEventRecord(c->ev1, c->st));
+        } else {
         HIP_TRY(hipEventRecord(c->ev0, c->st));
         for (int it = 0; it < nt; it++) {
             va.s = srcs[it];
@@ -642,6 +710,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             }
         }
         HIP_TRY(hipEventRecord(c->ev1, c->st));
+        }
     }
     HIP_TRY(hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -1339,9 +1408,13 @@ int legacy_run(Legacy& L, const char* FNsnap, const char* FNshot, FILE* fshot, i
 a3d_shot_desc base_desc(int op, int nx, int ny, int nz, float dx, float dy, float dz, int nb, int nt,
                         float dt, float fm) {
     a3d_shot_desc d{};
-    // A3D_SCHEME=1 selects the unsplit-interior stress update for the legacy entry points
-    // (default 0: the reference's split fields everywhere, bit for bit).
-    if (const char* e = getenv("A3D_SCHEME")) d.scheme = (op != A3D_OP_2ND && e[0] == '1') ? 1 : 0;
+    // A3D_SCHEME=1 selects the unsplit-interior stress update for the legacy entry points,
+    // A3D_SCHEME=2 the one-pass step for VTI (scheme 1 for ISO/TTI); default 0: the
+    // reference's split fields everywhere, bit for bit.
+    if (const char* e = getenv("A3D_SCHEME")) {
+        if (op != A3D_OP_2ND && (e[0] == '1' || e[0] == '2')) d.scheme = 1;
+        if (op == A3D_OP_VTI && e[0] == '2') d.scheme = 2;
+    }
     d.op = op;
     d.nx = nx;
     d.ny = ny;

@@ -104,6 +104,25 @@ struct StressArgs {
     Scales sc;
 };
 
+// One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out
+// (ping-pong buffers), split PML fields in place.
+struct FusedArgs {
+    Geo g;
+    Launch l;
+    Pml c;
+    const void* tab;
+    const float *P0, *Q0, *u0, *v0, *w0;   // stresses at step n, velocities at n-1/2
+    float *P1, *Q1, *u1, *v1, *w1;         // stresses at n+1, velocities at n+1/2
+    float *px, *py, *pz, *qx, *qy, *qz;
+    const float *vp, *rho, *eps, *del;
+    double dtx, dty, dtz;
+    Src s;                                 // this step's source, added to P0, Q0 on load
+    int sv, sxp, syp, szp;
+    const double* taper;
+    float* rec;
+    int zrec, nb, nx, ny;
+};
+
 struct Step2Args {
     Geo g;
     Launch l;
@@ -121,6 +140,8 @@ struct Step2Args {
 };
 
 hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st);
+hipError_t launch_fused_vti(int ty, const FusedArgs& a, hipStream_t st);
+int fused_blocks_per_cu(int ty);
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st);
 hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 

@@ -125,8 +125,11 @@ typedef struct a3d_shot_desc {
                               everywhere, the reference's arithmetic (FP32 bitwise equal to it);
                               1: split fields only in PML cells, P and Q updated directly where all
                               stress PML factors are 1 (same equations, different rounding: within
-                              1e-5 relative L2 of scheme 0, 80 instead of 120 B/point for VTI).
-                              Storage 1 (binary16) always uses scheme 1. */
+                              1e-5 relative L2 of scheme 0, 80 instead of 120 B/point for VTI);
+                              2: VTI with FP32 storage only: the scheme-1 arithmetic in one
+                              z-streaming pass per step (velocities made on the chip where the
+                              stress update needs them; bitwise equal to scheme 1, 56 B/point
+                              in the interior). Storage 1 (binary16) always uses scheme 1. */
 } a3d_shot_desc;
 
 typedef struct a3d_ctx a3d_ctx;

@@ -150,3 +150,48 @@ def test_unsplit_scheme_within_parity_bar(oracle, op):
           f"scheme 0 vs FP64 {rel_l2(r32, r64):.2e}")
     assert e <= 1e-5 and es <= 1e-5
     assert not np.array_equal(g, r32) or op == "iso"   # a different rounding path is really taken
+
+
+@pytest.mark.parametrize("launch", [None, (8, 1, 8, 1), (4, 3, 4, 3), (2, 5, 2, 5), (8, 7, 8, 7)])
+def test_one_pass_step_equals_two_sweeps(oracle, launch):
+    """scheme=2 (velocity and stress updates in one z-streaming kernel, ping-pong buffers) does
+    the scheme-1 arithmetic: bitwise equal to the two-sweep scheme-1 run for every launch shape
+    (partial x tiles, y tiles, z segments whose warm-up recomputes the velocities of the
+    segment below), and within the parity bar of the FP32 oracle."""
+    from a3dfd import Modeller, make_desc
+    from a3dfd.models import small_model
+    nx, ny, nz, nb = 70, 41, 30, 10
+    m = small_model("vti", nx, ny, nz, seed=19)
+    args = ("vti", nx, ny, nz, D, D, D, 36, 21, 4, 180, 1e-3, 25.0)
+    outs = {}
+    for scheme in (1, 2):
+        with Modeller(make_desc(*args, nb=nb, scheme=scheme, snap_interval=60)) as mdl:
+            mdl.set_model(**m)
+            if launch:
+                mdl.set_launch(*launch)
+            outs[scheme] = mdl.run(nsnap_max=4)
+    r32, s32, _ = oracle.run(oracle.make_shot(*args, nb=nb, snap_interval=60), 32, nsnap_max=4, **m)
+    (a, sa, _), (b, sb, _) = outs[1], outs[2]
+    print(f"\n[gpu-parity] one-pass step launch={launch}: vs two sweeps max|d| {np.abs(a - b).max():.3e}; "
+          f"vs FP32 oracle {rel_l2(b, r32):.2e}")
+    assert np.abs(b).max() > 0
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    assert rel_l2(b, r32) <= 1e-5 and rel_l2(sb, s32) <= 1e-5
+
+
+def test_one_pass_step_source_near_faces(oracle):
+    """Sources next to the low and high faces of a small grid: the halo velocities the one-pass
+    step recomputes reach into the PML and the unwritten ring on every side."""
+    from a3dfd import Modeller, make_desc
+    from a3dfd.models import small_model
+    nx, ny, nz, nb = 26, 22, 24, 10
+    m = small_model("vti", nx, ny, nz, seed=23)
+    for src in ((1, 1, 1), (nx, ny, nz), (1, ny, 12)):
+        args = ("vti", nx, ny, nz, D, D, D, *src, 150, 1e-3, 25.0)
+        res = []
+        for scheme in (1, 2):
+            with Modeller(make_desc(*args, nb=nb, scheme=scheme, snap_interval=149)) as mdl:
+                mdl.set_model(**m)
+                mdl.set_launch(4, 2, 4, 2)
+                res.append(mdl.run(nsnap_max=2))
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), src
