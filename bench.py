@@ -132,7 +132,7 @@ def batched(args):
     wins = models.uneven_windows(args.nwin, cfg["nx"], cfg["ny"], cfg["nz"])
     log(f"bench: TTI model and {len(wins)} windows generated in {time.time() - t0:.1f} s")
     desc = make_desc("tti", cfg["nx"], cfg["ny"], cfg["nz"], cfg["dx"], cfg["dy"], cfg["dz"], 1, 1, 1, nt,
-                     cfg["dt"], cfg["fm"])
+                     cfg["dt"], cfg["fm"], scheme=args.scheme)
     res = {}
     with Batch(desc, [Window(*w) for w in wins]) as b:
         b.set_model(vel=cfg["vel"], eps=cfg["eps"], delta=cfg["delta"], theta=cfg["theta"], phi=cfg["phi"])
@@ -153,7 +153,7 @@ def batched(args):
         "metric": "shots/hour, 64 uneven-window TTI shots (batched hipGraph vs sequential per-window runs)",
         "value": res[2]["shots_per_hour"], "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic seeded TTI model (layered + noise, smooth eps/delta/tilt/azimuth)",
-        "config": {"workload": f"tti_600x600x300_{len(wins)}win_nt{nt}", "npml": 40, "nt": nt,
+        "config": {"workload": f"tti_600x600x300_{len(wins)}win_nt{nt}", "npml": 40, "nt": nt, "scheme": args.scheme,
                    "windows": "uniform in [160..320]^2 x [200..300] cells, seed 4",
                    "updated_points_per_step": info["updated_points"], "size_classes": ncls},
         "sequential": res[0], "batched_graph": res[2],
@@ -203,6 +203,66 @@ def strip_bench(args):
     print(json.dumps(line), flush=True)
 
 
+# Algorithmic bytes per updated point (FP32): (velocity sweep or 2nd-order step, stress sweep with
+# split fields, stress sweep without them) -- reads + writes of each sweep, see DESIGN.md's table.
+OP_BYTES = {"2nd": (16, 0, 0), "iso": (32, 48, 28), "vti": (36, 84, 44), "tti": (48, 96, 56)}
+
+
+def operators(args):
+    """Every operator at its configuration's grid: 2nd order 100^3 (nt 500, sponge 20), ISO 256^3
+    (nt 2000, npml 20), VTI 400x400x300 (nt 3000), TTI 600x600x300 (one full-model shot, nt 500,
+    npml 40); the 1st-order ones with both stress schemes. One JSON line per run."""
+    from a3dfd import Modeller, make_desc, models
+    runs = []
+    c1 = models.cfg1()
+    runs.append(("2nd", c1, dict(nb=20), 0, c1["nt"]))
+    c2 = models.cfg2()
+    runs += [("iso", c2, dict(v_mute=2000.0), s, c2["nt"]) for s in (0, 1)]
+    c3 = models.cfg3()
+    runs += [("vti", c3, {}, s, c3["nt"]) for s in (0, 1)]
+    c4 = models.cfg4_model()
+    c4.update(sx=301, sy=301, sz=1)
+    runs += [("tti", c4, {}, s, 500) for s in (0, 1)]
+    for op, c, kw, scheme, nt in runs:
+        keys = {"2nd": ("vel",), "iso": ("vel", "rho"), "vti": ("vel", "rho", "eps", "delta"),
+                "tti": ("vel", "eps", "delta", "theta", "phi")}[op]
+        d = make_desc(op, c["nx"], c["ny"], c["nz"], c["dx"], c["dy"], c["dz"], c["sx"], c["sy"], c["sz"], nt,
+                      c["dt"], c["fm"], scheme=scheme, **kw)
+        with Modeller(d) as mdl:
+            mdl.set_model(**{k: c[k] for k in keys})
+            mdl.run(shot=False)                     # warm-up (and autotune)
+            mdl.set_kernel_timing(True)
+            mdl.kernel_times(reset=True)
+            _, _, ms = mdl.run(shot=False)
+            kms, kn = mdl.kernel_times(reset=True)
+            gi = mdl.grid_info()
+            launch = mdl.launch_info()
+        npts = gi["updated_points"]
+        a0, a_split, a_unsplit = OP_BYTES[op]
+        nbb = d.nb
+        inner = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1) if op != "2nd" else npts
+        f = min(1.0, inner / npts)
+        a1 = a_split if scheme == 0 else f * a_unsplit + (1 - f) * a_split
+        k0 = kms[0] / max(kn[0], 1)
+        k1 = kms[1] / max(kn[1], 1) if kn[1] else 0.0
+        g0 = a0 * npts / (k0 * 1e-3) / 1e9
+        g1 = a1 * npts / (k1 * 1e-3) / 1e9 if k1 else 0.0
+        gpts = npts * nt / ms / 1e6
+        line = {"operator": op, "workload": f"{op}_{c['nx']}x{c['ny']}x{c['nz']}_nt{nt}", "boundary": nbb,
+                "scheme": scheme, "Gpt_s": round(gpts, 2), "loop_ms": round(ms, 1), "updated_points": npts,
+                "launch": launch,
+                "kernels": {("k_step2" if op == "2nd" else "k_vel"): {"avg_ms": round(k0, 4), "bytes_per_point": a0,
+                                                                       "GBs": round(g0, 1),
+                                                                       "frac": round(g0 / HBM_PEAK_GBS, 4)}},
+                "whole_step_GBs": round((a0 + a1) * gpts, 1),
+                "whole_step_frac": round((a0 + a1) * gpts / HBM_PEAK_GBS, 4)}
+        if op != "2nd":
+            line["kernels"]["k_stress"] = {"avg_ms": round(k1, 4), "bytes_per_point": round(a1, 2),
+                                           "GBs": round(g1, 1), "frac": round(g1 / HBM_PEAK_GBS, 4)}
+        log(f"bench: {line['workload']} scheme {scheme}: {gpts:.2f} Gpt/s")
+        print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +282,8 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="64 uneven-window TTI shots: sequential per-window runs vs the batched hipGraph form")
     ap.add_argument("--nwin", type=int, default=64, help="windows of the batched run")
+    ap.add_argument("--operators", action="store_true",
+                    help="every operator at its configuration's grid, both stress schemes; one JSON line each")
     ap.add_argument("--strip", action="store_true",
                     help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
     ap.add_argument("--strip-n", type=int, default=256)
@@ -245,6 +307,8 @@ def main():
         return batched(args)
     if args.strip:
         return strip_bench(args)
+    if args.operators:
+        return operators(args)
     t0 = time.time()
     cfg = models.cfg3()
     cfg["nt"] = args.nt
