@@ -107,3 +107,25 @@ def test_parallel_run_on_streams(tmp_path, oracle):
         ref, _, _ = oracle.run(oracle.make_shot("vti", NX, NY, NZ, D, D, D, ishot[0], ishot[1], ishot[2], 60,
                                                 0.001, 25.0, nb=20), 32, **m)
         assert np.array_equal(got, ref)
+
+
+def test_legacy_scheme_environment(tmp_path, oracle, monkeypatch):
+    """A3D_SCHEME selects the stress update of the legacy entry points: 1 (split fields in PML
+    cells only) within the parity bar of the reference arithmetic, 2 (one-pass VTI step) bitwise
+    equal to 1."""
+    import readmodel as rm
+    m, model = _files(tmp_path, "vti", seed=29)
+    got = {}
+    for scheme in ("0", "1", "2"):
+        monkeypatch.setenv("A3D_SCHEME", scheme)
+        out = tmp_path / f"s{scheme}"
+        out.mkdir()
+        sg = rm.ShotReceiver(9, 8, 2, 1, 1, 1, 9, 8, 2, dgx=10.0, dgy=10.0, dgt=0.001, shotfolder=str(out) + "/")
+        fd = rm.Acoustic3dvti(model=model, sg=sg, nt=150, dt=0.001, fpeak=25.0, snap_folder=str(out) + "/",
+                              snap_interval=1000)
+        fd.run(sg.shotlist[0], 0)
+        got[scheme] = np.fromfile(fd.shot_name(sg.shotlist[0]), dtype=np.float32)
+    ref, _, _ = oracle.run(oracle.make_shot("vti", NX, NY, NZ, D, D, D, 9, 8, 2, 150, 0.001, 25.0, nb=20), 32, **m)
+    assert np.array_equal(got["0"], ref.ravel())
+    assert rel_l2(got["1"], ref.ravel()) <= 1e-5
+    assert np.array_equal(got["1"], got["2"])
