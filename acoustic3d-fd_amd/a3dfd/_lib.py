@@ -128,6 +128,7 @@ def load(path=None):
     lib.a3d_strip_info.argtypes = [vp, POINTER(c_int)]
     lib.a3d_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_set_strip_precision.argtypes = [vp, c_int]
+    lib.a3d_set_zchunk.argtypes = [vp, c_int]
     lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_int, c_float, ctypes.c_uint, fp, fp,
                                       fp, c_int, POINTER(c_int)]
     lib.a3d_mlp_create.argtypes = [c_int, POINTER(vp)]
@@ -137,7 +138,8 @@ def load(path=None):
     lib.a3d_mlp_predict.argtypes = [vp, fp, c_int, fp]
     lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
     lib.a3d_mlp_train.argtypes = [vp, fp, fp, c_int, c_int, c_int, c_float, c_float, ctypes.c_uint, POINTER(c_double)]
-    for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_set_strip_profile", "a3d_set_strip_precision", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
+    for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_set_strip_profile", "a3d_set_strip_precision",
+                 "a3d_set_zchunk", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
                  "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
                  "a3d_mlp_loss_grad", "a3d_mlp_train"):
         getattr(lib, name).restype = c_int
@@ -235,6 +237,10 @@ class Modeller:
 
     def set_launch(self, ty_vel, zseg_vel, ty_stress, zseg_stress):
         _check(self.lib, self.lib.a3d_set_launch(self._h, ty_vel, zseg_vel, ty_stress, zseg_stress))
+
+    def set_zchunk(self, planes):
+        """Two-sweep step in z chunks of `planes` planes (0: whole grid); bit-identical results."""
+        _check(self.lib, self.lib.a3d_set_zchunk(self._h, planes))
 
     def launch_info(self):
         v = (c_int * 6)()

@@ -63,8 +63,8 @@ __global__ __launch_bounds__(TX* TY) void k_fused_vti(const FusedArgs a) {
     const bool valid = y < g.nny - MM && xupd;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
-    const int zb = MM + seg * L.zlen;
-    const int ze = min(zb + L.zlen, g.nnz - MM);
+    const int zb = L.zbeg + seg * L.zlen;
+    const int ze = min(zb + L.zlen, L.zend);
     const float* T = (const float*)a.tab;
     const float dtx = (float)a.dtx, dty = (float)a.dty, dtz = (float)a.dtz;
     const Src S0 = a.s;

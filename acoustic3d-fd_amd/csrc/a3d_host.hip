@@ -223,20 +223,26 @@ struct a3d_ctx {
     float2* strip_delta = nullptr;
     float strip_amp = 10.0f;    // PML amplitude of the strip's 6-cell ramp
     int strip_mfma = 1;         // network on the matrix cores (binary16 operands) or FP32 per lane
+    int zchunk = 0;             // 1st-order two-sweep step in z chunks of this many planes (0: whole grid)
 };
 
 namespace {
 
-Launch make_launch(const Geo& g, int ty, int zseg) {
+// Tiles of 64 x ty points over the updated x-y range, the planes [z0, z1) (default: all
+// updated planes) cut into zseg segments.
+Launch make_launch(const Geo& g, int ty, int zseg, int z0 = MM, int z1 = -1) {
     Launch l{};
+    if (z1 < 0) z1 = g.nnz - MM;
     l.ntx = g.ld / TX;
     l.nty = (g.nny - 2 * MM + ty - 1) / ty;
-    const int nzi = g.nnz - 2 * MM;
+    const int nzi = z1 - z0;
     int nseg = std::max(1, std::min(zseg > 0 ? zseg : 1, nzi));
     l.zlen = (nzi + nseg - 1) / nseg;
     l.nseg = (nzi + l.zlen - 1) / l.zlen;
     l.ntiles = l.ntx * l.nty * l.nseg;
     l.grid = ((l.ntiles + 7) / 8) * 8;
+    l.zbeg = z0;
+    l.zend = z1;
     return l;
 }
 
@@ -694,16 +700,47 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             }
             HIP_TRY(hipEventRecord(c->ev1, c->st));
         } else {
+        // z chunks: velocity sweep of chunk k+1 before the stress sweep of chunk k (its w window
+        // reaches 3 planes into chunk k+1; the velocity sweep of chunk k+1 reads the stresses
+        // of chunk k before they are updated), so the velocities a stress chunk reads were
+        // written one launch earlier (the same arithmetic as whole-grid sweeps, bit for bit)
+        std::vector<VelArgs> vch;
+        std::vector<StressArgs> sch;
+        for (int z0 = MM; c->zchunk > 0 && z0 < g.nnz - MM; z0 += c->zchunk) {
+            const int z1 = std::min(z0 + c->zchunk, g.nnz - MM);
+            vch.push_back(va);
+            vch.back().l = make_launch(g, c->ty[0], c->zseg[0], z0, z1);
+            sch.push_back(sa);
+            sch.back().l = make_launch(g, c->ty[1], c->zseg[1], z0, z1);
+        }
+        const int nch = (int)vch.size();
         HIP_TRY(hipEventRecord(c->ev0, c->st));
         for (int it = 0; it < nt; it++) {
             va.s = srcs[it];
             sa.rec = c->rec + plane_rec * it;
-            timing_begin(c, 0);
-            HIP_TRY(launch_vel(prec, op, c->ty[0], va, c->st));
-            timing_end(c);
-            timing_begin(c, 1);
-            HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
-            timing_end(c);
+            if (nch > 0) {
+                for (int k = 0; k <= nch; ++k) {
+                    if (k < nch) {
+                        vch[k].s = srcs[it];
+                        timing_begin(c, 0);
+                        HIP_TRY(launch_vel(prec, op, c->ty[0], vch[k], c->st));
+                        timing_end(c);
+                    }
+                    if (k >= 1) {
+                        sch[k - 1].rec = sa.rec;
+                        timing_begin(c, 1);
+                        HIP_TRY(launch_stress(prec, op, c->ty[1], sch[k - 1], c->st));
+                        timing_end(c);
+                    }
+                }
+            } else {
+                timing_begin(c, 0);
+                HIP_TRY(launch_vel(prec, op, c->ty[0], va, c->st));
+                timing_end(c);
+                timing_begin(c, 1);
+                HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
+                timing_end(c);
+            }
             if (strip && (rc = strip_step(c, stp))) return rc;
             if (want_snap(it)) {
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
@@ -772,10 +809,10 @@ namespace {
 // process by (device, operator, padded grid).
 // ---------------------------------------------------------------------------
 struct TuneKey {
-    int dev, op, prec, scheme, nnx, nny, nnz;
+    int dev, op, prec, scheme, zchunk, nnx, nny, nnz;
     bool operator==(const TuneKey& o) const {
-        return dev == o.dev && op == o.op && prec == o.prec && scheme == o.scheme && nnx == o.nnx && nny == o.nny &&
-               nnz == o.nnz;
+        return dev == o.dev && op == o.op && prec == o.prec && scheme == o.scheme && zchunk == o.zchunk &&
+               nnx == o.nnx && nny == o.nny && nnz == o.nnz;
     }
 };
 struct TuneVal {
@@ -785,7 +822,7 @@ std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 std::mutex g_tune_mu;
 
 int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
-    const TuneKey key{c->device, c->d.op, c->prec, d->scheme, c->g.nnx, c->g.nny, c->g.nnz};
+    const TuneKey key{c->device, c->d.op, c->prec, d->scheme, c->zchunk, c->g.nnx, c->g.nny, c->g.nnz};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (auto& kv : g_tune_cache)
@@ -1205,6 +1242,13 @@ int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
         return fail(A3D_ENOMEM, "strip network allocation failed");
     }
     HIP_TRY(hipMemcpy(c->strip_net, params, LB_NPAR * sizeof(float), hipMemcpyHostToDevice));
+    return A3D_OK;
+}
+
+int a3d_set_zchunk(a3d_ctx* c, int planes) {
+    if (!c || planes < 0 || (planes > 0 && planes < 8)) return fail(A3D_EINVAL, "z chunk must be 0 or >= 8 planes");
+    c->zchunk = planes;
+    c->tuned = false;
     return A3D_OK;
 }
 

@@ -30,9 +30,11 @@ struct Geo {
 };
 
 // Work decomposition of one sweep: ntx*nty tiles per plane times nseg
-// z segments of zlen planes. grid = 8*ceil(ntiles/8) workgroups.
+// z segments of zlen planes covering the planes [zbeg, zend) (the whole
+// updated range, or one chunk of it). grid = 8*ceil(ntiles/8) workgroups.
 struct Launch {
     int ntx, nty, nseg, zlen, ntiles, grid;
+    int zbeg, zend;
 };
 
 // Precision instances of the sweep kernels.

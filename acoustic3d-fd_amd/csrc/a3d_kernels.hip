@@ -208,8 +208,8 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
-    const int zb = MM + seg * a.l.zlen;
-    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const int zb = a.l.zbeg + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, a.l.zend);
     const Scales sc = a.sc;
     const S* P = (const S*)a.P;
     const S* Q = (const S*)a.Q;
@@ -346,8 +346,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
-    const int zb = MM + seg * a.l.zlen;
-    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const int zb = a.l.zbeg + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, a.l.zend);
     const Scales sc = a.sc;
     const S* U = (const S*)a.u;
     const S* V = (const S*)a.v;
@@ -587,8 +587,8 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
-    const int zb = MM + seg * a.l.zlen;
-    const int ze = min(zb + a.l.zlen, g.nnz - MM);
+    const int zb = a.l.zbeg + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, a.l.zend);
     const float sp = a.sc.s[F_P], ip = a.sc.inv[F_P];
     S* X = (S*)a.X;
     const S* Y = (const S*)a.Y;

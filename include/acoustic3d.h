@@ -173,6 +173,12 @@ int a3d_set_launch(a3d_ctx* ctx, int ty_vel, int zseg_vel, int ty_stress, int zs
 /* out6 = {ty, z segments, workgroups} of sweep 0, then of sweep 1. */
 int a3d_get_launch(a3d_ctx* ctx, int* out6);
 
+/* 1st-order two-sweep step in z chunks of `planes` planes (0: whole-grid sweeps, the
+ * default; else >= 8): per step the velocity sweep of chunk k+1 runs before the stress
+ * sweep of chunk k, so the velocities a stress chunk reads were written one launch
+ * earlier and can still sit in the last-level cache. Bit-identical results. */
+int a3d_set_zchunk(a3d_ctx* ctx, int planes);
+
 /* Autotuner: times `steps` real time steps of *desc for each candidate
  * shape (TY in {2,4,8} x z segments in {1..24}) and keeps the fastest per
  * sweep; best_ms (may be NULL) receives the two best average launch times.
