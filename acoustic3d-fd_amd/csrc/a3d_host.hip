@@ -278,8 +278,10 @@ int check_desc_geometry(const a3d_shot_desc* d) {
     if (d->op == A3D_OP_2ND ? d->nb < 1 : d->nb < MM) return fail(A3D_EINVAL, "boundary width %d too small", d->nb);
     if (d->storage < 0 || d->storage > 2) return fail(A3D_EINVAL, "storage mode %d not in 0..2", d->storage);
     const int ring = d->op == A3D_OP_2ND ? 2 * MM + 2 * d->nb : 2 * d->nb;
-    const double n = (double)(d->nx + ring) * (d->ny + ring) * (d->nz + ring);
-    if (n * 1.1 > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
+    // kernels index the padded volume (x pitch rounded up to 64) with 32-bit ints
+    const double ld = (double)((d->nx + ring + TX - 1) / TX * TX);
+    const double n = ld * (d->ny + ring) * (d->nz + ring);
+    if (n > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
     return A3D_OK;
 }
 
