@@ -898,14 +898,14 @@ int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
     return A3D_OK;
 }
 
-// Tunes on the first shot of a large grid (>= 4M updated points, nt >= 100)
+// Tunes on the first shot of a grid of >= 1M updated points with nt >= 100
 // unless A3D_AUTOTUNE=0 or a shape was set explicitly.
 int maybe_autotune(a3d_ctx* c, const a3d_shot_desc* d) {
     if (c->tuned) return A3D_OK;
     const char* e = getenv("A3D_AUTOTUNE");
     if (e && e[0] == '0') return A3D_OK;
     const double pts = (double)(c->g.nnx - 2 * MM) * (c->g.nny - 2 * MM) * (c->g.nnz - 2 * MM);
-    if (pts < 4.0e6 || d->nt < 100) return A3D_OK;
+    if (pts < 1.0e6 || d->nt < 100) return A3D_OK;
     if (check_desc_shot(c, d) || !c->model_set) return A3D_OK;   // run_shot reports the error
     return autotune(c, d, 4, nullptr);
 }

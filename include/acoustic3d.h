@@ -184,7 +184,7 @@ int a3d_set_zchunk(a3d_ctx* ctx, int planes);
  * sweep; best_ms (may be NULL) receives the two best average launch times.
  * Results are cached per process by (device, operator, padded grid). a3d_run
  * and the legacy entry points call it on the first shot of a grid with at
- * least 4M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
+ * least 1M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
 int a3d_autotune(a3d_ctx* ctx, const a3d_shot_desc* desc, int steps, double* best_ms);
 
 /* ----------------------------------------------------------------------- */
