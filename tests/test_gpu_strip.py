@@ -170,3 +170,26 @@ def test_collect_samples():
     key = lambda A: np.lexsort(A.T[::-1])   # noqa: E731
     assert len(X2) == len(X)
     assert np.array_equal(X[key(X)], X2[key(X2)])
+
+
+def test_trained_network_gain_over_closed_head():
+    """What the committed trained network gains over the same 6-cell strip with its damping
+    head shut, on a held-out model, judged against an extended-domain FP64 run. The network
+    trained by teacher forcing keeps its head almost shut, so the gain is about 0 dB; the test
+    prints it and requires that the network does not make the strip worse (by more than
+    0.1 dB) and that both stay far below the reflection of an undamped truncation."""
+    from a3dfd import models, strip
+    params, amp = strip.load_params()
+    assert params is not None, "trained network missing (acoustic3d-fd_amd/a3dfd/strip_net.npz)"
+    n, nt, every, seed = 64, 400, 20, 90100
+    m = models.strip_model(seed, n)
+    src = models.strip_source(seed, n)
+    _, ref, _ = strip.run_case(m, src, nt, every, "extended", margin=32)
+    db = {}
+    for b in ("strip0", "strip"):
+        _, snaps, _ = strip.run_case(m, src, nt, every, b, params, amp=amp)
+        db[b] = float(strip.energy_db(snaps, ref))
+    print(f"\n[gpu-strip] held-out 64^3: strip without network {db['strip0']:.2f} dB, learned {db['strip']:.2f} dB, "
+          f"gain {db['strip0'] - db['strip']:+.2f} dB")
+    assert db["strip"] <= db["strip0"] + 0.1
+    assert db["strip0"] < -20.0
