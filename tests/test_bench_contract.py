@@ -1,0 +1,36 @@
+"""CPU checks of bench.py's contract pieces that need no GPU: the multi-GPU refusal and
+the algorithmic-bytes accounting behind roofline.achieved."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_algorithmic_bytes():
+    b = _bench()
+    cfg = dict(nx=400, ny=400, nz=300)
+    npts = 61959168                      # (440-8) * (440-8) * (340-8)
+    assert b.algorithmic_bytes(cfg, 0, 0, npts) == (36, 84.0)          # reference formulation, FP32
+    assert b.algorithmic_bytes(cfg, 2, 0, npts) == (72, 168.0)         # FP64
+    a_vel, a_str = b.algorithmic_bytes(cfg, 0, 1, npts)                # split fields in PML cells only
+    f = 401 * 401 * 301 / npts
+    assert a_vel == 36 and abs(a_str - (f * 44 + (1 - f) * 84)) < 1e-9
+    assert abs(a_str - 52.75) < 0.01
+    a_vel, a_str = b.algorithmic_bytes(cfg, 1, 0, npts)                # binary16 wavefields: always scheme 1
+    assert a_vel == 8 * 2 + 4 and abs(a_str - (f * (7 * 2 + 16) + (1 - f) * (5 * 2 + 12 * 4 + 16))) < 1e-9
+    assert b.OP_BYTES["vti"] == (36, 84, 44) and b.OP_BYTES["2nd"][0] == 16
