@@ -420,6 +420,19 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2, width);
     pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2, width);
     pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2, width);
+    // The planes whose z stress factor is exactly 1 form one run (the factor's argument
+    // grows away from the interior on both sides); the stress sweep takes the run from
+    // its bounds instead of reading the factor per plane.
+    int zi0 = -1, zi1 = -1;
+    for (int z = 0; z < g.nnz; ++z)
+        if (t[pm.az1 + z] == (T)1) {
+            if (zi0 < 0) zi0 = z;
+            else if (zi1 >= 0) return fail(A3D_EINVAL, "z stress factor: planes of factor 1 are not one run");
+        } else if (zi0 >= 0 && zi1 < 0) {
+            zi1 = z;
+        }
+    if (zi0 < 0) zi0 = zi1 = 0;
+    else if (zi1 < 0) zi1 = g.nnz;
     HIP_TRY(hipMemcpy(c->tab, tabh.data(), tabh.size() * sizeof(T), hipMemcpyHostToDevice));
     for (int v : volumes_of(op))
         if (is_wavefield(v))
@@ -486,6 +499,8 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.ny = ny;
     sa.sc = sc;
     sa.unsplit = d->scheme != 0 || prec == PREC_FP16;
+    sa.zi0 = zi0;
+    sa.zi1 = zi1;
     va.s.s0 = s0;
     // The time grid is the reference's float accumulation t = dt, t += dt and the gate
     // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.

@@ -103,6 +103,7 @@ struct StressArgs {
     float* rec;                // receiver plane of this step ([ny][nx]) or nullptr
     int zrec, nb, nx, ny;
     int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1
+    int zi0, zi1;              // planes whose z stress factor is exactly 1: zi0 <= z < zi1
     Scales sc;
 };
 

@@ -160,27 +160,52 @@ __device__ __forceinline__ C damp3(C v, double fy, double fx, double fz) {
     return v;
 }
 
-// Loads the halo of plane offset zo into an LDS tile s[TY+8][72] whose
-// interior rows (ty+4) were written by the owning threads. HY: 4 rows above
-// and below; HX: 4 columns left and right of the thread's own row.
+// Halo of one plane for an LDS tile s[TY+8][72]: HY the 4 rows above and below
+// the tile, HX the 4 columns left and right of each tile row. The values are
+// loaded into registers with every other load of the plane, before the
+// workgroup's barrier, and written to LDS after it, so a plane costs one
+// exposed memory latency (a load that fed LDS directly would wait inside its
+// own branch). Every lane loads: indices outside the grid are clamped to its
+// edge (entries no updated point reads), and lanes without an x-halo column
+// re-read their own point.
+template <int TY, bool HX, bool HY, class C>
+struct Halo {
+    static_assert((2 * MM * TX) % (TX * TY) == 0, "the halo rows split evenly over the workgroup");
+    static constexpr int NY = HY ? 2 * MM * TX / (TX * TY) : 0;   // halo-row values per thread
+    C y[NY > 0 ? NY : 1];
+    C x;
+};
+
 template <int TY, bool HX, bool HY, class C, class LD>
-__device__ __forceinline__ void fill_halo(C (*s)[TX + 2 * MM], const Geo& g, int zo, int x0, int y0, int tx,
-                                          int ty, int yl, bool yin, LD ld) {
+__device__ __forceinline__ void load_halo(Halo<TY, HX, HY, C>& h, const Geo& g, int zo, int x0, int y0, int tx,
+                                          int ty, int yl, LD ld) {
     if constexpr (HY) {
 #pragma unroll
-        for (int k = ty * TX + tx; k < 2 * MM * TX; k += TX * TY) {
+        for (int j = 0; j < Halo<TY, HX, HY, C>::NY; ++j) {
+            const int k = ty * TX + tx + j * TX * TY;
             const int r = k >> 6, c = k & 63;
             const int yy = r < MM ? y0 - MM + r : y0 + TY + (r - MM);
-            const int sr = r < MM ? r : TY + r;
-            if (yy < g.nny) s[sr][c + MM] = ld(zo + yy * g.ld + x0 + c);
+            h.y[j] = ld(zo + min(yy, g.nny - 1) * g.ld + x0 + c);
         }
     }
     if constexpr (HX) {
-        if (tx < 2 * MM && yin) {
-            const int xx = tx < MM ? x0 - MM + tx : x0 + TX + (tx - MM);
-            const int sc = tx < MM ? tx : TX + tx;
-            if (xx >= 0 && xx < g.nnx) s[ty + MM][sc] = ld(zo + yl * g.ld + xx);
+        const int xx = tx < MM ? x0 - MM + tx : (tx < 2 * MM ? x0 + TX + (tx - MM) : x0 + tx);
+        h.x = ld(zo + yl * g.ld + min(max(xx, 0), g.nnx - 1));
+    }
+}
+
+template <int TY, bool HX, bool HY, class C>
+__device__ __forceinline__ void store_halo(C (*s)[TX + 2 * MM], const Halo<TY, HX, HY, C>& h, int tx, int ty) {
+    if constexpr (HY) {
+#pragma unroll
+        for (int j = 0; j < Halo<TY, HX, HY, C>::NY; ++j) {
+            const int k = ty * TX + tx + j * TX * TY;
+            const int r = k >> 6, c = k & 63;
+            s[r < MM ? r : TY + r][c + MM] = h.y[j];
         }
+    }
+    if constexpr (HX) {
+        if (tx < 2 * MM) s[ty + MM][tx < MM ? tx : TX + tx] = h.x;
     }
 }
 
@@ -208,6 +233,9 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
+    // column of the nearest updated point: points outside the updated region load
+    // there, so no load sits in a branch (a wave then waits once per plane)
+    const int colc = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(y, g.nny - MM - 1);
     const int zb = a.l.zbeg + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, a.l.zend);
     const Scales sc = a.sc;
@@ -243,61 +271,66 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         const int id = zo + col;
         if constexpr (P_Z) wP[7] = ldP(id + 4 * g.pl);
         if constexpr (HAS_Q) wQ[7] = ldQ(id + 4 * g.pl);
-        C u0 = 0, v0 = 0, w0 = 0, r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
-        if (valid) {
-            u0 = ldw(U, id, sc.s[F_U]);
-            v0 = ldw(V, id, sc.s[F_V]);
-            w0 = ldw(Wv, id, sc.s[F_W]);
-            if constexpr (!TTI) r0 = (C)rho[id];
-            if constexpr (TTI) {
-                ct = (C)Ct[id];
-                st = (C)St[id];
-                cp = (C)Cp[id];
-                sp = (C)Sp[id];
-            }
+        const int idc = zo + colc;
+        C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
+        const C u0 = ldw(U, idc, sc.s[F_U]);
+        const C v0 = ldw(V, idc, sc.s[F_V]);
+        const C w0 = ldw(Wv, idc, sc.s[F_W]);
+        if constexpr (!TTI) r0 = (C)rho[idc];
+        if constexpr (TTI) {
+            ct = (C)Ct[idc];
+            st = (C)St[idc];
+            cp = (C)Cp[idc];
+            sp = (C)Sp[idc];
         }
         C pc;
         if constexpr (P_Z) pc = wP[3];
         else pc = ldP(id);
+        Halo<TY, true, true, C> hP;
+        load_halo(hP, g, zo, x0, y0, tx, ty, yl, ldP);
+        Halo<TY, Q_XY, Q_XY, C> hQ;
+        if constexpr (Q_XY) load_halo(hQ, g, zo, x0, y0, tx, ty, yl, ldQ);
+        const C cz1 = T[a.c.cz1 + z], cz2 = T[a.c.cz2 + z];
         __syncthreads();
         sP[ty + MM][tx + MM] = pc;
-        fill_halo<TY, true, true>(sP, g, zo, x0, y0, tx, ty, yl, yin, ldP);
+        store_halo(sP, hP, tx, ty);
         if constexpr (Q_XY) {
             sQ[ty + MM][tx + MM] = wQ[3];
-            fill_halo<TY, true, true>(sQ, g, zo, x0, y0, tx, ty, yl, yin, ldQ);
+            store_halo(sQ, hQ, tx, ty);
         }
         __syncthreads();
+        // z differences, then the window shift, before any store: the loop comes
+        // round without waiting for this plane's stores
+        C Pz = 0, Qz = 0;
+        if constexpr (P_Z) Pz = dst8<C>([&](int k) { return wP[k]; });
+        if constexpr (HAS_Q) Qz = dst8<C>([&](int k) { return wQ[k]; });
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if constexpr (P_Z) wP[k] = wP[k + 1];
+            if constexpr (HAS_Q) wQ[k] = wQ[k + 1];
+        }
         if (valid) {
             const int r = ty + MM, c = tx + MM;
             C xx, yy, zz;
             if constexpr (!TTI) {
                 yy = dst8<C>([&](int k) { return sP[r - 3 + k][c]; });
                 xx = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
-                if constexpr (ISO) zz = dst8<C>([&](int k) { return wP[k]; });
-                else zz = dst8<C>([&](int k) { return wQ[k]; });
+                zz = ISO ? Pz : Qz;
                 xx /= r0;
                 yy /= r0;
                 zz /= r0;
             } else {
                 const C Py = dst8<C>([&](int k) { return sP[r - 3 + k][c]; });
                 const C Px = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
-                const C Pz = dst8<C>([&](int k) { return wP[k]; });
                 const C Qy = dst8<C>([&](int k) { return sQ[r - 3 + k][c]; });
                 const C Qx = dst8<C>([&](int k) { return sQ[r][c - 3 + k]; });
-                const C Qz = dst8<C>([&](int k) { return wQ[k]; });
                 xx = ct * cp * Px - ct * sp * Py - st * Pz;
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
             }
-            const C cz1 = T[a.c.cz1 + z], cz2 = T[a.c.cz2 + z];
             stw(U, id, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
             stw(V, id, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
             stw(Wv, id, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
-        }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            if constexpr (P_Z) wP[k] = wP[k + 1];
-            if constexpr (HAS_Q) wQ[k] = wQ[k + 1];
         }
     }
 }
@@ -346,6 +379,9 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
+    // column of the nearest updated point: points outside the updated region load
+    // there, so no load sits in a branch (a wave then waits once per plane)
+    const int colc = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(y, g.nny - MM - 1);
     const int zb = a.l.zbeg + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, a.l.zend);
     const Scales sc = a.sc;
@@ -398,20 +434,19 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             wU[7] = ldU(id + 3 * g.pl);
             wV[7] = ldV(id + 3 * g.pl);
         }
-        C c0 = 0, r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
-        if (valid) {
-            c0 = (C)vp[id];
-            if constexpr (!TTI) r0 = (C)rho[id];
-            if constexpr (!ISO) {
-                e0 = (C)eps[id];
-                d0 = (C)del[id];
-            }
-            if constexpr (TTI) {
-                ct = (C)Ct[id];
-                st = (C)St[id];
-                cp = (C)Cp[id];
-                sp = (C)Sp[id];
-            }
+        const int idc = zo + colc;
+        C r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
+        const C c0 = (C)vp[idc];
+        if constexpr (!TTI) r0 = (C)rho[idc];
+        if constexpr (!ISO) {
+            e0 = (C)eps[idc];
+            d0 = (C)del[idc];
+        }
+        if constexpr (TTI) {
+            ct = (C)Ct[idc];
+            st = (C)St[idc];
+            cp = (C)Cp[idc];
+            sp = (C)Sp[idc];
         }
         C uc, vc;
         if constexpr (TTI) {
@@ -421,32 +456,68 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             uc = ldU(id);
             vc = ldV(id);
         }
+        Halo<TY, true, TTI, C> hU;
+        Halo<TY, TTI, true, C> hV;
+        Halo<TY, TTI, TTI, C> hW;
+        load_halo(hU, g, zo, x0, y0, tx, ty, yl, ldU);
+        load_halo(hV, g, zo, x0, y0, tx, ty, yl, ldV);
+        if constexpr (TTI) load_halo(hW, g, zo, x0, y0, tx, ty, yl, ldW);
+        const C az1 = T[a.c.az1 + z], az2 = T[a.c.az2 + z];
+        // Unsplit scheme: outside the PML (all stress factors 1) P and Q are updated
+        // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
+        // updates in exact arithmetic; the split fields are read and written only in
+        // PML cells. f0..f5: the split fields px..qz, or P (and Q) where unsplit.
+        const bool split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && z >= a.zi0 && z < a.zi1);
+        // Two one-sided branches with distinct destinations, left unset where not
+        // loaded: an if/else writing the same registers, or zeroing them first, makes
+        // the wave wait for earlier loads into those registers.
+        C f0, f1, f2, f3, f4, f5, p0, q0;
+        if (valid && split) {
+            f0 = ldw(px, id, sc.s[F_PX]);
+            f1 = ldw(py, id, sc.s[F_PY]);
+            f2 = ldw(pz, id, sc.s[F_PZ]);
+            if constexpr (!ISO) {
+                f3 = ldw(qx, id, sc.s[F_QX]);
+                f4 = ldw(qy, id, sc.s[F_QY]);
+                f5 = ldw(qz, id, sc.s[F_QZ]);
+            }
+        }
+        if (valid && !split) {
+            p0 = ldw((const S*)Po, id, sc.s[F_P]);
+            if constexpr (!ISO) q0 = ldw((const S*)Qo, id, sc.s[F_Q]);
+        }
         __syncthreads();
         sU[ty + MM][tx + MM] = uc;
         sV[ty + MM][tx + MM] = vc;
-        fill_halo<TY, true, TTI>(sU, g, zo, x0, y0, tx, ty, yl, yin, ldU);
-        fill_halo<TY, TTI, true>(sV, g, zo, x0, y0, tx, ty, yl, yin, ldV);
+        store_halo(sU, hU, tx, ty);
+        store_halo(sV, hV, tx, ty);
         if constexpr (TTI) {
             sW[ty + MM][tx + MM] = wW[4];
-            fill_halo<TY, true, true>(sW, g, zo, x0, y0, tx, ty, yl, yin, ldW);
+            store_halo(sW, hW, tx, ty);
         }
         __syncthreads();
+        // z differences, then the window shift, before any store: the loop comes
+        // round without waiting for this plane's stores
+        const C Wz = dst8<C>([&](int k) { return wW[k]; });
+        C Uz = 0;
+        if constexpr (TTI) Uz = dst8<C>([&](int k) { return wU[k]; });
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            wW[k] = wW[k + 1];
+            if constexpr (TTI) {
+                wU[k] = wU[k + 1];
+                wV[k] = wV[k + 1];
+            }
+        }
         if (valid) {
             const int r = ty + MM, c = tx + MM;
-            const C az1 = T[a.c.az1 + z], az2 = T[a.c.az2 + z];
-            // Unsplit scheme: outside the PML (all stress factors 1) P and Q are updated
-            // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
-            // updates in exact arithmetic; the split fields are read and written only in
-            // PML cells.
-            const bool split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && az1 == (C)1);
             C Pn, Qn = 0;
             if constexpr (ISO) {
                 const C yy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
                 const C xx = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
-                const C zz = dst8<C>([&](int k) { return wW[k]; });
+                const C zz = Wz;
                 if (split) {
-                    const C px0 = ldw(px, id, sc.s[F_PX]), py0 = ldw(py, id, sc.s[F_PY]);
-                    const C pz0 = ldw(pz, id, sc.s[F_PZ]);
+                    const C px0 = f0, py0 = f1, pz0 = f2;
                     const C npx = ax2 * px0 - ax1 * r0 * c0 * c0 * dtx * xx;
                     const C npy = ay2 * py0 - ay1 * r0 * c0 * c0 * dty * yy;
                     const C npz = az2 * pz0 - az1 * r0 * c0 * c0 * dtz * zz;
@@ -456,7 +527,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     Pn = npx + npy + npz;
                 } else {
                     const C K = r0 * c0 * c0;
-                    Pn = ldw((const S*)Po, id, sc.s[F_P]) - K * (dtx * xx + dty * yy + dtz * zz);
+                    Pn = p0 - K * (dtx * xx + dty * yy + dtz * zz);
                 }
             } else {
                 // SV taper (reference vti:154-175, tti:139-160)
@@ -479,16 +550,16 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 if constexpr (VTI) {
                     yy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
                     xx = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
-                    zz = dst8<C>([&](int k) { return wW[k]; });
+                    zz = Wz;
                 } else {
                     const C uy = dst8<C>([&](int k) { return sU[r - 4 + k][c]; });
                     const C ux = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
-                    const C uz = dst8<C>([&](int k) { return wU[k]; });
+                    const C uz = Uz;
                     const C vy = dst8<C>([&](int k) { return sV[r - 4 + k][c]; });
                     const C vx = dst8<C>([&](int k) { return sV[r][c - 4 + k]; });
                     const C wy = dst8<C>([&](int k) { return sW[r - 4 + k][c]; });
                     const C wx = dst8<C>([&](int k) { return sW[r][c - 4 + k]; });
-                    const C wz = dst8<C>([&](int k) { return wW[k]; });
+                    const C wz = Wz;
                     xx = ct * cp * ux - ct * sp * uy - st * uz;
                     yy = sp * vx + cp * vy;
                     zz = st * cp * wx - st * sp * wy + ct * wz;
@@ -496,9 +567,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 const C e2 = 1 + 2 * ee;
                 const C s2 = csqrt(1 + 2 * dd);
                 if (split) {
-                    const C px0 = ldw(px, id, sc.s[F_PX]), py0 = ldw(py, id, sc.s[F_PY]);
-                    const C pz0 = ldw(pz, id, sc.s[F_PZ]), qx0 = ldw(qx, id, sc.s[F_QX]);
-                    const C qy0 = ldw(qy, id, sc.s[F_QY]), qz0 = ldw(qz, id, sc.s[F_QZ]);
+                    const C px0 = f0, py0 = f1, pz0 = f2, qx0 = f3, qy0 = f4, qz0 = f5;
                     C npx, npy, npz, nqx, nqy, nqz;
                     if constexpr (VTI) {
                         npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
@@ -526,21 +595,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 } else {
                     const C K = VTI ? r0 * c0 * c0 : c0 * c0;
                     const C hxy = dtx * xx + dty * yy, hz = dtz * zz;
-                    Pn = ldw((const S*)Po, id, sc.s[F_P]) - K * (e2 * hxy + s2 * hz);
-                    Qn = ldw((const S*)Qo, id, sc.s[F_Q]) - K * (s2 * hxy + hz);
+                    Pn = p0 - K * (e2 * hxy + s2 * hz);
+                    Qn = q0 - K * (s2 * hxy + hz);
                 }
                 stw(Qo, id, Qn, sc.inv[F_Q]);
             }
             stw(Po, id, Pn, sc.inv[F_P]);
             if (recx && z == a.zrec) rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
-        }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            wW[k] = wW[k + 1];
-            if constexpr (TTI) {
-                wU[k] = wU[k + 1];
-                wV[k] = wV[k + 1];
-            }
         }
     }
 }
@@ -587,6 +648,9 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
     const int yl = yin ? y : g.nny - 1;
     const int col = x + g.ld * yl;
+    // column of the nearest updated point: points outside the updated region load
+    // there, so no load sits in a branch (a wave then waits once per plane)
+    const int colc = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(y, g.nny - MM - 1);
     const int zb = a.l.zbeg + seg * a.l.zlen;
     const int ze = min(zb + a.l.zlen, a.l.zend);
     const float sp = a.sc.s[F_P], ip = a.sc.inv[F_P];
@@ -608,33 +672,35 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
         const int zo = z * g.pl;
         const int id = zo + col;
         wY[8] = ldY(id + 4 * g.pl);
-        C xo = 0, vv = 0;
-        if (valid) {
-            xo = inj<C>(ldw((const S*)X, id, sp), id, a.s0, a.xa0, a.s1, a.xa1);
-            vv = (C)VV[id];
-        }
+        const int idc = zo + colc;
+        const C xo = inj<C>(ldw((const S*)X, idc, sp), idc, a.s0, a.xa0, a.s1, a.xa1);
+        const C vv = (C)VV[idc];
+        Halo<TY, true, true, C> hY;
+        load_halo(hY, g, zo, x0, y0, tx, ty, yl, ldY);
+        const double fz = a.fz[z];
         __syncthreads();
         sY[ty + MM][tx + MM] = wY[4];
-        fill_halo<TY, true, true>(sY, g, zo, x0, y0, tx, ty, yl, yin, ldY);
+        store_halo(sY, hY, tx, ty);
         __syncthreads();
+        // z difference, then the window shift, before any store: the loop comes round
+        // without waiting for this plane's stores
+        C c1 = lap9<C>([&](int k) { return wY[k]; });   // z
+        const C cur = wY[4];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) wY[k] = wY[k + 1];
         if (valid) {
             const int r = ty + MM, c = tx + MM;
-            const double fz = a.fz[z];
-            C c1 = lap9<C>([&](int k) { return wY[k]; });              // z
             C c2 = lap9<C>([&](int k) { return sY[r][c - 4 + k]; });   // x
             C c3 = lap9<C>([&](int k) { return sY[r - 4 + k][c]; });   // y
             c1 *= dz2;
             c2 *= dx2;
             c3 *= dy2;
-            const C cur = wY[4];
             const C prev = damp3<C>(xo, fy, fx, fz);
             const C lap = vv * (c1 + c2 + c3);
             const C nv = (C)(2.0 * (double)cur - (double)prev + (double)lap);
             stw(X, id, damp3<C>(nv, fy, fx, fz), ip);
             if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) wY[k] = wY[k + 1];
     }
 }
 
