@@ -282,6 +282,8 @@ int check_desc_geometry(const a3d_shot_desc* d) {
     const double ld = (double)((d->nx + ring + TX - 1) / TX * TX);
     const double n = ld * (d->ny + ring) * (d->nz + ring);
     if (n > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
+    // buffer loads address a plane with 31-bit byte offsets (8-byte elements at most)
+    if (ld * (d->ny + ring) * 8.0 >= 2147483648.0) return fail(A3D_EINVAL, "padded x-y plane too large");
     return A3D_OK;
 }
 

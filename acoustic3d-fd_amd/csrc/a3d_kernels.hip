@@ -29,6 +29,7 @@
 #include "a3d_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -75,13 +76,57 @@ template <> struct Coef<double> {
 __device__ __forceinline__ float csqrt(float v) { return sqrtf(v); }
 __device__ __forceinline__ double csqrt(double v) { return sqrt(v); }
 
-// Wavefield load / store with the per-field scale of binary16 storage.
+// Wavefield load with the per-field scale of binary16 storage (element i).
 __device__ __forceinline__ float ldw(const float* p, int i, float) { return p[i]; }
 __device__ __forceinline__ double ldw(const double* p, int i, float) { return p[i]; }
 __device__ __forceinline__ float ldw(const __half* p, int i, float sc) { return __half2float(p[i]) * sc; }
-__device__ __forceinline__ void stw(float* p, int i, float v, float) { p[i] = v; }
-__device__ __forceinline__ void stw(double* p, int i, double v, float) { p[i] = v; }
-__device__ __forceinline__ void stw(__half* p, int i, float v, float inv) { p[i] = __float2half_rn(v * inv); }
+
+// Buffer addressing of element o of the plane at offset zo: the plane base
+// p + zo goes into a buffer resource (scalar registers, rebuilt per plane) and
+// the lane passes its in-plane byte offset, so a load or store is one
+// instruction with no per-lane 64-bit address arithmetic and no address
+// registers live across the plane loop. In-plane byte offsets fit 31 bits (the
+// host bounds a plane's size). Resource word 3 (0x00020000) is the untyped
+// 32-bit data format of gfx9-family buffer resources.
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const T* p, int zo) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p + (ptrdiff_t)zo), 0, 0x7fffffff, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T bld(const T* p, int zo, int o) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(p, zo);
+    if constexpr (sizeof(T) == 2) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, o * 2, 0, 0));
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, o * 4, 0, 0));
+    } else {
+        static_assert(sizeof(T) == 8, "2, 4 or 8-byte elements");
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, o * 8, 0, 0));
+    }
+}
+template <class T>
+__device__ __forceinline__ void bst(T* p, int zo, int o, T v) {
+    const __amdgpu_buffer_rsrc_t r = plane_rsrc(p, zo);
+    if constexpr (sizeof(T) == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, o * 2, 0, 0);
+    } else if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, o * 4, 0, 0);
+    } else {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, o * 8, 0, 0);
+    }
+}
+// The same per plane (element o of the plane at offset zo), and the matching stores.
+__device__ __forceinline__ float ldw(const float* p, int zo, int o, float) { return bld(p, zo, o); }
+__device__ __forceinline__ double ldw(const double* p, int zo, int o, float) { return bld(p, zo, o); }
+__device__ __forceinline__ float ldw(const __half* p, int zo, int o, float sc) {
+    return (float)bld((const _Float16*)p, zo, o) * sc;
+}
+__device__ __forceinline__ void stw(float* p, int zo, int o, float v, float) { bst(p, zo, o, v); }
+__device__ __forceinline__ void stw(double* p, int zo, int o, double v, float) { bst(p, zo, o, v); }
+__device__ __forceinline__ void stw(__half* p, int zo, int o, float v, float inv) {
+    bst((_Float16*)p, zo, o, __builtin_bit_cast(_Float16, __float2half_rn(v * inv)));
+}
 
 // Staggered difference over an 8-point window w(0..7) whose centre pair is
 // (w(4), w(3)): forward windows hold F[-3..+4], backward windows F[-4..+3].
@@ -185,12 +230,12 @@ __device__ __forceinline__ void load_halo(Halo<TY, HX, HY, C>& h, const Geo& g, 
             const int k = ty * TX + tx + j * TX * TY;
             const int r = k >> 6, c = k & 63;
             const int yy = r < MM ? y0 - MM + r : y0 + TY + (r - MM);
-            h.y[j] = ld(zo + min(yy, g.nny - 1) * g.ld + x0 + c);
+            h.y[j] = ld(zo, min(yy, g.nny - 1) * g.ld + x0 + c);
         }
     }
     if constexpr (HX) {
         const int xx = tx < MM ? x0 - MM + tx : (tx < 2 * MM ? x0 + TX + (tx - MM) : x0 + tx);
-        h.x = ld(zo + yl * g.ld + min(max(xx, 0), g.nnx - 1));
+        h.x = ld(zo, yl * g.ld + min(max(xx, 0), g.nnx - 1));
     }
 }
 
@@ -251,15 +296,19 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const M* Sp = (const M*)a.sp;
     const C* T = (const C*)a.tab;
 
-    auto ldP = [&](int i) { return inj<C>(ldw(P, i, sc.s[F_P]), i, S0.s0, S0.p0, S0.s1, S0.p1); };
-    auto ldQ = [&](int i) { return inj<C>(ldw(Q, i, sc.s[F_Q]), i, S0.s0, S0.q0, S0.s1, S0.q1); };
+    auto ldP = [&](int zo, int o) {
+        return inj<C>(ldw(P, zo, o, sc.s[F_P]), zo + o, S0.s0, S0.p0, S0.s1, S0.p1);
+    };
+    auto ldQ = [&](int zo, int o) {
+        return inj<C>(ldw(Q, zo, o, sc.s[F_Q]), zo + o, S0.s0, S0.q0, S0.s1, S0.q1);
+    };
 
     C wP[8], wQ[8];   // forward windows: F(z-3+k), k = 0..7
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        const int zi = (zb - 3 + k) * g.pl + col;
-        if constexpr (P_Z) wP[k] = ldP(zi);
-        if constexpr (HAS_Q) wQ[k] = ldQ(zi);
+        const int zi = (zb - 3 + k) * g.pl;
+        if constexpr (P_Z) wP[k] = ldP(zi, col);
+        if constexpr (HAS_Q) wQ[k] = ldQ(zi, col);
     }
     const int xs = x < g.nnx ? x : 0;
     const C cx1 = T[a.c.cx1 + xs], cx2 = T[a.c.cx2 + xs];
@@ -268,24 +317,22 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
-        const int id = zo + col;
-        if constexpr (P_Z) wP[7] = ldP(id + 4 * g.pl);
-        if constexpr (HAS_Q) wQ[7] = ldQ(id + 4 * g.pl);
-        const int idc = zo + colc;
+        if constexpr (P_Z) wP[7] = ldP(zo + 4 * g.pl, col);
+        if constexpr (HAS_Q) wQ[7] = ldQ(zo + 4 * g.pl, col);
         C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
-        const C u0 = ldw(U, idc, sc.s[F_U]);
-        const C v0 = ldw(V, idc, sc.s[F_V]);
-        const C w0 = ldw(Wv, idc, sc.s[F_W]);
-        if constexpr (!TTI) r0 = (C)rho[idc];
+        const C u0 = ldw(U, zo, colc, sc.s[F_U]);
+        const C v0 = ldw(V, zo, colc, sc.s[F_V]);
+        const C w0 = ldw(Wv, zo, colc, sc.s[F_W]);
+        if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
         if constexpr (TTI) {
-            ct = (C)Ct[idc];
-            st = (C)St[idc];
-            cp = (C)Cp[idc];
-            sp = (C)Sp[idc];
+            ct = (C)bld(Ct, zo, colc);
+            st = (C)bld(St, zo, colc);
+            cp = (C)bld(Cp, zo, colc);
+            sp = (C)bld(Sp, zo, colc);
         }
         C pc;
         if constexpr (P_Z) pc = wP[3];
-        else pc = ldP(id);
+        else pc = ldP(zo, col);
         Halo<TY, true, true, C> hP;
         load_halo(hP, g, zo, x0, y0, tx, ty, yl, ldP);
         Halo<TY, Q_XY, Q_XY, C> hQ;
@@ -328,9 +375,9 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
             }
-            stw(U, id, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
-            stw(V, id, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
-            stw(Wv, id, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
+            stw(U, zo, col, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
+            stw(V, zo, col, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
+            stw(Wv, zo, col, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
         }
     }
 }
@@ -406,18 +453,18 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const M* Sp = (const M*)a.sp;
     const C* T = (const C*)a.tab;
 
-    auto ldU = [&](int i) { return ldw(U, i, sc.s[F_U]); };
-    auto ldV = [&](int i) { return ldw(V, i, sc.s[F_V]); };
-    auto ldW = [&](int i) { return ldw(Wv, i, sc.s[F_W]); };
+    auto ldU = [&](int zo, int o) { return ldw(U, zo, o, sc.s[F_U]); };
+    auto ldV = [&](int zo, int o) { return ldw(V, zo, o, sc.s[F_V]); };
+    auto ldW = [&](int zo, int o) { return ldw(Wv, zo, o, sc.s[F_W]); };
 
     C wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        const int zi = (zb - 4 + k) * g.pl + col;
-        wW[k] = ldW(zi);
+        const int zi = (zb - 4 + k) * g.pl;
+        wW[k] = ldW(zi, col);
         if constexpr (TTI) {
-            wU[k] = ldU(zi);
-            wV[k] = ldV(zi);
+            wU[k] = ldU(zi, col);
+            wV[k] = ldV(zi, col);
         }
     }
     const int xs = x < g.nnx ? x : 0;
@@ -428,33 +475,31 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
-        const int id = zo + col;
-        wW[7] = ldW(id + 3 * g.pl);
+        wW[7] = ldW(zo + 3 * g.pl, col);
         if constexpr (TTI) {
-            wU[7] = ldU(id + 3 * g.pl);
-            wV[7] = ldV(id + 3 * g.pl);
+            wU[7] = ldU(zo + 3 * g.pl, col);
+            wV[7] = ldV(zo + 3 * g.pl, col);
         }
-        const int idc = zo + colc;
         C r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
-        const C c0 = (C)vp[idc];
-        if constexpr (!TTI) r0 = (C)rho[idc];
+        const C c0 = (C)bld(vp, zo, colc);
+        if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
         if constexpr (!ISO) {
-            e0 = (C)eps[idc];
-            d0 = (C)del[idc];
+            e0 = (C)bld(eps, zo, colc);
+            d0 = (C)bld(del, zo, colc);
         }
         if constexpr (TTI) {
-            ct = (C)Ct[idc];
-            st = (C)St[idc];
-            cp = (C)Cp[idc];
-            sp = (C)Sp[idc];
+            ct = (C)bld(Ct, zo, colc);
+            st = (C)bld(St, zo, colc);
+            cp = (C)bld(Cp, zo, colc);
+            sp = (C)bld(Sp, zo, colc);
         }
         C uc, vc;
         if constexpr (TTI) {
             uc = wU[4];
             vc = wV[4];
         } else {
-            uc = ldU(id);
-            vc = ldV(id);
+            uc = ldU(zo, col);
+            vc = ldV(zo, col);
         }
         Halo<TY, true, TTI, C> hU;
         Halo<TY, TTI, true, C> hV;
@@ -473,18 +518,18 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         // the wave wait for earlier loads into those registers.
         C f0, f1, f2, f3, f4, f5, p0, q0;
         if (valid && split) {
-            f0 = ldw(px, id, sc.s[F_PX]);
-            f1 = ldw(py, id, sc.s[F_PY]);
-            f2 = ldw(pz, id, sc.s[F_PZ]);
+            f0 = ldw(px, zo, col, sc.s[F_PX]);
+            f1 = ldw(py, zo, col, sc.s[F_PY]);
+            f2 = ldw(pz, zo, col, sc.s[F_PZ]);
             if constexpr (!ISO) {
-                f3 = ldw(qx, id, sc.s[F_QX]);
-                f4 = ldw(qy, id, sc.s[F_QY]);
-                f5 = ldw(qz, id, sc.s[F_QZ]);
+                f3 = ldw(qx, zo, col, sc.s[F_QX]);
+                f4 = ldw(qy, zo, col, sc.s[F_QY]);
+                f5 = ldw(qz, zo, col, sc.s[F_QZ]);
             }
         }
         if (valid && !split) {
-            p0 = ldw((const S*)Po, id, sc.s[F_P]);
-            if constexpr (!ISO) q0 = ldw((const S*)Qo, id, sc.s[F_Q]);
+            p0 = ldw((const S*)Po, zo, col, sc.s[F_P]);
+            if constexpr (!ISO) q0 = ldw((const S*)Qo, zo, col, sc.s[F_Q]);
         }
         __syncthreads();
         sU[ty + MM][tx + MM] = uc;
@@ -521,9 +566,9 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     const C npx = ax2 * px0 - ax1 * r0 * c0 * c0 * dtx * xx;
                     const C npy = ay2 * py0 - ay1 * r0 * c0 * c0 * dty * yy;
                     const C npz = az2 * pz0 - az1 * r0 * c0 * c0 * dtz * zz;
-                    stw(px, id, npx, sc.inv[F_PX]);
-                    stw(py, id, npy, sc.inv[F_PY]);
-                    stw(pz, id, npz, sc.inv[F_PZ]);
+                    stw(px, zo, col, npx, sc.inv[F_PX]);
+                    stw(py, zo, col, npy, sc.inv[F_PY]);
+                    stw(pz, zo, col, npz, sc.inv[F_PZ]);
                     Pn = npx + npy + npz;
                 } else {
                     const C K = r0 * c0 * c0;
@@ -584,12 +629,12 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                         nqy = ay2 * qy0 - ay1 * c0 * c0 * s2 * dty * yy;
                         nqz = az2 * qz0 - az1 * c0 * c0 * dtz * zz;
                     }
-                    stw(px, id, npx, sc.inv[F_PX]);
-                    stw(py, id, npy, sc.inv[F_PY]);
-                    stw(pz, id, npz, sc.inv[F_PZ]);
-                    stw(qx, id, nqx, sc.inv[F_QX]);
-                    stw(qy, id, nqy, sc.inv[F_QY]);
-                    stw(qz, id, nqz, sc.inv[F_QZ]);
+                    stw(px, zo, col, npx, sc.inv[F_PX]);
+                    stw(py, zo, col, npy, sc.inv[F_PY]);
+                    stw(pz, zo, col, npz, sc.inv[F_PZ]);
+                    stw(qx, zo, col, nqx, sc.inv[F_QX]);
+                    stw(qy, zo, col, nqy, sc.inv[F_QY]);
+                    stw(qz, zo, col, nqz, sc.inv[F_QZ]);
                     Pn = npx + npy + npz;
                     Qn = nqx + nqy + nqz;
                 } else {
@@ -598,9 +643,9 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     Pn = p0 - K * (e2 * hxy + s2 * hz);
                     Qn = q0 - K * (s2 * hxy + hz);
                 }
-                stw(Qo, id, Qn, sc.inv[F_Q]);
+                stw(Qo, zo, col, Qn, sc.inv[F_Q]);
             }
-            stw(Po, id, Pn, sc.inv[F_P]);
+            stw(Po, zo, col, Pn, sc.inv[F_P]);
             if (recx && z == a.zrec) rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
         }
     }
@@ -659,22 +704,22 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     const M* VV = (const M*)a.vv;
     const C dx2 = (C)a.dx2, dy2 = (C)a.dy2, dz2 = (C)a.dz2;
 
-    auto ldY = [&](int i) { return inj<C>(ldw(Y, i, sp), i, a.s0, a.ya0, a.s1, a.ya1); };
+    auto ldY = [&](int zo, int o) {
+        return inj<C>(ldw(Y, zo, o, sp), zo + o, a.s0, a.ya0, a.s1, a.ya1);
+    };
 
     C wY[9];   // Y(z-4+k), k = 0..8
 #pragma unroll
-    for (int k = 0; k < 8; ++k) wY[k] = ldY((zb - 4 + k) * g.pl + col);
+    for (int k = 0; k < 8; ++k) wY[k] = ldY((zb - 4 + k) * g.pl, col);
     const int xs = x < g.nnx ? x : 0;
     const double fx = a.fx[xs], fy = a.fy[yl];
     const bool recx = a.rec && valid && x >= a.off && x < a.off + a.nx && y >= a.off && y < a.off + a.ny;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
-        const int id = zo + col;
-        wY[8] = ldY(id + 4 * g.pl);
-        const int idc = zo + colc;
-        const C xo = inj<C>(ldw((const S*)X, idc, sp), idc, a.s0, a.xa0, a.s1, a.xa1);
-        const C vv = (C)VV[idc];
+        wY[8] = ldY(zo + 4 * g.pl, col);
+        const C xo = inj<C>(ldw((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
+        const C vv = (C)bld(VV, zo, colc);
         Halo<TY, true, true, C> hY;
         load_halo(hY, g, zo, x0, y0, tx, ty, yl, ldY);
         const double fz = a.fz[z];
@@ -698,7 +743,7 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
             const C prev = damp3<C>(xo, fy, fx, fz);
             const C lap = vv * (c1 + c2 + c3);
             const C nv = (C)(2.0 * (double)cur - (double)prev + (double)lap);
-            stw(X, id, damp3<C>(nv, fy, fx, fz), ip);
+            stw(X, zo, col, damp3<C>(nv, fy, fx, fz), ip);
             if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
         }
     }
