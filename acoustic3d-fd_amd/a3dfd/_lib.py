@@ -129,6 +129,8 @@ def load(path=None):
     lib.a3d_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_set_strip_precision.argtypes = [vp, c_int]
     lib.a3d_set_zchunk.argtypes = [vp, c_int]
+    lib.a3d_set_w_sweep.argtypes = [vp, c_int]
+    lib.a3d_get_w_sweep.argtypes = [vp, POINTER(ShotDesc), POINTER(c_int)]
     lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_int, c_float, ctypes.c_uint, fp, fp,
                                       fp, c_int, POINTER(c_int)]
     lib.a3d_mlp_create.argtypes = [c_int, POINTER(vp)]
@@ -139,7 +141,7 @@ def load(path=None):
     lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
     lib.a3d_mlp_train.argtypes = [vp, fp, fp, c_int, c_int, c_int, c_float, c_float, ctypes.c_uint, POINTER(c_double)]
     for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_set_strip_profile", "a3d_set_strip_precision",
-                 "a3d_set_zchunk", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
+                 "a3d_set_zchunk", "a3d_set_w_sweep", "a3d_get_w_sweep", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
                  "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
                  "a3d_mlp_loss_grad", "a3d_mlp_train"):
         getattr(lib, name).restype = c_int
@@ -242,11 +244,22 @@ class Modeller:
         """Two-sweep step in z chunks of `planes` planes (0: whole grid); bit-identical results."""
         _check(self.lib, self.lib.a3d_set_zchunk(self._h, planes))
 
+    def set_w_sweep(self, mode):
+        """Which sweep updates w (ISO, VTI): 0 velocity sweep, 1 stress sweep (bit-identical,
+        fewer bytes per step), -1 the autotuner's choice (default)."""
+        _check(self.lib, self.lib.a3d_set_w_sweep(self._h, mode))
+
+    def w_sweep(self, desc=None):
+        """1 if a shot of desc (default: the handle's) runs with w updated by the stress sweep."""
+        m = c_int(0)
+        _check(self.lib, self.lib.a3d_get_w_sweep(self._h, ctypes.byref(desc or self.desc), ctypes.byref(m)))
+        return m.value
+
     def launch_info(self):
         v = (c_int * 6)()
         _check(self.lib, self.lib.a3d_get_launch(self._h, v))
         return dict(ty_vel=v[0], zseg_vel=v[1], blocks_vel=v[2], ty_stress=v[3], zseg_stress=v[4],
-                    blocks_stress=v[5])
+                    blocks_stress=v[5], w_sweep=self.w_sweep())
 
     def autotune(self, desc=None, steps=4):
         best = (c_double * 2)()

@@ -224,6 +224,10 @@ struct a3d_ctx {
     float strip_amp = 10.0f;    // PML amplitude of the strip's 6-cell ramp
     int strip_mfma = 1;         // network on the matrix cores (binary16 operands) or FP32 per lane
     int zchunk = 0;             // 1st-order two-sweep step in z chunks of this many planes (0: whole grid)
+    // which sweep updates w (ISO, VTI): requested mode (-1 tuner, 0 velocity sweep,
+    // 1 stress sweep) and the tuner's choice for mode -1
+    int wz_mode = -1;
+    int wz_tuned = 0;
 };
 
 namespace {
@@ -385,6 +389,15 @@ Scales field_scales(const a3d_ctx* c, const a3d_shot_desc* d) {
     return sc;
 }
 
+// The stress sweep can update w (from its own column's old P or Q, one z segment)
+// for ISO and VTI in the two-sweep step.
+bool wz_eligible(const a3d_ctx* c, const a3d_shot_desc* d) {
+    return (d->op == A3D_OP_ISO || d->op == A3D_OP_VTI) && d->scheme != 2 && c->zchunk == 0;
+}
+bool wz_active(const a3d_ctx* c, const a3d_shot_desc* d) {
+    return wz_eligible(c, d) && (c->wz_mode == 1 || (c->wz_mode < 0 && c->wz_tuned));
+}
+
 // Tables, zeroed wavefields, kernel arguments and the per-step sources of one
 // 1st-order shot (reference iso:372-671, vti:389-688, tti:390-676).
 template <class T>
@@ -503,6 +516,8 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.unsplit = d->scheme != 0 || prec == PREC_FP16;
     sa.zi0 = zi0;
     sa.zi1 = zi1;
+    va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
+    if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
     va.s.s0 = s0;
     // The time grid is the reference's float accumulation t = dt, t += dt and the gate
     // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
@@ -736,6 +751,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         HIP_TRY(hipEventRecord(c->ev0, c->st));
         for (int it = 0; it < nt; it++) {
             va.s = srcs[it];
+            sa.s = srcs[it];
             sa.rec = c->rec + plane_rec * it;
             if (nch > 0) {
                 for (int k = 0; k <= nch; ++k) {
@@ -835,13 +851,14 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int ty[2], zseg[2];
+    int ty[2], zseg[2], wz;
 };
 std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 std::mutex g_tune_mu;
 
 int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
-    const TuneKey key{c->device, c->d.op, c->prec, d->scheme, c->zchunk, c->g.nnx, c->g.nny, c->g.nnz};
+    const TuneKey key{c->device, c->d.op, c->prec, d->scheme * 4 + (c->wz_mode + 1), c->zchunk,
+                      c->g.nnx, c->g.nny, c->g.nnz};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (auto& kv : g_tune_cache)
@@ -850,6 +867,7 @@ int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
                     c->ty[k] = kv.second.ty[k];
                     c->zseg[k] = kv.second.zseg[k];
                 }
+                c->wz_tuned = kv.second.wz;
                 c->tuned = true;
                 return A3D_OK;
             }
@@ -860,58 +878,75 @@ int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
     td.mute = 0;
     const Timing saved = c->tm;
     const int nzi = c->g.nnz - 2 * MM;
-    double best[2] = {1e30, 1e30};
-    int bty[2] = {c->ty[0], c->ty[1]}, bzs[2] = {c->zseg[0], c->zseg[1]};
+    // best[m][k]: fastest launch of sweep k with w updated by sweep m (0 velocity, 1 stress);
+    // the two modes are compared on the sum of their two sweeps
+    double best[2][2] = {{1e30, 1e30}, {1e30, 1e30}};
+    int bty[2][2], bzs[2][2];
+    for (int m = 0; m < 2; ++m)
+        for (int k = 0; k < 2; ++k) {
+            bty[m][k] = c->ty[k];
+            bzs[m][k] = c->zseg[k];
+        }
     const int tys[3] = {2, 4, 8};
     const int segs[9] = {1, 2, 3, 4, 6, 8, 12, 16, 24};
+    const bool elig = wz_eligible(c, d);
+    const int m0 = elig && c->wz_mode == 1 ? 1 : 0, m1 = elig && c->wz_mode != 0 ? 1 : 0;
     int rc = A3D_OK;
     c->tm = Timing();
     c->tm.pool = saved.pool;   // reuse the event pool
     c->tm.on = true;
     c->tuned = true;           // run_shot below must not re-enter the tuner
-    for (int ty : tys) {
-        for (int ns : segs) {
-            if (ns > 1 && nzi / ns < 8) continue;
-            for (int k = 0; k < 2; ++k) {
-                c->ty[k] = ty;
-                c->zseg[k] = ns;
-            }
-            c->tm.ms[0] = c->tm.ms[1] = 0;
-            c->tm.n[0] = c->tm.n[1] = 0;
-            rc = run_shot(c, &td, nullptr, SnapFn(), nullptr, nullptr);
-            if (rc && rc != A3D_EUNSTABLE) break;
-            rc = A3D_OK;
-            for (int k = 0; k < 2; ++k) {
-                if (c->tm.n[k] == 0) continue;
-                const double avg = c->tm.ms[k] / c->tm.n[k];
-                if (avg < best[k]) {
-                    best[k] = avg;
-                    bty[k] = ty;
-                    bzs[k] = ns;
+    const int wz_mode = c->wz_mode;
+    for (int m = m0; m <= m1 && !rc; ++m) {
+        c->wz_mode = m;
+        for (int ty : tys) {
+            for (int ns : segs) {
+                if (ns > 1 && nzi / ns < 8) continue;
+                for (int k = 0; k < 2; ++k) {
+                    c->ty[k] = ty;
+                    c->zseg[k] = ns;
+                }
+                c->tm.ms[0] = c->tm.ms[1] = 0;
+                c->tm.n[0] = c->tm.n[1] = 0;
+                rc = run_shot(c, &td, nullptr, SnapFn(), nullptr, nullptr);
+                if (rc && rc != A3D_EUNSTABLE) break;
+                rc = A3D_OK;
+                for (int k = 0; k < 2; ++k) {
+                    if (c->tm.n[k] == 0) continue;
+                    const double avg = c->tm.ms[k] / c->tm.n[k];
+                    if (avg < best[m][k]) {
+                        best[m][k] = avg;
+                        bty[m][k] = ty;
+                        bzs[m][k] = ns;
+                    }
                 }
             }
+            if (rc) break;
         }
-        if (rc) break;
     }
+    c->wz_mode = wz_mode;
+    const int bm = m0 == m1 ? m0 : (best[1][0] + best[1][1] < best[0][0] + best[0][1] ? 1 : 0);
     std::vector<hipEvent_t> pool = c->tm.pool;
     c->tm = saved;
     c->tm.pool = pool;
     c->tm.used = 0;
     c->tm.marks.clear();
     for (int k = 0; k < 2; ++k) {
-        c->ty[k] = bty[k];
-        c->zseg[k] = bzs[k];
+        c->ty[k] = bty[bm][k];
+        c->zseg[k] = bzs[bm][k];
     }
+    if (bm == 1) c->zseg[1] = 1;   // the stress sweep that updates w runs one z segment
+    c->wz_tuned = bm;
     if (rc) {
         c->tuned = false;
         return rc;
     }
     if (best_ms) {
-        best_ms[0] = best[0] < 1e29 ? best[0] : 0.0;
-        best_ms[1] = best[1] < 1e29 ? best[1] : 0.0;
+        best_ms[0] = best[bm][0] < 1e29 ? best[bm][0] : 0.0;
+        best_ms[1] = best[bm][1] < 1e29 ? best[bm][1] : 0.0;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache.push_back({key, TuneVal{{bty[0], bty[1]}, {bzs[0], bzs[1]}}});
+    g_tune_cache.push_back({key, TuneVal{{c->ty[0], c->ty[1]}, {c->zseg[0], c->zseg[1]}, bm}});
     return A3D_OK;
 }
 
@@ -1170,6 +1205,19 @@ int a3d_get_launch(a3d_ctx* c, int* out6) {
 int a3d_autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
     if (!c || !d) return fail(A3D_EINVAL, "null argument");
     return autotune(c, d, steps, best_ms);
+}
+
+int a3d_set_w_sweep(a3d_ctx* c, int mode) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    if (mode < -1 || mode > 1) return fail(A3D_EINVAL, "w sweep mode %d not in -1..1", mode);
+    c->wz_mode = mode;
+    return A3D_OK;
+}
+
+int a3d_get_w_sweep(a3d_ctx* c, const a3d_shot_desc* d, int* mode) {
+    if (!c || !d || !mode) return fail(A3D_EINVAL, "null argument");
+    *mode = wz_active(c, d) ? 1 : 0;
+    return A3D_OK;
 }
 
 int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs) {

@@ -82,6 +82,7 @@ struct VelArgs {
     double dtx, dty, dtz;
     Src s;
     Scales sc;
+    int wz;                    // 1 (ISO, VTI): w is updated by the stress sweep instead
 };
 
 struct StressArgs {
@@ -105,6 +106,8 @@ struct StressArgs {
     int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1
     int zi0, zi1;              // planes whose z stress factor is exactly 1: zi0 <= z < zi1
     Scales sc;
+    int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
+    Src s;                     // wz: this step's source, added to P / Q as the velocity sweep reads them
 };
 
 // One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out

@@ -127,6 +127,12 @@ __device__ __forceinline__ void stw(double* p, int zo, int o, double v, float) {
 __device__ __forceinline__ void stw(__half* p, int zo, int o, float v, float inv) {
     bst((_Float16*)p, zo, o, __builtin_bit_cast(_Float16, __float2half_rn(v * inv)));
 }
+// The value a store of v followed by a load returns (binary16 storage rounds).
+__device__ __forceinline__ float stored(const float*, float v, float, float) { return v; }
+__device__ __forceinline__ double stored(const double*, double v, float, float) { return v; }
+__device__ __forceinline__ float stored(const __half*, float v, float inv, float sc) {
+    return __half2float(__float2half_rn(v * inv)) * sc;
+}
 
 // Staggered difference over an 8-point window w(0..7) whose centre pair is
 // (w(4), w(3)): forward windows hold F[-3..+4], backward windows F[-4..+3].
@@ -257,14 +263,17 @@ __device__ __forceinline__ void store_halo(C (*s)[TX + 2 * MM], const Halo<TY, H
 // --------------------------------------------------------------------------
 // 1st-order velocity sweep
 // --------------------------------------------------------------------------
-template <class PR, int OP, int TY>
+// WZ (ISO, VTI): the stress sweep makes w (from its own column's P or Q), so this
+// sweep updates u and v only and reads neither the z window nor w.
+template <class PR, int OP, int TY, bool WZ = false>
 __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using M = typename PR::M;
     using C = typename PR::C;
     constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
-    constexpr bool P_Z = !VTI;   // P differentiated along z (ISO: all axes; TTI: all axes)
-    constexpr bool HAS_Q = !ISO;
+    static_assert(!(WZ && TTI), "TTI's w needs x-y differences of Q");
+    constexpr bool P_Z = !VTI && !WZ;   // P differentiated along z (ISO: all axes; TTI: all axes)
+    constexpr bool HAS_Q = !ISO && !WZ;
     constexpr bool Q_XY = TTI;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
     __shared__ C sP[H][W];
@@ -322,7 +331,8 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
         const C u0 = ldw(U, zo, colc, sc.s[F_U]);
         const C v0 = ldw(V, zo, colc, sc.s[F_V]);
-        const C w0 = ldw(Wv, zo, colc, sc.s[F_W]);
+        C w0 = 0;
+        if constexpr (!WZ) w0 = ldw(Wv, zo, colc, sc.s[F_W]);
         if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
         if constexpr (TTI) {
             ct = (C)bld(Ct, zo, colc);
@@ -365,7 +375,7 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
                 zz = ISO ? Pz : Qz;
                 xx /= r0;
                 yy /= r0;
-                zz /= r0;
+                if constexpr (!WZ) zz /= r0;
             } else {
                 const C Py = dst8<C>([&](int k) { return sP[r - 3 + k][c]; });
                 const C Px = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
@@ -377,16 +387,16 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
             }
             stw(U, zo, col, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
             stw(V, zo, col, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
-            stw(Wv, zo, col, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
+            if constexpr (!WZ) stw(Wv, zo, col, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
         }
     }
 }
 
-template <class PR, int OP, int TY>
+template <class PR, int OP, int TY, bool WZ = false>
 __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
-    vel_body<PR, OP, TY>(a, a.s, tix, tiy, seg);
+    vel_body<PR, OP, TY, WZ>(a, a.s, tix, tiy, seg);
 }
 
 // Batched velocity sweep: the tiles of all windows of a batch in one launch; per-window
@@ -406,13 +416,18 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
 // --------------------------------------------------------------------------
 // 1st-order stress sweep
 // --------------------------------------------------------------------------
-template <class PR, int OP, int TY>
+// WZ (ISO, VTI; one z segment): this sweep also makes w, from its own column's old
+// P (ISO) or Q (VTI) -- the velocity sweep's arithmetic, bit for bit -- one plane
+// ahead of where the backward window needs it. The column reads F = P or Q at
+// planes z..z+7 before writing plane z, so no other tile's values are involved.
+template <class PR, int OP, int TY, bool WZ = false>
 __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using SP = typename PR::SP;
     using M = typename PR::M;
     using C = typename PR::C;
     constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
+    static_assert(!(WZ && TTI), "TTI's w needs x-y differences of Q");
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
     __shared__ C sU[H][W];
     __shared__ C sV[H][W];
@@ -457,32 +472,83 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     auto ldV = [&](int zo, int o) { return ldw(V, zo, o, sc.s[F_V]); };
     auto ldW = [&](int zo, int o) { return ldw(Wv, zo, o, sc.s[F_W]); };
 
+    const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
     C wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
+    // WZ: fF = old F (P for ISO, Q for VTI) at planes z..z+7, fr = density at z..z+3
+    C fF[8], fr[4];
+    const S* Fo = ISO ? (const S*)Po : (const S*)Qo;
+    const float sF = sc.s[ISO ? F_P : F_Q];
+    const double fa0 = ISO ? a.s.p0 : a.s.q0, fa1 = ISO ? a.s.p1 : a.s.q1;
+    // lanes of the source columns add this step's source to F as the velocity sweep does
+    const bool srccol = WZ && ((a.s.s0 >= 0 && a.s.s0 % g.pl == col) || (a.s.s1 >= 0 && a.s.s1 % g.pl == col));
+    // new w at plane p (velocity sweep arithmetic) from Fj(j) = F at plane p-3+j
+    auto wcalc = [&](int p, auto Fj, C wo, C r, C cz1, C cz2) -> C {
+        C zz = dst8<C>([&](int j) {
+            C v = Fj(j);
+            if (srccol) v = inj<C>(v, (p - 3 + j) * g.pl + col, a.s.s0, fa0, a.s.s1, fa1);
+            return v;
+        });
+        zz /= r;
+        return cz2 * wo - cz1 * dtz * zz;
+    };
+    if constexpr (WZ) {
+        // new w at zb-4 .. zb+2 (planes outside the updated range keep their value)
+        C fw[14];   // F at zb-7 .. zb+6
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        const int zi = (zb - 4 + k) * g.pl;
-        wW[k] = ldW(zi, col);
-        if constexpr (TTI) {
-            wU[k] = ldU(zi, col);
-            wV[k] = ldV(zi, col);
+        for (int i = 0; i < 14; ++i) fw[i] = ldw(Fo, min(max(zb - 7 + i, 0), g.nnz - 1) * g.pl, col, sF);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int p = zb - 4 + k;
+            const C wo = ldW(p * g.pl, col);
+            if (p >= MM && p < g.nnz - MM) {
+                const C r = (C)bld(rho, p * g.pl, colc);
+                const C wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, T[a.c.cz1 + p], T[a.c.cz2 + p]);
+                if (valid && p >= zb) stw((S*)Wv, p * g.pl, col, wn, sc.inv[F_W]);
+                wW[k] = stored(Wv, wn, sc.inv[F_W], sc.s[F_W]);   // as the stress sweep would read it
+            } else {
+                wW[k] = wo;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 7; ++j) fF[j] = fw[7 + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) fr[j] = (C)bld(rho, (zb + j) * g.pl, colc);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int zi = (zb - 4 + k) * g.pl;
+            wW[k] = ldW(zi, col);
+            if constexpr (TTI) {
+                wU[k] = ldU(zi, col);
+                wV[k] = ldV(zi, col);
+            }
         }
     }
     const int xs = x < g.nnx ? x : 0;
     const C ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
     const C ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
-    const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
     const bool recx = rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
-        wW[7] = ldW(zo + 3 * g.pl, col);
+        C wo = 0, czw1 = 0, czw2 = 0;   // WZ: old w, z velocity factors at plane z+3
+        if constexpr (WZ) {
+            fF[7] = ldw(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
+            wo = ldW(zo + 3 * g.pl, col);
+            fr[3] = (C)bld(rho, zo + 3 * g.pl, colc);
+            czw1 = T[a.c.cz1 + z + 3];
+            czw2 = T[a.c.cz2 + z + 3];
+        } else {
+            wW[7] = ldW(zo + 3 * g.pl, col);
+        }
         if constexpr (TTI) {
             wU[7] = ldU(zo + 3 * g.pl, col);
             wV[7] = ldV(zo + 3 * g.pl, col);
         }
         C r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
         const C c0 = (C)bld(vp, zo, colc);
-        if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
+        if constexpr (WZ) r0 = fr[0];
+        else if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
         if constexpr (!ISO) {
             e0 = (C)bld(eps, zo, colc);
             d0 = (C)bld(del, zo, colc);
@@ -528,8 +594,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             }
         }
         if (valid && !split) {
-            p0 = ldw((const S*)Po, zo, col, sc.s[F_P]);
-            if constexpr (!ISO) q0 = ldw((const S*)Qo, zo, col, sc.s[F_Q]);
+            // WZ: the old value of F at plane z is fF[0]
+            if constexpr (!(WZ && ISO)) p0 = ldw((const S*)Po, zo, col, sc.s[F_P]);
+            if constexpr (!ISO && !WZ) q0 = ldw((const S*)Qo, zo, col, sc.s[F_Q]);
+        }
+        if constexpr (WZ) {
+            if constexpr (ISO) p0 = fF[0];
+            else q0 = fF[0];
         }
         __syncthreads();
         sU[ty + MM][tx + MM] = uc;
@@ -541,6 +612,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             store_halo(sW, hW, tx, ty);
         }
         __syncthreads();
+        // WZ: new w at plane z+3 completes the backward window
+        const bool wupd = WZ && z + 3 < g.nnz - MM;
+        C wz3 = wo;
+        if constexpr (WZ) {
+            if (wupd) wz3 = wcalc(z + 3, [&](int j) { return fF[j]; }, wo, fr[3], czw1, czw2);
+            wW[7] = stored(Wv, wz3, sc.inv[F_W], sc.s[F_W]);   // as the stress sweep would read it
+        }
         // z differences, then the window shift, before any store: the loop comes
         // round without waiting for this plane's stores
         const C Wz = dst8<C>([&](int k) { return wW[k]; });
@@ -553,7 +631,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 wU[k] = wU[k + 1];
                 wV[k] = wV[k + 1];
             }
+            if constexpr (WZ) fF[k] = fF[k + 1];
         }
+        if constexpr (WZ) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) fr[k] = fr[k + 1];
+        }
+        if (valid && wupd) stw((S*)Wv, zo + 3 * g.pl, col, wz3, sc.inv[F_W]);
         if (valid) {
             const int r = ty + MM, c = tx + MM;
             C Pn, Qn = 0;
@@ -651,11 +735,11 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     }
 }
 
-template <class PR, int OP, int TY>
+template <class PR, int OP, int TY, bool WZ = false>
 __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
-    stress_body<PR, OP, TY>(a, a.rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, WZ>(a, a.rec, tix, tiy, seg);
 }
 
 // Batched stress sweep; a window's record plane of step it is rec + it * nx * ny.
@@ -864,25 +948,25 @@ __global__ void k_probe_read(const float4* __restrict__ a, size_t n, float* out)
 }
 
 // ---- launch dispatch -------------------------------------------------------
-template <class PR, int OP>
+template <class PR, int OP, bool WZ>
 hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_vel<PR, OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_vel<PR, OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_vel<PR, OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_vel<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_vel<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_vel<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <class PR, int OP>
+template <class PR, int OP, bool WZ>
 hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -951,9 +1035,9 @@ hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base,
 template <class PR>
 hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
     switch (op) {
-        case 1: return vel_ty<PR, 1>(ty, a, st);
-        case 2: return vel_ty<PR, 2>(ty, a, st);
-        case 3: return vel_ty<PR, 3>(ty, a, st);
+        case 1: return a.wz ? vel_ty<PR, 1, true>(ty, a, st) : vel_ty<PR, 1, false>(ty, a, st);
+        case 2: return a.wz ? vel_ty<PR, 2, true>(ty, a, st) : vel_ty<PR, 2, false>(ty, a, st);
+        case 3: return a.wz ? hipErrorInvalidValue : vel_ty<PR, 3, false>(ty, a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -961,9 +1045,9 @@ hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
 template <class PR>
 hipError_t stress_op(int op, int ty, const StressArgs& a, hipStream_t st) {
     switch (op) {
-        case 1: return stress_ty<PR, 1>(ty, a, st);
-        case 2: return stress_ty<PR, 2>(ty, a, st);
-        case 3: return stress_ty<PR, 3>(ty, a, st);
+        case 1: return a.wz ? stress_ty<PR, 1, true>(ty, a, st) : stress_ty<PR, 1, false>(ty, a, st);
+        case 2: return a.wz ? stress_ty<PR, 2, true>(ty, a, st) : stress_ty<PR, 2, false>(ty, a, st);
+        case 3: return a.wz ? hipErrorInvalidValue : stress_ty<PR, 3, false>(ty, a, st);
         default: return hipErrorInvalidValue;
     }
 }

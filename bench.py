@@ -37,19 +37,26 @@ HBM_PEAK_GBS = 8000.0
 A_VEL, A_STRESS = 36, 84
 
 
-def algorithmic_bytes(cfg, storage, scheme, npts, npml=20):
+def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
     """Algorithmic bytes per updated point of the VTI velocity and stress sweeps for the
     formulation run: wavefields of ws bytes (4 FP32, 2 binary16, 8 FP64), split PML fields
     of ss bytes (FP32 under binary16), model of ms bytes.
     velocity: R P,Q,rho,u,v,w; W u,v,w. stress, split cell: R u,v,w, 6 split, vp,rho,eps,delta;
     W 6 split, P, Q. stress, unsplit cell (scheme 1 outside the PML): R u,v,w, P,Q, 4 model; W P,Q.
+    wz (w updated by the stress sweep): velocity R P,rho,u,v; W u,v. stress reads u,v, the old Q
+    one plane ahead (also the unsplit cell's Q), the old w, and writes w; otherwise as above.
     scheme 2 (one pass, FP32): (0, R P,Q,u,v,w, 4 model; W P,Q,u,v,w; + R/W 6 split in PML cells)."""
     ws = {0: 4, 1: 2, 2: 8}[storage]
     ms = 8 if storage == 2 else 4
     ss = max(ws, 4)                     # split PML fields are FP32 under binary16 storage
-    a_vel = 8 * ws + ms
-    split = 5 * ws + 12 * ss + 4 * ms
-    unsplit = 7 * ws + 4 * ms
+    if wz:
+        a_vel = 5 * ws + ms
+        split = 7 * ws + 12 * ss + 4 * ms
+        unsplit = 8 * ws + 4 * ms
+    else:
+        a_vel = 8 * ws + ms
+        split = 5 * ws + 12 * ss + 4 * ms
+        unsplit = 7 * ws + 4 * ms
     inner = (cfg["nx"] + 1) * (cfg["ny"] + 1) * (cfg["nz"] + 1)   # cells whose stress PML factors are all 1
     f = inner / npts
     if scheme == 2:
@@ -204,8 +211,10 @@ def strip_bench(args):
 
 
 # Algorithmic bytes per updated point (FP32): (velocity sweep or 2nd-order step, stress sweep with
-# split fields, stress sweep without them) -- reads + writes of each sweep, see DESIGN.md's table.
+# split fields, stress sweep without them) -- reads + writes of each sweep; OP_BYTES_WZ with w
+# updated by the stress sweep (ISO: stress reads P one plane ahead and the old w, writes w).
 OP_BYTES = {"2nd": (16, 0, 0), "iso": (32, 48, 28), "vti": (36, 84, 44), "tti": (48, 96, 56)}
+OP_BYTES_WZ = {"iso": (24, 56, 32), "vti": (24, 92, 48)}
 
 
 def operators(args):
@@ -238,7 +247,7 @@ def operators(args):
             gi = mdl.grid_info()
             launch = mdl.launch_info()
         npts = gi["updated_points"]
-        a0, a_split, a_unsplit = OP_BYTES[op]
+        a0, a_split, a_unsplit = OP_BYTES_WZ[op] if launch["w_sweep"] else OP_BYTES[op]
         nbb = d.nb
         inner = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1) if op != "2nd" else npts
         f = min(1.0, inner / npts)
@@ -271,6 +280,8 @@ def main():
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--launch", type=str, default="", help="ty_vel,zseg_vel,ty_stress,zseg_stress (default: autotuned)")
+    ap.add_argument("--w-sweep", type=int, default=None, choices=(-1, 0, 1),
+                    help="sweep that updates w: 0 velocity, 1 stress, -1 autotuned (default)")
     ap.add_argument("--zchunk", type=int, default=0, help="two-sweep step in z chunks of this many planes (0: off)")
     ap.add_argument("--precision-study", action="store_true",
                     help="run the shot in FP32, FP16-storage and FP64 and print the errors vs FP64")
@@ -325,6 +336,8 @@ def main():
         mdl.set_zchunk(args.zchunk)
     if args.launch:
         mdl.set_launch(*[int(x) for x in args.launch.split(",")])
+    if args.w_sweep is not None:
+        mdl.set_w_sweep(args.w_sweep)
     gi = mdl.grid_info()
     npts = gi["updated_points"]
     log(f"bench: padded {gi['nnx']}x{gi['nny']}x{gi['nnz']}, updated points {npts}, launch {mdl.launch_info()}")
@@ -352,7 +365,8 @@ def main():
     avg_vel = kms[0] / steps_run     # per time step (a chunked step launches each sweep once per chunk)
     avg_str = kms[1] / steps_run
     main_kernel = "k_fused_vti" if args.scheme == 2 else "k_stress<VTI>"
-    a_vel, a_str = algorithmic_bytes(cfg, args.storage, args.scheme, npts)
+    wz = launch["w_sweep"]
+    a_vel, a_str = algorithmic_bytes(cfg, args.storage, args.scheme, npts, wz=wz)
     ach_str = a_str * npts / (avg_str * 1e-3) / 1e9
     ach_vel = a_vel * npts / (avg_vel * 1e-3) / 1e9 if kn[0] else 0.0
     step_gbs = (a_vel + a_str) * gpts
@@ -360,7 +374,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_vti_400x400x300.json")
     if os.path.exists(tpath) and args.storage == 0:
         with open(tpath) as f:
-            traffic = json.load(f).get(f"scheme{args.scheme}", {}).get("stress_bytes_per_launch")
+            traffic = json.load(f).get(f"scheme{args.scheme}" + ("_wz" if wz else ""), {}).get("stress_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:

@@ -173,6 +173,16 @@ int a3d_set_launch(a3d_ctx* ctx, int ty_vel, int zseg_vel, int ty_stress, int zs
 /* out6 = {ty, z segments, workgroups} of sweep 0, then of sweep 1. */
 int a3d_get_launch(a3d_ctx* ctx, int* out6);
 
+/* Which sweep updates the z velocity w (ISO and VTI two-sweep step; no counterpart in
+ * the reference, whose velocity kernel updates u, v and w): 0 the velocity sweep, as the
+ * reference; 1 the stress sweep, from its own column's old P (ISO) or Q (VTI), one plane
+ * ahead of where it needs w -- the same arithmetic, bit-identical results, 8 fewer
+ * bytes per point and step, the stress sweep in one z segment; -1 (default) the
+ * autotuner's choice (the velocity sweep when the tuner does not run).
+ * a3d_get_w_sweep reports the mode a shot of *desc runs with. */
+int a3d_set_w_sweep(a3d_ctx* ctx, int mode);
+int a3d_get_w_sweep(a3d_ctx* ctx, const a3d_shot_desc* desc, int* mode);
+
 /* 1st-order two-sweep step in z chunks of `planes` planes (0: whole-grid sweeps, the
  * default; else >= 8): per step the velocity sweep of chunk k+1 runs before the stress
  * sweep of chunk k, so the velocities a stress chunk reads were written one launch

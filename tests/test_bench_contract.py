@@ -34,3 +34,9 @@ def test_algorithmic_bytes():
     a_vel, a_str = b.algorithmic_bytes(cfg, 1, 0, npts)                # binary16 wavefields: always scheme 1
     assert a_vel == 8 * 2 + 4 and abs(a_str - (f * (7 * 2 + 16) + (1 - f) * (5 * 2 + 12 * 4 + 16))) < 1e-9
     assert b.OP_BYTES["vti"] == (36, 84, 44) and b.OP_BYTES["2nd"][0] == 16
+    # w updated by the stress sweep: velocity R P,rho,u,v W u,v; the stress sweep reads the
+    # old Q one plane ahead (its own Q too) and the old w, and writes w
+    a_vel, a_str = b.algorithmic_bytes(cfg, 0, 1, npts, wz=1)
+    assert a_vel == 24 and abs(a_str - (f * 48 + (1 - f) * 92)) < 1e-9
+    assert b.algorithmic_bytes(cfg, 0, 0, npts, wz=1) == (24, 92.0)
+    assert b.OP_BYTES_WZ["vti"] == (24, 92, 48) and b.OP_BYTES_WZ["iso"] == (24, 56, 32)
