@@ -92,40 +92,51 @@ template <class T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const T* p, int zo) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p + (ptrdiff_t)zo), 0, 0x7fffffff, 0x00020000);
 }
-template <class T>
+// Cache policy A: 0 default; NT (non-temporal) for streams no other tile re-reads
+// (pointwise fields, the z-window leads, every store), so the L2 keeps the rows
+// that neighbouring tiles read as halos.
+constexpr int NT = 2;
+template <int A = 0, class T>
 __device__ __forceinline__ T bld(const T* p, int zo, int o) {
     const __amdgpu_buffer_rsrc_t r = plane_rsrc(p, zo);
     if constexpr (sizeof(T) == 2) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, o * 2, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, o * 2, 0, A));
     } else if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, o * 4, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, o * 4, 0, A));
     } else {
         static_assert(sizeof(T) == 8, "2, 4 or 8-byte elements");
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, o * 8, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, o * 8, 0, A));
     }
 }
-template <class T>
+template <int A = 0, class T>
 __device__ __forceinline__ void bst(T* p, int zo, int o, T v) {
     const __amdgpu_buffer_rsrc_t r = plane_rsrc(p, zo);
     if constexpr (sizeof(T) == 2) {
-        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, o * 2, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, o * 2, 0, A);
     } else if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, o * 4, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, o * 4, 0, A);
     } else {
         typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, o * 8, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, o * 8, 0, A);
     }
 }
-// The same per plane (element o of the plane at offset zo), and the matching stores.
-__device__ __forceinline__ float ldw(const float* p, int zo, int o, float) { return bld(p, zo, o); }
-__device__ __forceinline__ double ldw(const double* p, int zo, int o, float) { return bld(p, zo, o); }
+// The same per plane (element o of the plane at offset zo), and the matching stores
+// (non-temporal by default: no sweep re-reads what it writes).
+template <int A = 0>
+__device__ __forceinline__ float ldw(const float* p, int zo, int o, float) { return bld<A>(p, zo, o); }
+template <int A = 0>
+__device__ __forceinline__ double ldw(const double* p, int zo, int o, float) { return bld<A>(p, zo, o); }
+template <int A = 0>
 __device__ __forceinline__ float ldw(const __half* p, int zo, int o, float sc) {
-    return (float)bld((const _Float16*)p, zo, o) * sc;
+    return (float)bld<A>((const _Float16*)p, zo, o) * sc;
 }
-__device__ __forceinline__ void stw(float* p, int zo, int o, float v, float) { bst(p, zo, o, v); }
-__device__ __forceinline__ void stw(double* p, int zo, int o, double v, float) { bst(p, zo, o, v); }
+template <int A = NT>
+__device__ __forceinline__ void stw(float* p, int zo, int o, float v, float) { bst<A>(p, zo, o, v); }
+template <int A = NT>
+__device__ __forceinline__ void stw(double* p, int zo, int o, double v, float) { bst<A>(p, zo, o, v); }
+template <int A = NT>
 __device__ __forceinline__ void stw(__half* p, int zo, int o, float v, float inv) {
-    bst((_Float16*)p, zo, o, __builtin_bit_cast(_Float16, __float2half_rn(v * inv)));
+    bst<A>((_Float16*)p, zo, o, __builtin_bit_cast(_Float16, __float2half_rn(v * inv)));
 }
 // The value a store of v followed by a load returns (binary16 storage rounds).
 __device__ __forceinline__ float stored(const float*, float v, float, float) { return v; }
@@ -329,16 +340,16 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         if constexpr (P_Z) wP[7] = ldP(zo + 4 * g.pl, col);
         if constexpr (HAS_Q) wQ[7] = ldQ(zo + 4 * g.pl, col);
         C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
-        const C u0 = ldw(U, zo, colc, sc.s[F_U]);
-        const C v0 = ldw(V, zo, colc, sc.s[F_V]);
+        const C u0 = ldw<NT>(U, zo, colc, sc.s[F_U]);
+        const C v0 = ldw<NT>(V, zo, colc, sc.s[F_V]);
         C w0 = 0;
-        if constexpr (!WZ) w0 = ldw(Wv, zo, colc, sc.s[F_W]);
-        if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
+        if constexpr (!WZ) w0 = ldw<NT>(Wv, zo, colc, sc.s[F_W]);
+        if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
         if constexpr (TTI) {
-            ct = (C)bld(Ct, zo, colc);
-            st = (C)bld(St, zo, colc);
-            cp = (C)bld(Cp, zo, colc);
-            sp = (C)bld(Sp, zo, colc);
+            ct = (C)bld<NT>(Ct, zo, colc);
+            st = (C)bld<NT>(St, zo, colc);
+            cp = (C)bld<NT>(Cp, zo, colc);
+            sp = (C)bld<NT>(Sp, zo, colc);
         }
         C pc;
         if constexpr (P_Z) pc = wP[3];
@@ -533,11 +544,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         const int zo = z * g.pl;
         C wo = 0, czw1 = 0, czw2 = 0;   // WZ: old w, z velocity factors at plane z+3
         if constexpr (WZ) {
-            fF[7] = ldw(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
-            wo = ldW(zo + 3 * g.pl, col);
-            fr[3] = (C)bld(rho, zo + 3 * g.pl, colc);
+            fF[7] = ldw<NT>(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
+            wo = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);
+            fr[3] = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
             czw1 = T[a.c.cz1 + z + 3];
             czw2 = T[a.c.cz2 + z + 3];
+        } else if constexpr (!TTI) {
+            wW[7] = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);   // TTI reads w's x-y halo too
         } else {
             wW[7] = ldW(zo + 3 * g.pl, col);
         }
@@ -546,18 +559,18 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             wV[7] = ldV(zo + 3 * g.pl, col);
         }
         C r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
-        const C c0 = (C)bld(vp, zo, colc);
+        const C c0 = (C)bld<NT>(vp, zo, colc);
         if constexpr (WZ) r0 = fr[0];
-        else if constexpr (!TTI) r0 = (C)bld(rho, zo, colc);
+        else if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
         if constexpr (!ISO) {
-            e0 = (C)bld(eps, zo, colc);
-            d0 = (C)bld(del, zo, colc);
+            e0 = (C)bld<NT>(eps, zo, colc);
+            d0 = (C)bld<NT>(del, zo, colc);
         }
         if constexpr (TTI) {
-            ct = (C)bld(Ct, zo, colc);
-            st = (C)bld(St, zo, colc);
-            cp = (C)bld(Cp, zo, colc);
-            sp = (C)bld(Sp, zo, colc);
+            ct = (C)bld<NT>(Ct, zo, colc);
+            st = (C)bld<NT>(St, zo, colc);
+            cp = (C)bld<NT>(Cp, zo, colc);
+            sp = (C)bld<NT>(Sp, zo, colc);
         }
         C uc, vc;
         if constexpr (TTI) {
@@ -584,19 +597,19 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         // the wave wait for earlier loads into those registers.
         C f0, f1, f2, f3, f4, f5, p0, q0;
         if (valid && split) {
-            f0 = ldw(px, zo, col, sc.s[F_PX]);
-            f1 = ldw(py, zo, col, sc.s[F_PY]);
-            f2 = ldw(pz, zo, col, sc.s[F_PZ]);
+            f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
+            f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
+            f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
             if constexpr (!ISO) {
-                f3 = ldw(qx, zo, col, sc.s[F_QX]);
-                f4 = ldw(qy, zo, col, sc.s[F_QY]);
-                f5 = ldw(qz, zo, col, sc.s[F_QZ]);
+                f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
+                f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
+                f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
             }
         }
         if (valid && !split) {
             // WZ: the old value of F at plane z is fF[0]
-            if constexpr (!(WZ && ISO)) p0 = ldw((const S*)Po, zo, col, sc.s[F_P]);
-            if constexpr (!ISO && !WZ) q0 = ldw((const S*)Qo, zo, col, sc.s[F_Q]);
+            if constexpr (!(WZ && ISO)) p0 = ldw<NT>((const S*)Po, zo, col, sc.s[F_P]);
+            if constexpr (!ISO && !WZ) q0 = ldw<NT>((const S*)Qo, zo, col, sc.s[F_Q]);
         }
         if constexpr (WZ) {
             if constexpr (ISO) p0 = fF[0];
@@ -802,8 +815,8 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
         wY[8] = ldY(zo + 4 * g.pl, col);
-        const C xo = inj<C>(ldw((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
-        const C vv = (C)bld(VV, zo, colc);
+        const C xo = inj<C>(ldw<NT>((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
+        const C vv = (C)bld<NT>(VV, zo, colc);
         Halo<TY, true, true, C> hY;
         load_halo(hY, g, zo, x0, y0, tx, ty, yl, ldY);
         const double fz = a.fz[z];
