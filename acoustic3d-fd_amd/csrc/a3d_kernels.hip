@@ -815,8 +815,10 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
         wY[8] = ldY(zo + 4 * g.pl, col);
-        const C xo = inj<C>(ldw<NT>((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
-        const C vv = (C)bld<NT>(VV, zo, colc);
+        // default cache policy: the 2nd-order grids are small enough to stay in the
+        // last-level cache between steps
+        const C xo = inj<C>(ldw((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
+        const C vv = (C)bld(VV, zo, colc);
         Halo<TY, true, true, C> hY;
         load_halo(hY, g, zo, x0, y0, tx, ty, yl, ldY);
         const double fz = a.fz[z];
@@ -840,7 +842,7 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
             const C prev = damp3<C>(xo, fy, fx, fz);
             const C lap = vv * (c1 + c2 + c3);
             const C nv = (C)(2.0 * (double)cur - (double)prev + (double)lap);
-            stw(X, zo, col, damp3<C>(nv, fy, fx, fz), ip);
+            stw<0>(X, zo, col, damp3<C>(nv, fy, fx, fz), ip);
             if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
         }
     }
