@@ -60,6 +60,7 @@ enum Vol {
     V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ,
     V_VV, V_PA, V_PB,
     V_P2, V_Q2, V_U2, V_V2, V_W2,   // second copies for the one-pass VTI step (scheme 2), allocated on first use
+    V_M4,                           // VTI, float model: (vp, rho, eps, delta) interleaved, built on first use
     NVOL
 };
 
@@ -207,6 +208,7 @@ struct a3d_ctx {
     float* snap_dev = nullptr;
     int* flag = nullptr;
     bool model_set = false;
+    bool m4_ok = false;         // V_M4 holds the current model
     float vel_first = 0.f, eps_first = 0.f, vel_d0 = 0.f, vel_raw1 = 0.f;
     float zmin = 1.0f;          // minimum acoustic impedance rho*v (binary16 velocity-field scale)
     // launch shape per sweep: [0] 2nd-order step / velocity sweep, [1] stress sweep
@@ -518,6 +520,20 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.zi1 = zi1;
     va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
     if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
+    // the stress sweep's interleaved model (VTI, float model), built once per model
+    sa.m4 = nullptr;
+    if (op == A3D_OP_VTI && prec != PREC_FP64) {
+        if (!c->vol[V_M4] && hipMalloc(&c->vol[V_M4], c->nvox * 16) != hipSuccess) {
+            c->vol[V_M4] = nullptr;
+            return fail(A3D_ENOMEM, "interleaved model: allocation failed");
+        }
+        if (!c->m4_ok) {
+            HIP_TRY(launch_pack4((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO], (const float*)c->vol[V_EPS],
+                                 (const float*)c->vol[V_DEL], c->vol[V_M4], c->nvox, c->st));
+            c->m4_ok = true;
+        }
+        sa.m4 = c->vol[V_M4];
+    }
     va.s.s0 = s0;
     // The time grid is the reference's float accumulation t = dt, t += dt and the gate
     // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
@@ -1143,6 +1159,7 @@ int a3d_set_model(a3d_ctx* c, const float* vel, const float* rho, const float* e
     if (rc) return rc;
     model_scalars(c, vel, rho, eps, Window{0, 0, 0, nx, ny, nz});
     c->model_set = true;
+    c->m4_ok = false;
     return A3D_OK;
 }
 
@@ -2041,6 +2058,7 @@ int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const 
         const Window win{b->win[w].ox, b->win[w].oy, b->win[w].oz, full.nx, full.ny, full.nz};
         model_scalars(b->ctx[w], vel, rho, eps, win);
         b->ctx[w]->model_set = true;
+        b->ctx[w]->m4_ok = false;
     }
     b->model_set = true;
     return A3D_OK;

@@ -108,6 +108,7 @@ struct StressArgs {
     Scales sc;
     int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
     Src s;                     // wz: this step's source, added to P / Q as the velocity sweep reads them
+    const void* m4;            // VTI, float model: (vp, rho, eps, delta) interleaved per point, or null
 };
 
 // One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out
@@ -179,6 +180,9 @@ hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const G
 // th -> cos(th) (in th), sin(th) (in st); ph -> cos(ph) (in ph), sin(ph) (in sp),
 // each (M)cos((double)angle) with M float or double (model_double).
 hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int model_double, hipStream_t s);
+// (vp, rho, eps, delta) of a float model interleaved into one float4 per point.
+hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, const float* del, void* out, size_t n,
+                        hipStream_t s);
 
 // Interior crop of a padded wavefield into z-fastest float order ([ny][nx][nz]).
 // 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied

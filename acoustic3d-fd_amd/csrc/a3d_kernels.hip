@@ -120,6 +120,12 @@ __device__ __forceinline__ void bst(T* p, int zo, int o, T v) {
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, o * 8, 0, A);
     }
 }
+// Four floats at element o of the plane at offset zo (16-byte elements).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int A = 0>
+__device__ __forceinline__ f32x4 bld4(const f32x4* p, int zo, int o) {
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(plane_rsrc(p, zo), o * 16, 0, A));
+}
 // The same per plane (element o of the plane at offset zo), and the matching stores
 // (non-temporal by default: no sweep re-reads what it writes).
 template <int A = 0>
@@ -487,6 +493,12 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     C wU[8], wV[8], wW[8];   // backward windows: F(z-4+k), k = 0..7
     // WZ: fF = old F (P for ISO, Q for VTI) at planes z..z+7, fr = density at z..z+3
     C fF[8], fr[4];
+    // VTI with a float model: (vp, rho, eps, delta) as one 16-byte load per point
+    // (one stream instead of four; the host always provides it); WZ keeps the
+    // planes z..z+3 in mq (fr is the density FIFO otherwise)
+    constexpr bool PACKED = VTI && sizeof(M) == 4;
+    const f32x4* m4 = (const f32x4*)a.m4;
+    f32x4 mq[4];
     const S* Fo = ISO ? (const S*)Po : (const S*)Qo;
     const float sF = sc.s[ISO ? F_P : F_Q];
     const double fa0 = ISO ? a.s.p0 : a.s.q0, fa1 = ISO ? a.s.p1 : a.s.q1;
@@ -523,7 +535,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
 #pragma unroll
         for (int j = 0; j < 7; ++j) fF[j] = fw[7 + j];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) fr[j] = (C)bld(rho, (zb + j) * g.pl, colc);
+        for (int j = 0; j < 3; ++j) {
+            if constexpr (PACKED) mq[j] = bld4(m4, (zb + j) * g.pl, colc);
+            else fr[j] = (C)bld(rho, (zb + j) * g.pl, colc);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -546,7 +561,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         if constexpr (WZ) {
             fF[7] = ldw<NT>(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
             wo = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);
-            fr[3] = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
+            if constexpr (PACKED) mq[3] = bld4<NT>(m4, zo + 3 * g.pl, colc);
+            else fr[3] = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
             czw1 = T[a.c.cz1 + z + 3];
             czw2 = T[a.c.cz2 + z + 3];
         } else if constexpr (!TTI) {
@@ -558,13 +574,23 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             wU[7] = ldU(zo + 3 * g.pl, col);
             wV[7] = ldV(zo + 3 * g.pl, col);
         }
-        C r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
-        const C c0 = (C)bld<NT>(vp, zo, colc);
-        if constexpr (WZ) r0 = fr[0];
-        else if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
-        if constexpr (!ISO) {
-            e0 = (C)bld<NT>(eps, zo, colc);
-            d0 = (C)bld<NT>(del, zo, colc);
+        C c0 = 0, r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
+        if constexpr (PACKED) {
+            f32x4 m;
+            if constexpr (WZ) m = mq[0];
+            else m = bld4<NT>(m4, zo, colc);
+            c0 = (C)m.x;
+            r0 = (C)m.y;
+            e0 = (C)m.z;
+            d0 = (C)m.w;
+        } else {
+            c0 = (C)bld<NT>(vp, zo, colc);
+            if constexpr (WZ) r0 = fr[0];
+            else if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
+            if constexpr (!ISO) {
+                e0 = (C)bld<NT>(eps, zo, colc);
+                d0 = (C)bld<NT>(del, zo, colc);
+            }
         }
         if constexpr (TTI) {
             ct = (C)bld<NT>(Ct, zo, colc);
@@ -629,7 +655,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         const bool wupd = WZ && z + 3 < g.nnz - MM;
         C wz3 = wo;
         if constexpr (WZ) {
-            if (wupd) wz3 = wcalc(z + 3, [&](int j) { return fF[j]; }, wo, fr[3], czw1, czw2);
+            const C rw = PACKED ? (C)mq[3].y : fr[3];   // density at plane z+3
+            if (wupd) wz3 = wcalc(z + 3, [&](int j) { return fF[j]; }, wo, rw, czw1, czw2);
             wW[7] = stored(Wv, wz3, sc.inv[F_W], sc.s[F_W]);   // as the stress sweep would read it
         }
         // z differences, then the window shift, before any store: the loop comes
@@ -648,7 +675,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         }
         if constexpr (WZ) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) fr[k] = fr[k + 1];
+            for (int k = 0; k < 3; ++k) {
+                if constexpr (PACKED) mq[k] = mq[k + 1];
+                else fr[k] = fr[k + 1];
+            }
         }
         if (valid && wupd) stw((S*)Wv, zo + 3 * g.pl, col, wz3, sc.inv[F_W]);
         if (valid) {
@@ -935,6 +965,12 @@ __global__ void k_trig(M* __restrict__ th, M* __restrict__ st, M* __restrict__ p
     }
 }
 
+__global__ void k_pack4(const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ eps,
+                        const float* __restrict__ del, f32x4* __restrict__ out, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f32x4{vp[i], rho[i], eps[i], del[i]};
+}
+
 __global__ void k_check_finite(const float* __restrict__ a, size_t n, int* flag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1215,6 +1251,13 @@ hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int mod
         k_trig<double><<<grid, 256, 0, s>>>((double*)th, (double*)st, (double*)ph, (double*)sp, n);
     else
         k_trig<float><<<grid, 256, 0, s>>>((float*)th, (float*)st, (float*)ph, (float*)sp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, const float* del, void* out, size_t n,
+                        hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_pack4, dim3(grid), dim3(256), 0, s, vp, rho, eps, del, (f32x4*)out, n);
     return hipGetLastError();
 }
 

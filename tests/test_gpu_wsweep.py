@@ -91,3 +91,30 @@ def test_w_sweep_not_for_tti_or_one_pass():
         with Modeller(make_desc(op, 20, 20, 20, D, D, D, 10, 10, 3, 10, 1e-3, 25.0, nb=10, scheme=scheme)) as mdl:
             mdl.set_w_sweep(1)
             assert mdl.w_sweep() == 0
+
+
+def test_vti_model_update_reaches_stress_sweep():
+    """The stress sweep reads an interleaved copy of the VTI model built at the first run
+    after a3d_set_model: a second set_model on the same handle must be picked up (the
+    record equals a fresh handle's), with w in either sweep."""
+    from a3dfd import Modeller, make_desc
+    from a3dfd.models import small_model
+    nx, ny, nz = 40, 30, 26
+    m1 = small_model("vti", nx, ny, nz, seed=51)
+    m2 = small_model("vti", nx, ny, nz, seed=52)
+    desc = make_desc("vti", nx, ny, nz, D, D, D, 20, 15, 5, 120, 1e-3, 25.0, nb=10, scheme=1)
+    for wz in (0, 1):
+        with Modeller(desc) as mdl:
+            mdl.set_launch(4, 2, 4, 1)
+            mdl.set_w_sweep(wz)
+            mdl.set_model(**m1)
+            first, _, _ = mdl.run()
+            mdl.set_model(**m2)
+            second, _, _ = mdl.run()
+        with Modeller(desc) as fresh:
+            fresh.set_launch(4, 2, 4, 1)
+            fresh.set_w_sweep(wz)
+            fresh.set_model(**m2)
+            ref, _, _ = fresh.run()
+        assert not np.array_equal(first, second)
+        assert np.array_equal(second, ref), wz
