@@ -18,7 +18,11 @@
 // per plane in LDS with a 4-cell halo on the needed sides only; pointwise
 // fields are read once. Workgroup ids are remapped so each XCD (ids b with
 // equal b % 8) walks a contiguous band of tiles and its L2 serves the halo
-// rows its neighbours share.
+// rows its neighbours share. Per plane, every global load is issued before
+// the workgroup barrier (none inside a branch), through buffer resources with
+// the plane base in scalar registers; single-use streams carry the
+// non-temporal hint. For ISO and VTI the stress sweep can also update w from
+// its own column (the WZ instances), which the velocity sweep then skips.
 //
 // Precision (template PR): Fp32 (wavefields, model and arithmetic in float,
 // written operation for operation as the reference -- float/double mix,
