@@ -8,17 +8,25 @@
 //   plane zc:   P(zc) staged in LDS with a 7-cell halo (x for u, y for v):
 //               u_new(zc) on the tile's rows over x0-4 .. x0+66,
 //               v_new(zc) on the tile's columns over y0-4 .. y0+TY+2;
-//   column:     w_new(zc+3) from the register window Q(zc .. zc+7);
+//   column:     w_new(zc+3) from the old Q at zc .. zc+7 (register window);
 //   stress:     P, Q at zc from u_new, v_new (LDS) and w_new(zc-4 .. zc+3)
 //               (register window), as the scheme-1 stress sweep.
 // The halo velocities are recomputed by every tile that needs them (the same
-// arithmetic, so the same bits) and stored only by their owner. Old and new
-// P, Q, u, v, w live in separate buffers (ping-pong across steps): a tile
-// must read its neighbours' old values after they may have written new ones.
-// Every operation is the scheme-1 two-sweep step's, in the same order: the
-// results are bitwise equal to it.
+// arithmetic, so the same bits) and stored only by their owner: the x-halo
+// columns of u by the first 7*TY threads (one value per lane), the y-halo rows
+// of v by whole waves. Old and new P, Q, u, v, w live in separate buffers
+// (ping-pong across steps): a tile reads its neighbours' old values after they
+// may have written new ones. Every operation is the two-sweep scheme-1 step's,
+// in the same order: the results are bitwise equal to it.
+//
+// Load phase as in the sweeps: every global load of a plane is issued before
+// the workgroup's first barrier, none in a branch (clamped indices), through
+// buffer resources with the plane base in scalar registers; single-use streams
+// carry the non-temporal hint; the model is read as the interleaved
+// (vp, rho, eps, delta) float4 of plane zc+3 and kept 4 planes deep.
 #include <hip/hip_runtime.h>
 
+#include "a3d_device.h"
 #include "a3d_internal.h"
 
 namespace a3d {
@@ -45,206 +53,242 @@ __device__ __forceinline__ int remap(const Launch& l) {
 
 template <int TY>
 __global__ __launch_bounds__(TX* TY) void k_fused_vti(const FusedArgs a) {
-    constexpr int HP = 2 * MM - 1;          // 7: P halo (u at x-4 .. x+3 needs P at x-7 .. x+7)
-    __shared__ float sP[TY + 2 * HP][TX + 2 * HP];
-    __shared__ float sU[TY][TX + 2 * MM - 1];   // u_new at x0-4 .. x0+66
-    __shared__ float sV[TY + 2 * MM - 1][TX];   // v_new at y0-4 .. y0+TY+2
+    constexpr int HP = 2 * MM - 1;                                   // 7: P halo (u at x-4 .. x+3 needs P at x-7 .. x+7)
+    constexpr int NH = 2 * MM - 1;                                   // 7 halo columns of u, rows of v
+    constexpr int NYH = (2 * HP * TX + TX * TY - 1) / (TX * TY);    // y-halo P loads per thread
+    constexpr int NPV = (NH + TY - 1) / TY;                          // halo v rows per thread
+    static_assert(2 * HP * TY <= TX * TY, "one x-halo P value per thread");
+    __shared__ float sP[TY + 2 * HP][TX + 2 * HP];   // row ri: y0-7+ri, column ci: x0-7+ci
+    __shared__ float sU[TY][TX + NH];                // u_new, column cu: x0-4+cu
+    __shared__ float sV[TY + NH][TX];                // v_new, row rv: y0-4+rv
 
     const Launch& L = a.l;
-    const int t = remap(L);
-    if (t >= L.ntiles) return;
-    const int tix = t % L.ntx, r0t = t / L.ntx, tiy = r0t % L.nty, seg = r0t / L.nty;
+    const int tile = remap(L);
+    if (tile >= L.ntiles) return;
+    const int tix = tile % L.ntx, r0t = tile / L.ntx, tiy = r0t % L.nty, seg = r0t / L.nty;
     const Geo g = a.g;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * TX + tx;
     const int x0 = tix * TX, y0 = MM + tiy * TY;
     const int x = x0 + tx, y = y0 + ty;
-    const bool yin = y < g.nny;
     const bool xupd = x >= MM && x < g.nnx - MM;
     const bool valid = y < g.nny - MM && xupd;
-    const int yl = yin ? y : g.nny - 1;
+    const int yl = min(y, g.nny - 1);
     const int col = x + g.ld * yl;
+    const int colc = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(y, g.nny - MM - 1);
     const int zb = L.zbeg + seg * L.zlen;
     const int ze = min(zb + L.zlen, L.zend);
     const float* T = (const float*)a.tab;
     const float dtx = (float)a.dtx, dty = (float)a.dty, dtz = (float)a.dtz;
     const Src S0 = a.s;
+    const f32x4* m4 = (const f32x4*)a.m4;
 
-    // the step's source is added to P and Q on every load (never stored)
-    auto ldP = [&](int i) {
-        float v = a.P0[i];
+    // the step's source is added to P and Q where the velocity updates read them (never stored)
+    auto injP = [&](float v, int i) {
         if (i == S0.s0) v += (float)S0.p0;
         if (i == S0.s1) v += (float)S0.p1;
         return v;
     };
-    auto ldQ = [&](int i) {
-        float v = a.Q0[i];
+    auto injQ = [&](float v, int i) {
         if (i == S0.s0) v += (float)S0.q0;
         if (i == S0.s1) v += (float)S0.q1;
         return v;
     };
-    auto ldQz = [&](int z) { return (z >= 0 && z < g.nnz) ? ldQ(z * g.pl + col) : 0.f; };
+    const bool srccol = (S0.s0 >= 0 && S0.s0 % g.pl == col) || (S0.s1 >= 0 && S0.s1 % g.pl == col);
 
-    // w_new at plane p of this column from wQ = Q(p-3 .. p+4); 0 where the
-    // velocity sweep never writes (outside the updated region)
-    float wQ[8], wW[8];
-    auto w_at = [&](int p) -> float {
-        if (!(valid && p >= MM && p < g.nnz - MM)) return 0.f;
-        const int id = p * g.pl + col;
-        float zz = d8([&](int k) { return wQ[k]; });
-        zz /= a.rho[id];
-        return T[a.c.cz2 + p] * a.w0[id] - T[a.c.cz1 + p] * dtz * zz;
+    // fF = old Q (without the source) at planes zc .. zc+7, wW = new w at zc-4 .. zc+3,
+    // mq = model at zc .. zc+3
+    float fF[8], wW[8];
+    f32x4 mq[4];
+    // new w at plane p (velocity sweep arithmetic) from Fj(j) = old Q at plane p-3+j
+    auto wcalc = [&](int p, auto Fj, float wo, float r, float cz1, float cz2) -> float {
+        float zz = d8([&](int j) {
+            float v = Fj(j);
+            if (srccol) v = injQ(v, (p - 3 + j) * g.pl + col);
+            return v;
+        });
+        zz /= r;
+        return cz2 * wo - cz1 * dtz * zz;
     };
-
+    {
+        // warm-up: new w at zb-4 .. zb+2 (stored by this segment on its own planes only;
+        // planes outside the updated range keep their value)
+        float fw[14];   // old Q at zb-7 .. zb+6
 #pragma unroll
-    for (int k = 0; k < 7; ++k) wQ[k] = ldQz(zb - 7 + k);
-    // warm-up: w_new at zb-4 .. zb+2 (stored by this segment only on its own planes)
+        for (int i = 0; i < 14; ++i) fw[i] = bld(a.Q0, min(max(zb - 7 + i, 0), g.nnz - 1) * g.pl, col);
 #pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const int p = zb - 4 + j;
-        wQ[7] = ldQz(p + 4);
-        const float wn = w_at(p);
-        wW[j] = wn;
-        if (p >= zb && p < ze && valid) a.w1[p * g.pl + col] = wn;
+        for (int k = 0; k < 7; ++k) {
+            const int p = zb - 4 + k;
+            const float wo = bld(a.w0, p * g.pl, col);
+            if (p >= MM && p < g.nnz - MM) {
+                const float r = bld(a.rho, p * g.pl, colc);
+                const float wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, T[a.c.cz1 + p], T[a.c.cz2 + p]);
+                if (valid && p >= zb && p < ze) bst<NT>(a.w1, p * g.pl, col, wn);
+                wW[k] = wn;
+            } else {
+                wW[k] = wo;
+            }
+        }
 #pragma unroll
-        for (int k = 0; k < 7; ++k) wQ[k] = wQ[k + 1];
+        for (int j = 0; j < 7; ++j) fF[j] = fw[7 + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) mq[j] = bld4(m4, (zb + j) * g.pl, colc);
     }
 
+    // per-thread constants: PML factors of the own point, the halo u column, the halo v rows
     const int xs = x < g.nnx ? x : 0;
     const float ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
     const float ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
+    const float cx1 = T[a.c.cx1 + xs], cx2 = T[a.c.cx2 + xs];
+    const float cy1 = T[a.c.cy1 + yl], cy2 = T[a.c.cy2 + yl];
     const bool recx = a.rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
-
-    constexpr int NYH = (2 * HP * TX + TX * TY - 1) / (TX * TY);   // y-halo P loads per thread
-    constexpr int NV = (TY + 2 * MM - 1 + TY - 1) / TY;           // v rows per thread
-    const bool xh = tx < 2 * HP;                                  // loads an x-halo P column
-    const int xxh = tx < HP ? x0 - HP + tx : x0 + TX + (tx - HP);
-    const int sch = tx < HP ? tx : TX + tx;
-    const bool uh = tx < 2 * MM - 1;                              // computes an x-halo u column
-    const int xcu = tx < MM ? x0 - MM + tx : x0 + TX + (tx - MM);
-    const int scu = tx < MM ? tx : TX + tx;
-    const bool uh_upd = uh && y < g.nny - MM && xcu >= MM && xcu < g.nnx - MM;
-    const bool u_upd = y < g.nny - MM && xupd;
+    // P x-halo value of this thread: row pxr, column pxj of the 14 halo columns
+    const bool pxh = t < 2 * HP * TY;
+    const int pxr = pxh ? t / (2 * HP) : ty, pxj = t % (2 * HP);
+    const int pxc = pxj < HP ? pxj : TX + pxj;   // sP column
+    const int pxo = min(max(x0 - HP + pxc, 0), g.nnx - 1) + g.ld * min(y0 + pxr, g.nny - 1);
+    // halo u of this thread: row hr, column hj of x0-4 .. x0-1, x0+64 .. x0+66
+    const bool uh = t < NH * TY;
+    const int hr = uh ? t / NH : 0, hj = t % NH;
+    const int xcu = hj < MM ? x0 - MM + hj : x0 + TX + (hj - MM);
+    const int cu = hj < MM ? hj : TX + hj;       // sU column
+    const bool uh_upd = uh && y0 + hr < g.nny - MM && xcu >= MM && xcu < g.nnx - MM;
+    const int xcs = min(max(xcu, 0), g.nnx - 1);
+    const int uho = min(max(xcu, MM), g.nnx - MM - 1) + g.ld * min(y0 + hr, g.nny - MM - 1);
+    const float hcx1 = T[a.c.cx1 + xcs], hcx2 = T[a.c.cx2 + xcs];
+    // halo v rows of this thread: hv = ty + j*TY < 7
+    int vho[NPV], vrv[NPV];
+    bool vh_upd[NPV];
+    float hcy1[NPV], hcy2[NPV];
+#pragma unroll
+    for (int j = 0; j < NPV; ++j) {
+        const int hv = ty + j * TY;
+        const int yv = hv < MM ? y0 - MM + hv : y0 + TY + (hv - MM);
+        vrv[j] = hv < MM ? hv : TY + hv;
+        vh_upd[j] = hv < NH && xupd && yv >= MM && yv < g.nny - MM;
+        vho[j] = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(max(yv, MM), g.nny - MM - 1);
+        const int yvs = min(max(yv, 0), g.nny - 1);
+        hcy1[j] = T[a.c.cy1 + yvs];
+        hcy2[j] = T[a.c.cy2 + yvs];
+    }
 
     for (int zc = zb; zc < ze; ++zc) {
         const int zo = zc * g.pl;
-        const int id = zo + col;
-        // ---- every global load of this plane first (one exposed latency per plane)
-        const int pw = zc + 3;
-        const bool w_upd = valid && pw < g.nnz - MM;
-        const float q7 = ldQz(pw + 4);
-        const float w0p = w_upd ? a.w0[pw * g.pl + col] : 0.f;
-        const float rhop = w_upd ? a.rho[pw * g.pl + col] : 1.f;
-        const float pc = (yin && x < g.nnx) ? ldP(zo + yl * g.ld + x) : 0.f;
-        const float phx = (xh && yin && xxh >= 0 && xxh < g.nnx) ? ldP(zo + yl * g.ld + xxh) : 0.f;
-        float phy[NYH];
-#pragma unroll
-        for (int j = 0; j < NYH; ++j) {
-            const int k = ty * TX + tx + j * TX * TY;
-            const int r = k >> 6, c = k & 63;
-            const int yy = r < HP ? y0 - HP + r : y0 + TY + (r - HP);
-            phy[j] = (k < 2 * HP * TX && yy >= 0 && yy < g.nny && x0 + c < g.nnx) ? ldP(zo + yy * g.ld + x0 + c) : 0.f;
-        }
-        const float u0c = u_upd ? a.u0[id] : 0.f;
-        const float rhoc = u_upd ? a.rho[id] : 1.f;
-        const int iuh = zo + yl * g.ld + xcu;
-        const float u0h = uh_upd ? a.u0[iuh] : 0.f;
-        const float rhoh = uh_upd ? a.rho[iuh] : 1.f;
-        float v0r[NV], rhov[NV];
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const int rr = ty + j * TY;
-            const int yv = y0 - MM + rr;
-            const bool ok = rr < TY + 2 * MM - 1 && xupd && yv >= MM && yv < g.nny - MM;
-            v0r[j] = ok ? a.v0[zo + yv * g.ld + x] : 0.f;
-            rhov[j] = ok ? a.rho[zo + yv * g.ld + x] : 1.f;
-        }
+        // ---- every global load of this plane first
+        fF[7] = bld<NT>(a.Q0, min(zc + 7, g.nnz - 1) * g.pl, col);
+        const float wo = bld<NT>(a.w0, zo + 3 * g.pl, col);
+        mq[3] = bld4<NT>(m4, zo + 3 * g.pl, colc);
+        const float czw1 = T[a.c.cz1 + zc + 3], czw2 = T[a.c.cz2 + zc + 3];
         const float az1 = T[a.c.az1 + zc], az2 = T[a.c.az2 + zc];
-        const bool split = !(ax1 == 1.0f && ay1 == 1.0f && az1 == 1.0f);
-        float c0 = 0.f, e0 = 0.f, d0 = 0.f, P0c = 0.f, Q0c = 0.f;
-        float px0 = 0.f, py0 = 0.f, pz0 = 0.f, qx0 = 0.f, qy0 = 0.f, qz0 = 0.f;
-        if (valid) {
-            c0 = a.vp[id];
-            e0 = a.eps[id];
-            d0 = a.del[id];
-            if (split) {
-                px0 = a.px[id];
-                py0 = a.py[id];
-                pz0 = a.pz[id];
-                qx0 = a.qx[id];
-                qy0 = a.qy[id];
-                qz0 = a.qz[id];
-            } else {
-                P0c = a.P0[id];
-                Q0c = a.Q0[id];
-            }
+        const bool split = !(ax1 == 1.0f && ay1 == 1.0f && zc >= a.zi0 && zc < a.zi1);
+        float f0, f1, f2, f3, f4, f5;   // split fields, loaded in PML cells only
+        if (valid && split) {
+            f0 = bld<NT>(a.px, zo, col);
+            f1 = bld<NT>(a.py, zo, col);
+            f2 = bld<NT>(a.pz, zo, col);
+            f3 = bld<NT>(a.qx, zo, col);
+            f4 = bld<NT>(a.qy, zo, col);
+            f5 = bld<NT>(a.qz, zo, col);
         }
-        // ---- column: w_new(zc+3) from the Q window
-        {
-            wQ[7] = q7;
-            float wn = 0.f;
-            if (w_upd) {
-                float zz = d8([&](int k) { return wQ[k]; });
-                zz /= rhop;
-                wn = T[a.c.cz2 + pw] * w0p - T[a.c.cz1 + pw] * dtz * zz;
-                if (pw < ze) a.w1[pw * g.pl + col] = wn;
-            }
-            wW[7] = wn;
+        const float u0c = bld<NT>(a.u0, zo, colc);
+        const float v0c = bld<NT>(a.v0, zo, colc);
+        const float u0h = bld(a.u0, zo, uho);
+        const float rhoh = bld(a.rho, zo, uho);
+        float v0h[NPV], rhov[NPV];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) wQ[k] = wQ[k + 1];
+        for (int j = 0; j < NPV; ++j) {
+            v0h[j] = bld(a.v0, zo, vho[j]);
+            rhov[j] = bld(a.rho, zo, vho[j]);
         }
-        // ---- P(zc) with halos into LDS
-        __syncthreads();
-        sP[ty + HP][tx + HP] = pc;
-        if (xh) sP[ty + HP][sch] = phx;
+        const float pc = bld(a.P0, zo, col);   // old P of the own point (without the source)
+        const float phx = bld(a.P0, zo, pxo);
+        float phy[NYH];
+        int phyo[NYH];
 #pragma unroll
         for (int j = 0; j < NYH; ++j) {
-            const int k = ty * TX + tx + j * TX * TY;
-            if (k < 2 * HP * TX) {
-                const int r = k >> 6, c = k & 63;
-                sP[r < HP ? r : TY + r][c + HP] = phy[j];
-            }
+            const int k = min(t + j * TX * TY, 2 * HP * TX - 1);
+            const int rr = k >> 6, cc = k & 63;
+            const int yy = rr < HP ? y0 - HP + rr : y0 + TY + (rr - HP);
+            phyo[j] = (x0 + cc) + g.ld * min(max(yy, 0), g.nny - 1);
+            phy[j] = bld(a.P0, zo, phyo[j]);
+        }
+        // keep the halo loads ahead of the barrier: a load whose only use sits in a
+        // branch below would otherwise be moved into it and waited for there
+#pragma unroll
+        for (int j = 0; j < NPV; ++j) asm volatile("" ::"v"(v0h[j]), "v"(rhov[j]));
+        asm volatile("" ::"v"(u0h), "v"(rhoh));
+        // ---- P(zc) with the step's source, and its halos, into LDS. Threads past the
+        // halo counts write a halo cell another thread also writes, with the same value
+        // (their indices were clamped onto it), so the stores need no branch.
+        sP[ty + HP][tx + HP] = injP(pc, zo + col);
+        sP[pxr + HP][pxc] = injP(phx, zo + pxo);
+#pragma unroll
+        for (int j = 0; j < NYH; ++j) {
+            const int k = min(t + j * TX * TY, 2 * HP * TX - 1);
+            const int rr = k >> 6, cc = k & 63;
+            sP[rr < HP ? rr : TY + rr][cc + HP] = injP(phy[j], zo + phyo[j]);
         }
         __syncthreads();
-        // ---- u_new(zc) on the tile's rows (x0-4 .. x0+66), v_new(zc) on its columns (y0-4 .. y0+TY+2)
+        // ---- u_new(zc) on the tile's rows and the 7 halo columns, v_new(zc) on its
+        // columns and the 7 halo rows (0 where the velocity sweep never writes)
+        // (the plane's global stores all come after the stress update: a store here
+        // would make later register reuse wait for it)
+        const float r0 = mq[0].y;
+        float un = 0.f, vn = 0.f;
         {
-            float un = 0.f;
-            if (u_upd) {
+            if (valid) {
                 float xx = d8([&](int k) { return sP[ty + HP][tx + HP - 3 + k]; });
-                xx /= rhoc;
-                un = T[a.c.cx2 + x] * u0c - T[a.c.cx1 + x] * dtx * xx;
-                if (valid) a.u1[id] = un;
+                xx /= r0;
+                un = cx2 * u0c - cx1 * dtx * xx;
             }
             sU[ty][tx + MM] = un;
             if (uh) {
                 float uhn = 0.f;
                 if (uh_upd) {
                     const int spc = xcu - (x0 - HP);
-                    float xx = d8([&](int k) { return sP[ty + HP][spc - 3 + k]; });
+                    float xx = d8([&](int k) { return sP[hr + HP][spc - 3 + k]; });
                     xx /= rhoh;
-                    uhn = T[a.c.cx2 + xcu] * u0h - T[a.c.cx1 + xcu] * dtx * xx;
+                    uhn = hcx2 * u0h - hcx1 * dtx * xx;
                 }
-                sU[ty][scu] = uhn;
+                sU[hr][cu] = uhn;
             }
+            if (valid) {
+                float yy = d8([&](int k) { return sP[ty + HP - 3 + k][tx + HP]; });
+                yy /= r0;
+                vn = cy2 * v0c - cy1 * dty * yy;
+            }
+            sV[ty + MM][tx] = vn;
+#pragma unroll
+            for (int j = 0; j < NPV; ++j) {
+                if (ty + j * TY < NH) {
+                    float vhn = 0.f;
+                    if (vh_upd[j]) {
+                        const int spr = vrv[j] + (HP - MM);   // sP row of the halo v row
+                        float yy = d8([&](int k) { return sP[spr - 3 + k][tx + HP]; });
+                        yy /= rhov[j];
+                        vhn = hcy2[j] * v0h[j] - hcy1[j] * dty * yy;
+                    }
+                    sV[vrv[j]][tx] = vhn;
+                }
+            }
+        }
+        // ---- column: w_new(zc+3) completes the backward window
+        const bool wupd = valid && zc + 3 < g.nnz - MM;
+        const float wz3 = wupd ? wcalc(zc + 3, [&](int j) { return fF[j]; }, wo, mq[3].y, czw1, czw2) : wo;
+        wW[7] = wz3;
+        const float Wz = d8([&](int k) { return wW[k]; });
+        const float q0 = fF[0];
+        const f32x4 m = mq[0];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            wW[k] = wW[k + 1];
+            fF[k] = fF[k + 1];
         }
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const int rr = ty + j * TY;
-            if (rr < TY + 2 * MM - 1) {
-                const int yv = y0 - MM + rr;
-                float vn = 0.f;
-                if (xupd && yv >= MM && yv < g.nny - MM) {
-                    float yy = d8([&](int k) { return sP[rr + k][tx + HP]; });
-                    yy /= rhov[j];
-                    vn = T[a.c.cy2 + yv] * v0r[j] - T[a.c.cy1 + yv] * dty * yy;
-                    if (rr >= MM && rr < TY + MM) a.v1[zo + yv * g.ld + x] = vn;
-                }
-                sV[rr][tx] = vn;
-            }
-        }
+        for (int k = 0; k < 3; ++k) mq[k] = mq[k + 1];
         __syncthreads();
         // ---- stress at (x, y, zc): the scheme-1 update (split fields in PML cells only)
         if (valid) {
-            const float r0 = rhoc;
+            const float c0 = m.x, e0 = m.z, d0 = m.w;
             float ee = e0, dd = d0;
             if (a.sv) {
                 const int rx = x - a.sxp, ry = y - a.syp, rz = zc - a.szp;
@@ -262,37 +306,38 @@ __global__ __launch_bounds__(TX* TY) void k_fused_vti(const FusedArgs a) {
             }
             const float yy = d8([&](int k) { return sV[ty + k][tx]; });
             const float xx = d8([&](int k) { return sU[ty][tx + k]; });
-            const float zz = d8([&](int k) { return wW[k]; });
+            const float zz = Wz;
             const float e2 = 1 + 2 * ee;
             const float s2 = sqrtf(1 + 2 * dd);
             float Pn, Qn;
             if (split) {
-                const float npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
-                const float npy = ay2 * py0 - ay1 * r0 * c0 * c0 * e2 * dty * yy;
-                const float npz = az2 * pz0 - az1 * r0 * c0 * c0 * s2 * dtz * zz;
-                const float nqx = ax2 * qx0 - ax1 * r0 * c0 * c0 * s2 * dtx * xx;
-                const float nqy = ay2 * qy0 - ay1 * r0 * c0 * c0 * s2 * dty * yy;
-                const float nqz = az2 * qz0 - az1 * r0 * c0 * c0 * dtz * zz;
-                a.px[id] = npx;
-                a.py[id] = npy;
-                a.pz[id] = npz;
-                a.qx[id] = nqx;
-                a.qy[id] = nqy;
-                a.qz[id] = nqz;
+                const float npx = ax2 * f0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
+                const float npy = ay2 * f1 - ay1 * r0 * c0 * c0 * e2 * dty * yy;
+                const float npz = az2 * f2 - az1 * r0 * c0 * c0 * s2 * dtz * zz;
+                const float nqx = ax2 * f3 - ax1 * r0 * c0 * c0 * s2 * dtx * xx;
+                const float nqy = ay2 * f4 - ay1 * r0 * c0 * c0 * s2 * dty * yy;
+                const float nqz = az2 * f5 - az1 * r0 * c0 * c0 * dtz * zz;
+                bst<NT>(a.px, zo, col, npx);
+                bst<NT>(a.py, zo, col, npy);
+                bst<NT>(a.pz, zo, col, npz);
+                bst<NT>(a.qx, zo, col, nqx);
+                bst<NT>(a.qy, zo, col, nqy);
+                bst<NT>(a.qz, zo, col, nqz);
                 Pn = npx + npy + npz;
                 Qn = nqx + nqy + nqz;
             } else {
                 const float K = r0 * c0 * c0;
                 const float hxy = dtx * xx + dty * yy, hz = dtz * zz;
-                Pn = P0c - K * (e2 * hxy + s2 * hz);
-                Qn = Q0c - K * (s2 * hxy + hz);
+                Pn = pc - K * (e2 * hxy + s2 * hz);
+                Qn = q0 - K * (s2 * hxy + hz);
             }
-            a.P1[id] = Pn;
-            a.Q1[id] = Qn;
+            bst<NT>(a.P1, zo, col, Pn);
+            bst<NT>(a.Q1, zo, col, Qn);
+            bst<NT>(a.u1, zo, col, un);
+            bst<NT>(a.v1, zo, col, vn);
             if (recx && zc == a.zrec) a.rec[(y - a.nb) * a.nx + (x - a.nb)] = Pn;
         }
-#pragma unroll
-        for (int k = 0; k < 7; ++k) wW[k] = wW[k + 1];
+        if (wupd && zc + 3 < ze) bst<NT>(a.w1, zo + 3 * g.pl, col, wz3);
     }
 }
 

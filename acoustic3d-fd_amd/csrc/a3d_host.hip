@@ -520,10 +520,10 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.zi1 = zi1;
     va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
     if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
-    // the stress sweep's interleaved model (VTI, float model; the one-pass step reads the
-    // separate volumes), built once per model
+    // the stress sweep's (and the one-pass step's) interleaved model (VTI, float model),
+    // built once per model
     sa.m4 = nullptr;
-    if (op == A3D_OP_VTI && prec != PREC_FP64 && d->scheme != 2) {
+    if (op == A3D_OP_VTI && prec != PREC_FP64) {
         if (!c->vol[V_M4] && hipMalloc(&c->vol[V_M4], c->nvox * 16) != hipSuccess) {
             c->vol[V_M4] = nullptr;
             return fail(A3D_ENOMEM, "interleaved model: allocation failed");
@@ -723,6 +723,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             fa.nb = sa.nb;
             fa.nx = sa.nx;
             fa.ny = sa.ny;
+            fa.m4 = sa.m4;
+            fa.zi0 = sa.zi0;
+            fa.zi1 = sa.zi1;
             float* cur[5] = {(float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_U], (float*)c->vol[V_V],
                              (float*)c->vol[V_W]};
             float* nxt[5] = {(float*)c->vol[V_P2], (float*)c->vol[V_Q2], (float*)c->vol[V_U2], (float*)c->vol[V_V2],

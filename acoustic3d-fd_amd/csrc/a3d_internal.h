@@ -128,6 +128,8 @@ struct FusedArgs {
     const double* taper;
     float* rec;
     int zrec, nb, nx, ny;
+    const void* m4;                        // (vp, rho, eps, delta) interleaved per point
+    int zi0, zi1;                          // planes whose z stress factor is exactly 1
 };
 
 struct Step2Args {
