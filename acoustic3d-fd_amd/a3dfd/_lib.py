@@ -228,8 +228,9 @@ class Modeller:
             snaps = snaps[: min(ns.value, nsnap_max)]
         return out, snaps, ms.value
 
-    def set_kernel_timing(self, on=True):
-        _check(self.lib, self.lib.a3d_set_kernel_timing(self._h, int(on)))
+    def set_kernel_timing(self, on=True, stride=1):
+        """Time the hot kernels with HIP events; stride N > 1 times every N-th time step."""
+        _check(self.lib, self.lib.a3d_set_kernel_timing(self._h, int(stride) if on else 0))
 
     def kernel_times(self, reset=True):
         ms = (c_double * 2)()

@@ -178,6 +178,8 @@ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 struct Timing {
     bool on = false;
+    int stride = 1;       // time steps sampled: every stride-th one
+    bool sample = true;   // the current time step is sampled
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<std::pair<int, size_t>> marks;   // (kernel index, first event)
@@ -340,8 +342,10 @@ hipEvent_t timing_event(Timing& t) {
     return t.pool[t.used++];
 }
 
+void timing_step(a3d_ctx* c, int it) { c->tm.sample = c->tm.stride <= 1 || it % c->tm.stride == 0; }
+
 void timing_begin(a3d_ctx* c, int k) {
-    if (!c->tm.on) return;
+    if (!c->tm.on || !c->tm.sample) return;
     hipEvent_t e = timing_event(c->tm);
     if (!e) return;
     c->tm.marks.push_back({k, c->tm.used - 1});
@@ -349,7 +353,7 @@ void timing_begin(a3d_ctx* c, int k) {
 }
 
 void timing_end(a3d_ctx* c) {
-    if (!c->tm.on || c->tm.marks.empty()) return;
+    if (!c->tm.on || !c->tm.sample || c->tm.marks.empty()) return;
     hipEvent_t e = timing_event(c->tm);
     if (e) (void)hipEventRecord(e, c->st);
 }
@@ -669,6 +673,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             a.xa0 = it > 0 ? (double)wlt[it - 1] : 0.0;
             a.xa1 = it > 0 ? -(double)wlt[it - 1] : 0.0;
             a.rec = c->rec + plane_rec * it;
+            timing_step(c, it);
             timing_begin(c, 0);
             HIP_TRY(launch_step2(prec, c->ty[0], a, c->st));
             timing_end(c);
@@ -744,6 +749,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                 fa.w1 = nxt[4];
                 fa.s = srcs[it];
                 fa.rec = c->rec + plane_rec * it;
+                timing_step(c, it);
                 timing_begin(c, 1);
                 HIP_TRY(launch_fused_vti(c->ty[1], fa, c->st));
                 timing_end(c);
@@ -773,6 +779,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             va.s = srcs[it];
             sa.s = srcs[it];
             sa.rec = c->rec + plane_rec * it;
+            timing_step(c, it);
             if (nch > 0) {
                 for (int k = 0; k <= nch; ++k) {
                     if (k < nch) {
@@ -1182,7 +1189,10 @@ int a3d_run(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, float* snaps_ou
 
 int a3d_set_kernel_timing(a3d_ctx* c, int enable) {
     if (!c) return fail(A3D_EINVAL, "null context");
+    if (enable < 0) return fail(A3D_EINVAL, "kernel timing: negative stride %d", enable);
     c->tm.on = enable != 0;
+    c->tm.stride = enable > 1 ? enable : 1;
+    c->tm.sample = true;
     return A3D_OK;
 }
 

@@ -67,6 +67,11 @@ def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
     return a_vel, f * unsplit + (1 - f) * split
 
 
+def sampled_steps(nt, stride):
+    """Time steps of one shot whose kernels a3d_set_kernel_timing(ctx, stride) times."""
+    return (nt + stride - 1) // stride if stride > 1 else nt
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -240,7 +245,7 @@ def operators(args):
         with Modeller(d) as mdl:
             mdl.set_model(**{k: c[k] for k in keys})
             mdl.run(shot=False)                     # warm-up (and autotune)
-            mdl.set_kernel_timing(True)
+            mdl.set_kernel_timing(True, stride=8)
             mdl.kernel_times(reset=True)
             _, _, ms = mdl.run(shot=False)
             kms, kn = mdl.kernel_times(reset=True)
@@ -279,6 +284,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-stride", type=int, default=8,
+                    help="HIP events around the kernels of every N-th time step of the timed shots")
     ap.add_argument("--launch", type=str, default="", help="ty_vel,zseg_vel,ty_stress,zseg_stress (default: autotuned)")
     ap.add_argument("--w-sweep", type=int, default=None, choices=(-1, 0, 1),
                     help="sweep that updates w: 0 velocity, 1 stress, -1 autotuned (default)")
@@ -347,7 +354,7 @@ def main():
         log(f"bench: warmup shot {w}: {ms:.1f} ms loop, launch {mdl.launch_info()}")
     launch = mdl.launch_info()
 
-    mdl.set_kernel_timing(True)
+    mdl.set_kernel_timing(True, stride=args.time_stride)
     mdl.kernel_times(reset=True)
     loop_ms = []
     t_start = time.perf_counter()
@@ -361,9 +368,9 @@ def main():
 
     total_loop_s = sum(loop_ms) / 1e3
     gpts = npts * args.nt * args.steps / total_loop_s / 1e9
-    steps_run = args.steps * args.nt
-    avg_vel = kms[0] / steps_run     # per time step (a chunked step launches each sweep once per chunk)
-    avg_str = kms[1] / steps_run
+    steps_timed = args.steps * sampled_steps(args.nt, args.time_stride)
+    avg_vel = kms[0] / steps_timed   # per time step (a chunked step launches each sweep once per chunk)
+    avg_str = kms[1] / steps_timed
     main_kernel = "k_fused_vti" if args.scheme == 2 else "k_stress<VTI>"
     wz = launch["w_sweep"]
     a_vel, a_str = algorithmic_bytes(cfg, args.storage, args.scheme, npts, wz=wz)
@@ -409,7 +416,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": main_kernel, "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_point": round(a_str, 2),
-                     "avg_launch_ms": round(avg_str, 4), "launches": kn[1], "zchunk": args.zchunk},
+                     "avg_launch_ms": round(avg_str, 4), "launches": kn[1], "zchunk": args.zchunk,
+                     "timed_every": args.time_stride},
         "kernels": {"k_vel<VTI>": {"avg_launch_ms": round(avg_vel, 4), "achieved_GBs": round(ach_vel, 1),
                                    "bytes_per_point": a_vel, "launches": kn[0]},
                     "whole_step": {"achieved_GBs": round(step_gbs, 1),

@@ -156,11 +156,14 @@ int a3d_set_model(a3d_ctx* ctx, const float* vel, const float* rho, const float*
 int a3d_run(a3d_ctx* ctx, const a3d_shot_desc* desc, float* shot_out, float* snaps_out,
             int nsnap_max, int* nsnap_taken, double* loop_ms);
 
-/* Per-kernel timing: when enabled, every launch of the two hot kernels is
- * bracketed by HIP events on the context stream. a3d_kernel_times returns
- * the summed milliseconds and launch counts since the last reset (reset=1
- * clears after reading). Index 0: the 2nd-order step or 1st-order
- * velocity sweep; index 1: the 1st-order stress sweep. */
+/* Per-kernel timing: when enabled, the launches of the two hot kernels are
+ * bracketed by HIP events on the context stream. enable = 0 turns it off,
+ * 1 times every time step, N > 1 times every N-th time step (it % N == 0,
+ * fewer events in the stream). a3d_kernel_times returns the summed
+ * milliseconds and launch counts of the timed launches since the last reset
+ * (reset=1 clears after reading). Index 0: the 2nd-order step or 1st-order
+ * velocity sweep; index 1: the 1st-order stress sweep (or the one-pass
+ * step). A negative enable is A3D_EINVAL. */
 int a3d_set_kernel_timing(a3d_ctx* ctx, int enable);
 int a3d_kernel_times(a3d_ctx* ctx, double* ms, long long* launches, int reset);
 

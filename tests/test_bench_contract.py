@@ -40,3 +40,11 @@ def test_algorithmic_bytes():
     assert a_vel == 24 and abs(a_str - (f * 48 + (1 - f) * 92)) < 1e-9
     assert b.algorithmic_bytes(cfg, 0, 0, npts, wz=1) == (24, 92.0)
     assert b.OP_BYTES_WZ["vti"] == (24, 92, 48) and b.OP_BYTES_WZ["iso"] == (24, 56, 32)
+
+
+def test_sampled_steps():
+    """Time steps timed per shot at a3d_set_kernel_timing(ctx, N): it % N == 0 for it < nt."""
+    b = _bench()
+    for nt in (1, 7, 8, 9, 3000):
+        for stride in (1, 2, 8, 40):
+            assert b.sampled_steps(nt, stride) == sum(1 for it in range(nt) if it % stride == 0)

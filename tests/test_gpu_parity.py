@@ -217,3 +217,37 @@ def test_z_chunks_bitwise(op, zchunk):
     (a, sa, _), (b, sb, _) = outs
     assert np.abs(a).max() > 0
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+@pytest.mark.parametrize("scheme,zchunk", [(1, 0), (1, 9), (2, 0)])
+def test_kernel_timing_stride(scheme, zchunk):
+    """a3d_set_kernel_timing(ctx, N) times the kernels of every N-th time step: the launch
+    counts follow (z chunks launch each sweep once per chunk), timing leaves the record
+    unchanged, and a negative stride is refused."""
+    from a3dfd import A3DError, Modeller, make_desc
+    from a3dfd.models import small_model
+    nx, ny, nz, nt = 30, 26, 28, 21
+    m = small_model("vti", nx, ny, nz, seed=61)
+    desc = make_desc("vti", nx, ny, nz, 10.0, 10.0, 10.0, 15, 13, 5, nt, 1e-3, 25.0, nb=10, scheme=scheme)
+    print(f"\n[gpu-timing] scheme={scheme} zchunk={zchunk}", flush=True)
+    with Modeller(desc) as mdl:
+        mdl.set_model(**m)
+        mdl.set_launch(4, 1, 4, 1)
+        if zchunk:
+            mdl.set_zchunk(zchunk)
+        ref, _, _ = mdl.run()
+        nch = mdl.grid_info()["nnz"] - 8
+        nch = -(-nch // zchunk) if zchunk else 1
+        for stride, steps in ((1, nt), (4, 6), (8, 3), (40, 1)):
+            mdl.set_kernel_timing(True, stride=stride)
+            mdl.kernel_times(reset=True)
+            out, _, _ = mdl.run()
+            ms, n = mdl.kernel_times(reset=True)
+            assert np.array_equal(out, ref)
+            assert n[1] == steps * nch and ms[1] > 0, (stride, n)
+            assert n[0] == (0 if scheme == 2 else steps * nch), (stride, n)
+        mdl.set_kernel_timing(False)
+        mdl.run()
+        assert mdl.kernel_times(reset=True)[1][1] == 0
+        with pytest.raises(A3DError):
+            mdl.set_kernel_timing(True, stride=-2)
