@@ -262,7 +262,7 @@ class Modeller:
         return dict(ty_vel=v[0], zseg_vel=v[1], blocks_vel=v[2], ty_stress=v[3], zseg_stress=v[4],
                     blocks_stress=v[5], w_sweep=self.w_sweep())
 
-    def autotune(self, desc=None, steps=4):
+    def autotune(self, desc=None, steps=10):
         best = (c_double * 2)()
         _check(self.lib, self.lib.a3d_autotune(self._h, ctypes.byref(desc or self.desc), steps, best))
         return [best[0], best[1]]

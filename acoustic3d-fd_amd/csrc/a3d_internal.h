@@ -103,7 +103,8 @@ struct StressArgs {
     const double* taper;       // taper factor by r^2, valid for 16 < r^2 <= 225
     float* rec;                // receiver plane of this step ([ny][nx]) or nullptr
     int zrec, nb, nx, ny;
-    int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1
+    int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1;
+                               // 2: also split fields only for the damped directions of PML cells
     int zi0, zi1;              // planes whose z stress factor is exactly 1: zi0 <= z < zi1
     Scales sc;
     int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
@@ -159,8 +160,9 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 // lb: the combined decomposition (ntiles = base[nwin], grid = 8*ceil(ntiles/8)).
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
                             const Src* srcs, int it, hipStream_t st);
+// dirs: the windows' stress args have unsplit == 2 (all windows of a batch share it)
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, hipStream_t st);
+                               const Launch& lb, int it, bool dirs, hipStream_t st);
 
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.

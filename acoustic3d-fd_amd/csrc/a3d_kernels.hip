@@ -397,7 +397,7 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
 // P (ISO) or Q (VTI) -- the velocity sweep's arithmetic, bit for bit -- one plane
 // ahead of where the backward window needs it. The column reads F = P or Q at
 // planes z..z+7 before writing plane z, so no other tile's values are involved.
-template <class PR, int OP, int TY, bool WZ = false>
+template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false>
 __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using SP = typename PR::SP;
@@ -514,6 +514,11 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const C ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
     const C ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
     const bool recx = rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
+    // Directional split (DIRS, unsplit == 2): a PML cell keeps the split fields of its
+    // damped directions only (factor != 1); P minus those fields (Q likewise) is the
+    // undamped remainder, which takes the undamped directions' updates without damping.
+    constexpr bool dirs = DIRS;
+    const bool dX = ax1 != (C)1, dY = ay1 != (C)1;
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
@@ -577,22 +582,34 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
         // updates in exact arithmetic; the split fields are read and written only in
         // PML cells. f0..f5: the split fields px..qz, or P (and Q) where unsplit.
-        const bool split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && z >= a.zi0 && z < a.zi1);
-        // Two one-sided branches with distinct destinations, left unset where not
-        // loaded: an if/else writing the same registers, or zeroing them first, makes
-        // the wave wait for earlier loads into those registers.
+        // split: the split update; rem: the remainder needs the old P (Q); sx, sy, sz: the
+        // split fields kept
+        bool split, dZ = false, rem = false, sx = true, sy = true, sz = true;
+        if constexpr (DIRS) {
+            dZ = !(z >= a.zi0 && z < a.zi1);
+            split = dX || dY || dZ;
+            rem = !(dX && dY && dZ);
+            sx = dX;
+            sy = dY;
+            sz = dZ;
+        } else {
+            split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && z >= a.zi0 && z < a.zi1);
+        }
+        // One-sided branches with distinct destinations, left unset where not loaded:
+        // an if/else writing the same registers, or zeroing them first, makes the wave
+        // wait for earlier loads into those registers.
         C f0, f1, f2, f3, f4, f5, p0, q0;
         if (valid && split) {
-            f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
-            f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
-            f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
+            if (sx) f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
+            if (sy) f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
+            if (sz) f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
             if constexpr (!ISO) {
-                f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
-                f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
-                f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
+                if (sx) f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
+                if (sy) f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
+                if (sz) f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
             }
         }
-        if (valid && !split) {
+        if (valid && (!split || rem)) {
             // WZ: the old value of F at plane z is fF[0]
             if constexpr (!(WZ && ISO)) p0 = ldw<NT>((const S*)Po, zo, col, sc.s[F_P]);
             if constexpr (!ISO && !WZ) q0 = ldw<NT>((const S*)Qo, zo, col, sc.s[F_Q]);
@@ -649,14 +666,22 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 const C xx = dst8<C>([&](int k) { return sU[r][c - 4 + k]; });
                 const C zz = Wz;
                 if (split) {
-                    const C px0 = f0, py0 = f1, pz0 = f2;
+                    C px0 = f0, py0 = f1, pz0 = f2, rP = 0;
+                    if (dirs) {
+                        // undamped directions: no split field (factors 1, old value 0)
+                        px0 = dX ? f0 : (C)0;
+                        py0 = dY ? f1 : (C)0;
+                        pz0 = dZ ? f2 : (C)0;
+                        if (rem) rP = p0 - px0 - py0 - pz0;
+                    }
                     const C npx = ax2 * px0 - ax1 * r0 * c0 * c0 * dtx * xx;
                     const C npy = ay2 * py0 - ay1 * r0 * c0 * c0 * dty * yy;
                     const C npz = az2 * pz0 - az1 * r0 * c0 * c0 * dtz * zz;
-                    stw(px, zo, col, npx, sc.inv[F_PX]);
-                    stw(py, zo, col, npy, sc.inv[F_PY]);
-                    stw(pz, zo, col, npz, sc.inv[F_PZ]);
+                    if (sx) stw(px, zo, col, npx, sc.inv[F_PX]);
+                    if (sy) stw(py, zo, col, npy, sc.inv[F_PY]);
+                    if (sz) stw(pz, zo, col, npz, sc.inv[F_PZ]);
                     Pn = npx + npy + npz;
+                    if (dirs) Pn += rP;
                 } else {
                     const C K = r0 * c0 * c0;
                     Pn = p0 - K * (dtx * xx + dty * yy + dtz * zz);
@@ -699,7 +724,20 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 const C e2 = 1 + 2 * ee;
                 const C s2 = csqrt(1 + 2 * dd);
                 if (split) {
-                    const C px0 = f0, py0 = f1, pz0 = f2, qx0 = f3, qy0 = f4, qz0 = f5;
+                    C px0 = f0, py0 = f1, pz0 = f2, qx0 = f3, qy0 = f4, qz0 = f5, rP = 0, rQ = 0;
+                    if (dirs) {
+                        // undamped directions: no split field (factors 1, old value 0)
+                        px0 = dX ? f0 : (C)0;
+                        py0 = dY ? f1 : (C)0;
+                        pz0 = dZ ? f2 : (C)0;
+                        qx0 = dX ? f3 : (C)0;
+                        qy0 = dY ? f4 : (C)0;
+                        qz0 = dZ ? f5 : (C)0;
+                        if (rem) {
+                            rP = p0 - px0 - py0 - pz0;
+                            rQ = q0 - qx0 - qy0 - qz0;
+                        }
+                    }
                     C npx, npy, npz, nqx, nqy, nqz;
                     if constexpr (VTI) {
                         npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
@@ -716,14 +754,18 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                         nqy = ay2 * qy0 - ay1 * c0 * c0 * s2 * dty * yy;
                         nqz = az2 * qz0 - az1 * c0 * c0 * dtz * zz;
                     }
-                    stw(px, zo, col, npx, sc.inv[F_PX]);
-                    stw(py, zo, col, npy, sc.inv[F_PY]);
-                    stw(pz, zo, col, npz, sc.inv[F_PZ]);
-                    stw(qx, zo, col, nqx, sc.inv[F_QX]);
-                    stw(qy, zo, col, nqy, sc.inv[F_QY]);
-                    stw(qz, zo, col, nqz, sc.inv[F_QZ]);
+                    if (sx) stw(px, zo, col, npx, sc.inv[F_PX]);
+                    if (sy) stw(py, zo, col, npy, sc.inv[F_PY]);
+                    if (sz) stw(pz, zo, col, npz, sc.inv[F_PZ]);
+                    if (sx) stw(qx, zo, col, nqx, sc.inv[F_QX]);
+                    if (sy) stw(qy, zo, col, nqy, sc.inv[F_QY]);
+                    if (sz) stw(qz, zo, col, nqz, sc.inv[F_QZ]);
                     Pn = npx + npy + npz;
                     Qn = nqx + nqy + nqz;
+                    if (dirs) {
+                        Pn += rP;
+                        Qn += rQ;
+                    }
                 } else {
                     const C K = VTI ? r0 * c0 * c0 : c0 * c0;
                     const C hxy = dtx * xx + dty * yy, hz = dtz * zz;
@@ -738,15 +780,15 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     }
 }
 
-template <class PR, int OP, int TY, bool WZ = false>
+template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false>
 __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
-    stress_body<PR, OP, TY, WZ>(a, a.rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, WZ, DIRS>(a, a.rec, tix, tiy, seg);
 }
 
 // Batched stress sweep; a window's record plane of step it is rec + it * nx * ny.
-template <class PR, int OP, int TY>
+template <class PR, int OP, int TY, bool DIRS = false>
 __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __restrict__ wins,
                                                          const int* __restrict__ base, int nwin, Launch lb, int it) {
     const int t = remap_tile(lb);
@@ -756,7 +798,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __res
     int tix, tiy, seg;
     split_tile(a.l, t - base[w], tix, tiy, seg);
     float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
-    stress_body<PR, OP, TY>(a, rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, false, DIRS>(a, rec, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -971,13 +1013,13 @@ hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool WZ>
+template <class PR, int OP, bool WZ, bool DIRS>
 hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2, WZ, DIRS>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4, WZ, DIRS>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8, WZ, DIRS>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1008,14 +1050,14 @@ hipError_t vel_batch_ty(int ty, const VelArgs* w, const int* base, int nwin, con
     return hipGetLastError();
 }
 
-template <class PR, int OP>
+template <class PR, int OP, bool DIRS>
 hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
                            hipStream_t st) {
     const dim3 grid(lb.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it); break;
-        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it); break;
-        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it); break;
+        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2, DIRS>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it); break;
+        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4, DIRS>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it); break;
+        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8, DIRS>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1034,11 +1076,14 @@ hipError_t vel_batch_op(int op, int ty, const VelArgs* w, const int* base, int n
 
 template <class PR>
 hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
-                           hipStream_t st) {
+                           bool dirs, hipStream_t st) {
     switch (op) {
-        case 1: return stress_batch_ty<PR, 1>(ty, w, base, nwin, lb, it, st);
-        case 2: return stress_batch_ty<PR, 2>(ty, w, base, nwin, lb, it, st);
-        case 3: return stress_batch_ty<PR, 3>(ty, w, base, nwin, lb, it, st);
+        case 1: return dirs ? stress_batch_ty<PR, 1, true>(ty, w, base, nwin, lb, it, st)
+                            : stress_batch_ty<PR, 1, false>(ty, w, base, nwin, lb, it, st);
+        case 2: return dirs ? stress_batch_ty<PR, 2, true>(ty, w, base, nwin, lb, it, st)
+                            : stress_batch_ty<PR, 2, false>(ty, w, base, nwin, lb, it, st);
+        case 3: return dirs ? stress_batch_ty<PR, 3, true>(ty, w, base, nwin, lb, it, st)
+                            : stress_batch_ty<PR, 3, false>(ty, w, base, nwin, lb, it, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1056,9 +1101,15 @@ hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
 template <class PR>
 hipError_t stress_op(int op, int ty, const StressArgs& a, hipStream_t st) {
     switch (op) {
-        case 1: return a.wz ? stress_ty<PR, 1, true>(ty, a, st) : stress_ty<PR, 1, false>(ty, a, st);
-        case 2: return a.wz ? stress_ty<PR, 2, true>(ty, a, st) : stress_ty<PR, 2, false>(ty, a, st);
-        case 3: return a.wz ? hipErrorInvalidValue : stress_ty<PR, 3, false>(ty, a, st);
+        case 1:
+            if (a.unsplit == 2) return a.wz ? stress_ty<PR, 1, true, true>(ty, a, st) : stress_ty<PR, 1, false, true>(ty, a, st);
+            return a.wz ? stress_ty<PR, 1, true, false>(ty, a, st) : stress_ty<PR, 1, false, false>(ty, a, st);
+        case 2:
+            if (a.unsplit == 2) return a.wz ? stress_ty<PR, 2, true, true>(ty, a, st) : stress_ty<PR, 2, false, true>(ty, a, st);
+            return a.wz ? stress_ty<PR, 2, true, false>(ty, a, st) : stress_ty<PR, 2, false, false>(ty, a, st);
+        case 3:
+            if (a.wz) return hipErrorInvalidValue;
+            return a.unsplit == 2 ? stress_ty<PR, 3, false, true>(ty, a, st) : stress_ty<PR, 3, false, false>(ty, a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1132,11 +1183,11 @@ hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const
 }
 
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, hipStream_t st) {
+                               const Launch& lb, int it, bool dirs, hipStream_t st) {
     switch (prec) {
-        case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, st);
-        case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, st);
-        case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, st);
+        case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, st);
+        case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, dirs, st);
+        case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, dirs, st);
         default: return hipErrorInvalidValue;
     }
 }

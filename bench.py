@@ -45,7 +45,9 @@ def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
     W 6 split, P, Q. stress, unsplit cell (scheme 1 outside the PML): R u,v,w, P,Q, 4 model; W P,Q.
     wz (w updated by the stress sweep): velocity R P,rho,u,v; W u,v. stress reads u,v, the old Q
     one plane ahead (also the unsplit cell's Q), the old w, and writes w; otherwise as above.
-    scheme 2 (one pass, FP32): (0, R P,Q,u,v,w, 4 model; W P,Q,u,v,w; + R/W 6 split in PML cells)."""
+    scheme 2 (one pass, FP32): (0, R P,Q,u,v,w, 4 model; W P,Q,u,v,w; + R/W 6 split in PML cells).
+    scheme 3: the unsplit cell plus R/W of 2 split fields per damped direction; a cell damped in
+    all three directions is the split cell (it reads no old P)."""
     ws = {0: 4, 1: 2, 2: 8}[storage]
     ms = 8 if storage == 2 else 4
     ss = max(ws, 4)                     # split PML fields are FP32 under binary16 storage
@@ -59,12 +61,34 @@ def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
         unsplit = 7 * ws + 4 * ms
     inner = (cfg["nx"] + 1) * (cfg["ny"] + 1) * (cfg["nz"] + 1)   # cells whose stress PML factors are all 1
     f = inner / npts
+    if scheme == 3:
+        return a_vel, dir_split_bytes(cfg, npml, unsplit, split, 4 * ss)
     if scheme == 2:
         one = 10 * ws + 4 * ms
         return 0, f * one + (1 - f) * (one + 12 * ss)
     if scheme == 0 and storage != 1:
         return a_vel, float(split)
     return a_vel, f * unsplit + (1 - f) * split
+
+
+def damped_counts(cfg, npml):
+    """Updated cells by the number of directions (0..3) whose stress PML factor is not 1: per
+    axis of n cells the padded extent n + 2 npml less the 4-cell ring on each side, of which
+    n + 1 cells (x = npml .. npml + n) have factor 1."""
+    c = [1]
+    for k in ("nx", "ny", "nz"):
+        inner = cfg[k] + 1
+        damped = cfg[k] + 2 * npml - 8 - inner
+        c = [(c[j] if j < len(c) else 0) * inner + (c[j - 1] * damped if j >= 1 else 0) for j in range(len(c) + 1)]
+    return c
+
+
+def dir_split_bytes(cfg, npml, unsplit, split, per_dir):
+    """Mean stress bytes per updated point of scheme 3: unsplit + per_dir per damped direction,
+    the split cell where all three are damped."""
+    c = damped_counts(cfg, npml)
+    tot = c[0] * unsplit + c[1] * (unsplit + per_dir) + c[2] * (unsplit + 2 * per_dir) + c[3] * split
+    return tot / sum(c)
 
 
 def sampled_steps(nt, stride):
@@ -105,7 +129,8 @@ def precision_study(cfg):
     from a3dfd import Modeller, make_desc
     nz = cfg["nz"]
     out = {}
-    for name, storage, scheme in (("fp64", 2, 0), ("fp32", 0, 0), ("fp32_unsplit", 0, 1), ("fp16", 1, 1)):
+    for name, storage, scheme in (("fp64", 2, 0), ("fp32", 0, 0), ("fp32_unsplit", 0, 1), ("fp32_dirsplit", 0, 3),
+                                  ("fp16", 1, 1)):
         d = make_desc("vti", cfg["nx"], cfg["ny"], nz, cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"], cfg["sy"],
                       cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], storage=storage, scheme=scheme,
                       snap_interval=1000, snap_skip0=1)
@@ -119,7 +144,7 @@ def precision_study(cfg):
     ref = out["fp64"]
     res = {"workload": "vti_400x400x300_nt%d" % cfg["nt"], "reference": "GPU FP64 instance"}
     rl = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))  # noqa
-    for name in ("fp32", "fp32_unsplit", "fp16"):
+    for name in ("fp32", "fp32_unsplit", "fp32_dirsplit", "fp16"):
         o = out[name]
         res[name] = {"shot_rel_l2_vs_fp64": rl(o["shot"], ref["shot"]),
                      "snapshot_it1000_rel_l2_vs_fp64": rl(o["snap"], ref["snap"]),
@@ -128,6 +153,9 @@ def precision_study(cfg):
     a, b = out["fp32_unsplit"], out["fp32"]
     res["fp32_unsplit_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
                                    "snapshot_it2000": rl(a["snap2"], b["snap2"])}
+    a = out["fp32_dirsplit"]
+    res["fp32_dirsplit_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
+                                    "snapshot_it2000": rl(a["snap2"], b["snap2"])}
     res["fp64"] = {"loop_ms": round(ref["ms"], 1), "Gpt_s": round(ref["gpts"], 2)}
     print(json.dumps(res), flush=True)
 
@@ -231,12 +259,12 @@ def operators(args):
     c1 = models.cfg1()
     runs.append(("2nd", c1, dict(nb=20), 0, c1["nt"]))
     c2 = models.cfg2()
-    runs += [("iso", c2, dict(v_mute=2000.0), s, c2["nt"]) for s in (0, 1)]
+    runs += [("iso", c2, dict(v_mute=2000.0), s, c2["nt"]) for s in (0, 1, 3)]
     c3 = models.cfg3()
-    runs += [("vti", c3, {}, s, c3["nt"]) for s in (0, 1)]
+    runs += [("vti", c3, {}, s, c3["nt"]) for s in (0, 1, 3)]
     c4 = models.cfg4_model()
     c4.update(sx=301, sy=301, sz=1)
-    runs += [("tti", c4, {}, s, 500) for s in (0, 1)]
+    runs += [("tti", c4, {}, s, 500) for s in (0, 1, 3)]
     for op, c, kw, scheme, nt in runs:
         keys = {"2nd": ("vel",), "iso": ("vel", "rho"), "vti": ("vel", "rho", "eps", "delta"),
                 "tti": ("vel", "eps", "delta", "theta", "phi")}[op]
@@ -257,6 +285,11 @@ def operators(args):
         inner = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1) if op != "2nd" else npts
         f = min(1.0, inner / npts)
         a1 = a_split if scheme == 0 else f * a_unsplit + (1 - f) * a_split
+        if scheme == 3:
+            # old P (Q) reads a split cell skips: those the sweep does not take from its w window
+            drop = {"iso": 0 if launch["w_sweep"] else 4, "vti": 4 if launch["w_sweep"] else 8, "tti": 8}[op]
+            per_dir = (a_split - a_unsplit + drop) / 3
+            a1 = dir_split_bytes(c, nbb, a_unsplit, a_split, per_dir)
         k0 = kms[0] / max(kn[0], 1)
         k1 = kms[1] / max(kn[1], 1) if kn[1] else 0.0
         g0 = a0 * npts / (k0 * 1e-3) / 1e9
@@ -295,7 +328,8 @@ def main():
     ap.add_argument("--storage", type=int, default=0, help="0 FP32 (default), 1 FP16 wavefield storage, 2 FP64")
     ap.add_argument("--scheme", type=int, default=1,
                     help="0: split PML fields everywhere (bitwise reference arithmetic); 1: split only in PML cells; "
-                         "2: scheme 1 in one pass per step (bitwise equal to 1)")
+                         "2: scheme 1 in one pass per step (bitwise equal to 1); 3: split fields only for the "
+                         "damped directions of PML cells")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run the calibration streams (2 GiB float4 copy and read) and print GB/s")
     ap.add_argument("--batched", action="store_true",
@@ -412,7 +446,9 @@ def main():
                    "parity": {0: "bitwise equal to the FP32 reference arithmetic",
                               1: "split PML fields in PML cells only: <=1e-5 rel. L2 of the reference arithmetic "
                                  "(3.9e-7 on this shot's record)",
-                              2: "scheme 1 in one pass: bitwise equal to scheme 1"}[args.scheme]},
+                              2: "scheme 1 in one pass: bitwise equal to scheme 1",
+                              3: "split PML fields for damped directions only: <=1e-5 rel. L2 of the reference "
+                                 "arithmetic"}[args.scheme]},
         "roofline": {"bound": "hbm", "kernel": main_kernel, "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_point": round(a_str, 2),

@@ -129,7 +129,11 @@ typedef struct a3d_shot_desc {
                               2: VTI with FP32 storage only: the scheme-1 arithmetic in one
                               z-streaming pass per step (velocities made on the chip where the
                               stress update needs them; bitwise equal to scheme 1, 56 B/point
-                              in the interior). Storage 1 (binary16) always uses scheme 1. */
+                              in the interior); 3: scheme 1, and a PML cell keeps the split
+                              fields of its damped directions only (P, Q minus them carry the
+                              undamped directions' updates): 64 instead of 92 B/point in the
+                              faces' PML cells, within 1e-5 relative L2 of scheme 0. Storage 1
+                              (binary16) uses scheme 1 or 3. */
 } a3d_shot_desc;
 
 typedef struct a3d_ctx a3d_ctx;
@@ -193,11 +197,13 @@ int a3d_get_w_sweep(a3d_ctx* ctx, const a3d_shot_desc* desc, int* mode);
 int a3d_set_zchunk(a3d_ctx* ctx, int planes);
 
 /* Autotuner: times `steps` real time steps of *desc for each candidate
- * shape (TY in {2,4,8} x z segments in {1..24}) and keeps the fastest per
- * sweep; best_ms (may be NULL) receives the two best average launch times.
- * Results are cached per process by (device, operator, padded grid). a3d_run
- * and the legacy entry points call it on the first shot of a grid with at
- * least 1M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
+ * shape (TY in {2,4,8} x z segments in {1..24}; with w made by either sweep
+ * where both can) and keeps the fastest per sweep; with steps >= 6 the first
+ * two launches of each kernel (cold caches) are not counted. best_ms (may be
+ * NULL) receives the two best average launch times. Results are cached per
+ * process by (device, operator, precision, scheme, padded grid). a3d_run and
+ * the legacy entry points call it (10 steps) on the first shot of a grid with
+ * at least 1M updated points and nt >= 100 unless A3D_AUTOTUNE=0. */
 int a3d_autotune(a3d_ctx* ctx, const a3d_shot_desc* desc, int steps, double* best_ms);
 
 /* ----------------------------------------------------------------------- */

@@ -48,3 +48,31 @@ def test_sampled_steps():
     for nt in (1, 7, 8, 9, 3000):
         for stride in (1, 2, 8, 40):
             assert b.sampled_steps(nt, stride) == sum(1 for it in range(nt) if it % stride == 0)
+
+
+def test_damped_counts_and_dir_split_bytes():
+    """Cells by number of damped directions, against an enumeration of the updated cells with
+    the PML profile's rule (factor 1 where the ramp argument (npml - id) or (id - n - npml) is
+    <= 0), and the scheme-3 mean bytes at its extremes."""
+    b = _bench()
+    cfg = dict(nx=7, ny=5, nz=6)
+    npml, MM = 6, 4
+
+    def damped(i, n):
+        ra = (npml - i) if i < npml else (i - n - npml if i >= npml + n else 0)
+        return ra > 0
+
+    cnt = [0, 0, 0, 0]
+    for x in range(MM, cfg["nx"] + 2 * npml - MM):
+        for y in range(MM, cfg["ny"] + 2 * npml - MM):
+            for z in range(MM, cfg["nz"] + 2 * npml - MM):
+                cnt[damped(x, cfg["nx"]) + damped(y, cfg["ny"]) + damped(z, cfg["nz"])] += 1
+    assert b.damped_counts(cfg, npml) == cnt
+    big = dict(nx=400, ny=400, nz=300)
+    assert sum(b.damped_counts(big, 20)) == 61959168
+    # VTI with w made by the stress sweep: 48 / 64 / 80 / 92 B by damped directions
+    c = b.damped_counts(big, 20)
+    want = (c[0] * 48 + c[1] * 64 + c[2] * 80 + c[3] * 92) / sum(c)
+    assert abs(b.algorithmic_bytes(big, 0, 3, 61959168, wz=1)[1] - want) < 1e-9
+    c1 = b.algorithmic_bytes(big, 0, 1, 61959168, wz=1)[1]
+    assert want < c1 and abs(want - 51.79) < 0.01

@@ -116,7 +116,7 @@ def test_closed_network_equals_oracle_bitwise(oracle):
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("scheme,mfma", [(0, 1), (1, 1), (0, 0)])
+@pytest.mark.parametrize("scheme,mfma", [(0, 1), (1, 1), (3, 1), (0, 0)])
 def test_strip_matches_oracle(oracle, scheme, mfma):
     """The strip correction fused into the time loop against the oracle's restatement: the
     matrix-core (binary16 operand) network against the oracle's binary16 form, the FP32
