@@ -584,6 +584,7 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     a.cdt[0] = d->dt / d->dx;
     a.cdt[1] = d->dt / d->dy;
     a.cdt[2] = d->dt / d->dz;
+    a.dirsplit = d->scheme == 3;
     return a;
 }
 
@@ -1322,8 +1323,13 @@ int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
                         if (hi < best) best = hi, axis = a, side = 1;
                     }
                     if (best > STRIP_CELLS) continue;
+                    // directions whose stress PML factor is not 1 (padded index below nb or
+                    // above nn - nb, as pml_axis builds the tables)
+                    int dirs = 0;
+                    for (int a = 0; a < 3; ++a)
+                        if (p[a] < STRIP_NB || p[a] > n3[a] - STRIP_NB) dirs |= 1 << a;
                     ids.push_back(x + g.ld * y + g.pl * z);
-                    meta.push_back(axis | side << 2 | best << 3);
+                    meta.push_back(axis | side << 2 | best << 3 | dirs << 8);
                 }
         const size_t nb = ids.size() * sizeof(int);
         if (hipMalloc(&c->strip_ids, nb) != hipSuccess || hipMalloc(&c->strip_meta, nb) != hipSuccess ||

@@ -213,8 +213,9 @@ hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, i
 constexpr int STRIP_CELLS = 6;
 constexpr int STRIP_NB = STRIP_CELLS + MM;
 
-// Strip cells of one grid: padded linear index and meta = axis | side << 2 | depth << 3
-// (axis of the nearest face, side 1 for the high face, depth 1 = outermost updated cell).
+// Strip cells of one grid: padded linear index and meta = axis | side << 2 | depth << 3 |
+// dirs << 8 (axis of the nearest face, side 1 for the high face, depth 1 = outermost
+// updated cell, dirs bit a set when the cell's stress PML factor along axis a is not 1).
 struct StripArgs {
     Geo g;
     const int* ids;
@@ -222,6 +223,7 @@ struct StripArgs {
     int n;
     const float *P, *Q, *u, *v, *w, *vp, *rho, *eps, *del;
     float cdt[3];   // dt/dx, dt/dy, dt/dz
+    int dirsplit;   // the stress sweep keeps split fields only for a cell's damped directions
 };
 
 // delta[k] = the damping factors (g_P, g_Q) of every strip cell (after the stress sweep);

@@ -29,7 +29,7 @@ namespace {
 constexpr int SB = 256;   // threads per block
 
 __device__ __forceinline__ float strip_features(const StripArgs& a, int id, int meta, float (&x)[LB_NIN]) {
-    const int axis = meta & 3, side = (meta >> 2) & 1, dist = meta >> 3;
+    const int axis = meta & 3, side = (meta >> 2) & 1, dist = (meta >> 3) & 7;
     const int sa = axis == 0 ? 1 : (axis == 1 ? a.g.ld : a.g.pl);
     const int sn = side ? sa : -sa;   // one cell outward
     const int s1 = axis == 0 ? a.g.ld : 1;
@@ -76,109 +76,136 @@ __global__ __launch_bounds__(SB) void k_strip_infer(const StripArgs a, const flo
     delta[k] = make_float2(g[0], g[1]);
 }
 
-// The same network on the matrix cores: one wavefront evaluates 64 strip cells as
-// a 64 x 16 by 16 x 64 and a 64 x 64 by 64 x 64 product with v_mfma_f32_32x32x8_f16
-// (operands rounded to binary16, products exact, sums in FP32), the 2-output last
-// layer per lane in FP32. Weights, features and hidden activations pass through LDS
-// in binary16. MFMA 32x32x8 operand layout (lane l, r = 0..15): A[i][k] with
-// i = l % 32, k = 4 (l / 32) + 0..3; B[k][j] with j = l % 32, same k; D[i][j] with
-// j = l % 32, i = 8 (r / 4) + 4 (l / 32) + r % 4.
-typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+// The same network on the matrix cores (binary16 operands, exact products, FP32
+// sums), one wavefront per 64 strip cells, in transposed form: every layer computes
+// H^T = W X^T with v_mfma_f32_32x32x16_f16, the weights as the A operand (held in
+// registers for the whole kernel) and the cells as the B operand. MFMA 32x32 operand
+// layout (lane l, group g = l / 32, r = 0..15): A[i][k] with i = l % 32, B[k][j] with
+// j = l % 32, a lane supplying 8 values of k for group g; D[i][j] with j = l % 32,
+// i = 8 (r / 4) + 4 g + r % 4. The k order within a product is free as long as A and
+// B use the same one, so layer 2 takes layer 1's accumulator registers as its B
+// operand directly: lane slot (ks, g, e) carries hidden unit
+//   pi(ks, g, e) = 16 ks + 8 (e / 4) + 4 g + e % 4,
+// which is what the layer-1 tile cb = ks / 2 holds in r = 8 (ks % 2) + e. Layer 3
+// (2 outputs) is a per-lane sum over the 32 hidden units a lane holds plus its
+// partner lane's (l ^ 32). Activations tanh(a) = 1 - 2 / (2^(2a log2 e) + 1) with the
+// hardware exponential and reciprocal, rounded to binary16 like the operands.
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
 typedef float float16v __attribute__((ext_vector_type(16)));
-constexpr int XP = LB_NIN + 4;   // padded LDS rows (halves), 8-byte aligned
-constexpr int HP = LB_NH + 8;
+union Half8 {
+    half8v h;
+    int i[4];
+};
+
+__device__ __forceinline__ float act_tanh_hw(float a) {
+    const float t = __builtin_amdgcn_exp2f(a * 2.8853900817779268f);   // 2 log2(e)
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(t + 1.0f), 1.0f);
+}
 
 __global__ __launch_bounds__(SB) void k_strip_infer_mfma(const StripArgs a, const float* __restrict__ net,
                                                          float2* __restrict__ delta) {
-    __shared__ _Float16 sW1[LB_NH][XP];
-    __shared__ _Float16 sW2[LB_NH][HP];
-    __shared__ float sB1[LB_NH], sB2[LB_NH], sW3[LB_NOUT][LB_NH], sB3[LB_NOUT];
-    __shared__ _Float16 sX[SB / 64][64][XP];
-    __shared__ _Float16 sH[SB / 64][64][HP];
-    for (int i = threadIdx.x; i < LB_NH * LB_NIN; i += SB) sW1[i / LB_NIN][i % LB_NIN] = (_Float16)net[LB_W1 + i];
-    for (int i = threadIdx.x; i < LB_NH * LB_NH; i += SB) sW2[i / LB_NH][i % LB_NH] = (_Float16)net[LB_W2 + i];
+    __shared__ float sB1[LB_NH], sB2[LB_NH], sW3[LB_NOUT][LB_NH];
     for (int i = threadIdx.x; i < LB_NH; i += SB) {
         sB1[i] = net[LB_B1 + i];
         sB2[i] = net[LB_B2 + i];
         sW3[0][i] = net[LB_W3 + i];
         sW3[1][i] = net[LB_W3 + LB_NH + i];
     }
-    if (threadIdx.x < LB_NOUT) sB3[threadIdx.x] = net[LB_B3 + threadIdx.x];
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int lr = l & 31, lk = (l >> 5) * 4;
-    // the block stages the weights once and walks the strip cells 256 at a time
-    for (int base = blockIdx.x * SB; base < a.n; base += gridDim.x * SB) {
-        const int k = base + threadIdx.x;
+    const float b3[LB_NOUT] = {net[LB_B3], net[LB_B3 + 1]};
+    const int l = threadIdx.x & 63, g = l >> 5, lr = l & 31;
+    // A operands: W1 row 32 cb + lr, inputs 8 g + e; W2 row 32 cb + lr, inputs pi(ks, g, e)
+    half8v w1[2], w2[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const float* r1 = net + LB_W1 + (32 * cb + lr) * LB_NIN + 8 * g;
+        const float* r2 = net + LB_W2 + (32 * cb + lr) * LB_NH + 4 * g;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w1[cb][e] = (_Float16)r1[e];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) w2[cb][ks][e] = (_Float16)r2[16 * ks + 8 * (e >> 2) + (e & 3)];
+    }
+    __syncthreads();
+    const int nwave = gridDim.x * (SB / 64);
+    for (int base = (blockIdx.x * SB + threadIdx.x - l); base < a.n; base += nwave * 64) {
+        const int k = base + l;   // this lane's cell
         float x[LB_NIN];
 #pragma unroll
         for (int i = 0; i < LB_NIN; ++i) x[i] = 0.f;
         if (k < a.n) (void)strip_features(a, a.ids[k], a.meta[k], x);
-        __syncthreads();   // the previous group's last reads of sX / sH are done
+        // layer-1 B operands: cells 32 rb + lr, inputs 8 g + e. Cell 32 g + lr is this
+        // lane's own; the other half comes from the partner lane, which sends the
+        // inputs of group g of its own cell.
+        Half8 own, send, recv;
 #pragma unroll
-        for (int i = 0; i < LB_NIN; ++i) sX[wv][l][i] = (_Float16)x[i];
-        __syncthreads();
-        float16v acc[2][2];
-        // layer 1: H1 = tanh(X W1^T + b1)
+        for (int e = 0; e < 8; ++e) {
+            own.h[e] = (_Float16)(g ? x[8 + e] : x[e]);
+            send.h[e] = (_Float16)(g ? x[e] : x[8 + e]);
+        }
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int q = 0; q < 4; ++q) recv.i[q] = __shfl_xor(send.i[q], 32);
+        float y[LB_NOUT] = {0.f, 0.f};
+#pragma nounroll
+        for (int rb = 0; rb < 2; ++rb) {
+            Half8 xb;
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                float16v c = {};
-#pragma unroll
-                for (int ks = 0; ks < LB_NIN / 8; ++ks) {
-                    const half4v av = *(const half4v*)&sX[wv][32 * rb + lr][8 * ks + lk];
-                    const half4v bv = *(const half4v*)&sW1[32 * cb + lr][8 * ks + lk];
-                    c = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
-                }
-                acc[rb][cb] = c;
-            }
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int i = 32 * rb + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), j = 32 * cb + lr;
-                    sH[wv][i][j] = (_Float16)act_tanh(acc[rb][cb][r] + sB1[j]);
-                }
-        __syncthreads();
-        // layer 2: H2 = tanh(H1 W2^T + b2)
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+            for (int q = 0; q < 4; ++q) xb.i[q] = (g == rb) ? own.i[q] : recv.i[q];
+            // biases and last-layer weights are re-read from LDS where they are used
+            // (kept live across the loop they would take 128 registers)
+            __asm__ volatile("" ::: "memory");
+            // layer 1: H1^T tiles cb (hidden 32 cb ..) x cells 32 rb ..
+            half8v hb[4];
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb) {
-                float16v c = {};
+                const float16v zero = {};
+                const float16v d = __builtin_amdgcn_mfma_f32_32x32x16_f16(w1[cb], xb.h, zero, 0, 0, 0);
 #pragma unroll
-                for (int ks = 0; ks < LB_NH / 8; ++ks) {
-                    const half4v av = *(const half4v*)&sH[wv][32 * rb + lr][8 * ks + lk];
-                    const half4v bv = *(const half4v*)&sW2[32 * cb + lr][8 * ks + lk];
-                    c = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+                for (int q = 0; q < 4; ++q) {
+                    const float4 bq = *(const float4*)&sB1[32 * cb + 8 * q + 4 * g];
+                    const float bb[4] = {bq.x, bq.y, bq.z, bq.w};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int r = 4 * q + t;
+                        hb[2 * cb + (r >> 3)][r & 7] = (_Float16)act_tanh_hw(d[r] + bb[t]);
+                    }
                 }
-                acc[rb][cb] = c;
             }
-        __syncthreads();
+            // layer 2: H2^T tiles cb (hidden 32 cb ..) x cells 32 rb .., then the partial
+            // last layer over the hidden units this lane holds
+            float part[LB_NOUT] = {0.f, 0.f};
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+            for (int cb = 0; cb < 2; ++cb) {
+                float16v d = {};
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
+                for (int ks = 0; ks < 4; ++ks) d = __builtin_amdgcn_mfma_f32_32x32x16_f16(w2[cb][ks], hb[ks], d, 0, 0, 0);
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int i = 32 * rb + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3), j = 32 * cb + lr;
-                    sH[wv][i][j] = (_Float16)act_tanh(acc[rb][cb][r] + sB2[j]);
+                for (int q = 0; q < 4; ++q) {
+                    const int j = 32 * cb + 8 * q + 4 * g;
+                    const float4 bq = *(const float4*)&sB2[j];
+                    const float4 u0 = *(const float4*)&sW3[0][j];
+                    const float4 u1 = *(const float4*)&sW3[1][j];
+                    const float bb[4] = {bq.x, bq.y, bq.z, bq.w};
+                    const float c0[4] = {u0.x, u0.y, u0.z, u0.w};
+                    const float c1[4] = {u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const float h = (float)(_Float16)act_tanh_hw(d[4 * q + t] + bb[t]);
+                        part[0] = fmaf(c0[t], h, part[0]);
+                        part[1] = fmaf(c1[t], h, part[1]);
+                    }
                 }
-        __syncthreads();
-        if (k < a.n) {
-            // layer 3 (2 outputs) and the damping head, one cell per lane
-            float y[LB_NOUT], g[LB_NOUT], pred[LB_NOUT];
+            }
 #pragma unroll
             for (int o = 0; o < LB_NOUT; ++o) {
-                float t = 0.f;
-#pragma unroll
-                for (int j = 0; j < LB_NH; ++j) t = fmaf(sW3[o][j], (float)sH[wv][l][j], t);
-                y[o] = t + sB3[o];
+                const float full = part[o] + __shfl_xor(part[o], 32);
+                if (g == rb) y[o] = full + b3[o];
             }
-            strip_head(y, x, g, pred);
-            delta[k] = make_float2(g[0], g[1]);
+        }
+        if (k < a.n) {
+            float gg[LB_NOUT], pred[LB_NOUT];
+            strip_head(y, x, gg, pred);
+            delta[k] = make_float2(gg[0], gg[1]);
         }
     }
 }
@@ -193,16 +220,25 @@ __global__ __launch_bounds__(SB) void k_strip_apply(const StripArgs a, const flo
     const int k = blockIdx.x * SB + threadIdx.x;
     if (k >= a.n) return;
     const int id = a.ids[k];
+    // split fields of the directions whose stress factor is 1 stay zero when the stress
+    // sweep splits only the damped directions: scaling them is skipped
+    const int dirs = a.dirsplit ? (a.meta[k] >> 8) & 7 : 7;
     const float2 d = delta[k];
     const float fp = 1.0f - d.x, fq = 1.0f - d.y;
     P[id] *= fp;
-    px[id] *= fp;
-    py[id] *= fp;
-    pz[id] *= fp;
     Q[id] *= fq;
-    qx[id] *= fq;
-    qy[id] *= fq;
-    qz[id] *= fq;
+    if (dirs & 1) {
+        px[id] *= fp;
+        qx[id] *= fq;
+    }
+    if (dirs & 2) {
+        py[id] *= fp;
+        qy[id] *= fq;
+    }
+    if (dirs & 4) {
+        pz[id] *= fp;
+        qz[id] *= fq;
+    }
     u[id] *= fp;
     v[id] *= fp;
     w[id] *= fq;
@@ -254,8 +290,8 @@ __global__ void k_strip_copy(const float* __restrict__ src, Geo gs, float* __res
 
 hipError_t launch_strip_infer(const StripArgs& a, const float* net, float2* delta, int mfma, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-    if (mfma)   // two resident blocks per CU (LDS), each walking its share of the cells
-        k_strip_infer_mfma<<<std::min((a.n + SB - 1) / SB, 512), SB, 0, st>>>(a, net, delta);
+    if (mfma)   // a few resident blocks per CU, each wave walking its share of 64-cell groups
+        k_strip_infer_mfma<<<std::min((a.n + SB - 1) / SB, 1024), SB, 0, st>>>(a, net, delta);
     else
         k_strip_infer<<<(a.n + SB - 1) / SB, SB, 0, st>>>(a, net, delta);
     return hipGetLastError();
