@@ -112,6 +112,34 @@ def closed_params():
     return p
 
 
+def profile_params(gains, steep=60.0):
+    """A network whose damping head gives the fixed factor gains[d - 1] (P and Q alike) at strip
+    depth d = 1..6 (1 = outermost), whatever the local state: hidden unit j of layer 1 is a step
+    tanh(steep * (depth/6 - (j + 0.5)/6)) (+1 for d >= j + 1, -1 below), layer 2 passes each step
+    on as tanh(steps / 2), and the output weights make y(d) = logit(gains[d - 1] / 0.25), from the
+    differences y(d) - y(d - 1) = 2 c_(d-1), c_j = W3_j tanh(1/2). A zero gain is taken as 1e-14
+    (1 - g rounds to 1 in FP32). Used to measure what fixed damping profiles on top of the strip's
+    PML ramp do to the reflected energy."""
+    gains = np.asarray(gains, np.float64)
+    if gains.shape != (6,) or np.any(gains < 0) or np.any(gains >= 0.25):
+        raise ValueError("six damping factors in [0, 0.25) expected")
+    r = np.maximum(gains, 1e-14) / 0.25
+    y = np.log(r / (1.0 - r))
+    c = np.zeros(6)
+    c[1:] = (y[1:] - y[:-1]) / 2.0
+    s = np.tanh(0.5)
+    w1, b1 = np.zeros((64, 16)), np.zeros(64)
+    w2, b2 = np.zeros((64, 64)), np.zeros(64)
+    w3, b3 = np.zeros((2, 64)), np.zeros(2)
+    for j in range(6):
+        w1[j, 14] = steep
+        b1[j] = -steep * (j + 0.5) / 6.0
+        w2[j, j] = 0.5
+        w3[:, j] = c[j] / s
+    b3[:] = y[0] + c[1:].sum()
+    return np.concatenate([a.ravel() for a in (w1, b1, w2, b2, w3, b3)]).astype(np.float32)
+
+
 def save(params, path=NET_PATH, **meta):
     np.savez(path, params=np.asarray(params, np.float32), **{k: np.asarray(v) for k, v in meta.items()})
 

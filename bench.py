@@ -131,7 +131,7 @@ def precision_study(cfg):
     nz = cfg["nz"]
     out = {}
     for name, storage, scheme in (("fp64", 2, 0), ("fp32", 0, 0), ("fp32_unsplit", 0, 1), ("fp32_dirsplit", 0, 3),
-                                  ("fp16", 1, 1)):
+                                  ("fp16", 1, 1), ("fp16_dirsplit", 1, 3)):
         d = make_desc("vti", cfg["nx"], cfg["ny"], nz, cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"], cfg["sy"],
                       cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], storage=storage, scheme=scheme,
                       snap_interval=1000, snap_skip0=1)
@@ -145,7 +145,7 @@ def precision_study(cfg):
     ref = out["fp64"]
     res = {"workload": "vti_400x400x300_nt%d" % cfg["nt"], "reference": "GPU FP64 instance"}
     rl = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))  # noqa
-    for name in ("fp32", "fp32_unsplit", "fp32_dirsplit", "fp16"):
+    for name in ("fp32", "fp32_unsplit", "fp32_dirsplit", "fp16", "fp16_dirsplit"):
         o = out[name]
         res[name] = {"shot_rel_l2_vs_fp64": rl(o["shot"], ref["shot"]),
                      "snapshot_it1000_rel_l2_vs_fp64": rl(o["snap"], ref["snap"]),
@@ -254,6 +254,43 @@ def strip_bench(args):
     print(json.dumps(line), flush=True)
 
 
+STRIP_PROFILES = {
+    "none": (0.0,) * 6,
+    "uniform_0.002": (0.002,) * 6, "uniform_0.005": (0.005,) * 6, "uniform_0.01": (0.01,) * 6,
+    "uniform_0.02": (0.02,) * 6,
+    "outer1": (0.02, 0.0, 0.0, 0.0, 0.0, 0.0), "outer2": (0.03, 0.01, 0.0, 0.0, 0.0, 0.0),
+    "outer3": (0.05, 0.02, 0.01, 0.0, 0.0, 0.0), "ramp": (0.03, 0.025, 0.02, 0.015, 0.01, 0.005),
+    "inner": (0.0, 0.0, 0.0, 0.005, 0.01, 0.01),
+}
+
+
+def strip_profiles(args):
+    """Reflected energy (dB, against the extended-domain FP64 run) of the 6-cell strip with fixed
+    damping-head profiles (strip.profile_params: factor per depth, 1 = outermost cell) on top of
+    its PML ramp, beside the 20-cell PML, on seeded 128^3 models (nt 800, every 20th step)."""
+    from a3dfd import models, strip
+    n, nt, every = 128, 800, 20
+    rows = []
+    for seed in (10000, 10001, 10002, 90000, 90001, 90002):
+        m = models.strip_model(seed, n)
+        src = models.strip_source(seed, n)
+        _, ref, _ = strip.run_case(m, src, nt, every, "extended", margin=64)
+        _, sp, _ = strip.run_case(m, src, nt, every, "pml")
+        row = {"seed": seed, "pml20": round(float(strip.energy_db(sp, ref)), 2)}
+        for name, gains in STRIP_PROFILES.items():
+            _, ss, _ = strip.run_case(m, src, nt, every, "strip", strip.profile_params(gains), amp=10.0)
+            row[name] = round(float(strip.energy_db(ss, ref)), 2)
+        rows.append(row)
+        log(f"bench: strip profiles seed {seed}: {row}")
+    line = {"metric": "reflected energy (dB) of the 6-cell strip with fixed damping-head profiles",
+            "unit": "dB", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+            "config": {"workload": f"vti_{n}^3_nt{nt}_strip_profiles", "strip_amp": 10.0,
+                       "profiles": {k: list(v) for k, v in STRIP_PROFILES.items()},
+                       "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64, every 20th step"},
+            "rows": rows}
+    print(json.dumps(line), flush=True)
+
+
 # Algorithmic bytes per updated point (FP32): (velocity sweep or 2nd-order step, stress sweep with
 # split fields, stress sweep without them) -- reads + writes of each sweep; OP_BYTES_WZ with w
 # updated by the stress sweep (ISO: stress reads P one plane ahead and the old w, writes w).
@@ -350,6 +387,8 @@ def main():
                     help="every operator at its configuration's grid, both stress schemes; one JSON line each")
     ap.add_argument("--strip", action="store_true",
                     help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
+    ap.add_argument("--strip-profiles", action="store_true",
+                    help="reflected energy of the strip with fixed damping profiles on top of its PML ramp")
     ap.add_argument("--strip-n", type=int, default=256)
     ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()
@@ -371,6 +410,8 @@ def main():
         return batched(args)
     if args.strip:
         return strip_bench(args)
+    if args.strip_profiles:
+        return strip_profiles(args)
     if args.operators:
         return operators(args)
     t0 = time.time()

@@ -226,3 +226,21 @@ def test_oracle_strip_closed_network(oracle):
     assert np.array_equal(b, c) and np.array_equal(sb, sc)
     assert not np.array_equal(a, b)
     assert np.abs(a).max() > 0
+
+
+@pytest.mark.parametrize("gains", [(0.05, 0.02, 0.01, 0.0, 0.0, 0.0), (0.002,) * 6, (0.0, 0.0, 0.0, 0.005, 0.01, 0.01),
+                                   (0.03, 0.025, 0.02, 0.015, 0.01, 0.005)])
+def test_profile_network_gives_its_depth_profile(gains):
+    """strip.profile_params: the FP64 network's damping factor at every depth is the requested
+    one, whatever the 11 local-state and 3 model features are."""
+    from a3dfd import strip
+    p = strip.profile_params(gains)
+    rng = np.random.default_rng(5)
+    for d in range(1, 7):
+        X = np.zeros((200, mlp.NIN))
+        X[:, :11] = rng.uniform(-1, 1, (200, 11))
+        X[:, 11:14] = rng.uniform(0, 0.5, (200, 3))
+        X[:, 14] = d / 6.0
+        _, (_, _, _, g, _) = mlp.forward(p, X)
+        want = max(gains[d - 1], 1e-14)
+        assert np.allclose(g, want, rtol=2e-3, atol=0), (d, g.min(), g.max(), want)
