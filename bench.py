@@ -313,6 +313,8 @@ def operators(args):
     c4 = models.cfg4_model()
     c4.update(sx=301, sy=301, sz=1)
     runs += [("tti", c4, {}, s, 500) for s in (0, 1, 3)]
+    if args.ops:
+        runs = [r for r in runs if r[0] in args.ops.split(",")]
     for op, c, kw, scheme, nt in runs:
         keys = {"2nd": ("vel",), "iso": ("vel", "rho"), "vti": ("vel", "rho", "eps", "delta"),
                 "tti": ("vel", "eps", "delta", "theta", "phi")}[op]
@@ -387,6 +389,7 @@ def main():
                     help="every operator at its configuration's grid, both stress schemes; one JSON line each")
     ap.add_argument("--strip", action="store_true",
                     help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
+    ap.add_argument("--ops", type=str, default="", help="with --operators: comma list of 2nd,iso,vti,tti (default all)")
     ap.add_argument("--strip-profiles", action="store_true",
                     help="reflected energy of the strip with fixed damping profiles on top of its PML ramp")
     ap.add_argument("--strip-n", type=int, default=256)
