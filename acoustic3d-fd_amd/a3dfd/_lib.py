@@ -128,6 +128,7 @@ def load(path=None):
     lib.a3d_strip_info.argtypes = [vp, POINTER(c_int)]
     lib.a3d_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_set_strip_precision.argtypes = [vp, c_int]
+    lib.a3d_set_strip_head.argtypes = [vp, c_int]
     lib.a3d_set_zchunk.argtypes = [vp, c_int]
     lib.a3d_set_w_sweep.argtypes = [vp, c_int]
     lib.a3d_get_w_sweep.argtypes = [vp, POINTER(ShotDesc), POINTER(c_int)]
@@ -141,6 +142,7 @@ def load(path=None):
     lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
     lib.a3d_mlp_train.argtypes = [vp, fp, fp, c_int, c_int, c_int, c_float, c_float, ctypes.c_uint, POINTER(c_double)]
     for name in ("a3d_set_strip_net", "a3d_strip_info", "a3d_set_strip_profile", "a3d_set_strip_precision",
+                 "a3d_set_strip_head",
                  "a3d_set_zchunk", "a3d_set_w_sweep", "a3d_get_w_sweep", "a3d_strip_collect", "a3d_mlp_nparams", "a3d_mlp_create",
                  "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
                  "a3d_mlp_loss_grad", "a3d_mlp_train"):
@@ -279,6 +281,11 @@ class Modeller:
     def set_strip_profile(self, amp):
         """PML amplitude of the learned strip's 6-cell ramp (default 10)."""
         _check(self.lib, self.lib.a3d_set_strip_profile(self._h, amp))
+
+    def set_strip_head(self, mode):
+        """What the strip's damping factors act on: 0 every field (sponge), 1 the split
+        components along the outward normal axis and the normal velocity (PML-like)."""
+        _check(self.lib, self.lib.a3d_set_strip_head(self._h, int(mode)))
 
     def set_strip_precision(self, mfma):
         """1: strip network on the matrix cores (binary16 operands, FP32 sums); 0: FP32."""

@@ -152,6 +152,14 @@ def load_params(path=NET_PATH):
         return f["params"].astype(np.float32), float(f["amp"]) if "amp" in f else None
 
 
+def load_head(path=NET_PATH):
+    """The head mode the committed network was trained for (0 when absent or not recorded)."""
+    if not os.path.exists(path):
+        return 0
+    with np.load(path, allow_pickle=False) as f:
+        return int(f["head"]) if "head" in f else 0
+
+
 def extended(m, src, margin):
     """Model and source of the extended-domain run: every face padded by `margin` cells
     (edge replication, as the boundary padding itself does)."""
@@ -162,11 +170,12 @@ def extended(m, src, margin):
     return me, se
 
 
-def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None, scheme=0, device=0):
+def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None, scheme=0, device=0, head=0):
     """One shot under a boundary: 'strip' (learned, needs params), 'strip0' (the 6-cell strip
     with its damping head shut: its own PML only), 'pml' (20-cell PML), 'extended' (FP64, model
     padded by margin, 40-cell PML). amp: the strip's PML amplitude (None: the library default).
     scheme: the stress update of the non-extended runs (0 reference, 1 split fields in PML cells only).
+    head: what the damping factors act on (0 every field, 1 the normal-axis PML components).
     Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original interior."""
     if boundary == "extended":
         me, se = extended(m, src, margin)
@@ -183,6 +192,7 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
         if boundary in ("strip", "strip0"):
             if amp is not None:
                 mdl.set_strip_profile(amp)
+            mdl.set_strip_head(head)
             mdl.set_strip_net(params if boundary == "strip" else closed_params())
         return mdl.run(nsnap_max=nt // snap_every)
 
@@ -202,9 +212,35 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
 
 
-def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, device=0, log=None):
+ADAPT_GAINS = (0.0, 0.001, 0.002, 0.004, 0.007, 0.01, 0.015, 0.02)
+
+
+def best_uniform_gain(seed, n=128, nt=800, snap_every=20, amp=5.0, head=1, gains=ADAPT_GAINS, margin=64, device=0):
+    """The per-model optimum of a uniform damping factor on top of the strip's ramp: reflected
+    energy of the strip with profile_params((g,) * 6) for every g in gains, against the
+    extended-domain FP64 run of seeded model `seed`. Returns (g*, {g: dB})."""
+    m = models.strip_model(seed, n)
+    src = models.strip_source(seed, n)
+    _, ref, _ = run_case(m, src, nt, snap_every, "extended", margin=margin, device=device)
+    tab = {}
+    for g in gains:
+        _, snaps, _ = run_case(m, src, nt, snap_every, "strip", profile_params((g,) * 6), amp=amp, head=head,
+                               device=device)
+        tab[g] = round(float(energy_db(snaps, ref)), 2)
+    return min(tab, key=tab.get), tab
+
+
+def adaptive_targets(X, gstar):
+    """Regression targets that make the damping head give g* (the head's prediction is
+    (-x_0 g_P, -x_7 g_Q), x_0 and x_7 the cell's own P/s and Q/s)."""
+    return (-X[:, [0, 7]] * np.float32(gstar)).astype(np.float32)
+
+
+def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, device=0, log=None, head=0,
+          extra_amps=()):
     """Reflected energy (dB) and shot-record rel. L2 against the extended FP64 run for the
-    learned strip, the strip without network and the 20-cell PML, on held-out models."""
+    learned strip, the strip without network and the 20-cell PML, on held-out models; also
+    the strip without network at each ramp amplitude in extra_amps (key strip0_amp<a>)."""
     rows = []
     for seed in seeds:
         m = models.strip_model(seed, n)
@@ -212,16 +248,18 @@ def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, devi
         t0 = time.time()
         ref_shot, ref_snaps, _ = run_case(m, src, nt, snap_every, "extended", margin=margin, device=device)
         row = {"seed": seed, "fm": round(src["fm"], 2), "src": [src["sx"], src["sy"], src["sz"]]}
-        for b in ("pml", "strip0", "strip"):
+        cases = [("pml", "pml", amp), ("strip0", "strip0", amp), ("strip", "strip", amp)]
+        cases += [(f"strip0_amp{a:g}", "strip0", a) for a in extra_amps]
+        for key, b, a in cases:
             if b == "strip" and params is None:
                 continue
             try:
-                shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, amp=amp, device=device)
+                shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, amp=a, device=device, head=head)
             except A3DError as e:   # an unstable boundary is judged as failed
-                row[b] = {"reflected_db": float("inf"), "shot_rel_l2": float("inf"), "loop_ms": None, "error": str(e)}
+                row[key] = {"reflected_db": float("inf"), "shot_rel_l2": float("inf"), "loop_ms": None, "error": str(e)}
                 continue
-            row[b] = {"reflected_db": round(float(energy_db(snaps, ref_snaps)), 2),
-                      "shot_rel_l2": rel_l2(shot, ref_shot), "loop_ms": round(ms, 2)}
+            row[key] = {"reflected_db": round(float(energy_db(snaps, ref_snaps)), 2),
+                        "shot_rel_l2": rel_l2(shot, ref_shot), "loop_ms": round(ms, 2)}
         rows.append(row)
         if log:
             log(f"strip judge: seed {seed} ({time.time() - t0:.1f} s): {row}")

@@ -228,6 +228,7 @@ struct a3d_ctx {
     float2* strip_delta = nullptr;
     float strip_amp = 10.0f;    // PML amplitude of the strip's 6-cell ramp
     int strip_mfma = 1;         // network on the matrix cores (binary16 operands) or FP32 per lane
+    int strip_head = 0;         // 0: damping of every field (sponge); 1: normal-axis PML components
     int zchunk = 0;             // 1st-order two-sweep step in z chunks of this many planes (0: whole grid)
     // which sweep updates w (ISO, VTI): requested mode (-1 tuner, 0 velocity sweep,
     // 1 stress sweep) and the tuner's choice for mode -1
@@ -585,6 +586,7 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     a.cdt[1] = d->dt / d->dy;
     a.cdt[2] = d->dt / d->dz;
     a.dirsplit = d->scheme == 3;
+    a.normal = c->strip_head;
     return a;
 }
 
@@ -1362,6 +1364,12 @@ int a3d_set_zchunk(a3d_ctx* c, int planes) {
 int a3d_set_strip_precision(a3d_ctx* c, int mfma) {
     if (!c || mfma < 0 || mfma > 1) return fail(A3D_EINVAL, "strip precision must be 0 or 1");
     c->strip_mfma = mfma;
+    return A3D_OK;
+}
+
+int a3d_set_strip_head(a3d_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 1) return fail(A3D_EINVAL, "strip head mode must be 0 or 1");
+    c->strip_head = mode;
     return A3D_OK;
 }
 

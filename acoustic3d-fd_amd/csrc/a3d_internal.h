@@ -224,6 +224,7 @@ struct StripArgs {
     const float *P, *Q, *u, *v, *w, *vp, *rho, *eps, *del;
     float cdt[3];   // dt/dx, dt/dy, dt/dz
     int dirsplit;   // the stress sweep keeps split fields only for a cell's damped directions
+    int normal;     // head mode 1: damp the normal-axis split components and velocity only
 };
 
 // delta[k] = the damping factors (g_P, g_Q) of every strip cell (after the stress sweep);

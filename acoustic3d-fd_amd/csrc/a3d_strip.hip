@@ -7,6 +7,9 @@
 //   features  f = local state / s, s = max |local state| (amplitude-free)
 //   factors   (g_P, g_Q) = STRIP_GMAX * sigmoid(net(f)) in (0, STRIP_GMAX)
 //   update    P, px, py, pz, u, v *= 1 - g_P;  Q, qx, qy, qz, w *= 1 - g_Q
+//             (head mode 1: only the split components along the cell's normal axis a
+//             and the normal velocity, d = g p_a, p_a -= d, P -= d, as a local
+//             increase of the PML's damping along a)
 // (the velocities with their stresses, as a sponge damps every field: damping
 // the stresses alone would leave a net inflow into the strip and a drifting
 // static field). A factor in (0, 1) only removes energy, so the strip stays
@@ -220,10 +223,26 @@ __global__ __launch_bounds__(SB) void k_strip_apply(const StripArgs a, const flo
     const int k = blockIdx.x * SB + threadIdx.x;
     if (k >= a.n) return;
     const int id = a.ids[k];
+    const float2 d = delta[k];
+    if (a.normal) {
+        // the split components along the outward normal axis and the normal velocity:
+        // a local increase of the PML's damping along that axis
+        const int meta = a.meta[k], axis = meta & 3;
+        if (!((meta >> (8 + axis)) & 1)) return;   // stress factor 1 along the normal
+        float* pa = axis == 0 ? px : (axis == 1 ? py : pz);
+        float* qa = axis == 0 ? qx : (axis == 1 ? qy : qz);
+        float* va = axis == 0 ? u : (axis == 1 ? v : w);
+        const float dp = d.x * pa[id], dq = d.y * qa[id];
+        pa[id] = pa[id] - dp;
+        P[id] = P[id] - dp;
+        qa[id] = qa[id] - dq;
+        Q[id] = Q[id] - dq;
+        va[id] *= 1.0f - (axis == 2 ? d.y : d.x);
+        return;
+    }
     // split fields of the directions whose stress factor is 1 stay zero when the stress
     // sweep splits only the damped directions: scaling them is skipped
     const int dirs = a.dirsplit ? (a.meta[k] >> 8) & 7 : 7;
-    const float2 d = delta[k];
     const float fp = 1.0f - d.x, fq = 1.0f - d.y;
     P[id] *= fp;
     Q[id] *= fq;

@@ -50,8 +50,13 @@ def main():
                     help="PML amplitude of the strip's 6-cell ramp (10: the reference's constant)")
     ap.add_argument("--tune-amp", action="store_true",
                     help="pick the amplitude with the best worst-case reflection over 8 training models")
+    ap.add_argument("--adaptive", action="store_true",
+                    help="normal head (PML-like damping) on a weaker ramp: each training model's best uniform "
+                         "extra damping (scanned against its extended-domain run) is the network's target")
     ap.add_argument("--out", default=strip.NET_PATH)
     args = ap.parse_args()
+    if args.adaptive:
+        return adaptive(args)
 
     seeds = [10000 + k for k in range(args.models)]
     t0 = time.time()
@@ -85,6 +90,41 @@ def main():
                "loss_first_epoch": losses[0][0], "loss_last_epoch": losses[-1][-1],
                "target_mean_square": float((T ** 2).sum(1).mean()), "judge": rows, "choice": choice}
     strip.save(params, args.out, amp=np.float32(amp), loss=np.concatenate(losses), summary=json.dumps(summary))
+    print(json.dumps(summary), flush=True)
+
+
+def adaptive(args):
+    """Normal head (damping of the normal-axis PML components: a local increase of the PML's own
+    damping) on the strip's ramp at amplitude --amp. Per training model, the uniform damping
+    factor g* with the lowest reflected energy against the model's extended-domain FP64 run
+    (strip.best_uniform_gain) labels that model's teacher-forced feature samples; the network,
+    trained on the GPU, learns g* from the local wavefield and model features, so it can damp
+    more where the ramp alone lets the wall's reflection through and less where the ramp's own
+    onset reflects. Judged on held-out models against the ramp alone at --amp and at 10."""
+    seeds = [10000 + k for k in range(args.models)]
+    Xs, Ts, labels = [], [], []
+    t0 = time.time()
+    for seed in seeds:
+        gstar, tab = strip.best_uniform_gain(seed, n=args.n, nt=args.judge_nt, amp=args.amp, head=1)
+        X, _, _ = strip.collect([seed], n=args.n, nt=args.nt, every=args.every, keep=args.keep, amp=args.amp)
+        Xs.append(X)
+        Ts.append(strip.adaptive_targets(X, gstar))
+        labels.append({"seed": seed, "g_star": gstar, "db": {str(k): v for k, v in tab.items()}, "samples": len(X)})
+        log(f"train_strip: seed {seed}: g* {gstar} {tab} ({len(X)} samples, {time.time() - t0:.1f} s)")
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    g_mean = float(np.mean([r["g_star"] for r in labels]))
+    r = min(max(g_mean, 1e-4) / 0.25, 0.5)   # the head's factor is 0.25 sigmoid(y)
+    init = strip.init_params(0, out_bias=float(np.log(r / (1 - r))))
+    params, hist = strip.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, init=init)
+    log(f"train_strip: {len(X)} samples, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    rows = strip.judge([90000 + k for k in range(args.heldout)], params, n=args.n, nt=args.judge_nt, amp=args.amp,
+                       head=1, extra_amps=(10.0,), log=log)
+    choice = strip.choose(rows)
+    summary = {"mode": "adaptive", "head": 1, "samples": int(len(X)), "models": args.models, "n": args.n,
+               "nt": args.nt, "amp": args.amp, "labels": labels, "loss_first_epoch": hist[0],
+               "loss_last_epoch": hist[-1], "judge": rows, "choice": choice}
+    strip.save(params, args.out, amp=np.float32(args.amp), head=np.int32(1), loss=np.asarray(hist),
+               summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
 
 

@@ -267,9 +267,12 @@ STRIP_PROFILES = {
 def strip_profiles(args):
     """Reflected energy (dB, against the extended-domain FP64 run) of the 6-cell strip with fixed
     damping-head profiles (strip.profile_params: factor per depth, 1 = outermost cell) on top of
-    its PML ramp, beside the 20-cell PML, on seeded 128^3 models (nt 800, every 20th step)."""
+    its PML ramp, beside the 20-cell PML, on seeded 128^3 models (nt 800, every 20th step), for
+    the head mode --strip-head (0 every field, 1 the normal-axis PML components); also the ramp
+    alone at amplitudes 5, 15, 20, 30 (10 is the default)."""
     from a3dfd import models, strip
     n, nt, every = 128, 800, 20
+    head = args.strip_head
     rows = []
     for seed in (10000, 10001, 10002, 90000, 90001, 90002):
         m = models.strip_model(seed, n)
@@ -278,13 +281,16 @@ def strip_profiles(args):
         _, sp, _ = strip.run_case(m, src, nt, every, "pml")
         row = {"seed": seed, "pml20": round(float(strip.energy_db(sp, ref)), 2)}
         for name, gains in STRIP_PROFILES.items():
-            _, ss, _ = strip.run_case(m, src, nt, every, "strip", strip.profile_params(gains), amp=10.0)
+            _, ss, _ = strip.run_case(m, src, nt, every, "strip", strip.profile_params(gains), amp=10.0, head=head)
             row[name] = round(float(strip.energy_db(ss, ref)), 2)
+        for amp in (5.0, 15.0, 20.0, 30.0):
+            _, ss, _ = strip.run_case(m, src, nt, every, "strip0", amp=amp)
+            row[f"ramp_amp{amp:g}"] = round(float(strip.energy_db(ss, ref)), 2)
         rows.append(row)
         log(f"bench: strip profiles seed {seed}: {row}")
     line = {"metric": "reflected energy (dB) of the 6-cell strip with fixed damping-head profiles",
             "unit": "dB", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
-            "config": {"workload": f"vti_{n}^3_nt{nt}_strip_profiles", "strip_amp": 10.0,
+            "config": {"workload": f"vti_{n}^3_nt{nt}_strip_profiles", "strip_amp": 10.0, "head": head,
                        "profiles": {k: list(v) for k, v in STRIP_PROFILES.items()},
                        "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64, every 20th step"},
             "rows": rows}
@@ -392,6 +398,8 @@ def main():
     ap.add_argument("--ops", type=str, default="", help="with --operators: comma list of 2nd,iso,vti,tti (default all)")
     ap.add_argument("--strip-profiles", action="store_true",
                     help="reflected energy of the strip with fixed damping profiles on top of its PML ramp")
+    ap.add_argument("--strip-head", type=int, default=0, choices=(0, 1),
+                    help="with --strip-profiles: 0 damping of every field, 1 of the normal-axis PML components")
     ap.add_argument("--strip-n", type=int, default=256)
     ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()

@@ -267,6 +267,14 @@ int a3d_set_strip_precision(a3d_ctx* ctx, int mfma);
  * d0 = amp * v * ln(1e5) / (2 * 6 * d) (the reference's formula with 6 cells in place
  * of npml and amp in place of 10; default amp = 10). */
 int a3d_set_strip_profile(a3d_ctx* ctx, float amp);
+/* What the damping factors act on. 0 (default, "sponge"): P, its split components, u
+ * and v are scaled by 1 - g_P; Q, its split components and w by 1 - g_Q. 1
+ * ("normal"): only the split components along the cell's outward normal axis a
+ * (nearest face) and the normal velocity: d = g_P * p_a; p_a -= d; P -= d (Q, q_a
+ * with g_Q likewise), and the normal velocity (u, v for a = x, y with g_P; w with g_Q)
+ * scaled by 1 - g -- a local increase of the PML's own damping along a. Cells whose
+ * stress PML factor along a is 1 are left alone. */
+int a3d_set_strip_head(a3d_ctx* ctx, int mode);
 /* Number of strip cells of the context's grid (0 before a network was set). */
 int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
 /* Teacher-forced training samples: runs shot *desc on `teacher` (same interior,

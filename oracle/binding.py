@@ -66,6 +66,8 @@ def load():
     lib.oracle_strip_apply.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float, c_float] + [fp] * 15
     lib.oracle_strip_apply.restype = c_int
     lib.oracle_set_strip_profile.argtypes = [c_float]
+    lib.oracle_set_strip_head.argtypes = [c_int]
+    lib.oracle_set_strip_head.restype = c_int
     lib.oracle_set_strip_half.argtypes = [c_int]
     lib.oracle_set_strip_half.restype = c_int
     lib.oracle_round_half.argtypes = [c_float]
@@ -166,3 +168,10 @@ def time_steps(shot, nsteps, vel=None, rho=None, eps=None, delta=None, theta=Non
     shape = (shot.ny, shot.nx, shot.nz)
     arrs = [_arr(a, shape) for a in (vel, rho, eps, delta, theta, phi)]
     return lib.oracle_time_steps(ctypes.byref(shot), *[_fp(a) for a in arrs], nsteps)
+
+
+def set_strip_head(mode):
+    """0: the strip's damping factors scale every field of a cell; 1: the split components
+    along the normal axis and the normal velocity."""
+    if load().oracle_set_strip_head(int(mode)) != 0:
+        raise ValueError("bad strip head mode")

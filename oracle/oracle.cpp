@@ -409,6 +409,7 @@ constexpr int kW1 = 0, kB1 = kW1 + kNh * kNin, kW2 = kB1 + kNh, kB2 = kW2 + kNh 
 std::vector<float> g_strip;   /* empty: plain PML */
 double g_strip_amp = 10.0;    /* PML amplitude of the strip's 6-cell ramp */
 bool g_strip_half = true;     /* network operands rounded to binary16 (matrix-core evaluation) */
+int g_strip_head = 0;         /* 0: damping of every field; 1: normal-axis split components and velocity */
 
 /* Nearest binary16 value (ties to even), as a float; finite inputs below 65504 in magnitude. */
 float round_half(float f) {
@@ -475,7 +476,8 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
     const long stride[3] = {nnz, pl, 1};   /* x, y, z */
     const int n3[3] = {nnx, nny, nnz};
     const float cdt[3] = {dt / dx, dt / dy, dt / dz};
-    struct Cell { long id; float gp, gq; };
+    struct Cell { long id; float gp, gq; int axis; bool normal_damped; };
+    const int nb = kStripCells + mm;   /* the strip grid's padding */
     std::vector<std::vector<Cell>> rows(nny);
 #pragma omp parallel for schedule(static)
     for (int iy = mm; iy < nny - mm; ++iy)
@@ -509,10 +511,28 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
                 x[14] = (float)best * (1.0f / kStripCells);
                 x[15] = 0.f;
                 strip_net(g_strip.data(), x, y);
-                rows[iy].push_back(Cell{id, kGmax / (1.0f + std::exp(-y[0])), kGmax / (1.0f + std::exp(-y[1]))});
+                /* stress PML factor along the normal differs from 1 (the ramp's tables) */
+                const bool nd = pp[axis] < nb || pp[axis] > n3[axis] - nb;
+                rows[iy].push_back(Cell{id, kGmax / (1.0f + std::exp(-y[0])), kGmax / (1.0f + std::exp(-y[1])), axis,
+                                        nd});
             }
     for (const auto& r : rows)
         for (const Cell& c : r) {
+            if (g_strip_head == 1) {
+                /* normal head: the split components along the normal axis and the normal
+                 * velocity (a local increase of the PML damping along that axis) */
+                if (!c.normal_damped) continue;
+                float* pa = c.axis == 0 ? px : (c.axis == 1 ? py : pz);
+                float* qa = c.axis == 0 ? qx : (c.axis == 1 ? qy : qz);
+                float* va = c.axis == 0 ? u : (c.axis == 1 ? v : w);
+                const float dp = c.gp * pa[c.id], dq = c.gq * qa[c.id];
+                pa[c.id] = pa[c.id] - dp;
+                P[c.id] = P[c.id] - dp;
+                qa[c.id] = qa[c.id] - dq;
+                Q[c.id] = Q[c.id] - dq;
+                va[c.id] *= 1.0f - (c.axis == 2 ? c.gq : c.gp);
+                continue;
+            }
             const float fp = 1.0f - c.gp, fq = 1.0f - c.gq;
             P[c.id] *= fp;
             px[c.id] *= fp;
@@ -821,6 +841,12 @@ int oracle_set_strip_half(int on) {
 }
 
 float oracle_round_half(float f) { return round_half(f); }
+
+int oracle_set_strip_head(int mode) {
+    if (mode < 0 || mode > 1) return -1;
+    g_strip_head = mode;
+    return 0;
+}
 
 int oracle_set_strip_profile(float amp) {
     if (!(amp >= 0.0f)) return -1;

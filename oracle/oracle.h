@@ -64,6 +64,10 @@ float oracle_round_half(float f);
 
 /* PML amplitude of the strip's 6-cell ramp while a strip network is set (default 10). */
 int oracle_set_strip_profile(float amp);
+/* What the damping factors act on: 0 (default) every field of the cell, 1 the split
+ * components along the cell's normal axis and the normal velocity (cells whose stress
+ * PML factor along the normal is 1 are left alone). */
+int oracle_set_strip_head(int mode);
 
 /* One strip correction on padded arrays (oracle layout: z fastest, then x, then y;
  * extents nnz, nnx, nny) with the network set above; -1 if none is set. */

@@ -116,8 +116,9 @@ def test_closed_network_equals_oracle_bitwise(oracle):
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("scheme,mfma", [(0, 1), (1, 1), (3, 1), (0, 0)])
-def test_strip_matches_oracle(oracle, scheme, mfma):
+@pytest.mark.parametrize("scheme,mfma,head", [(0, 1, 0), (1, 1, 0), (3, 1, 0), (0, 0, 0), (0, 1, 1), (1, 1, 1),
+                                              (3, 1, 1), (0, 0, 1)])
+def test_strip_matches_oracle(oracle, scheme, mfma, head):
     """The strip correction fused into the time loop against the oracle's restatement: the
     matrix-core (binary16 operand) network against the oracle's binary16 form, the FP32
     per-lane network against its FP32 form."""
@@ -129,6 +130,7 @@ def test_strip_matches_oracle(oracle, scheme, mfma):
     with Modeller(d) as mdl:
         mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
         mdl.set_strip_precision(mfma)
+        mdl.set_strip_head(head)
         mdl.set_strip_net(net)
         shot, snaps, _ = mdl.run(nsnap_max=8)
     plain, _, _ = _run(m, d)
@@ -136,14 +138,16 @@ def test_strip_matches_oracle(oracle, scheme, mfma):
                            mute_tt=0)
     oracle.set_strip_net(net)
     oracle.set_strip_half(mfma)
+    oracle.set_strip_head(head)
     try:
         rshot, rsnaps, _ = oracle.run(os_, 32, vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"], nsnap_max=8)
     finally:
         oracle.set_strip_net(None)
         oracle.set_strip_half(1)
+        oracle.set_strip_head(0)
     e_shot, e_snap = rel_l2(shot, rshot), rel_l2(snaps, rsnaps)
     effect = rel_l2(plain, rshot)
-    print(f"[gpu-strip] scheme {scheme} mfma {mfma}: shot rel L2 {e_shot:.2e}, snapshots {e_snap:.2e}; "
+    print(f"[gpu-strip] scheme {scheme} mfma {mfma} head {head}: shot rel L2 {e_shot:.2e}, snapshots {e_snap:.2e}; "
           f"network effect on the record {effect:.2e}")
     assert effect > 1e-3          # the correction matters
     assert e_shot <= 1e-5 and e_snap <= 1e-5

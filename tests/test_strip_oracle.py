@@ -244,3 +244,51 @@ def test_profile_network_gives_its_depth_profile(gains):
         _, (_, _, _, g, _) = mlp.forward(p, X)
         want = max(gains[d - 1], 1e-14)
         assert np.allclose(g, want, rtol=2e-3, atol=0), (d, g.min(), g.max(), want)
+
+
+def test_oracle_normal_head_damps_only_the_normal_pml_components(oracle):
+    """Head mode 1 with a network of known per-depth factors: in a strip cell whose stress
+    PML factor along its normal axis a differs from 1, p_a and q_a lose the fraction g (P and
+    Q lose the same amount), the normal velocity is scaled by 1 - g, and every other field is
+    untouched; all other cells are untouched."""
+    from a3dfd import strip
+    gains = (0.05, 0.02, 0.01, 0.004, 0.003, 0.002)
+    shape = (26, 28, 24)   # (nny, nnx, nnz): interior 6 x 8 x 4 plus nb = 10 per side
+    nb = CELLS + MM
+    f = _state(shape, 21)
+    before = {k: v.copy() for k, v in f.items()}
+    oracle.set_strip_net(strip.profile_params(gains))
+    oracle.set_strip_head(1)
+    try:
+        oracle.strip_apply(1e-3, 10.0, 10.0, 10.0, f)
+    finally:
+        oracle.set_strip_head(0)
+        oracle.set_strip_net(None)
+    depth, axis, side, is_strip = _owner(shape)
+    grids = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), np.arange(shape[2]), indexing="ij")
+    pos = {0: grids[1], 1: grids[0], 2: grids[2]}
+    ext = {0: shape[1], 1: shape[0], 2: shape[2]}
+    split = {0: ("px", "qx", "u"), 1: ("py", "qy", "v"), 2: ("pz", "qz", "w")}
+    n_damped = 0
+    for idx in zip(*np.nonzero(np.ones(shape, bool))):
+        a = int(axis[idx])
+        damped = bool(is_strip[idx]) and (pos[a][idx] < nb or pos[a][idx] > ext[a] - nb)
+        if not damped:
+            for k in before:
+                assert f[k][idx] == before[k][idx], (k, idx)
+            continue
+        n_damped += 1
+        g = gains[int(depth[idx]) - 1]
+        pa, qa, va = split[a]
+        for k in ("px", "py", "pz", "qx", "qy", "qz", "u", "v", "w"):
+            if k not in (pa, qa, va):
+                assert f[k][idx] == before[k][idx], (k, idx)
+        for name, tot in ((pa, "P"), (qa, "Q")):
+            b, n = float(before[name][idx]), float(f[name][idx])
+            assert abs(n - b * (1 - g)) <= 3e-3 * g * abs(b) + 1e-6 * abs(b), (name, idx, b, n, g)
+            # the total loses what the component loses
+            lost = b - n
+            assert abs((float(before[tot][idx]) - float(f[tot][idx])) - lost) <= 1e-6 * (abs(b) + abs(float(before[tot][idx]))), (tot, idx)
+        b, n = float(before[va][idx]), float(f[va][idx])
+        assert abs(n - b * (1 - g)) <= 3e-3 * g * abs(b) + 1e-6 * abs(b), (va, idx)
+    assert n_damped > 0.8 * int(is_strip.sum())
