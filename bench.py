@@ -480,6 +480,20 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get(f"scheme{args.scheme}" + ("_wz" if wz else ""), {}).get("stress_bytes_per_launch")
 
+    # error of this formulation against the GPU FP64 instance on the same shot, from the
+    # precision study (bench.py --precision-study) kept with the profiles
+    err64 = None
+    ppath = os.path.join(ROOT, "profiles", "precision_vti_400x400x300.json")
+    pkey = {(0, 0): "fp32", (0, 1): "fp32_unsplit", (0, 3): "fp32_dirsplit", (1, 1): "fp16", (1, 0): "fp16",
+            (1, 3): "fp16_dirsplit"}.get((args.storage, args.scheme))
+    if pkey and args.nt == 3000 and os.path.exists(ppath):
+        with open(ppath) as f:
+            r = json.load(f).get("result", {}).get(pkey)
+        if r:
+            err64 = {"shot_rel_l2": r["shot_rel_l2_vs_fp64"], "snapshot_it1000_rel_l2": r["snapshot_it1000_rel_l2_vs_fp64"],
+                     "snapshot_it2000_rel_l2": r["snapshot_it2000_rel_l2_vs_fp64"],
+                     "source": "profiles/precision_vti_400x400x300.json (bench.py --precision-study)"}
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (FP32 oracle) ...")
@@ -516,7 +530,8 @@ def main():
                                  "(3.9e-7 on this shot's record)",
                               2: "scheme 1 in one pass: bitwise equal to scheme 1",
                               3: "split PML fields for damped directions only: <=1e-5 rel. L2 of the reference "
-                                 "arithmetic (4.3e-7 on this shot's record)"}[args.scheme]},
+                                 "arithmetic (4.3e-7 on this shot's record)"}[args.scheme],
+                   "error_vs_fp64": err64},
         "roofline": {"bound": "hbm", "kernel": main_kernel, "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_point": round(a_str, 2),
