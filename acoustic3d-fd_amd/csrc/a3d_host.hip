@@ -60,7 +60,8 @@ enum Vol {
     V_U, V_V, V_W, V_P, V_Q, V_PX, V_PY, V_PZ, V_QX, V_QY, V_QZ,
     V_VV, V_PA, V_PB,
     V_P2, V_Q2, V_U2, V_V2, V_W2,   // second copies for the one-pass VTI step (scheme 2), allocated on first use
-    V_M4,                           // VTI, float model: (vp, rho, eps, delta) interleaved, built on first use
+    V_M4,                           // float model, built on first use: VTI (vp, rho, eps, delta) interleaved,
+                                    // TTI the rotation terms (cos, sin tilt, cos, sin azimuth) interleaved
     NVOL
 };
 
@@ -292,8 +293,9 @@ int check_desc_geometry(const a3d_shot_desc* d) {
     const double ld = (double)((d->nx + ring + TX - 1) / TX * TX);
     const double n = ld * (d->ny + ring) * (d->nz + ring);
     if (n > 2.0e9) return fail(A3D_EINVAL, "padded volume too large for 32-bit indexing");
-    // buffer loads address a plane with 31-bit byte offsets (8-byte elements at most)
-    if (ld * (d->ny + ring) * 8.0 >= 2147483648.0) return fail(A3D_EINVAL, "padded x-y plane too large");
+    // buffer loads address a plane with 31-bit byte offsets (16-byte elements at most: the
+    // interleaved model and rotation terms)
+    if (ld * (d->ny + ring) * 16.0 >= 2147483648.0) return fail(A3D_EINVAL, "padded x-y plane too large");
     return A3D_OK;
 }
 
@@ -529,18 +531,28 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
     // the stress sweep's (and the one-pass step's) interleaved model (VTI, float model),
     // built once per model
+    // TTI (float model): the four rotation terms interleaved, one 16-byte load per point in
+    // both sweeps
     sa.m4 = nullptr;
-    if (op == A3D_OP_VTI && prec != PREC_FP64) {
+    va.t4 = nullptr;
+    if ((op == A3D_OP_VTI || op == A3D_OP_TTI) && prec != PREC_FP64) {
         if (!c->vol[V_M4] && hipMalloc(&c->vol[V_M4], c->nvox * 16) != hipSuccess) {
             c->vol[V_M4] = nullptr;
             return fail(A3D_ENOMEM, "interleaved model: allocation failed");
         }
         if (!c->m4_ok) {
-            HIP_TRY(launch_pack4((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO], (const float*)c->vol[V_EPS],
-                                 (const float*)c->vol[V_DEL], c->vol[V_M4], c->nvox, c->st));
+            if (op == A3D_OP_VTI)
+                HIP_TRY(launch_pack4((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO],
+                                     (const float*)c->vol[V_EPS], (const float*)c->vol[V_DEL], c->vol[V_M4], c->nvox,
+                                     c->st));
+            else
+                HIP_TRY(launch_pack4((const float*)c->vol[V_CT], (const float*)c->vol[V_ST],
+                                     (const float*)c->vol[V_CP], (const float*)c->vol[V_SP], c->vol[V_M4], c->nvox,
+                                     c->st));
             c->m4_ok = true;
         }
         sa.m4 = c->vol[V_M4];
+        if (op == A3D_OP_TTI) va.t4 = c->vol[V_M4];
     }
     va.s.s0 = s0;
     // The time grid is the reference's float accumulation t = dt, t += dt and the gate

@@ -76,6 +76,7 @@ struct VelArgs {
     const void* Q;
     const void* rho;
     const void *ct, *st, *cp, *sp;   // TTI: cos/sin of tilt and azimuth, per point
+    const void* t4;            // TTI, float model: (cos, sin tilt, cos, sin azimuth) interleaved, or null
     void* u;
     void* v;
     void* w;
@@ -109,7 +110,8 @@ struct StressArgs {
     Scales sc;
     int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
     Src s;                     // wz: this step's source, added to P / Q as the velocity sweep reads them
-    const void* m4;            // VTI, float model: (vp, rho, eps, delta) interleaved per point, or null
+    const void* m4;            // VTI, float model: (vp, rho, eps, delta) interleaved per point, or null;
+                               // TTI, float model: (cos, sin tilt, cos, sin azimuth)
 };
 
 // One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out

@@ -280,6 +280,9 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const M* St = (const M*)a.st;
     const M* Cp = (const M*)a.cp;
     const M* Sp = (const M*)a.sp;
+    // TTI with a float model: the four rotation terms as one 16-byte load per point
+    constexpr bool PTRIG = TTI && sizeof(M) == 4;
+    const f32x4* T4 = (const f32x4*)a.t4;
     const C* T = (const C*)a.tab;
 
     auto ldP = [&](int zo, int o) {
@@ -311,7 +314,13 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         C w0 = 0;
         if constexpr (!WZ) w0 = ldw<NT>(Wv, zo, colc, sc.s[F_W]);
         if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
-        if constexpr (TTI) {
+        if constexpr (PTRIG) {
+            const f32x4 t4 = bld4<NT>(T4, zo, colc);
+            ct = t4.x;
+            st = t4.y;
+            cp = t4.z;
+            sp = t4.w;
+        } else if constexpr (TTI) {
             ct = (C)bld<NT>(Ct, zo, colc);
             st = (C)bld<NT>(St, zo, colc);
             cp = (C)bld<NT>(Cp, zo, colc);
@@ -457,6 +466,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     // (one stream instead of four; the host always provides it); WZ keeps the
     // planes z..z+3 in mq (fr is the density FIFO otherwise)
     constexpr bool PACKED = VTI && sizeof(M) == 4;
+    // TTI with a float model: the four rotation terms as one 16-byte load per point
+    constexpr bool PTRIG = TTI && sizeof(M) == 4;
     const f32x4* m4 = (const f32x4*)a.m4;
     f32x4 mq[4];
     const S* Fo = ISO ? (const S*)Po : (const S*)Qo;
@@ -557,7 +568,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 d0 = (C)bld<NT>(del, zo, colc);
             }
         }
-        if constexpr (TTI) {
+        if constexpr (PTRIG) {
+            const f32x4 t4 = bld4<NT>(m4, zo, colc);
+            ct = t4.x;
+            st = t4.y;
+            cp = t4.z;
+            sp = t4.w;
+        } else if constexpr (TTI) {
             ct = (C)bld<NT>(Ct, zo, colc);
             st = (C)bld<NT>(St, zo, colc);
             cp = (C)bld<NT>(Cp, zo, colc);
