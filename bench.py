@@ -215,6 +215,7 @@ def strip_bench(args):
     m = models.strip_model(95000, n)
     src = models.strip_source(95000, n)
     params, amp = strip.load_params()
+    head = strip.load_head()
     if params is None:
         log("bench: no trained strip network found; the strip runs with its damping head shut")
         params = strip.closed_params()
@@ -225,8 +226,8 @@ def strip_bench(args):
     for name, b, sch in (("pml20", "pml", 0), ("strip_closed", "strip0", 0), ("strip_learned", "strip", 0),
                          ("pml20_scheme1", "pml", 1), ("strip_learned_scheme1", "strip", 1),
                          ("pml20_scheme3", "pml", 3), ("strip_learned_scheme3", "strip", 3)):
-        shot, snaps, ms = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch)
-        _, _, ms2 = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch)   # second run: steady timing
+        shot, snaps, ms = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch, head=head)
+        _, _, ms2 = strip.run_case(m, src, nt, every, b, params, amp=amp, scheme=sch, head=head)   # steady timing
         out[name] = {"loop_ms": round(min(ms, ms2), 1), "shots_per_hour": round(3600e3 / min(ms, ms2), 1),
                      "reflected_db": round(float(strip.energy_db(snaps, ref_snaps)), 2),
                      "shot_rel_l2_vs_fp64": strip.rel_l2(shot, ref_shot)}

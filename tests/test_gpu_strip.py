@@ -191,7 +191,7 @@ def test_trained_network_gain_over_closed_head():
     _, ref, _ = strip.run_case(m, src, nt, every, "extended", margin=32)
     db = {}
     for b in ("strip0", "strip"):
-        _, snaps, _ = strip.run_case(m, src, nt, every, b, params, amp=amp)
+        _, snaps, _ = strip.run_case(m, src, nt, every, b, params, amp=amp, head=strip.load_head())
         db[b] = float(strip.energy_db(snaps, ref))
     print(f"\n[gpu-strip] held-out 64^3: strip without network {db['strip0']:.2f} dB, learned {db['strip']:.2f} dB, "
           f"gain {db['strip0'] - db['strip']:+.2f} dB")
