@@ -121,6 +121,10 @@ def load(path=None):
     lib.a3d_batch_run.argtypes = [vp, c_int, POINTER(fp), POINTER(c_double)]
     lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
     lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
+    lib.a3d_batch_set_zseg.argtypes = [vp, c_int, c_int]
+    lib.a3d_batch_get_zseg.argtypes = [vp, POINTER(c_int), POINTER(c_int)]
+    for name in ("a3d_batch_set_zseg", "a3d_batch_get_zseg"):
+        getattr(lib, name).restype = c_int
     for name in ("a3d_batch_create", "a3d_batch_destroy", "a3d_batch_set_model", "a3d_batch_tune", "a3d_batch_run",
                  "a3d_batch_info", "a3d_batch_set_graph_chunk"):
         getattr(lib, name).restype = c_int
@@ -440,7 +444,14 @@ class Batch:
         pts, n = c_longlong(), c_int()
         tr = (c_int * (2 * nw))()
         _check(self.lib, self.lib.a3d_batch_info(self._h, ctypes.byref(pts), ctypes.byref(n), tr))
-        return dict(updated_points=pts.value, nwin=n.value, tile_rows=[(tr[2 * w], tr[2 * w + 1]) for w in range(nw)])
+        zv, zs = c_int(), c_int()
+        _check(self.lib, self.lib.a3d_batch_get_zseg(self._h, ctypes.byref(zv), ctypes.byref(zs)))
+        return dict(updated_points=pts.value, nwin=n.value, tile_rows=[(tr[2 * w], tr[2 * w + 1]) for w in range(nw)],
+                    zseg=(zv.value, zs.value))
+
+    def set_zseg(self, vel, stress):
+        """z segments per tile column of every window, per sweep (1..64); tune() picks them."""
+        _check(self.lib, self.lib.a3d_batch_set_zseg(self._h, int(vel), int(stress)))
 
     def close(self):
         if self._h:

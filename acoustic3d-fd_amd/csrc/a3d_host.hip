@@ -1764,6 +1764,7 @@ struct a3d_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool model_set = false;
     std::vector<int> ty_vel, ty_str; // tile rows per window and sweep
+    int zs_vel = 1, zs_str = 1;      // z segments per tile column of every window, per sweep
     bool tuned = false;
     // state of the captured time loop, kept so later runs replay the same graph
     std::vector<BatchGroup> gv, gs;
@@ -1800,9 +1801,11 @@ int group_sources(BatchGroup& g, const std::vector<Src>& all, int nt, int nw) {
     return A3D_OK;
 }
 
-// Groups windows by tile rows and uploads their arguments and tile prefix sums.
+// Groups windows by tile rows and uploads their arguments and tile prefix sums; every
+// window's tile columns are cut into zs segments (one where a stress sweep updates w).
 template <class A>
-int build_groups(a3d_batch* b, const std::vector<A>& args, const std::vector<int>& tys, std::vector<BatchGroup>& out) {
+int build_groups(a3d_batch* b, const std::vector<A>& args, const std::vector<int>& tys, int zs, bool stress,
+                 std::vector<BatchGroup>& out) {
     for (int ty : {2, 4, 8}) {
         BatchGroup g;
         g.ty = ty;
@@ -1813,7 +1816,7 @@ int build_groups(a3d_batch* b, const std::vector<A>& args, const std::vector<int
         std::vector<int> base(1, 0);
         for (int w : g.wins) {
             A a = args[w];
-            a.l = make_launch(a.g, ty, 1);
+            a.l = make_launch(a.g, ty, stress && a.wz ? 1 : zs);
             ga.push_back(a);
             base.push_back(base.back() + a.l.ntiles);
         }
@@ -1878,7 +1881,8 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     };
     if (!replay) {
         if (mode == 2) drop_graph(b);
-        if ((rc = build_groups(b, va, b->ty_vel, gv)) || (rc = build_groups(b, sa, b->ty_str, gs)))
+        if ((rc = build_groups(b, va, b->ty_vel, b->zs_vel, false, gv)) ||
+            (rc = build_groups(b, sa, b->ty_str, b->zs_str, true, gs)))
             return cleanup(rc);
         for (auto& g : gv)
             if ((rc = group_sources(g, src_all, nt, nw))) return cleanup(rc);
@@ -2201,10 +2205,54 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
         b->ty_vel[w] = tv[cls[w]];
         b->ty_str[w] = ts[cls[w]];
     }
+    // then the z segments of every window's tile columns (all windows alike), per sweep,
+    // on the tuned tile rows: the sum of each sweep's group times over the timed steps
+    {
+        auto distinct = [&](const std::vector<int>& v) {
+            int n = 0;
+            for (int t : {2, 4, 8}) n += std::find(v.begin(), v.end(), t) != v.end() ? 1 : 0;
+            return n;
+        };
+        const int nv = distinct(b->ty_vel), ns = distinct(b->ty_str);
+        double bv = 1e30, bs = 1e30;
+        int zv = 1, zst = 1;
+        for (int zs : {1, 2, 4, 8, 12, 16}) {
+            if (rc) break;
+            b->zs_vel = b->zs_str = zs;
+            double gms[6] = {0, 0, 0, 0, 0, 0};
+            double ms = 0.0;
+            rc = batch_run(b, 1, nullptr, &ms, gms);
+            if (rc == A3D_EUNSTABLE) rc = A3D_OK;
+            if (rc) break;
+            double sv = 0.0, ss = 0.0;
+            for (int k = 0; k < nv; ++k) sv += gms[k];
+            for (int k = 0; k < ns; ++k) ss += gms[nv + k];
+            if (sv < bv) bv = sv, zv = zs;
+            if (ss < bs) bs = ss, zst = zs;
+        }
+        b->zs_vel = zv;
+        b->zs_str = zst;
+    }
     for (auto& d : b->wd) d.nt = saved_nt;
     b->d.nt = saved_nt;
     b->tuned = rc == A3D_OK;
     return rc;
+}
+
+int a3d_batch_set_zseg(a3d_batch* b, int zseg_vel, int zseg_stress) {
+    if (!b || zseg_vel < 1 || zseg_vel > 64 || zseg_stress < 1 || zseg_stress > 64)
+        return fail(A3D_EINVAL, "z segments must be in 1..64");
+    if (zseg_vel != b->zs_vel || zseg_stress != b->zs_str) drop_graph(b);
+    b->zs_vel = zseg_vel;
+    b->zs_str = zseg_stress;
+    return A3D_OK;
+}
+
+int a3d_batch_get_zseg(const a3d_batch* b, int* zseg_vel, int* zseg_stress) {
+    if (!b || !zseg_vel || !zseg_stress) return fail(A3D_EINVAL, "null argument");
+    *zseg_vel = b->zs_vel;
+    *zseg_stress = b->zs_str;
+    return A3D_OK;
 }
 
 int a3d_batch_set_graph_chunk(a3d_batch* b, int steps) {

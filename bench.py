@@ -196,7 +196,8 @@ def batched(args):
         "dtype": "f32", "data": "synthetic seeded TTI model (layered + noise, smooth eps/delta/tilt/azimuth)",
         "config": {"workload": f"tti_600x600x300_{len(wins)}win_nt{nt}", "npml": 40, "nt": nt, "scheme": args.scheme,
                    "windows": "uniform in [160..320]^2 x [200..300] cells, seed 4",
-                   "updated_points_per_step": info["updated_points"], "size_classes": ncls},
+                   "updated_points_per_step": info["updated_points"], "size_classes": ncls,
+                   "z_segments": list(info["zseg"])},
         "sequential": res[0], "batched_graph": res[2],
         "speedup": round(res[0]["loop_s"] / res[2]["loop_s"], 3),
     }

@@ -245,6 +245,11 @@ int a3d_batch_set_graph_chunk(a3d_batch* b, int steps);
 /* Updated points per step summed over windows; window count; tile rows per
  * window (2 ints each: velocity, stress), may be NULL. */
 int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile_rows_out);
+/* z segments per tile column of every window (1..64), per sweep; a stress sweep that
+ * updates w keeps one. a3d_batch_tune picks them after the tile rows; all choices give
+ * bitwise the same records. */
+int a3d_batch_set_zseg(a3d_batch* b, int zseg_vel, int zseg_stress);
+int a3d_batch_get_zseg(const a3d_batch* b, int* zseg_vel, int* zseg_stress);
 
 /* ----------------------------------------------------------------------- */
 /* Learned absorbing strip (no counterpart in the reference): VTI operator,   */

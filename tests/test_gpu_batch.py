@@ -36,10 +36,13 @@ def test_batch_modes_equal_single_runs_and_oracle(oracle, op):
         bat2, _ = b.run(mode=1)        # eager work between replays
         g3, _ = b.run(mode=2)
         info = b.info()
-    print(f"[gpu-batch] {op}: {ncls} size classes, tile rows {info['tile_rows']}")
+        b.set_zseg(3, 2)               # other z segments: same records
+        bz, _ = b.run(mode=1)
+        gz, _ = b.run(mode=2)
+    print(f"[gpu-batch] {op}: {ncls} size classes, tile rows {info['tile_rows']}, z segments {info['zseg']}")
     for w, win in enumerate(wins):
         ox, oy, oz, wx, wy, wz, sx, sy, sz = win
-        for other in (bat[w], g1[w], g2[w], bat2[w], g3[w]):
+        for other in (bat[w], g1[w], g2[w], bat2[w], g3[w], bz[w], gz[w]):
             assert np.array_equal(other, seq[w])
         sub = {k: np.ascontiguousarray(a[oy:oy + wy, ox:ox + wx, oz:oz + wz]) for k, a in m.items()}
         ref, _, _ = oracle.run(oracle.make_shot(op, wx, wy, wz, 10.0, 10.0, 10.0, sx, sy, sz, 70, 1e-3, 25.0, nb=8),
