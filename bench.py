@@ -263,18 +263,23 @@ STRIP_PROFILES = {
     "outer1": (0.02, 0.0, 0.0, 0.0, 0.0, 0.0), "outer2": (0.03, 0.01, 0.0, 0.0, 0.0, 0.0),
     "outer3": (0.05, 0.02, 0.01, 0.0, 0.0, 0.0), "ramp": (0.03, 0.025, 0.02, 0.015, 0.01, 0.005),
     "inner": (0.0, 0.0, 0.0, 0.005, 0.01, 0.01),
+    # shaped like the extra per-step damping that doubling a 5-amplitude ramp adds
+    # (1 - a2(2D)/a2(D), D = dt d0 (k/6)^2, capped below the head's 0.25), and fractions of it
+    "ramp_shaped": (0.2499, 0.2499, 0.2499, 0.22, 0.10, 0.027),
+    "ramp_shaped_half": (0.125, 0.125, 0.125, 0.11, 0.05, 0.0135),
+    "ramp_shaped_quarter": (0.0625, 0.0625, 0.0625, 0.055, 0.025, 0.00675),
 }
 
 
 def strip_profiles(args):
     """Reflected energy (dB, against the extended-domain FP64 run) of the 6-cell strip with fixed
     damping-head profiles (strip.profile_params: factor per depth, 1 = outermost cell) on top of
-    its PML ramp, beside the 20-cell PML, on seeded 128^3 models (nt 800, every 20th step), for
-    the head mode --strip-head (0 every field, 1 the normal-axis PML components); also the ramp
-    alone at amplitudes 5, 15, 20, 30 (10 is the default)."""
+    its PML ramp (amplitude --strip-amp, default 10), beside the 20-cell PML, on seeded 128^3
+    models (nt 800, every 20th step), for the head mode --strip-head (0 every field, 1 the
+    normal-axis PML components); also the ramp alone at amplitudes 5, 7, 10, 15, 20, 30."""
     from a3dfd import models, strip
     n, nt, every = 128, 800, 20
-    head = args.strip_head
+    head, base = args.strip_head, args.strip_amp
     rows = []
     for seed in (10000, 10001, 10002, 90000, 90001, 90002):
         m = models.strip_model(seed, n)
@@ -283,16 +288,16 @@ def strip_profiles(args):
         _, sp, _ = strip.run_case(m, src, nt, every, "pml")
         row = {"seed": seed, "pml20": round(float(strip.energy_db(sp, ref)), 2)}
         for name, gains in STRIP_PROFILES.items():
-            _, ss, _ = strip.run_case(m, src, nt, every, "strip", strip.profile_params(gains), amp=10.0, head=head)
+            _, ss, _ = strip.run_case(m, src, nt, every, "strip", strip.profile_params(gains), amp=base, head=head)
             row[name] = round(float(strip.energy_db(ss, ref)), 2)
-        for amp in (5.0, 15.0, 20.0, 30.0):
+        for amp in (5.0, 7.0, 10.0, 15.0, 20.0, 30.0):
             _, ss, _ = strip.run_case(m, src, nt, every, "strip0", amp=amp)
             row[f"ramp_amp{amp:g}"] = round(float(strip.energy_db(ss, ref)), 2)
         rows.append(row)
         log(f"bench: strip profiles seed {seed}: {row}")
     line = {"metric": "reflected energy (dB) of the 6-cell strip with fixed damping-head profiles",
             "unit": "dB", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
-            "config": {"workload": f"vti_{n}^3_nt{nt}_strip_profiles", "strip_amp": 10.0, "head": head,
+            "config": {"workload": f"vti_{n}^3_nt{nt}_strip_profiles", "strip_amp": base, "head": head,
                        "profiles": {k: list(v) for k, v in STRIP_PROFILES.items()},
                        "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64, every 20th step"},
             "rows": rows}
@@ -402,6 +407,8 @@ def main():
                     help="reflected energy of the strip with fixed damping profiles on top of its PML ramp")
     ap.add_argument("--strip-head", type=int, default=0, choices=(0, 1),
                     help="with --strip-profiles: 0 damping of every field, 1 of the normal-axis PML components")
+    ap.add_argument("--strip-amp", type=float, default=10.0,
+                    help="with --strip-profiles: PML amplitude of the ramp under the profiles")
     ap.add_argument("--strip-n", type=int, default=256)
     ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()
