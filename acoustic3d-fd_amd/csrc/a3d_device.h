@@ -48,6 +48,8 @@ __device__ __forceinline__ void bst(T* p, int zo, int o, T v) {
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, o * 8, 0, A);
     }
 }
+// Two floats as one 8-byte element (bld with an 8-byte element type).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 // Four floats at element o of the plane at offset zo (16-byte elements).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <int A = 0>

@@ -61,7 +61,7 @@ enum Vol {
     V_VV, V_PA, V_PB,
     V_P2, V_Q2, V_U2, V_V2, V_W2,   // second copies for the one-pass VTI step (scheme 2), allocated on first use
     V_M4,                           // float model, built on first use: VTI (vp, rho, eps, delta) interleaved,
-                                    // TTI the rotation terms (cos, sin tilt, cos, sin azimuth) interleaved
+                                    // TTI the rotation terms (cos, sin tilt, cos, sin azimuth), ISO (vp, rho)
     NVOL
 };
 
@@ -532,11 +532,11 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     // the stress sweep's (and the one-pass step's) interleaved model (VTI, float model),
     // built once per model
     // TTI (float model): the four rotation terms interleaved, one 16-byte load per point in
-    // both sweeps
+    // both sweeps; ISO (float model): (vp, rho), one 8-byte load in the stress sweep
     sa.m4 = nullptr;
     va.t4 = nullptr;
-    if ((op == A3D_OP_VTI || op == A3D_OP_TTI) && prec != PREC_FP64) {
-        if (!c->vol[V_M4] && hipMalloc(&c->vol[V_M4], c->nvox * 16) != hipSuccess) {
+    if (op != A3D_OP_2ND && prec != PREC_FP64) {
+        if (!c->vol[V_M4] && hipMalloc(&c->vol[V_M4], c->nvox * (op == A3D_OP_ISO ? 8 : 16)) != hipSuccess) {
             c->vol[V_M4] = nullptr;
             return fail(A3D_ENOMEM, "interleaved model: allocation failed");
         }
@@ -544,6 +544,9 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
             if (op == A3D_OP_VTI)
                 HIP_TRY(launch_pack4((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO],
                                      (const float*)c->vol[V_EPS], (const float*)c->vol[V_DEL], c->vol[V_M4], c->nvox,
+                                     c->st));
+            else if (op == A3D_OP_ISO)
+                HIP_TRY(launch_pack2((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO], c->vol[V_M4], c->nvox,
                                      c->st));
             else
                 HIP_TRY(launch_pack4((const float*)c->vol[V_CT], (const float*)c->vol[V_ST],

@@ -111,7 +111,7 @@ struct StressArgs {
     int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
     Src s;                     // wz: this step's source, added to P / Q as the velocity sweep reads them
     const void* m4;            // VTI, float model: (vp, rho, eps, delta) interleaved per point, or null;
-                               // TTI, float model: (cos, sin tilt, cos, sin azimuth)
+                               // TTI, float model: (cos, sin tilt, cos, sin azimuth); ISO: (vp, rho)
 };
 
 // One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out
@@ -189,6 +189,8 @@ hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int mod
 // (vp, rho, eps, delta) of a float model interleaved into one float4 per point.
 hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, const float* del, void* out, size_t n,
                         hipStream_t s);
+// (vp, rho) of a float model interleaved into one float2 per point.
+hipError_t launch_pack2(const float* vp, const float* rho, void* out, size_t n, hipStream_t s);
 
 // Interior crop of a padded wavefield into z-fastest float order ([ny][nx][nz]).
 // 2nd-order snapshots: dipole (s0,a0),(s1,a1) added and the sponge applied

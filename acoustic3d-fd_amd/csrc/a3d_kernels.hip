@@ -466,9 +466,12 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     // (one stream instead of four; the host always provides it); WZ keeps the
     // planes z..z+3 in mq (fr is the density FIFO otherwise)
     constexpr bool PACKED = VTI && sizeof(M) == 4;
-    // TTI with a float model: the four rotation terms as one 16-byte load per point
+    // TTI with a float model: the four rotation terms as one 16-byte load per point;
+    // ISO with a float model and w in the velocity sweep: (vp, rho) as one 8-byte load
     constexpr bool PTRIG = TTI && sizeof(M) == 4;
+    constexpr bool PACK2 = ISO && sizeof(M) == 4 && !WZ;
     const f32x4* m4 = (const f32x4*)a.m4;
+    const f32x2* m2 = (const f32x2*)a.m4;
     f32x4 mq[4];
     const S* Fo = ISO ? (const S*)Po : (const S*)Qo;
     const float sF = sc.s[ISO ? F_P : F_Q];
@@ -559,6 +562,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             r0 = (C)m.y;
             e0 = (C)m.z;
             d0 = (C)m.w;
+        } else if constexpr (PACK2) {
+            const f32x2 m = bld<NT>(m2, zo, colc);
+            c0 = (C)m.x;
+            r0 = (C)m.y;
         } else {
             c0 = (C)bld<NT>(vp, zo, colc);
             if constexpr (WZ) r0 = fr[0];
@@ -990,6 +997,12 @@ __global__ void k_pack4(const float* __restrict__ vp, const float* __restrict__ 
         out[i] = f32x4{vp[i], rho[i], eps[i], del[i]};
 }
 
+__global__ void k_pack2(const float* __restrict__ vp, const float* __restrict__ rho, f32x2* __restrict__ out,
+                        size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f32x2{vp[i], rho[i]};
+}
+
 __global__ void k_check_finite(const float* __restrict__ a, size_t n, int* flag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1286,6 +1299,12 @@ hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, con
                         hipStream_t s) {
     const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_pack4, dim3(grid), dim3(256), 0, s, vp, rho, eps, del, (f32x4*)out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack2(const float* vp, const float* rho, void* out, size_t n, hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_pack2, dim3(grid), dim3(256), 0, s, vp, rho, (f32x2*)out, n);
     return hipGetLastError();
 }
 
