@@ -86,3 +86,18 @@ def test_legacy_symbol_reports_missing_files(tmp_path, capfd):
                            c_float(10), c_float(10), c_float(0.001), c_int(10), c_float(0.001), c_float(20.0),
                            c_bool(False), c_bool(False), c_int(1), c_int(0))
     assert rc == 1   # reference tti:429-432
+
+
+def test_setters_reject_bad_arguments_without_touching_a_gpu():
+    """Argument checks of the newer setters run before any device call: a null handle or an
+    out-of-range value gives A3D_EINVAL (-1) and a message."""
+    from ctypes import c_int, c_void_p
+    from a3dfd import load
+    lib = load()
+    null = c_void_p()
+    assert lib.a3d_set_strip_head(null, 1) == -1
+    assert lib.a3d_batch_set_zseg(null, 2, 2) == -1
+    assert lib.a3d_batch_get_zseg(null, None, None) == -1
+    v, s = c_int(), c_int()
+    assert lib.a3d_batch_get_zseg(null, v, s) == -1
+    assert lib.a3d_last_error()
