@@ -122,8 +122,10 @@ def load(path=None):
     lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
     lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
     lib.a3d_batch_set_zseg.argtypes = [vp, c_int, c_int]
+    lib.a3d_batch_set_strip_net.argtypes = [vp, fp, c_int]
+    lib.a3d_batch_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_batch_get_zseg.argtypes = [vp, POINTER(c_int), POINTER(c_int)]
-    for name in ("a3d_batch_set_zseg", "a3d_batch_get_zseg"):
+    for name in ("a3d_batch_set_zseg", "a3d_batch_get_zseg", "a3d_batch_set_strip_net", "a3d_batch_set_strip_profile"):
         getattr(lib, name).restype = c_int
     for name in ("a3d_batch_create", "a3d_batch_destroy", "a3d_batch_set_model", "a3d_batch_tune", "a3d_batch_run",
                  "a3d_batch_info", "a3d_batch_set_graph_chunk"):
@@ -448,6 +450,16 @@ class Batch:
         _check(self.lib, self.lib.a3d_batch_get_zseg(self._h, ctypes.byref(zv), ctypes.byref(zs)))
         return dict(updated_points=pts.value, nwin=n.value, tile_rows=[(tr[2 * w], tr[2 * w + 1]) for w in range(nw)],
                     zseg=(zv.value, zs.value))
+
+    def set_strip_net(self, params, amp=None):
+        """The learned strip on every window (VTI, FP32, nb = STRIP_NB); None: the plain PML."""
+        if amp is not None:
+            _check(self.lib, self.lib.a3d_batch_set_strip_profile(self._h, float(amp)))
+        if params is None:
+            _check(self.lib, self.lib.a3d_batch_set_strip_net(self._h, None, 0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        _check(self.lib, self.lib.a3d_batch_set_strip_net(self._h, _fptr(p), p.size))
 
     def set_zseg(self, vel, stress):
         """z segments per tile column of every window, per sweep (1..64); tune() picks them."""

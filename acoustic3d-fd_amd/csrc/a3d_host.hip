@@ -605,15 +605,17 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     return a;
 }
 
-// Strip correction after a stress sweep (learned absorbing strip).
-int strip_step(a3d_ctx* c, const StripArgs& a) {
-    HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, c->st));
+// Strip correction after a stress sweep (learned absorbing strip), on stream st.
+int strip_step_on(a3d_ctx* c, const StripArgs& a, hipStream_t st) {
+    HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
     HIP_TRY(launch_strip_apply(a, c->strip_delta, (float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_PX],
                                (float*)c->vol[V_PY], (float*)c->vol[V_PZ], (float*)c->vol[V_QX], (float*)c->vol[V_QY],
                                (float*)c->vol[V_QZ], (float*)c->vol[V_U], (float*)c->vol[V_V], (float*)c->vol[V_W],
-                               c->st));
+                               st));
     return A3D_OK;
 }
+
+int strip_step(a3d_ctx* c, const StripArgs& a) { return strip_step_on(c, a, c->st); }
 
 template <class T>
 int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
@@ -1890,6 +1892,14 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
         for (auto& g : gv)
             if ((rc = group_sources(g, src_all, nt, nw))) return cleanup(rc);
     }
+    // windows with a learned strip: their strip passes follow each step's stress sweeps
+    std::vector<int> swin;
+    std::vector<StripArgs> sargs;
+    for (int w = 0; w < nw; ++w)
+        if (b->ctx[w]->strip_net) {
+            swin.push_back(w);
+            sargs.push_back(strip_args(b->ctx[w], &b->wd[w]));
+        }
     // per-group timing (used by the tuner): events around each group's launches
     std::vector<hipEvent_t> gev;
     const bool timed = group_ms != nullptr && mode != 2;
@@ -1908,6 +1918,10 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
                                         (int)g.wins.size(), g.lb, it, b->d.scheme == 3, b->st));
             if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
+        }
+        for (size_t k = 0; k < swin.size(); ++k) {
+            const int rc2 = strip_step_on(b->ctx[swin[k]], sargs[k], b->st);
+            if (rc2) return rc2;
         }
         return A3D_OK;
     };
@@ -2240,6 +2254,26 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
     b->d.nt = saved_nt;
     b->tuned = rc == A3D_OK;
     return rc;
+}
+
+int a3d_batch_set_strip_net(a3d_batch* b, const float* params, int n) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    drop_graph(b);
+    for (a3d_ctx* c : b->ctx) {
+        const int rc = a3d_set_strip_net(c, params, n);
+        if (rc) return rc;
+    }
+    return A3D_OK;
+}
+
+int a3d_batch_set_strip_profile(a3d_batch* b, float amp) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    drop_graph(b);
+    for (a3d_ctx* c : b->ctx) {
+        const int rc = a3d_set_strip_profile(c, amp);
+        if (rc) return rc;
+    }
+    return A3D_OK;
 }
 
 int a3d_batch_set_zseg(a3d_batch* b, int zseg_vel, int zseg_stress) {

@@ -256,6 +256,63 @@ def strip_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def batched_strip(args):
+    """Shots/hour of the learned-strip batched form against 20-cell-PML per-shot runs: --nwin
+    shots (default 8) on the 256^3 VTI study model (d = 10 m, dt = 0.8 ms, nt = --strip-nt),
+    sources at seeded positions, every window the full model. Runs (stress scheme 3 unless
+    noted): the strip batched with the time loop in hipGraphs (mode 2) and as per-shot runs
+    (mode 0); the 20-cell PML as per-shot runs under schemes 3 and 0 (reference formulation)
+    and batched in graphs."""
+    from a3dfd import STRIP_NB, Batch, Window, make_desc, models, strip
+    os.environ.setdefault("A3D_PROGRESS", "1")
+    n, nt = args.strip_n, args.strip_nt
+    nshot = 8 if args.nwin == 64 else min(args.nwin, 16)   # --nwin defaults to the TTI batch's 64
+    m = models.strip_model(95000, n)
+    fm = models.strip_source(95000, n)["fm"]
+    srcs = [models.strip_source(95000 + k, n) for k in range(nshot)]
+    wins = [Window(0, 0, 0, n, n, n, s["sx"], s["sy"], s["sz"]) for s in srcs]
+    params, amp = strip.load_params()
+    head = strip.load_head()
+    if params is None:
+        params = strip.closed_params()
+    if head != 0:
+        raise SystemExit("bench.py --batched-strip: batches run the strip's default head mode (0)")
+    model = dict(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+    res = {}
+
+    def run(name, nb, scheme, mode, net):
+        d = make_desc("vti", n, n, n, m["dx"], m["dy"], m["dz"], 1, 1, 1, nt, m["dt"], fm, nb=nb, scheme=scheme)
+        with Batch(d, wins) as b:
+            b.set_model(**model)
+            if net is not None:
+                b.set_strip_net(net, amp=amp)
+            b.tune()
+            b.set_graph_chunk(250)
+            _, ms = b.run(mode=mode, shots=False)
+        res[name] = {"loop_s": round(ms / 1e3, 2), "shots_per_hour": round(nshot * 3600e3 / ms, 1)}
+        log(f"bench: {name}: {res[name]}")
+
+    run("strip_batched_graph", STRIP_NB, 3, 2, params)
+    run("strip_per_shot", STRIP_NB, 3, 0, params)
+    run("pml20_per_shot", 20, 3, 0, None)
+    run("pml20_per_shot_scheme0", 20, 0, 0, None)
+    run("pml20_batched_graph", 20, 3, 2, None)
+    sph = lambda k: res[k]["shots_per_hour"]   # noqa: E731
+    line = {
+        "metric": "shots/hour, learned-strip batched form vs 20-cell-PML per-shot runs (VTI 256^3)",
+        "value": sph("strip_batched_graph"), "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic seeded VTI study model, sources at seeded positions",
+        "config": {"workload": f"vti_{n}^3_nt{nt}_{nshot}shots_strip_batched", "strip_amp": amp,
+                   "fm": round(fm, 2), "scheme": 3},
+        "runs": res,
+        "speedup_vs_pml20_per_shot": round(sph("strip_batched_graph") / sph("pml20_per_shot"), 3),
+        "speedup_vs_pml20_per_shot_reference_formulation": round(sph("strip_batched_graph")
+                                                                 / sph("pml20_per_shot_scheme0"), 3),
+        "speedup_vs_pml20_batched": round(sph("strip_batched_graph") / sph("pml20_batched_graph"), 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 STRIP_PROFILES = {
     "none": (0.0,) * 6,
     "uniform_0.002": (0.002,) * 6, "uniform_0.005": (0.005,) * 6, "uniform_0.01": (0.01,) * 6,
@@ -403,6 +460,8 @@ def main():
     ap.add_argument("--strip", action="store_true",
                     help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
     ap.add_argument("--ops", type=str, default="", help="with --operators: comma list of 2nd,iso,vti,tti (default all)")
+    ap.add_argument("--batched-strip", action="store_true",
+                    help="learned-strip batched form vs 20-cell-PML per-shot runs (shots/hour, VTI 256^3)")
     ap.add_argument("--strip-profiles", action="store_true",
                     help="reflected energy of the strip with fixed damping profiles on top of its PML ramp")
     ap.add_argument("--strip-head", type=int, default=0, choices=(0, 1),
@@ -432,6 +491,8 @@ def main():
         return strip_bench(args)
     if args.strip_profiles:
         return strip_profiles(args)
+    if args.batched_strip:
+        return batched_strip(args)
     if args.operators:
         return operators(args)
     t0 = time.time()

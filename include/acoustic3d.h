@@ -249,6 +249,13 @@ int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile
  * updates w keeps one. a3d_batch_tune picks them after the tile rows; all choices give
  * bitwise the same records. */
 int a3d_batch_set_zseg(a3d_batch* b, int zseg_vel, int zseg_stress);
+/* The learned absorbing strip on every window (VTI, FP32, nb = A3D_STRIP_NB): the
+ * same network (params, a3d_mlp_nparams() floats, or NULL for the plain PML) and ramp
+ * amplitude as a3d_set_strip_net / a3d_set_strip_profile give one context. The strip
+ * passes of all windows follow each step's batched stress sweeps (and are captured
+ * with them in mode 2); every mode gives bitwise the records of single runs. */
+int a3d_batch_set_strip_net(a3d_batch* b, const float* params, int n);
+int a3d_batch_set_strip_profile(a3d_batch* b, float amp);
 int a3d_batch_get_zseg(const a3d_batch* b, int* zseg_vel, int* zseg_stress);
 
 /* ----------------------------------------------------------------------- */

@@ -63,3 +63,45 @@ def test_chunked_graphs_equal_single_graph():
         again, _ = b.run(mode=2)
     for w in range(len(wins)):
         assert np.array_equal(chunked[w], one[w]) and np.array_equal(again[w], one[w])
+
+
+@pytest.mark.parametrize("scheme", [0, 3])
+def test_batch_with_learned_strip_equals_single_runs(oracle, scheme):
+    """The learned strip on every window of a VTI batch (nb = 10, a network with an open
+    damping head): batched launches and captured graphs give bitwise the sequential
+    per-window runs, and a window equals the oracle's strip restatement on its sub-model."""
+    from a3dfd import STRIP_NB, Batch, Window, make_desc
+    from a3dfd.models import small_model
+    NX, NY, NZ, nt = 64, 52, 30, 90
+    m = small_model("vti", NX, NY, NZ, seed=41)
+    wins = [(0, 0, 0, 30, 28, 26, 12, 10, 2), (20, 10, 0, 44, 40, 30, 20, 21, 2), (33, 0, 4, 31, 26, 22, 15, 12, 2)]
+    desc = make_desc("vti", NX, NY, NZ, 10.0, 10.0, 10.0, 1, 1, 1, nt, 1e-3, 25.0, nb=STRIP_NB, scheme=scheme)
+    from a3dfd import strip
+    rng = np.random.default_rng(5)
+    net = strip.init_params(5, out_bias=0.0)
+    net[-130:] += rng.normal(0, 0.2, 130).astype(np.float32)
+    with Batch(desc, [Window(*w) for w in wins]) as b:
+        b.set_model(**m)
+        b.set_strip_net(net, amp=10.0)
+        b.tune(steps=3)
+        seq, _ = b.run(mode=0)
+        bat, _ = b.run(mode=1)
+        g1, _ = b.run(mode=2)
+        g2, _ = b.run(mode=2)
+    for w in range(len(wins)):
+        assert np.abs(seq[w]).max() > 0
+        for other in (bat[w], g1[w], g2[w]):
+            assert np.array_equal(other, seq[w])
+    # the first window against the oracle's strip restatement on its sub-model (the
+    # matrix-core network against the oracle's binary16 form: within 1e-5)
+    ox, oy, oz, wx, wy, wz, sx, sy, sz = wins[0]
+    sub = {k: np.ascontiguousarray(a[oy:oy + wy, ox:ox + wx, oz:oz + wz]) for k, a in m.items()}
+    oracle.set_strip_net(net)
+    try:
+        ref, _, _ = oracle.run(oracle.make_shot("vti", wx, wy, wz, 10.0, 10.0, 10.0, sx, sy, sz, nt, 1e-3, 25.0,
+                                                nb=STRIP_NB), 32, **sub)
+    finally:
+        oracle.set_strip_net(None)
+    e = rel_l2(seq[0], ref)
+    print(f"\n[gpu-batch] strip scheme {scheme}: window 0 rel L2 vs oracle {e:.2e}")
+    assert e <= 1e-5
