@@ -123,9 +123,11 @@ def load(path=None):
     lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
     lib.a3d_batch_set_zseg.argtypes = [vp, c_int, c_int]
     lib.a3d_batch_set_strip_net.argtypes = [vp, fp, c_int]
+    lib.a3d_batch_set_w_sweep.argtypes = [vp, c_int]
     lib.a3d_batch_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_batch_get_zseg.argtypes = [vp, POINTER(c_int), POINTER(c_int)]
-    for name in ("a3d_batch_set_zseg", "a3d_batch_get_zseg", "a3d_batch_set_strip_net", "a3d_batch_set_strip_profile"):
+    for name in ("a3d_batch_set_zseg", "a3d_batch_get_zseg", "a3d_batch_set_strip_net", "a3d_batch_set_strip_profile",
+                 "a3d_batch_set_w_sweep"):
         getattr(lib, name).restype = c_int
     for name in ("a3d_batch_create", "a3d_batch_destroy", "a3d_batch_set_model", "a3d_batch_tune", "a3d_batch_run",
                  "a3d_batch_info", "a3d_batch_set_graph_chunk"):
@@ -460,6 +462,10 @@ class Batch:
             return
         p = np.ascontiguousarray(params, dtype=np.float32).ravel()
         _check(self.lib, self.lib.a3d_batch_set_strip_net(self._h, _fptr(p), p.size))
+
+    def set_w_sweep(self, mode):
+        """Batched modes (ISO, VTI): w updated by 0 the velocity sweep, 1 the stress sweep, -1 tuned."""
+        _check(self.lib, self.lib.a3d_batch_set_w_sweep(self._h, int(mode)))
 
     def set_zseg(self, vel, stress):
         """z segments per tile column of every window, per sweep (1..64); tune() picks them."""

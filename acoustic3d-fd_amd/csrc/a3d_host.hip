@@ -1770,6 +1770,8 @@ struct a3d_batch {
     bool model_set = false;
     std::vector<int> ty_vel, ty_str; // tile rows per window and sweep
     int zs_vel = 1, zs_str = 1;      // z segments per tile column of every window, per sweep
+    int wz_mode = -1;                // batched sweeps: w updated by 0 the velocity, 1 the stress
+    int wz_tuned = 0;                //   sweep, -1 the tuner's choice (wz_tuned) (ISO, VTI)
     bool tuned = false;
     // state of the captured time loop, kept so later runs replay the same graph
     std::vector<BatchGroup> gv, gs;
@@ -1870,6 +1872,10 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     }
     // the wavefield memsets of prepare_first ran on the windows' streams
     for (auto* c : b->ctx) HIP_TRY(hipStreamSynchronize(c->st));
+    // the batched sweeps' w mode, the same for every window
+    const bool wz = (op == A3D_OP_ISO || op == A3D_OP_VTI) && b->d.scheme != 2 &&
+                    (b->wz_mode == 1 || (b->wz_mode < 0 && b->wz_tuned));
+    for (int w = 0; w < nw; ++w) va[w].wz = sa[w].wz = wz ? 1 : 0;
     const bool replay = mode == 2 && !b->exec.empty();
     const bool progress = getenv("A3D_PROGRESS") != nullptr;
     std::vector<BatchGroup> gv_local, gs_local;
@@ -1891,6 +1897,9 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             return cleanup(rc);
         for (auto& g : gv)
             if ((rc = group_sources(g, src_all, nt, nw))) return cleanup(rc);
+        if (wz)   // the stress sweep that updates w adds the step's source as the velocity sweep does
+            for (auto& g : gs)
+                if ((rc = group_sources(g, src_all, nt, nw))) return cleanup(rc);
     }
     // windows with a learned strip: their strip passes follow each step's stress sweeps
     std::vector<int> swin;
@@ -1908,7 +1917,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const BatchGroup& g = gv[k];
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k], b->st));
             HIP_TRY(launch_vel_batch(b->ctx[0]->prec, op, g.ty, (const VelArgs*)g.args, g.base, (int)g.wins.size(),
-                                     g.lb, g.src, it, b->st));
+                                     g.lb, g.src, it, wz, b->st));
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k + 1], b->st));
         }
         for (size_t k = 0; k < gs.size(); ++k) {
@@ -1916,7 +1925,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const size_t e = 2 * (gv.size() + k);
             if (timed) HIP_TRY(hipEventRecord(gev[e], b->st));
             HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
-                                        (int)g.wins.size(), g.lb, it, b->d.scheme == 3, b->st));
+                                        (int)g.wins.size(), g.lb, it, b->d.scheme == 3, wz, g.src, b->st));
             if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
         }
         for (size_t k = 0; k < swin.size(); ++k) {
@@ -2250,10 +2259,29 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
         b->zs_vel = zv;
         b->zs_str = zst;
     }
+    // then which sweep updates w (ISO, VTI): the whole step timed both ways
+    if (!rc && (b->d.op == A3D_OP_ISO || b->d.op == A3D_OP_VTI) && b->d.scheme != 2 && b->wz_mode < 0) {
+        double ms[2] = {1e30, 1e30};
+        for (int m = 0; m < 2 && !rc; ++m) {
+            b->wz_mode = m;
+            rc = batch_run(b, 1, nullptr, &ms[m], nullptr);
+            if (rc == A3D_EUNSTABLE) rc = A3D_OK;
+        }
+        b->wz_mode = -1;
+        b->wz_tuned = ms[1] < ms[0] ? 1 : 0;
+    }
     for (auto& d : b->wd) d.nt = saved_nt;
     b->d.nt = saved_nt;
     b->tuned = rc == A3D_OK;
     return rc;
+}
+
+int a3d_batch_set_w_sweep(a3d_batch* b, int mode) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    if (mode < -1 || mode > 1) return fail(A3D_EINVAL, "w sweep mode %d not in -1..1", mode);
+    if (mode != b->wz_mode) drop_graph(b);
+    b->wz_mode = mode;
+    return A3D_OK;
 }
 
 int a3d_batch_set_strip_net(a3d_batch* b, const float* params, int n) {

@@ -160,11 +160,14 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 // Batched sweeps over several windows (each an independent padded grid with its own
 // arguments in device memory). base: nwin+1 prefix sums of the windows' tile counts;
 // lb: the combined decomposition (ntiles = base[nwin], grid = 8*ceil(ntiles/8)).
+// wz (ISO, VTI): w is updated by the stress sweep (all windows of a batch share it; the
+// stress sweep then takes the per-step sources srcs[it * nwin + w] too, and its windows
+// run one z segment per tile column).
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
-                            const Src* srcs, int it, hipStream_t st);
+                            const Src* srcs, int it, bool wz, hipStream_t st);
 // dirs: the windows' stress args have unsplit == 2 (all windows of a batch share it)
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, hipStream_t st);
+                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st);
 
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.

@@ -386,8 +386,9 @@ __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
 }
 
 // Batched velocity sweep: the tiles of all windows of a batch in one launch; per-window
-// arguments in device memory, per-step sources srcs[it * nwin + w].
-template <class PR, int OP, int TY>
+// arguments in device memory, per-step sources srcs[it * nwin + w]. WZ (ISO, VTI): the
+// stress sweep updates w.
+template <class PR, int OP, int TY, bool WZ = false>
 __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict__ wins, const int* __restrict__ base,
                                                       int nwin, Launch lb, const Src* __restrict__ srcs, int it) {
     const int t = remap_tile(lb);
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
     const VelArgs a = wins[w];
     int tix, tiy, seg;
     split_tile(a.l, t - base[w], tix, tiy, seg);
-    vel_body<PR, OP, TY>(a, srcs[(size_t)it * nwin + w], tix, tiy, seg);
+    vel_body<PR, OP, TY, WZ>(a, srcs[(size_t)it * nwin + w], tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -811,18 +812,22 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     stress_body<PR, OP, TY, WZ, DIRS>(a, a.rec, tix, tiy, seg);
 }
 
-// Batched stress sweep; a window's record plane of step it is rec + it * nx * ny.
-template <class PR, int OP, int TY, bool DIRS = false>
+// Batched stress sweep; a window's record plane of step it is rec + it * nx * ny. WZ
+// (ISO, VTI; one z segment per tile column): this sweep also updates w, with the
+// step's source of the window (srcs[it * nwin + w], as the velocity sweep adds it).
+template <class PR, int OP, int TY, bool DIRS = false, bool WZ = false>
 __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __restrict__ wins,
-                                                         const int* __restrict__ base, int nwin, Launch lb, int it) {
+                                                         const int* __restrict__ base, int nwin, Launch lb, int it,
+                                                         const Src* __restrict__ srcs) {
     const int t = remap_tile(lb);
     if (t >= lb.ntiles) return;
     const int w = find_window(base, nwin, t);
-    const StressArgs a = wins[w];
+    StressArgs a = wins[w];
+    if constexpr (WZ) a.s = srcs[(size_t)it * nwin + w];
     int tix, tiy, seg;
     split_tile(a.l, t - base[w], tix, tiy, seg);
     float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
-    stress_body<PR, OP, TY, false, DIRS>(a, rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, WZ, DIRS>(a, rec, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -1067,27 +1072,27 @@ hipError_t step2_ty(int ty, const Step2Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class PR, int OP>
+template <class PR, int OP, bool WZ>
 hipError_t vel_batch_ty(int ty, const VelArgs* w, const int* base, int nwin, const Launch& lb, const Src* srcs, int it,
                         hipStream_t st) {
     const dim3 grid(lb.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_vel_batch<PR, OP, 2>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, srcs, it); break;
-        case 4: hipLaunchKernelGGL((k_vel_batch<PR, OP, 4>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, srcs, it); break;
-        case 8: hipLaunchKernelGGL((k_vel_batch<PR, OP, 8>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 2: hipLaunchKernelGGL((k_vel_batch<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 4: hipLaunchKernelGGL((k_vel_batch<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 8: hipLaunchKernelGGL((k_vel_batch<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, srcs, it); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool DIRS>
+template <class PR, int OP, bool DIRS, bool WZ>
 hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
-                           hipStream_t st) {
+                           const Src* srcs, hipStream_t st) {
     const dim3 grid(lb.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2, DIRS>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it); break;
-        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4, DIRS>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it); break;
-        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8, DIRS>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it); break;
+        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2, DIRS, WZ>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it, srcs); break;
+        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4, DIRS, WZ>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it, srcs); break;
+        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8, DIRS, WZ>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it, srcs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1095,25 +1100,37 @@ hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwi
 
 template <class PR>
 hipError_t vel_batch_op(int op, int ty, const VelArgs* w, const int* base, int nwin, const Launch& lb, const Src* srcs,
-                        int it, hipStream_t st) {
+                        int it, bool wz, hipStream_t st) {
     switch (op) {
-        case 1: return vel_batch_ty<PR, 1>(ty, w, base, nwin, lb, srcs, it, st);
-        case 2: return vel_batch_ty<PR, 2>(ty, w, base, nwin, lb, srcs, it, st);
-        case 3: return vel_batch_ty<PR, 3>(ty, w, base, nwin, lb, srcs, it, st);
+        case 1: return wz ? vel_batch_ty<PR, 1, true>(ty, w, base, nwin, lb, srcs, it, st)
+                          : vel_batch_ty<PR, 1, false>(ty, w, base, nwin, lb, srcs, it, st);
+        case 2: return wz ? vel_batch_ty<PR, 2, true>(ty, w, base, nwin, lb, srcs, it, st)
+                          : vel_batch_ty<PR, 2, false>(ty, w, base, nwin, lb, srcs, it, st);
+        case 3: return wz ? hipErrorInvalidValue : vel_batch_ty<PR, 3, false>(ty, w, base, nwin, lb, srcs, it, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class PR>
 hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
-                           bool dirs, hipStream_t st) {
+                           bool dirs, bool wz, const Src* srcs, hipStream_t st) {
     switch (op) {
-        case 1: return dirs ? stress_batch_ty<PR, 1, true>(ty, w, base, nwin, lb, it, st)
-                            : stress_batch_ty<PR, 1, false>(ty, w, base, nwin, lb, it, st);
-        case 2: return dirs ? stress_batch_ty<PR, 2, true>(ty, w, base, nwin, lb, it, st)
-                            : stress_batch_ty<PR, 2, false>(ty, w, base, nwin, lb, it, st);
-        case 3: return dirs ? stress_batch_ty<PR, 3, true>(ty, w, base, nwin, lb, it, st)
-                            : stress_batch_ty<PR, 3, false>(ty, w, base, nwin, lb, it, st);
+        case 1:
+            if (wz)
+                return dirs ? stress_batch_ty<PR, 1, true, true>(ty, w, base, nwin, lb, it, srcs, st)
+                            : stress_batch_ty<PR, 1, false, true>(ty, w, base, nwin, lb, it, srcs, st);
+            return dirs ? stress_batch_ty<PR, 1, true, false>(ty, w, base, nwin, lb, it, srcs, st)
+                        : stress_batch_ty<PR, 1, false, false>(ty, w, base, nwin, lb, it, srcs, st);
+        case 2:
+            if (wz)
+                return dirs ? stress_batch_ty<PR, 2, true, true>(ty, w, base, nwin, lb, it, srcs, st)
+                            : stress_batch_ty<PR, 2, false, true>(ty, w, base, nwin, lb, it, srcs, st);
+            return dirs ? stress_batch_ty<PR, 2, true, false>(ty, w, base, nwin, lb, it, srcs, st)
+                        : stress_batch_ty<PR, 2, false, false>(ty, w, base, nwin, lb, it, srcs, st);
+        case 3:
+            if (wz) return hipErrorInvalidValue;
+            return dirs ? stress_batch_ty<PR, 3, true, false>(ty, w, base, nwin, lb, it, srcs, st)
+                        : stress_batch_ty<PR, 3, false, false>(ty, w, base, nwin, lb, it, srcs, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1203,21 +1220,21 @@ hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStrea
 }
 
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
-                            const Src* srcs, int it, hipStream_t st) {
+                            const Src* srcs, int it, bool wz, hipStream_t st) {
     switch (prec) {
-        case PREC_FP32: return vel_batch_op<Fp32>(op, ty, wins, base, nwin, lb, srcs, it, st);
-        case PREC_FP16: return vel_batch_op<Fp16>(op, ty, wins, base, nwin, lb, srcs, it, st);
-        case PREC_FP64: return vel_batch_op<Fp64>(op, ty, wins, base, nwin, lb, srcs, it, st);
+        case PREC_FP32: return vel_batch_op<Fp32>(op, ty, wins, base, nwin, lb, srcs, it, wz, st);
+        case PREC_FP16: return vel_batch_op<Fp16>(op, ty, wins, base, nwin, lb, srcs, it, wz, st);
+        case PREC_FP64: return vel_batch_op<Fp64>(op, ty, wins, base, nwin, lb, srcs, it, wz, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, hipStream_t st) {
+                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st) {
     switch (prec) {
-        case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, st);
-        case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, dirs, st);
-        case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, dirs, st);
+        case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
+        case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
+        case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
         default: return hipErrorInvalidValue;
     }
 }

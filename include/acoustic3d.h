@@ -249,6 +249,11 @@ int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile
  * updates w keeps one. a3d_batch_tune picks them after the tile rows; all choices give
  * bitwise the same records. */
 int a3d_batch_set_zseg(a3d_batch* b, int zseg_vel, int zseg_stress);
+/* Which sweep updates w in the batched modes (ISO, VTI): 0 the velocity sweep, 1 the
+ * stress sweep (one z segment per tile column), -1 (default) the choice of
+ * a3d_batch_tune (velocity sweep until tuned); as a3d_set_w_sweep for one context.
+ * Bitwise the same records either way. */
+int a3d_batch_set_w_sweep(a3d_batch* b, int mode);
 /* The learned absorbing strip on every window (VTI, FP32, nb = A3D_STRIP_NB): the
  * same network (params, a3d_mlp_nparams() floats, or NULL for the plain PML) and ramp
  * amplitude as a3d_set_strip_net / a3d_set_strip_profile give one context. The strip

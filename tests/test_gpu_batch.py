@@ -39,10 +39,15 @@ def test_batch_modes_equal_single_runs_and_oracle(oracle, op):
         b.set_zseg(3, 2)               # other z segments: same records
         bz, _ = b.run(mode=1)
         gz, _ = b.run(mode=2)
+        extra = [bz, gz]
+        if op in ("iso", "vti"):
+            for wzm in (1, 0):         # w updated by the stress sweep, then by the velocity sweep
+                b.set_w_sweep(wzm)
+                extra += [b.run(mode=1)[0], b.run(mode=2)[0]]
     print(f"[gpu-batch] {op}: {ncls} size classes, tile rows {info['tile_rows']}, z segments {info['zseg']}")
     for w, win in enumerate(wins):
         ox, oy, oz, wx, wy, wz, sx, sy, sz = win
-        for other in (bat[w], g1[w], g2[w], bat2[w], g3[w], bz[w], gz[w]):
+        for other in [bat[w], g1[w], g2[w], bat2[w], g3[w]] + [x[w] for x in extra]:
             assert np.array_equal(other, seq[w])
         sub = {k: np.ascontiguousarray(a[oy:oy + wy, ox:ox + wx, oz:oz + wz]) for k, a in m.items()}
         ref, _, _ = oracle.run(oracle.make_shot(op, wx, wy, wz, 10.0, 10.0, 10.0, sx, sy, sz, 70, 1e-3, 25.0, nb=8),
@@ -88,9 +93,12 @@ def test_batch_with_learned_strip_equals_single_runs(oracle, scheme):
         bat, _ = b.run(mode=1)
         g1, _ = b.run(mode=2)
         g2, _ = b.run(mode=2)
+        b.set_w_sweep(1)               # w updated by the stress sweep
+        bw, _ = b.run(mode=1)
+        gw, _ = b.run(mode=2)
     for w in range(len(wins)):
         assert np.abs(seq[w]).max() > 0
-        for other in (bat[w], g1[w], g2[w]):
+        for other in (bat[w], g1[w], g2[w], bw[w], gw[w]):
             assert np.array_equal(other, seq[w])
     # the first window against the oracle's strip restatement on its sub-model (the
     # matrix-core network against the oracle's binary16 form: within 1e-5)
