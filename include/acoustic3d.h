@@ -228,8 +228,10 @@ int a3d_batch_destroy(a3d_batch* b);
  * padded model is cut from the device copy. */
 int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const float* eps, const float* del,
                         const float* theta, const float* phi);
-/* Tile rows per window-size class (windows with equal x-tile and y-block counts),
- * from `steps` timed batched steps per candidate; *nclasses receives the class count. */
+/* Tile rows per window-size class (windows with equal x-tile and y-block counts), then
+ * the z segments per sweep and (ISO, VTI) which sweep updates w, each candidate timed
+ * over max(steps, 6) batched steps with the first two left out; *nclasses receives the
+ * class count. */
 int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses);
 /* mode 0: the windows one after another as plain single-shot runs (baseline);
  * mode 1: all windows advanced together, one batched launch per sweep and
