@@ -73,6 +73,8 @@ def load():
     lib.oracle_round_half.argtypes = [c_float]
     lib.oracle_round_half.restype = c_float
     lib.oracle_set_strip_profile.restype = c_int
+    lib.oracle_set_tti_fault.argtypes = [c_int]
+    lib.oracle_set_tti_fault.restype = c_int
     _lib = lib
     return lib
 
@@ -175,3 +177,11 @@ def set_strip_head(mode):
     along the normal axis and the normal velocity."""
     if load().oracle_set_strip_head(int(mode)) != 0:
         raise ValueError("bad strip head mode")
+
+
+def set_tti_fault(code):
+    """Plants a mistake in the TTI rotation (0: none). Codes 1..8 flip the sign of a
+    velocity-sweep rotation term, 9..16 of a stress-sweep term, 17 and 18 swap an
+    operand (oracle.cpp lists them); used only by tests that must show they catch one."""
+    if load().oracle_set_tti_fault(int(code)) != 0:
+        raise ValueError(f"bad TTI fault code {code}")

@@ -411,6 +411,14 @@ double g_strip_amp = 10.0;    /* PML amplitude of the strip's 6-cell ramp */
 bool g_strip_half = true;     /* network operands rounded to binary16 (matrix-core evaluation) */
 int g_strip_head = 0;         /* 0: damping of every field; 1: normal-axis split components and velocity */
 
+/* Planted mistake in the TTI rotation, for tests that must show they catch one (0: none).
+ * 1..8 flip the sign of velocity-sweep term k-1, 9..16 of stress-sweep term k-9, in the
+ * order ct*cp*d_x, ct*sp*d_y, st*d_z (xx), sp*d_x, cp*d_y (yy), st*cp*d_x, st*sp*d_y,
+ * ct*d_z (zz); 17: the velocity sweep's zz takes P's x derivative in place of Q's;
+ * 18: the stress sweep's xx takes v's x derivative in place of u's. A sign of +1 is an
+ * exact multiplication, so fault 0 leaves every result bit for bit unchanged. */
+int g_tti_fault = 0;
+
 /* Nearest binary16 value (ties to even), as a float; finite inputs below 65504 in magnitude. */
 float round_half(float f) {
     const float a = std::fabs(f);
@@ -600,6 +608,12 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
     initial_coffe<T>(dt, d0, nz, npml, cz1.data(), cz2.data(), az1.data(), az2.data(), width);
 
     const T dtx = dt / (T)p.dx, dty = dt / (T)p.dy, dtz = dt / (T)p.dz;   /* iso:109-111 */
+    /* signs of the TTI rotation terms: all +1 unless a test plants a mistake */
+    T vsgn[8], ssgn[8];
+    for (int k = 0; k < 8; ++k) {
+        vsgn[k] = g_tti_fault == k + 1 ? (T)-1 : (T)1;
+        ssgn[k] = g_tti_fault == k + 9 ? (T)-1 : (T)1;
+    }
 
     std::vector<T> u(N, 0), v(N, 0), w(N, 0), P(N, 0), Q, px(N, 0), py(N, 0), pz(N, 0), qx, qy, qz;
     if (!iso) { Q.assign(N, 0); qx.assign(N, 0); qy.assign(N, 0); qz.assign(N, 0); }
@@ -652,9 +666,11 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                         T Qy = dfwd<T>(Qp, id, pl), Qx = dfwd<T>(Qp, id, nnz), Qz = dfwd<T>(Qp, id, 1);
                         T ct = fcos<T>(th[id]), st = fsin<T>(th[id]);
                         T cp = fcos<T>(ph[id]), sp = fsin<T>(ph[id]);
-                        xx = ct * cp * Px - ct * sp * Py - st * Pz;     /* tti:101 */
-                        yy = sp * Px + cp * Py;                         /* tti:102 */
-                        zz = st * cp * Qx - st * sp * Qy + ct * Qz;     /* tti:103 */
+                        const T* f = vsgn;
+                        if (g_tti_fault == 17) Qx = Px;
+                        xx = f[0] * ct * cp * Px - f[1] * ct * sp * Py - f[2] * st * Pz;     /* tti:101 */
+                        yy = f[3] * sp * Px + f[4] * cp * Py;                                /* tti:102 */
+                        zz = f[5] * st * cp * Qx - f[6] * st * sp * Qy + f[7] * ct * Qz;     /* tti:103 */
                     }
                     u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
                     v[id] = cy2[iy] * v[id] - cy1[iy] * dty * yy;
@@ -713,9 +729,11 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                         (void)vz;   /* computed but unused by the reference (tti:179, :185) */
                         T ct = fcos<T>(th[id]), st = fsin<T>(th[id]);
                         T cp = fcos<T>(ph[id]), sp = fsin<T>(ph[id]);
-                        xx = ct * cp * ux - ct * sp * uy - st * uz;     /* tti:184 */
-                        yy = sp * vx + cp * vy;                         /* tti:185 */
-                        zz = st * cp * wx - st * sp * wy + ct * wz;     /* tti:186 */
+                        const T* f = ssgn;
+                        if (g_tti_fault == 18) ux = vx;
+                        xx = f[0] * ct * cp * ux - f[1] * ct * sp * uy - f[2] * st * uz;     /* tti:184 */
+                        yy = f[3] * sp * vx + f[4] * cp * vy;                                /* tti:185 */
+                        zz = f[5] * st * cp * wx - f[6] * st * sp * wy + f[7] * ct * wz;     /* tti:186 */
                         const T c = vp[id];
                         px[id] = ax2[ix] * px[id] - ax1[ix] * c * c * (1 + 2 * ee) * dtx * xx;
                         py[id] = ay2[iy] * py[id] - ay1[iy] * c * c * (1 + 2 * ee) * dty * yy;
@@ -845,6 +863,12 @@ float oracle_round_half(float f) { return round_half(f); }
 int oracle_set_strip_head(int mode) {
     if (mode < 0 || mode > 1) return -1;
     g_strip_head = mode;
+    return 0;
+}
+
+int oracle_set_tti_fault(int code) {
+    if (code < 0 || code > 18) return -1;
+    g_tti_fault = code;
     return 0;
 }
 

@@ -69,6 +69,12 @@ int oracle_set_strip_profile(float amp);
  * PML factor along the normal is 1 are left alone). */
 int oracle_set_strip_head(int mode);
 
+/* Plants a mistake in the TTI rotation so a test can show that it catches it
+ * (0 = none, the default; 1..8 flip a velocity-sweep term's sign, 9..16 a
+ * stress-sweep term's, 17/18 swap an operand; oracle.cpp lists them).
+ * Returns -1 for an unknown code. */
+int oracle_set_tti_fault(int code);
+
 /* One strip correction on padded arrays (oracle layout: z fastest, then x, then y;
  * extents nnz, nnx, nny) with the network set above; -1 if none is set. */
 int oracle_strip_apply(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,

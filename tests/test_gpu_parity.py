@@ -53,6 +53,39 @@ def test_parity_small(oracle, op):
         assert np.array_equal(snaps, rsn), "FP32 snapshots not bitwise equal to the oracle"
 
 
+HALF_PI = float(np.float32(np.pi / 2))
+
+
+@pytest.mark.parametrize("angles", [(0.0, HALF_PI), (HALF_PI, 0.0), (HALF_PI, HALF_PI), "field"])
+@pytest.mark.parametrize("scheme", [0, 3])
+def test_tti_rotation_angles(oracle, angles, scheme):
+    """TTI at the right-angle tilts/azimuths where the oracle's rotation is pinned against
+    VTI with two axes exchanged, and at smooth random tilt (0-45 deg) and azimuth (0-360 deg)
+    fields: the GPU against the FP32 oracle."""
+    from a3dfd import Modeller, make_desc
+    from a3dfd.models import small_model
+    nx, ny, nz, nb = 36, 30, 34, 12
+    m = small_model("tti", nx, ny, nz, seed=29)
+    if angles != "field":
+        m["theta"] = np.full_like(m["vel"], angles[0])
+        m["phi"] = np.full_like(m["vel"], angles[1])
+    else:
+        rng = np.random.default_rng(29)
+        m["theta"] = (np.pi / 4 * rng.random(m["vel"].shape)).astype(np.float32)
+        m["phi"] = (2 * np.pi * rng.random(m["vel"].shape)).astype(np.float32)
+    args = ("tti", nx, ny, nz, D, D, D, 18, 15, 9, 150, 1e-3, 25.0)
+    kw = dict(nb=nb, snap_interval=149)
+    print(f"\n[gpu-parity] TTI angles={angles} scheme={scheme}", flush=True)
+    with Modeller(make_desc(*args, scheme=scheme, **kw)) as mdl:
+        mdl.set_model(**m)
+        out, snaps, _ = mdl.run(nsnap_max=2)
+    ref, rsn, _ = oracle.run(oracle.make_shot(*args, **kw), 32, nsnap_max=2, **m)
+    e, es = rel_l2(out, ref), rel_l2(snaps, rsn)
+    print(f"[gpu-parity] TTI angles={angles} scheme={scheme}: shot {e:.2e} snapshots {es:.2e}")
+    assert np.abs(ref).max() > 0 and np.isfinite(out).all()
+    assert e <= 1e-5 and es <= 1e-5
+
+
 @pytest.mark.parametrize("op", ["2nd", "iso", "vti", "tti"])
 def test_parity_multi_tile(oracle, op):
     """Several x tiles (partial last tile), y tiles and z segments."""

@@ -13,7 +13,9 @@ stagger offset, a transposed operand, a wrong spacing) fails at least one:
   (velocity, density and dt/dx scaling), x<->y swap symmetry (bitwise) and
   swapped spacings (catches dtx/dty mix-ups), mirror symmetry (stagger
   offsets), VTI(eps=delta=0) against the ISO dipole by superposition, TTI with
-  zero tilt against VTI with unit density (bitwise), power-of-two linearity
+  zero tilt against VTI with unit density (bitwise), TTI at right-angle tilts
+  and azimuths against VTI with two axes exchanged (every rotation term's sign
+  and operand; planted mistakes shown to fail), power-of-two linearity
   (bitwise), receivers against snapshot planes, PML energy absorption, the
   mute cut, and FP32 against the FP64 instance.
 """
@@ -350,6 +352,100 @@ def test_tti_with_zero_tilt_equals_vti_with_unit_density_bitwise(oracle):
     b, sb, _ = run(oracle, "tti", n3, (10, 11, 4), 120, model=tti_m, nsnap=3, **kw)
     assert np.abs(a).max() > 0
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
+
+
+# ------------------------------------------------ TTI rotation at non-zero angles
+# The reference's rotation (tti:101-103 in the velocity sweep, tti:184-186 in the stress
+# sweep) at a tilt theta and azimuth phi whose cosines and sines are each 0 or 1 turns the
+# TTI operator into the VTI operator (rho = 1) with two grid axes exchanged:
+#   (0, pi/2):    xx = -d_y, yy = d_x, zz = d_z   -> VTI with the roles of u and v swapped
+#                 (v carries -dt d_x P as VTI's u does, -u carries VTI's v); no transpose;
+#   (pi/2, 0):    xx = -d_z, yy = d_y, zz = d_x   -> VTI with the x and z axes exchanged;
+#   (pi/2, pi/2): xx = -d_z, yy = d_x, zz = -d_y  -> VTI with the y and z axes exchanged.
+# Every one of the 8 terms of each sweep has coefficient +-1 in one of these cases or at
+# theta = phi = 0, so a flipped sign, a wrong trig factor or a P/Q or u/v operand mix-up
+# in any term breaks one equality. The PML tables differ between the exchanged axes, so
+# the fields are compared before the wavefront reaches the boundary; float cos(pi/2) is
+# -4.4e-8, not 0, which leaves ~3e-7 relative L2.
+HALF_PI = float(np.float32(np.pi / 2))
+ROT_N, ROT_NT = 36, 50
+# name: ((theta, phi), permutation of the (y, x, z) model axes for the VTI run)
+ROT_CASES = {
+    "theta0_phi0": ((0.0, 0.0), (0, 1, 2)),
+    "theta0_phi90": ((0.0, HALF_PI), (0, 1, 2)),
+    "theta90_phi0": ((HALF_PI, 0.0), (0, 2, 1)),
+    "theta90_phi90": ((HALF_PI, HALF_PI), (2, 1, 0)),
+}
+
+
+def _rot_model():
+    rng = np.random.default_rng(7)
+    n3 = (ROT_N,) * 3
+    vel = rng.uniform(1900, 2300, n3).astype(np.float32)
+    eps = rng.uniform(0.05, 0.2, n3).astype(np.float32)
+    dl = (eps * rng.uniform(0.2, 0.6, n3)).astype(np.float32)   # epsilon != delta: P != Q
+    return vel, eps, dl
+
+
+_ROT_VTI = {}
+
+
+def _rot_run(oracle, name):
+    """(TTI snapshot, VTI snapshot mapped back to the TTI axes, TTI record, VTI record)."""
+    (th, ph), perm = ROT_CASES[name]
+    vel, eps, dl = _rot_model()
+    n = ROT_N
+    src = (n // 2, n // 2 + 1, n // 2 - 1)                      # (sx, sy, sz), 1-based
+    kw = dict(nb=10, sv=1, snap_interval=ROT_NT - 1, snap_skip0=1)
+    tm = dict(vel=vel, eps=eps, delta=dl, theta=np.full(vel.shape, th, np.float32),
+              phi=np.full(vel.shape, ph, np.float32))
+    a, sa, _ = run(oracle, "tti", vel.shape, src, ROT_NT, model=tm, nsnap=1, **kw)
+    if name not in _ROT_VTI:
+        vm = dict(vel=vel, rho=np.ones_like(vel), eps=eps, delta=dl)
+        vm = {k: np.ascontiguousarray(x.transpose(perm)) for k, x in vm.items()}
+        axes = {0: src[1], 1: src[0], 2: src[2]}                 # model axis -> source index
+        ps = (axes[perm[1]], axes[perm[0]], axes[perm[2]])       # (sx, sy, sz) of the VTI run
+        b, sb, _ = run(oracle, "vti", vm["vel"].shape, ps, ROT_NT, model=vm, nsnap=1, **kw)
+        _ROT_VTI[name] = (sb[0].transpose(np.argsort(perm)), b)
+    sb, b = _ROT_VTI[name]
+    return sa[0], sb, a, b
+
+
+@pytest.mark.parametrize("name", ["theta0_phi90", "theta90_phi0", "theta90_phi90"])
+def test_tti_rotation_at_right_angles_equals_axis_exchanged_vti(oracle, name):
+    sa, sb, a, b = _rot_run(oracle, name)
+    assert np.abs(sa).max() > 1e-3
+    # the wavefront has not reached the PML: the outer cells are still quiet
+    edge = max(np.abs(sa[:2]).max(), np.abs(sa[-2:]).max(), np.abs(sa[:, :2]).max(), np.abs(sa[:, -2:]).max(),
+               np.abs(sa[:, :, :2]).max(), np.abs(sa[:, :, -2:]).max())
+    assert edge < 1e-5 * np.abs(sa).max()
+    assert rel_l2(sa, sb) < 1e-6
+    if name == "theta0_phi90":
+        assert rel_l2(a, b) < 1e-6                               # receivers: no axis exchanged
+
+
+def test_tti_rotation_checks_catch_planted_mistakes(oracle):
+    """Each of 18 planted mistakes in the oracle's rotation (a flipped sign of any of the 16
+    terms, P's x derivative in place of Q's in the velocity sweep's zz, v's x derivative in
+    place of u's in the stress sweep's xx) fails at least one of the four right-angle
+    equalities (theta = phi = 0 included, where it is bitwise)."""
+    missed = []
+    try:
+        for code in range(1, 19):
+            oracle.set_tti_fault(code)
+            caught = False
+            for name in ROT_CASES:
+                sa, sb, _, _ = _rot_run(oracle, name)
+                if not np.all(np.isfinite(sa)) or rel_l2(sa, sb) > 1e-4:
+                    caught = True
+                    break
+            if not caught:
+                missed.append(code)
+    finally:
+        oracle.set_tti_fault(0)
+    assert missed == []
+    sa, sb, _, _ = _rot_run(oracle, "theta0_phi0")
+    assert np.array_equal(sa, sb)
 
 
 @pytest.mark.parametrize("op", ["2nd", "iso", "vti", "tti"])
