@@ -155,6 +155,9 @@ def load(path=None):
                  "a3d_mlp_destroy", "a3d_mlp_set_params", "a3d_mlp_get_params", "a3d_mlp_predict",
                  "a3d_mlp_loss_grad", "a3d_mlp_train"):
         getattr(lib, name).restype = c_int
+    # allocation-free shot: model arrays and the record are device addresses (c_void_p)
+    lib.a3d_run_static.argtypes = [POINTER(ShotDesc), c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.a3d_run_static.restype = c_int
     lib.a3d_probe_bandwidth.argtypes = [c_int, ctypes.c_size_t, c_int, POINTER(c_double), POINTER(c_double)]
     lib.a3d_probe_bandwidth.restype = c_int
     lib.a3d_last_error.restype = c_char_p

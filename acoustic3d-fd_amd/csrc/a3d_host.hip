@@ -177,6 +177,9 @@ double sponge_value(int i, int n, int npd) {
 
 int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Records e on s; a context without loop events (the allocation-free entry point) skips it.
+hipError_t mark(hipEvent_t e, hipStream_t s) { return e ? hipEventRecord(e, s) : hipSuccess; }
+
 struct Timing {
     bool on = false;
     int stride = 1;       // time steps sampled: every stride-th one
@@ -235,6 +238,7 @@ struct a3d_ctx {
     // 1 stress sweep) and the tuner's choice for mode -1
     int wz_mode = -1;
     int wz_tuned = 0;
+    bool out_device = false;    // the shot record is transposed into shot_dev, the caller's device buffer
 };
 
 namespace {
@@ -686,7 +690,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         a.ny = ny;
         void* X = c->vol[V_PA];
         void* Y = c->vol[V_PB];
-        HIP_TRY(hipEventRecord(c->ev0, c->st));
+        HIP_TRY(mark(c->ev0, c->st));
         for (int it = 0; it < nt; ++it) {
             a.X = X;
             a.Y = Y;
@@ -704,7 +708,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             }
             std::swap(X, Y);
         }
-        HIP_TRY(hipEventRecord(c->ev1, c->st));
+        HIP_TRY(mark(c->ev1, c->st));
     } else {
         // -------- 1st-order family
         VelArgs va;
@@ -757,7 +761,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                              (float*)c->vol[V_W]};
             float* nxt[5] = {(float*)c->vol[V_P2], (float*)c->vol[V_Q2], (float*)c->vol[V_U2], (float*)c->vol[V_V2],
                              (float*)c->vol[V_W2]};
-            HIP_TRY(hipEventRecord(c->ev0, c->st));
+            HIP_TRY(mark(c->ev0, c->st));
             for (int it = 0; it < nt; it++) {
                 fa.P0 = cur[0];
                 fa.Q0 = cur[1];
@@ -780,7 +784,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                     if ((rc = take_snap(it, cur[0], -1, 0.0, -1, 0.0, false))) return rc;
                 }
             }
-            HIP_TRY(hipEventRecord(c->ev1, c->st));
+            HIP_TRY(mark(c->ev1, c->st));
         } else {
         // z chunks: velocity sweep of chunk k+1 before the stress sweep of chunk k (its w window
         // reaches 3 planes into chunk k+1; the velocity sweep of chunk k+1 reads the stresses
@@ -796,7 +800,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             sch.back().l = make_launch(g, c->ty[1], c->zseg[1], z0, z1);
         }
         const int nch = (int)vch.size();
-        HIP_TRY(hipEventRecord(c->ev0, c->st));
+        HIP_TRY(mark(c->ev0, c->st));
         for (int it = 0; it < nt; it++) {
             va.s = srcs[it];
             sa.s = srcs[it];
@@ -830,12 +834,14 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
             }
         }
-        HIP_TRY(hipEventRecord(c->ev1, c->st));
+        HIP_TRY(mark(c->ev1, c->st));
         }
     }
-    HIP_TRY(hipEventSynchronize(c->ev1));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (c->ev1) {
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    }
     if (loop_ms) *loop_ms = ms;
     if (c->tm.on) timing_collect(c);
     if (nsnap_taken) *nsnap_taken = nsnap;
@@ -847,8 +853,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
     HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
 
-    if (shot_out) {
-        if ((rc = ensure(&c->shot_dev, &c->shot_cap, plane_rec * nt))) return rc;
+    if (shot_out || c->out_device) {
+        // out_device: c->shot_dev is the caller's device buffer of plane_rec * nt floats
+        if (!c->out_device && (rc = ensure(&c->shot_dev, &c->shot_cap, plane_rec * nt))) return rc;
         MuteArgs m{};
         m.mute = d->mute;
         m.dx = d->dx;
@@ -865,7 +872,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         else if (d->op == A3D_OP_ISO) m.vcut = c->vel_first;
         else m.vcut = c->vel_first * sqrtf(1 + 2 * c->eps_first);
         HIP_TRY(launch_shot_transpose(c->rec, nt, nx, ny, m, c->shot_dev, c->st));
-        HIP_TRY(hipMemcpyAsync(shot_out, c->shot_dev, plane_rec * nt * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        if (shot_out)
+            HIP_TRY(hipMemcpyAsync(shot_out, c->shot_dev, plane_rec * nt * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->st));
         HIP_TRY(hipStreamSynchronize(c->st));
     }
     if (hflag) return fail(A3D_EUNSTABLE, "non-finite values in the shot record (unstable time step?)");
@@ -1022,22 +1031,48 @@ namespace {
 // Model-derived scalars of a context whose interior is the window (w.o*, c->d.n*) of a
 // z-fastest host model of size w.n*: the d0 sample, the mute velocity samples, the
 // smallest impedance (binary16 scales).
+// Interior cells (iz, ix, iy) of the model samples a shot's scalars come from. 1st order:
+// the d0 sample, padded z-fastest linear index N/2 (reference get_d0, iso:194), and the
+// mute-velocity sample, padded index 1 = (iz=1, ix=0, iy=0) (mute_directwave, iso:340,
+// vti:357), both clamped into the interior (edge replication); 2nd order: the interior
+// sample (iz=1, ix=0, iy=0) (Toa_3dfd_cuda.cu:403) in `first`.
+struct Samples {
+    int d0[3], first[3];
+};
+
+Samples model_samples(const a3d_ctx* c) {
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    Samples s{};
+    if (c->d.op != A3D_OP_2ND) {
+        const int nb = c->d.nb;
+        const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
+        const long long i = (long long)nny * nnx * nnz / 2;
+        const int iz = (int)(i % nnz), ix = (int)((i / nnz) % nnx), iy = (int)(i / ((long long)nnz * nnx));
+        s.d0[0] = clampi(iz - nb, 0, nz - 1);
+        s.d0[1] = clampi(ix - nb, 0, nx - 1);
+        s.d0[2] = clampi(iy - nb, 0, ny - 1);
+        s.first[0] = clampi(1 - nb, 0, nz - 1);
+        s.first[1] = clampi(-nb, 0, nx - 1);
+        s.first[2] = clampi(-nb, 0, ny - 1);
+    } else {
+        s.first[0] = nz > 1 ? 1 : 0;
+    }
+    return s;
+}
+
+// Model-derived scalars of a context whose interior is the window (w.o*, c->d.n*) of a
+// z-fastest host model of size w.n*: the d0 sample, the mute velocity samples, the
+// smallest impedance (binary16 scales).
 void model_scalars(a3d_ctx* c, const float* vel, const float* rho, const float* eps, const Window& w) {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
     auto at = [&](const float* a, int iz, int ix, int iy) {
         return a[(size_t)(iz + w.oz) + (size_t)w.nz * (ix + w.ox) + (size_t)w.nz * w.nx * (iy + w.oy)];
     };
+    const Samples sm = model_samples(c);
     if (c->d.op != A3D_OP_2ND) {
-        const int nb = c->d.nb;
-        const int nnx = nx + 2 * nb, nny = ny + 2 * nb, nnz = nz + 2 * nb;
-        // padded z-fastest linear index N/2 (reference get_d0, iso:194)
-        const long long i = (long long)nny * nnx * nnz / 2;
-        const int iz = (int)(i % nnz), ix = (int)((i / nnz) % nnx), iy = (int)(i / ((long long)nnz * nnx));
-        c->vel_d0 = at(vel, clampi(iz - nb, 0, nz - 1), clampi(ix - nb, 0, nx - 1), clampi(iy - nb, 0, ny - 1));
-        // padded index 1 = (iz=1, ix=0, iy=0) (reference mute_directwave, iso:340, vti:357)
-        const int z1 = clampi(1 - nb, 0, nz - 1), x1 = clampi(-nb, 0, nx - 1), y1 = clampi(-nb, 0, ny - 1);
-        c->vel_first = at(vel, z1, x1, y1);
-        c->eps_first = eps ? at(eps, z1, x1, y1) : 0.0f;
+        c->vel_d0 = at(vel, sm.d0[0], sm.d0[1], sm.d0[2]);
+        c->vel_first = at(vel, sm.first[0], sm.first[1], sm.first[2]);
+        c->eps_first = eps ? at(eps, sm.first[0], sm.first[1], sm.first[2]) : 0.0f;
         // smallest impedance rho*v (TTI has no density: rho = 1)
         double zmin = 1e300;
         for (int iy = 0; iy < ny; ++iy)
@@ -1046,9 +1081,54 @@ void model_scalars(a3d_ctx* c, const float* vel, const float* rho, const float* 
                     zmin = std::min(zmin, (double)at(vel, iz, ix, iy) * (rho ? (double)at(rho, iz, ix, iy) : 1.0));
         c->zmin = (float)std::max(zmin, 1e-30);
     } else {
-        // reference Toa_3dfd_cuda.cu:403: v0[1], the interior sample (iz=1, ix=0, iy=0)
-        c->vel_raw1 = at(vel, nz > 1 ? 1 : 0, 0, 0);
+        c->vel_raw1 = at(vel, sm.first[0], sm.first[1], sm.first[2]);
     }
+}
+
+// Geometry, precision and element sizes of a context for *desc on `device`.
+void init_geometry(a3d_ctx* c, const a3d_shot_desc* desc, int device) {
+    c->d = *desc;
+    c->device = device;
+    const int op = desc->op;
+    c->off = op == A3D_OP_2ND ? MM + desc->nb : desc->nb;
+    const int ring = op == A3D_OP_2ND ? 2 * MM + 2 * desc->nb : 2 * desc->nb;
+    c->g.nnx = desc->nx + ring;
+    c->g.nny = desc->ny + ring;
+    c->g.nnz = desc->nz + ring;
+    c->g.ld = (c->g.nnx + TX - 1) / TX * TX;
+    c->g.pl = c->g.ld * c->g.nny;
+    c->nvox = (size_t)c->g.pl * c->g.nnz;
+    c->prec = desc->storage == 1 ? PREC_FP16 : (desc->storage == 2 ? PREC_FP64 : PREC_FP32);
+    c->wsize = c->prec == PREC_FP16 ? 2 : (c->prec == PREC_FP64 ? 8 : 4);
+    c->msize = c->prec == PREC_FP64 ? 8 : 4;
+    c->csize = c->msize;
+    if (int ncu = 0; hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+        c->ncu = ncu;
+}
+
+// Elements of the PML table (arithmetic type) and of the double table (sponge, taper).
+size_t table_count(const a3d_ctx* c) { return 4 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz); }
+size_t dtable_count(const a3d_ctx* c) { return (size_t)c->g.nnx + c->g.nny + c->g.nnz + 226; }
+
+// Double tables: sponge factors (2nd order) and SV taper (VTI/TTI), uploaded to c->dtab.
+int upload_dtab(a3d_ctx* c) {
+    const a3d_shot_desc* desc = &c->d;
+    std::vector<double> dt(dtable_count(c), 1.0);
+    if (desc->op == A3D_OP_2ND) {
+        for (int i = 0; i < c->g.nnx; ++i) dt[i] = sponge_value(i, desc->nx, desc->nb);
+        for (int i = 0; i < c->g.nny; ++i) dt[c->g.nnx + i] = sponge_value(i, desc->ny, desc->nb);
+        for (int i = 0; i < c->g.nnz; ++i) dt[c->g.nnx + c->g.nny + i] = sponge_value(i, desc->nz, desc->nb);
+    }
+    double* taper = dt.data() + c->g.nnx + c->g.nny + c->g.nnz;
+    for (int r2 = 0; r2 < 226; ++r2) {
+        // reference vti:163-166: 0.5*(1-cos(pi*((sqrtf(r2)-r)*4.0/(R*3.0-1)))), R=15, r=4;
+        // the FP64 instance takes the square root in double
+        const double rr = c->prec == PREC_FP64 ? std::sqrt((double)r2) - 4 : (double)(sqrtf((float)r2) - 4);
+        taper[r2] = r2 <= 16 ? 0.0 : 0.5 * (1 - std::cos(kPi * (rr * 4.0 / (15 * 3.0 - 1))));
+    }
+    if (hipMemcpy(c->dtab, dt.data(), dt.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(A3D_EHIP, "table upload failed");
+    return A3D_OK;
 }
 
 }  // namespace
@@ -1068,21 +1148,8 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
     if (device < 0 || device >= ndev) return fail(A3D_EINVAL, "device %d not present (%d devices)", device, ndev);
     HIP_TRY(hipSetDevice(device));
     a3d_ctx* c = new a3d_ctx();
-    c->d = *desc;
-    c->device = device;
+    init_geometry(c, desc, device);
     const int op = desc->op;
-    c->off = op == A3D_OP_2ND ? MM + desc->nb : desc->nb;
-    const int ring = op == A3D_OP_2ND ? 2 * MM + 2 * desc->nb : 2 * desc->nb;
-    c->g.nnx = desc->nx + ring;
-    c->g.nny = desc->ny + ring;
-    c->g.nnz = desc->nz + ring;
-    c->g.ld = (c->g.nnx + TX - 1) / TX * TX;
-    c->g.pl = c->g.ld * c->g.nny;
-    c->nvox = (size_t)c->g.pl * c->g.nnz;
-    c->prec = desc->storage == 1 ? PREC_FP16 : (desc->storage == 2 ? PREC_FP64 : PREC_FP32);
-    c->wsize = c->prec == PREC_FP16 ? 2 : (c->prec == PREC_FP64 ? 8 : 4);
-    c->msize = c->prec == PREC_FP64 ? 8 : 4;
-    c->csize = c->msize;
     auto cleanup = [&](int code) {
         a3d_destroy(c);
         return code;
@@ -1101,29 +1168,11 @@ int a3d_create(const a3d_shot_desc* desc, int device, a3d_ctx** out) {
             return cleanup(fail(A3D_ENOMEM, "device allocation of %zu MiB failed", bytes >> 20));
         }
     }
-    const size_t ntab = 4 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz);
-    const size_t ndtab = (size_t)c->g.nnx + c->g.nny + c->g.nnz + 226;
+    const size_t ntab = table_count(c), ndtab = dtable_count(c);
     if (hipMalloc(&c->tab, ntab * c->csize) != hipSuccess) { c->tab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
     if (hipMalloc(&c->dtab, ndtab * sizeof(double)) != hipSuccess) { c->dtab = nullptr; return cleanup(fail(A3D_ENOMEM, "table allocation failed")); }
     if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) { c->flag = nullptr; return cleanup(fail(A3D_ENOMEM, "flag allocation failed")); }
-    // double tables: sponge factors (2nd order) and SV taper (VTI/TTI)
-    std::vector<double> dt(ndtab, 1.0);
-    if (op == A3D_OP_2ND) {
-        for (int i = 0; i < c->g.nnx; ++i) dt[i] = sponge_value(i, desc->nx, desc->nb);
-        for (int i = 0; i < c->g.nny; ++i) dt[c->g.nnx + i] = sponge_value(i, desc->ny, desc->nb);
-        for (int i = 0; i < c->g.nnz; ++i) dt[c->g.nnx + c->g.nny + i] = sponge_value(i, desc->nz, desc->nb);
-    }
-    double* taper = dt.data() + c->g.nnx + c->g.nny + c->g.nnz;
-    for (int r2 = 0; r2 < 226; ++r2) {
-        // reference vti:163-166: 0.5*(1-cos(pi*((sqrtf(r2)-r)*4.0/(R*3.0-1)))), R=15, r=4;
-        // the FP64 instance takes the square root in double
-        const double rr = c->prec == PREC_FP64 ? std::sqrt((double)r2) - 4 : (double)(sqrtf((float)r2) - 4);
-        taper[r2] = r2 <= 16 ? 0.0 : 0.5 * (1 - std::cos(kPi * (rr * 4.0 / (15 * 3.0 - 1))));
-    }
-    if (hipMemcpy(c->dtab, dt.data(), ndtab * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(A3D_EHIP, "table upload failed"));
-    if (int ncu = 0; hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
-        c->ncu = ncu;
+    if (int rc2 = upload_dtab(c)) return cleanup(rc2);
     default_shapes(c);
     *out = c;
     return A3D_OK;
@@ -2347,3 +2396,139 @@ int a3d_batch_info(a3d_batch* b, long long* updated_points, int* nwin, int* tile
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Allocation-free shot: every buffer the shot needs beyond its inputs and its
+// output lives in one file-scope device array, `scratch`, of fixed size. The
+// runtime gives each device its own copy of the array; its address on a device
+// is taken once with hipGetSymbolAddress, with that device current, and only
+// that address is handed to kernels and copies (the array's name in host code
+// is a host-side stand-in, not a device address).
+// ===========================================================================
+
+static __device__ __align__(256) unsigned char scratch[(size_t)A3D_STATIC_SCRATCH_MIB << 20];
+
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)A3D_STATIC_SCRATCH_MIB << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `device` (taken on first use).
+int scratch_base(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(A3D_EINVAL, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(A3D_EHIP, "address of the scratch array on device %d: %s", device, hipGetErrorString(e));
+        g_scratch[device] = (unsigned char*)p;
+    }
+    *out = g_scratch[device];
+    return A3D_OK;
+}
+
+// Bump allocation of 256-byte aligned slices; base == nullptr only counts bytes.
+struct Carver {
+    unsigned char* base;
+    size_t used = 0;
+    void* take(size_t n) {
+        used = (used + 255) & ~(size_t)255;
+        void* p = base ? base + used : nullptr;
+        used += n;
+        return p;
+    }
+};
+
+// Lays out every device buffer of a FP32 shot of context *c in cv (volumes of the operator,
+// interleaved model, PML and double tables, sentinel flag, receiver record).
+void carve(a3d_ctx* c, Carver& cv) {
+    const int op = c->d.op;
+    for (int v : volumes_of(op)) c->vol[v] = cv.take(c->nvox * c->wsize);
+    if (op != A3D_OP_2ND) c->vol[V_M4] = cv.take(c->nvox * (op == A3D_OP_ISO ? 8 : 16));
+    c->tab = cv.take(table_count(c) * c->csize);
+    c->dtab = (double*)cv.take(dtable_count(c) * sizeof(double));
+    c->flag = (int*)cv.take(sizeof(int));
+    c->rec_cap = (size_t)c->d.nx * c->d.ny * c->d.nt;
+    c->rec = (float*)cv.take(c->rec_cap * sizeof(float));
+}
+
+// One float of a device array (a model sample) to the host.
+int sample(const float* a, size_t i, float* out) {
+    HIP_TRY(hipMemcpy(out, a + i, sizeof(float), hipMemcpyDeviceToHost));
+    return A3D_OK;
+}
+
+}  // namespace
+
+extern "C" int a3d_run_static(const a3d_shot_desc* desc, int device, const float* vel, const float* rho,
+                              const float* eps, const float* del, const float* theta, const float* phi, float* shot,
+                              void* stream) {
+    int rc = check_desc_geometry(desc);
+    if (rc) return rc;
+    const int op = desc->op;
+    if (!vel || !shot) return fail(A3D_EINVAL, "velocity model and shot record are required");
+    if ((op == A3D_OP_ISO || op == A3D_OP_VTI) && !rho) return fail(A3D_EINVAL, "density model required");
+    if ((op == A3D_OP_VTI || op == A3D_OP_TTI) && (!eps || !del)) return fail(A3D_EINVAL, "epsilon and delta required");
+    if (op == A3D_OP_TTI && (!theta || !phi)) return fail(A3D_EINVAL, "theta and phi required");
+    if (desc->storage != 0) return fail(A3D_EINVAL, "the allocation-free shot runs FP32 storage only");
+    if (desc->scheme == 2) return fail(A3D_EINVAL, "the allocation-free shot runs schemes 0, 1 and 3");
+    if (desc->snap_interval != 0) return fail(A3D_EINVAL, "the allocation-free shot takes no snapshots");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(A3D_EINVAL, "device %d not present (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    unsigned char* base = nullptr;
+    if ((rc = scratch_base(device, &base))) return rc;
+
+    a3d_ctx c;   // on the host stack: nothing of it is allocated
+    init_geometry(&c, desc, device);
+    if ((rc = check_desc_shot(&c, desc))) return rc;
+    Carver count{nullptr};
+    carve(&c, count);
+    if (count.used > kScratchBytes)
+        return fail(A3D_EINVAL, "shot needs %zu MiB of scratch, the allocation-free entry point has %zu MiB",
+                    (count.used + (1u << 20) - 1) >> 20, kScratchBytes >> 20);
+    Carver cv{base};
+    carve(&c, cv);
+    c.st = (hipStream_t)stream;
+    c.out_device = true;
+    c.shot_dev = shot;
+    c.shot_cap = c.rec_cap;
+    // the caller's work that produced the inputs, and an earlier call's, has finished
+    HIP_TRY(hipStreamSynchronize(c.st));
+    if ((rc = upload_dtab(&c))) return rc;
+
+    const int nx = desc->nx, ny = desc->ny, nz = desc->nz;
+    const Samples sm = model_samples(&c);
+    auto lin = [&](const int* s) { return (size_t)s[0] + (size_t)nz * s[1] + (size_t)nz * nx * s[2]; };
+    if (op == A3D_OP_2ND) {
+        c.stage = const_cast<float*>(vel);   // run_shot pads (v*dt)^2 from it; never freed here
+        c.vv_dt = 0.0f;
+        if ((rc = sample(vel, lin(sm.first), &c.vel_raw1))) return rc;
+    } else {
+        auto pad = [&](const float* src, int v) -> int {
+            HIP_TRY(launch_pad(src, nx, ny, nz, c.off, c.g, c.vol[v], 0, 0, 0.f, c.st));
+            return A3D_OK;
+        };
+        if (!rc) rc = pad(vel, V_VP);
+        if (!rc && (op == A3D_OP_ISO || op == A3D_OP_VTI)) rc = pad(rho, V_RHO);
+        if (!rc && op != A3D_OP_ISO) rc = pad(eps, V_EPS);
+        if (!rc && op != A3D_OP_ISO) rc = pad(del, V_DEL);
+        if (!rc && op == A3D_OP_TTI) rc = pad(theta, V_CT);
+        if (!rc && op == A3D_OP_TTI) rc = pad(phi, V_CP);
+        if (!rc && op == A3D_OP_TTI)
+            HIP_TRY(launch_trig(c.vol[V_CT], c.vol[V_ST], c.vol[V_CP], c.vol[V_SP], c.nvox, 0, c.st));
+        if (!rc) rc = sample(vel, lin(sm.d0), &c.vel_d0);
+        if (!rc) rc = sample(vel, lin(sm.first), &c.vel_first);
+        if (!rc && eps) rc = sample(eps, lin(sm.first), &c.eps_first);
+        if (rc) return rc;
+    }
+    c.model_set = true;
+    c.m4_ok = false;
+    default_shapes(&c);
+    c.tuned = true;
+    return run_shot(&c, desc, nullptr, SnapFn(), nullptr, nullptr);
+}

@@ -324,6 +324,30 @@ int a3d_mlp_loss_grad(a3d_mlp* m, const float* X, const float* T, int n, double*
 int a3d_mlp_train(a3d_mlp* m, const float* X, const float* T, int n, int epochs, int batch, float lr, float lr_end,
                   unsigned seed, double* loss_per_epoch);
 
+/* ----------------------------------------------------------------------- */
+/* Allocation-free shot. No counterpart in the reference: for callers that  */
+/* cannot allocate device memory (tight serving loops, harnesses that        */
+/* forbid hipMalloc).                                                        */
+/* ----------------------------------------------------------------------- */
+/* Size of the library's static scratch array, held by every device the library runs
+ * the entry point below on, until the process ends. */
+#define A3D_STATIC_SCRATCH_MIB 256
+/* One shot of *desc, as a3d_run with FP32 storage (storage 0), stress scheme 0, 1 or 3,
+ * no snapshots, the default launch shape and w in the velocity sweep, calling no
+ * hipMalloc/hipFree or other allocating routine: the padded model, wavefields, tables
+ * and receiver record live in the static scratch array (A3D_STATIC_SCRATCH_MIB MiB per
+ * device); a shot that needs more returns A3D_EINVAL with the size it would need.
+ * vel..phi: DEVICE pointers to the interior model arrays (nx*ny*nz floats, z fastest, as
+ * for a3d_set_model; NULL where the operator uses none). shot: DEVICE pointer to the
+ * record [ny][nx][nt] (time fastest, muted if desc->mute). stream: a hipStream_t, or
+ * NULL for the default stream. The call synchronizes with the stream on entry and
+ * returns after the record is written (it cannot be captured into a graph). Results
+ * equal a3d_run's bit for bit. All calls on one device share its scratch array: calls
+ * that overlap on one device (several host threads) race and must be serialized by the
+ * caller. */
+int a3d_run_static(const a3d_shot_desc* desc, int device, const float* vel, const float* rho, const float* eps,
+                   const float* del, const float* theta, const float* phi, float* shot, void* stream);
+
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
 
