@@ -138,6 +138,8 @@ def load(path=None):
     lib.a3d_set_strip_precision.argtypes = [vp, c_int]
     lib.a3d_set_strip_head.argtypes = [vp, c_int]
     lib.a3d_set_zchunk.argtypes = [vp, c_int]
+    lib.a3d_set_graph.argtypes = [vp, c_int]
+    lib.a3d_set_graph.restype = c_int
     lib.a3d_set_w_sweep.argtypes = [vp, c_int]
     lib.a3d_get_w_sweep.argtypes = [vp, POINTER(ShotDesc), POINTER(c_int)]
     lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_int, c_float, ctypes.c_uint, fp, fp,
@@ -257,6 +259,10 @@ class Modeller:
     def set_zchunk(self, planes):
         """Two-sweep step in z chunks of `planes` planes (0: whole grid); bit-identical results."""
         _check(self.lib, self.lib.a3d_set_zchunk(self._h, planes))
+
+    def set_graph(self, steps):
+        """Time loop as hipGraphs of at most `steps` steps (0: host launches)."""
+        _check(self.lib, self.lib.a3d_set_graph(self._h, int(steps)))
 
     def set_w_sweep(self, mode):
         """Which sweep updates w (ISO, VTI): 0 velocity sweep, 1 stress sweep (bit-identical,

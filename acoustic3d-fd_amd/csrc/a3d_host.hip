@@ -239,6 +239,7 @@ struct a3d_ctx {
     int wz_mode = -1;
     int wz_tuned = 0;
     bool out_device = false;    // the shot record is transposed into shot_dev, the caller's device buffer
+    int graph_steps = 0;        // > 0: the time loop runs as hipGraphs of at most this many steps
 };
 
 namespace {
@@ -380,6 +381,54 @@ void timing_collect(a3d_ctx* c) {
     t.marks.clear();
     t.used = 0;
 }
+
+// The time loop of one shot captured into hipGraphs of at most c->graph_steps steps, each
+// launched as soon as it is instantiated (the host captures the next chunk while the GPU
+// runs this one); a chunk ends before a snapshot, whose host copy cannot be captured.
+// Off while per-kernel timing is on (its events stay ordinary stream events).
+struct StepGraphs {
+    a3d_ctx* c;
+    bool on;
+    bool open = false;
+    int n = 0;
+    std::vector<hipGraphExec_t> execs;
+    explicit StepGraphs(a3d_ctx* ctx) : c(ctx), on(ctx->graph_steps > 0 && !ctx->tm.on) {}
+    int begin() {
+        if (!on || open) return A3D_OK;
+        HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        open = true;
+        n = 0;
+        return A3D_OK;
+    }
+    int end() {
+        if (!open) return A3D_OK;
+        open = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->st, &g);
+        if (e != hipSuccess || !g) return fail(A3D_EHIP, "time-loop capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(A3D_EHIP, "time-loop graph instantiation failed: %s", hipGetErrorString(ei));
+        execs.push_back(ex);
+        HIP_TRY(hipGraphLaunch(ex, c->st));
+        return A3D_OK;
+    }
+    // after the launches of one step; cut = a snapshot or the last step follows
+    int step_done(bool cut) {
+        if (!open) return A3D_OK;
+        return (++n >= c->graph_steps || cut) ? end() : A3D_OK;
+    }
+    ~StepGraphs() {
+        if (open) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->st, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (!execs.empty()) (void)hipStreamSynchronize(c->st);
+        for (auto e : execs) (void)hipGraphExecDestroy(e);
+    }
+};
 
 using SnapFn = std::function<void(int k, int it, const float* host)>;
 
@@ -691,7 +740,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         void* X = c->vol[V_PA];
         void* Y = c->vol[V_PB];
         HIP_TRY(mark(c->ev0, c->st));
+        StepGraphs sg(c);
         for (int it = 0; it < nt; ++it) {
+            if ((rc = sg.begin())) return rc;
             a.X = X;
             a.Y = Y;
             a.ya0 = wlt[it];
@@ -703,6 +754,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             timing_begin(c, 0);
             HIP_TRY(launch_step2(prec, c->ty[0], a, c->st));
             timing_end(c);
+            if ((rc = sg.step_done(want_snap(it) || it == nt - 1))) return rc;
             if (want_snap(it)) {
                 if ((rc = take_snap(it, Y, a.s0, a.ya0, a.s1, a.ya1, true))) return rc;
             }
@@ -762,7 +814,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
             float* nxt[5] = {(float*)c->vol[V_P2], (float*)c->vol[V_Q2], (float*)c->vol[V_U2], (float*)c->vol[V_V2],
                              (float*)c->vol[V_W2]};
             HIP_TRY(mark(c->ev0, c->st));
+            StepGraphs sg(c);
             for (int it = 0; it < nt; it++) {
+                if ((rc = sg.begin())) return rc;
                 fa.P0 = cur[0];
                 fa.Q0 = cur[1];
                 fa.u0 = cur[2];
@@ -779,6 +833,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                 timing_begin(c, 1);
                 HIP_TRY(launch_fused_vti(c->ty[1], fa, c->st));
                 timing_end(c);
+                if ((rc = sg.step_done(want_snap(it) || it == nt - 1))) return rc;
                 for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
                 if (want_snap(it)) {
                     if ((rc = take_snap(it, cur[0], -1, 0.0, -1, 0.0, false))) return rc;
@@ -801,7 +856,9 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         }
         const int nch = (int)vch.size();
         HIP_TRY(mark(c->ev0, c->st));
+        StepGraphs sg(c);
         for (int it = 0; it < nt; it++) {
+            if ((rc = sg.begin())) return rc;
             va.s = srcs[it];
             sa.s = srcs[it];
             sa.rec = c->rec + plane_rec * it;
@@ -830,6 +887,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                 timing_end(c);
             }
             if (strip && (rc = strip_step(c, stp))) return rc;
+            if ((rc = sg.step_done(want_snap(it) || it == nt - 1))) return rc;
             if (want_snap(it)) {
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
             }
@@ -1417,6 +1475,12 @@ int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
         return fail(A3D_ENOMEM, "strip network allocation failed");
     }
     HIP_TRY(hipMemcpy(c->strip_net, params, LB_NPAR * sizeof(float), hipMemcpyHostToDevice));
+    return A3D_OK;
+}
+
+int a3d_set_graph(a3d_ctx* c, int steps) {
+    if (!c || steps < 0) return fail(A3D_EINVAL, "graph steps must be >= 0 (0: no graphs)");
+    c->graph_steps = steps;
     return A3D_OK;
 }
 

@@ -395,8 +395,15 @@ def operators(args):
             mdl.run(shot=False)                     # warm-up (and autotune)
             mdl.set_kernel_timing(True, stride=8)
             mdl.kernel_times(reset=True)
-            _, _, ms = mdl.run(shot=False)
+            _, _, ms_timed = mdl.run(shot=False)
             kms, kn = mdl.kernel_times(reset=True)
+            mdl.set_kernel_timing(False)
+            # loop times without per-kernel events: host-launched steps, then the loop in hipGraphs
+            _, _, ms = mdl.run(shot=False)
+            mdl.set_graph(args.graph_steps)
+            mdl.run(shot=False)
+            _, _, ms_graph = mdl.run(shot=False)
+            mdl.set_graph(0)
             gi = mdl.grid_info()
             launch = mdl.launch_info()
         npts = gi["updated_points"]
@@ -416,7 +423,10 @@ def operators(args):
         g1 = a1 * npts / (k1 * 1e-3) / 1e9 if k1 else 0.0
         gpts = npts * nt / ms / 1e6
         line = {"operator": op, "workload": f"{op}_{c['nx']}x{c['ny']}x{c['nz']}_nt{nt}", "boundary": nbb,
-                "scheme": scheme, "Gpt_s": round(gpts, 2), "loop_ms": round(ms, 1), "updated_points": npts,
+                "scheme": scheme, "Gpt_s": round(gpts, 2), "loop_ms": round(ms, 1),
+                "graph": {"steps_per_graph": args.graph_steps, "loop_ms": round(ms_graph, 2),
+                          "Gpt_s": round(npts * nt / ms_graph / 1e6, 2)},
+                "loop_ms_kernel_events": round(ms_timed, 1), "updated_points": npts,
                 "launch": launch,
                 "kernels": {("k_step2" if op == "2nd" else "k_vel"): {"avg_ms": round(k0, 4), "bytes_per_point": a0,
                                                                        "GBs": round(g0, 1),
@@ -460,6 +470,8 @@ def main():
     ap.add_argument("--strip", action="store_true",
                     help="learned absorbing strip against the 20-cell PML: shots/hour and reflected energy")
     ap.add_argument("--ops", type=str, default="", help="with --operators: comma list of 2nd,iso,vti,tti (default all)")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="with --operators: time steps per captured hipGraph of the graph-mode loop")
     ap.add_argument("--batched-strip", action="store_true",
                     help="learned-strip batched form vs 20-cell-PML per-shot runs (shots/hour, VTI 256^3)")
     ap.add_argument("--strip-profiles", action="store_true",

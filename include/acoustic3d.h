@@ -190,6 +190,13 @@ int a3d_get_launch(a3d_ctx* ctx, int* out6);
 int a3d_set_w_sweep(a3d_ctx* ctx, int mode);
 int a3d_get_w_sweep(a3d_ctx* ctx, const a3d_shot_desc* desc, int* mode);
 
+/* Time loop as hipGraphs (no counterpart in the reference, which launches every kernel
+ * from the host): steps > 0 captures the loop of each a3d_run into graphs of at most
+ * `steps` time steps, cut before every snapshot, each launched as soon as it is
+ * instantiated; 0 (default) launches every step from the host. Bit-identical results.
+ * Not used while per-kernel timing is on. */
+int a3d_set_graph(a3d_ctx* ctx, int steps);
+
 /* 1st-order two-sweep step in z chunks of `planes` planes (0: whole-grid sweeps, the
  * default; else >= 8): per step the velocity sweep of chunk k+1 runs before the stress
  * sweep of chunk k, so the velocities a stress chunk reads were written one launch
