@@ -12,6 +12,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# the same source with a*b+c contracted into fused multiply-adds (the reference's nvcc
+# default): bounds what contraction changes
+FMA_PATH = os.path.join(HERE, "_build", "liboracle_fma.so")
 
 OPS = {"2nd": 0, "iso": 1, "vti": 2, "tti": 3}
 
@@ -37,13 +40,30 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+_fma = None
+
+
+def load_contracted():
+    """The oracle built with contracted multiply-adds (oracle/_build/liboracle_fma.so)."""
+    global _fma
+    if _fma is None:
+        if not os.path.exists(FMA_PATH):
+            raise RuntimeError(f"{FMA_PATH} missing: run make -C oracle")
+        _fma = _declare(ctypes.CDLL(FMA_PATH))
+    return _fma
+
+
 def load():
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
-    lib = ctypes.CDLL(LIB_PATH)
+    _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _declare(lib):
     fp = POINTER(c_float)
     lib.oracle_run.argtypes = [c_int, POINTER(OracleShot), fp, fp, fp, fp, fp, fp, c_void_p, c_void_p, c_int,
                                POINTER(c_double)]
@@ -75,7 +95,6 @@ def load():
     lib.oracle_set_strip_profile.restype = c_int
     lib.oracle_set_tti_fault.argtypes = [c_int]
     lib.oracle_set_tti_fault.restype = c_int
-    _lib = lib
     return lib
 
 
@@ -109,9 +128,11 @@ def make_shot(op, nx, ny, nz, dx, dy, dz, sx, sy, sz, nt, dt, fm, nthreads=0, **
                       d["mute_tt"], nthreads)
 
 
-def run(shot, prec=32, vel=None, rho=None, eps=None, delta=None, theta=None, phi=None, nsnap_max=0):
-    """Runs the oracle; returns (shot (ny,nx,nt), snaps (k,ny,nx,nz) or None, loop seconds)."""
-    lib = load()
+def run(shot, prec=32, vel=None, rho=None, eps=None, delta=None, theta=None, phi=None, nsnap_max=0,
+        contracted=False):
+    """Runs the oracle; returns (shot (ny,nx,nt), snaps (k,ny,nx,nz) or None, loop seconds).
+    contracted=True runs the build with fused multiply-adds."""
+    lib = load_contracted() if contracted else load()
     shape = (shot.ny, shot.nx, shot.nz)
     arrs = [_arr(a, shape) for a in (vel, rho, eps, delta, theta, phi)]
     dt = np.float32 if prec == 32 else np.float64
