@@ -2296,6 +2296,9 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
     }
     if (nclasses) *nclasses = (int)keys.size();
     drop_graph(b);
+    // tile rows and z segments are tuned with w in the velocity sweep (a stress sweep that
+    // updates w runs one z segment), unless the caller fixed the stress sweep for w
+    if (b->wz_mode < 0) b->wz_tuned = 0;
     const int saved_nt = b->d.nt;
     const int tsteps = std::max(6, steps);   // timed from the third step on
     for (auto& d : b->wd) d.nt = tsteps;
@@ -2374,18 +2377,35 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
         b->zs_str = zst;
     }
     // then which sweep updates w (ISO, VTI): both sweeps' group times summed, both ways
+    // (the stress sweep that updates w keeps one z segment, so its tile rows are re-timed
+    // for that form: the tuned ones and each uniform choice; the faster form is kept)
     if (!rc && (b->d.op == A3D_OP_ISO || b->d.op == A3D_OP_VTI) && b->d.scheme != 2 && b->wz_mode < 0) {
-        double ms[2] = {1e30, 1e30};
-        for (int m = 0; m < 2 && !rc; ++m) {
-            b->wz_mode = m;
+        auto timed = [&](double* out) -> int {
             double gms[6] = {0, 0, 0, 0, 0, 0};
             double loop = 0.0;
-            rc = batch_run(b, 1, nullptr, &loop, gms);
-            if (rc == A3D_EUNSTABLE) rc = A3D_OK;
-            ms[m] = gms[0] + gms[1] + gms[2] + gms[3] + gms[4] + gms[5];
+            int r = batch_run(b, 1, nullptr, &loop, gms);
+            if (r == A3D_EUNSTABLE) r = A3D_OK;
+            *out = gms[0] + gms[1] + gms[2] + gms[3] + gms[4] + gms[5];
+            return r;
+        };
+        double ms0 = 1e30, ms1 = 1e30;
+        b->wz_mode = 0;
+        rc = timed(&ms0);
+        const std::vector<int> ty_tuned = b->ty_str;
+        std::vector<int> ty_best1 = ty_tuned;
+        b->wz_mode = 1;
+        for (int cand = 0; cand < 4 && !rc; ++cand) {
+            if (cand > 0) b->ty_str.assign(nw, cand == 1 ? 2 : (cand == 2 ? 4 : 8));
+            double t = 1e30;
+            rc = timed(&t);
+            if (t < ms1) {
+                ms1 = t;
+                ty_best1 = b->ty_str;
+            }
         }
         b->wz_mode = -1;
-        b->wz_tuned = ms[1] < ms[0] ? 1 : 0;
+        b->wz_tuned = ms1 < ms0 ? 1 : 0;
+        b->ty_str = b->wz_tuned ? ty_best1 : ty_tuned;
     }
     for (auto& d : b->wd) d.nt = saved_nt;
     b->d.nt = saved_nt;

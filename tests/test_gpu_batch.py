@@ -113,3 +113,39 @@ def test_batch_with_learned_strip_equals_single_runs(oracle, scheme):
     e = rel_l2(seq[0], ref)
     print(f"\n[gpu-batch] strip scheme {scheme}: window 0 rel L2 vs oracle {e:.2e}")
     assert e <= 1e-5
+
+
+def test_cfg4_size_windows(oracle):
+    """Four windows of cfg4's size range (160-320 x 160-320 x 200-300 cells, npml 40) cut
+    from the 600x600x300 TTI model, scheme 3 as the batched bench runs: sequential plain
+    runs, batched launches and captured graphs bitwise equal after the batch tuner picked
+    tile rows and z segments at these sizes, and the second window against the oracle on
+    its sub-model (40 steps)."""
+    from a3dfd import Batch, Window, make_desc, models
+    c = models.cfg4_model()
+    wins = models.uneven_windows(4, c["nx"], c["ny"], c["nz"], seed=44)
+    nt = 40
+    m = {k: c[k] for k in ("vel", "eps", "delta", "theta", "phi")}
+    desc = make_desc("tti", c["nx"], c["ny"], c["nz"], c["dx"], c["dy"], c["dz"], 1, 1, 1, nt, c["dt"], c["fm"],
+                     scheme=3)
+    print(f"\n[gpu-batch] cfg4-size windows {wins}", flush=True)
+    with Batch(desc, [Window(*w) for w in wins]) as b:
+        b.set_model(**m)
+        ncls = b.tune(steps=6)
+        info = b.info()
+        seq, _ = b.run(mode=0)
+        bat, _ = b.run(mode=1)
+        g1, _ = b.run(mode=2)
+        g2, _ = b.run(mode=2)
+    print(f"[gpu-batch] {ncls} size classes, tile rows {info['tile_rows']}, z segments {info['zseg']}", flush=True)
+    for w in range(len(wins)):
+        assert np.abs(seq[w]).max() > 0
+        for other in (bat[w], g1[w], g2[w]):
+            assert np.array_equal(other, seq[w]), w
+    ox, oy, oz, wx, wy, wz, sx, sy, sz = wins[1]
+    sub = {k: np.ascontiguousarray(a[oy:oy + wy, ox:ox + wx, oz:oz + wz]) for k, a in m.items()}
+    ref, _, _ = oracle.run(oracle.make_shot("tti", wx, wy, wz, c["dx"], c["dy"], c["dz"], sx, sy, sz, nt, c["dt"],
+                                            c["fm"]), 32, **sub)
+    e = rel_l2(seq[1], ref)
+    print(f"[gpu-batch] window 1 ({wx}x{wy}x{wz}) vs oracle: {e:.2e}", flush=True)
+    assert np.abs(ref).max() > 0 and e <= 1e-5

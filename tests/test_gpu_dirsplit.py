@@ -73,7 +73,10 @@ def test_dir_split_fp16(oracle, op):
     e3, e1, es = rel_l2(g3, r64), rel_l2(g1, r64), rel_l2(s3, s64)
     print(f"[gpu-dirsplit] {op}: FP16 scheme 3 vs FP64 {e3:.2e} (snapshot {es:.2e}); scheme 1 {e1:.2e}")
     assert np.isfinite(g3).all()
-    assert e3 <= 2e-2 and es <= 1.5e-1
+    # about twice the errors measured on MI355X (shot 3.3e-3 / 2.7e-3 / 3.1e-3, snapshot
+    # 4.5e-2 / 4.5e-2 / 3.8e-2 for ISO / VTI / TTI)
+    shot_max, snap_max = {"iso": (6.6e-3, 9e-2), "vti": (5.5e-3, 9e-2), "tti": (6.2e-3, 7.6e-2)}[op]
+    assert e3 <= shot_max and es <= snap_max
 
 
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])

@@ -157,8 +157,12 @@ def test_fp16_storage_error_vs_fp64(oracle, op):
     print(f"[gpu-parity] {op}: FP16-storage shot rel-L2 vs FP64 {e:.2e}, snapshot {es:.2e}")
     assert np.isfinite(g16).all()
     # binary16 keeps 11 significant bits: ~5e-4 relative per stored value, accumulated over
-    # the run; the late snapshot is dominated by weak, absorbed energy and has the larger error
-    assert e <= 2e-2 and es <= 1.5e-1
+    # the run; the late snapshot is dominated by weak, absorbed energy and has the larger error.
+    # Bounds: about twice the errors measured on MI355X (shot 2.1e-3 / 3.1e-3 / 2.6e-3 / 3.0e-3,
+    # snapshot 9.2e-3 / 4.2e-2 / 4.2e-2 / 3.7e-2 for 2nd / ISO / VTI / TTI)
+    shot_max, snap_max = {"2nd": (4.2e-3, 1.9e-2), "iso": (6.2e-3, 8.4e-2), "vti": (5.2e-3, 8.4e-2),
+                          "tti": (6.0e-3, 7.4e-2)}[op]
+    assert e <= shot_max and es <= snap_max
 
 
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])
