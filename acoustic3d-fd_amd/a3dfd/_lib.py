@@ -137,6 +137,12 @@ def load(path=None):
     lib.a3d_set_strip_profile.argtypes = [vp, c_float]
     lib.a3d_set_strip_precision.argtypes = [vp, c_int]
     lib.a3d_set_strip_head.argtypes = [vp, c_int]
+    lib.a3d_set_strip_period.argtypes = [vp, c_int]
+    lib.a3d_set_strip_period.restype = c_int
+    lib.a3d_batch_set_strip_head.argtypes = [vp, c_int]
+    lib.a3d_batch_set_strip_head.restype = c_int
+    lib.a3d_batch_set_strip_period.argtypes = [vp, c_int]
+    lib.a3d_batch_set_strip_period.restype = c_int
     lib.a3d_set_zchunk.argtypes = [vp, c_int]
     lib.a3d_set_graph.argtypes = [vp, c_int]
     lib.a3d_set_graph.restype = c_int
@@ -300,9 +306,14 @@ class Modeller:
         _check(self.lib, self.lib.a3d_set_strip_profile(self._h, amp))
 
     def set_strip_head(self, mode):
-        """What the strip's damping factors act on: 0 every field (sponge), 1 the split
-        components along the outward normal axis and the normal velocity (PML-like)."""
+        """What the strip's network does: 0 damp every field (sponge), 1 damp the split
+        components along the outward normal axis and the normal velocity (PML-like), 2 scale
+        each strip cell's PML damping inside the sweeps (coefficient head)."""
         _check(self.lib, self.lib.a3d_set_strip_head(self._h, int(mode)))
+
+    def set_strip_period(self, steps):
+        """Coefficient head: the network runs after the stress sweep of every `steps`-th step."""
+        _check(self.lib, self.lib.a3d_set_strip_period(self._h, int(steps)))
 
     def set_strip_precision(self, mfma):
         """1: strip network on the matrix cores (binary16 operands, FP32 sums); 0: FP32."""
@@ -462,10 +473,15 @@ class Batch:
         return dict(updated_points=pts.value, nwin=n.value, tile_rows=[(tr[2 * w], tr[2 * w + 1]) for w in range(nw)],
                     zseg=(zv.value, zs.value))
 
-    def set_strip_net(self, params, amp=None):
-        """The learned strip on every window (VTI, FP32, nb = STRIP_NB); None: the plain PML."""
+    def set_strip_net(self, params, amp=None, head=None, period=None):
+        """The learned strip on every window (VTI, FP32, nb = STRIP_NB); None: the plain PML.
+        head / period: as Modeller.set_strip_head / set_strip_period."""
         if amp is not None:
             _check(self.lib, self.lib.a3d_batch_set_strip_profile(self._h, float(amp)))
+        if head is not None:
+            _check(self.lib, self.lib.a3d_batch_set_strip_head(self._h, int(head)))
+        if period is not None:
+            _check(self.lib, self.lib.a3d_batch_set_strip_period(self._h, int(period)))
         if params is None:
             _check(self.lib, self.lib.a3d_batch_set_strip_net(self._h, None, 0))
             return

@@ -160,6 +160,29 @@ void pml_axis(T dt, T d0, int n, int npml, T* c1, T* c2, T* a1, T* a2, int width
     }
 }
 
+// Half the damping step of each index, hv = dt d (r) / 2 on the velocity (half-cell) and
+// hs = dt d (ra) / 2 on the stress nodes of initial_coffe's profile, 0 outside the PML: the
+// learned strip's coefficient head rebuilds c1, c2 per cell from a multiple of them.
+template <class T>
+void pml_half_steps(T dt, T d0, int n, int npml, T* hv, T* hs, int width) {
+    const T dtd0 = dt * d0;
+    for (int id = 0; id < n + 2 * npml; ++id) {
+        double r, ra;
+        if (id < npml) {
+            r = (npml - 0.5 - id) / width;
+            ra = ((npml - id) * 1.0) / width;
+        } else if (id < npml + n) {
+            hv[id] = hs[id] = (T)0;
+            continue;
+        } else {
+            r = (0.5 + id - n - npml) / width;
+            ra = ((id - n - npml) * 1.0) / width;
+        }
+        hv[id] = (T)((dtd0 * (r * r)) / 2.0);
+        hs[id] = (T)((dtd0 * (ra * ra)) / 2.0);
+    }
+}
+
 // cuda_absorb_bndr factor (reference Toa_3dfd_cuda.cu:138-151): qp*r^2 + 1
 // inside the npd-wide bands, 1.0 elsewhere.
 double sponge_value(int i, int n, int npd) {
@@ -232,7 +255,10 @@ struct a3d_ctx {
     float2* strip_delta = nullptr;
     float strip_amp = 10.0f;    // PML amplitude of the strip's 6-cell ramp
     int strip_mfma = 1;         // network on the matrix cores (binary16 operands) or FP32 per lane
-    int strip_head = 0;         // 0: damping of every field (sponge); 1: normal-axis PML components
+    int strip_head = 0;         // 0: damping of every field (sponge); 1: normal-axis PML components;
+                                // 2: multiplier of the PML damping inside the sweeps (coefficient head)
+    float* strip_mfac = nullptr;   // head 2: damping multiplier per padded point (1 outside the strip)
+    int strip_period = 1;          // head 2: the network runs after the stress sweep of every period-th step
     int zchunk = 0;             // 1st-order two-sweep step in z chunks of this many planes (0: whole grid)
     // which sweep updates w (ISO, VTI): requested mode (-1 tuner, 0 velocity sweep,
     // 1 stress sweep) and the tuner's choice for mode -1
@@ -456,7 +482,8 @@ Scales field_scales(const a3d_ctx* c, const a3d_shot_desc* d) {
 // The stress sweep can update w (from its own column's old P or Q, one z segment)
 // for ISO and VTI in the two-sweep step.
 bool wz_eligible(const a3d_ctx* c, const a3d_shot_desc* d) {
-    return (d->op == A3D_OP_ISO || d->op == A3D_OP_VTI) && d->scheme != 2 && c->zchunk == 0;
+    return (d->op == A3D_OP_ISO || d->op == A3D_OP_VTI) && d->scheme != 2 && c->zchunk == 0 &&
+           !(c->strip_net && c->strip_head == 2);
 }
 bool wz_active(const a3d_ctx* c, const a3d_shot_desc* d) {
     return wz_eligible(c, d) && (c->wz_mode == 1 || (c->wz_mode < 0 && c->wz_tuned));
@@ -481,7 +508,7 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     const int width = strip ? STRIP_CELLS : nb;
     const double amp = strip ? (double)c->strip_amp : 10.0;
     const T d0 = (T)(amp * (T)c->vel_d0 * std::log(100000.0) / (2.0 * width * (dsum / 3.0)));
-    std::vector<T> tabh(4 * (size_t)(g.nnx + g.nny + g.nnz));
+    std::vector<T> tabh(6 * (size_t)(g.nnx + g.nny + g.nnz));
     Pml pm{};
     pm.cx1 = 0;
     pm.cx2 = g.nnx;
@@ -499,6 +526,15 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     pml_axis<T>(dt, d0, nx, nb, t + pm.cx1, t + pm.cx2, t + pm.ax1, t + pm.ax2, width);
     pml_axis<T>(dt, d0, ny, nb, t + pm.cy1, t + pm.cy2, t + pm.ay1, t + pm.ay2, width);
     pml_axis<T>(dt, d0, nz, nb, t + pm.cz1, t + pm.cz2, t + pm.az1, t + pm.az2, width);
+    pm.hvx = 4 * (g.nnx + g.nny + g.nnz);
+    pm.hsx = pm.hvx + g.nnx;
+    pm.hvy = pm.hsx + g.nnx;
+    pm.hsy = pm.hvy + g.nny;
+    pm.hvz = pm.hsy + g.nny;
+    pm.hsz = pm.hvz + g.nnz;
+    pml_half_steps<T>(dt, d0, nx, nb, t + pm.hvx, t + pm.hsx, width);
+    pml_half_steps<T>(dt, d0, ny, nb, t + pm.hvy, t + pm.hsy, width);
+    pml_half_steps<T>(dt, d0, nz, nb, t + pm.hvz, t + pm.hsz, width);
     // The planes whose z stress factor is exactly 1 form one run (the factor's argument
     // grows away from the interior on both sides); the stress sweep takes the run from
     // its bounds instead of reading the factor per plane.
@@ -610,6 +646,17 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
         sa.m4 = c->vol[V_M4];
         if (op == A3D_OP_TTI) va.t4 = c->vol[V_M4];
     }
+    // learned strip, coefficient head: the multipliers start at 1 (the plain ramp) until the
+    // network first runs
+    va.mfac = sa.mfac = nullptr;
+    if (c->strip_net && c->strip_head == 2) {
+        if (!c->strip_mfac && hipMalloc(&c->strip_mfac, c->nvox * sizeof(float)) != hipSuccess) {
+            c->strip_mfac = nullptr;
+            return fail(A3D_ENOMEM, "strip multiplier allocation failed");
+        }
+        HIP_TRY(launch_fill(c->strip_mfac, c->nvox, 1.0f, c->st));
+        va.mfac = sa.mfac = c->strip_mfac;
+    }
     va.s.s0 = s0;
     // The time grid is the reference's float accumulation t = dt, t += dt and the gate
     // t <= 2*tdelay is taken in float (iso:90, :521) in every precision.
@@ -654,12 +701,19 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     a.cdt[1] = d->dt / d->dy;
     a.cdt[2] = d->dt / d->dz;
     a.dirsplit = d->scheme == 3;
-    a.normal = c->strip_head;
+    a.normal = c->strip_head == 1;
+    a.mfac = c->strip_head == 2 ? c->strip_mfac : nullptr;
     return a;
 }
 
-// Strip correction after a stress sweep (learned absorbing strip), on stream st.
-int strip_step_on(a3d_ctx* c, const StripArgs& a, hipStream_t st) {
+// Learned absorbing strip after the stress sweep of step it, on stream st: the damping
+// heads (0, 1) evaluate the network and apply its factors; the coefficient head (2) writes
+// the cells' damping multipliers, read by the next steps' sweeps, every strip_period steps.
+int strip_step_on(a3d_ctx* c, const StripArgs& a, hipStream_t st, int it) {
+    if (c->strip_head == 2) {
+        if (it % c->strip_period == 0) HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
+        return A3D_OK;
+    }
     HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
     HIP_TRY(launch_strip_apply(a, c->strip_delta, (float*)c->vol[V_P], (float*)c->vol[V_Q], (float*)c->vol[V_PX],
                                (float*)c->vol[V_PY], (float*)c->vol[V_PZ], (float*)c->vol[V_QX], (float*)c->vol[V_QY],
@@ -668,7 +722,7 @@ int strip_step_on(a3d_ctx* c, const StripArgs& a, hipStream_t st) {
     return A3D_OK;
 }
 
-int strip_step(a3d_ctx* c, const StripArgs& a) { return strip_step_on(c, a, c->st); }
+int strip_step(a3d_ctx* c, const StripArgs& a, int it) { return strip_step_on(c, a, c->st, it); }
 
 template <class T>
 int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
@@ -886,7 +940,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                 HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
                 timing_end(c);
             }
-            if (strip && (rc = strip_step(c, stp))) return rc;
+            if (strip && (rc = strip_step(c, stp, it))) return rc;
             if ((rc = sg.step_done(want_snap(it) || it == nt - 1))) return rc;
             if (want_snap(it)) {
                 if ((rc = take_snap(it, c->vol[V_P], -1, 0.0, -1, 0.0, false))) return rc;
@@ -1165,7 +1219,7 @@ void init_geometry(a3d_ctx* c, const a3d_shot_desc* desc, int device) {
 }
 
 // Elements of the PML table (arithmetic type) and of the double table (sponge, taper).
-size_t table_count(const a3d_ctx* c) { return 4 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz); }
+size_t table_count(const a3d_ctx* c) { return 6 * (size_t)(c->g.nnx + c->g.nny + c->g.nnz); }
 size_t dtable_count(const a3d_ctx* c) { return (size_t)c->g.nnx + c->g.nny + c->g.nnz + 226; }
 
 // Double tables: sponge factors (2nd order) and SV taper (VTI/TTI), uploaded to c->dtab.
@@ -1246,7 +1300,8 @@ int a3d_destroy(a3d_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->dtab) (void)hipFree(c->dtab);
     if (c->flag) (void)hipFree(c->flag);
-    for (void* p : {(void*)c->strip_net, (void*)c->strip_ids, (void*)c->strip_meta, (void*)c->strip_delta})
+    for (void* p : {(void*)c->strip_net, (void*)c->strip_ids, (void*)c->strip_meta, (void*)c->strip_delta,
+                    (void*)c->strip_mfac})
         if (p) (void)hipFree(p);
     for (auto e : c->tm.pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1498,8 +1553,14 @@ int a3d_set_strip_precision(a3d_ctx* c, int mfma) {
 }
 
 int a3d_set_strip_head(a3d_ctx* c, int mode) {
-    if (!c || mode < 0 || mode > 1) return fail(A3D_EINVAL, "strip head mode must be 0 or 1");
+    if (!c || mode < 0 || mode > 2) return fail(A3D_EINVAL, "strip head mode must be 0, 1 or 2");
     c->strip_head = mode;
+    return A3D_OK;
+}
+
+int a3d_set_strip_period(a3d_ctx* c, int steps) {
+    if (!c || steps < 1) return fail(A3D_EINVAL, "strip network period must be >= 1 step");
+    c->strip_period = steps;
     return A3D_OK;
 }
 
@@ -1526,6 +1587,7 @@ int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every,
         return fail(A3D_EINVAL, "strip sampling: same interior, teacher boundary wider than the student's");
     if (!t->model_set || !s->model_set) return fail(A3D_ESTATE, "model not set");
     if (!s->strip_ids) return fail(A3D_ESTATE, "student strip cells not built (set a strip network first)");
+    if (s->strip_head == 2) return fail(A3D_EINVAL, "teacher-forced samples are taken for the damping heads (0, 1)");
     int rc = check_desc_shot(t, d);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(t->device));
@@ -1986,7 +2048,9 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     // the wavefield memsets of prepare_first ran on the windows' streams
     for (auto* c : b->ctx) HIP_TRY(hipStreamSynchronize(c->st));
     // the batched sweeps' w mode, the same for every window
-    const bool wz = (op == A3D_OP_ISO || op == A3D_OP_VTI) && b->d.scheme != 2 &&
+    // learned strip with the coefficient head on the windows: w stays in the velocity sweep
+    const bool sm = b->ctx[0]->strip_net && b->ctx[0]->strip_head == 2;
+    const bool wz = (op == A3D_OP_ISO || op == A3D_OP_VTI) && b->d.scheme != 2 && !sm &&
                     (b->wz_mode == 1 || (b->wz_mode < 0 && b->wz_tuned));
     for (int w = 0; w < nw; ++w) va[w].wz = sa[w].wz = wz ? 1 : 0;
     const bool replay = mode == 2 && !b->exec.empty();
@@ -2030,7 +2094,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const BatchGroup& g = gv[k];
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k], b->st));
             HIP_TRY(launch_vel_batch(b->ctx[0]->prec, op, g.ty, (const VelArgs*)g.args, g.base, (int)g.wins.size(),
-                                     g.lb, g.src, it, wz, b->st));
+                                     g.lb, g.src, it, wz, b->st, sm));
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k + 1], b->st));
         }
         for (size_t k = 0; k < gs.size(); ++k) {
@@ -2038,11 +2102,11 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const size_t e = 2 * (gv.size() + k);
             if (timed) HIP_TRY(hipEventRecord(gev[e], b->st));
             HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
-                                        (int)g.wins.size(), g.lb, it, b->d.scheme == 3, wz, g.src, b->st));
+                                        (int)g.wins.size(), g.lb, it, b->d.scheme == 3, wz, g.src, b->st, sm));
             if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
         }
         for (size_t k = 0; k < swin.size(); ++k) {
-            const int rc2 = strip_step_on(b->ctx[swin[k]], sargs[k], b->st);
+            const int rc2 = strip_step_on(b->ctx[swin[k]], sargs[k], b->st, it);
             if (rc2) return rc2;
         }
         return A3D_OK;
@@ -2379,7 +2443,8 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {
     // then which sweep updates w (ISO, VTI): both sweeps' group times summed, both ways
     // (the stress sweep that updates w keeps one z segment, so its tile rows are re-timed
     // for that form: the tuned ones and each uniform choice; the faster form is kept)
-    if (!rc && (b->d.op == A3D_OP_ISO || b->d.op == A3D_OP_VTI) && b->d.scheme != 2 && b->wz_mode < 0) {
+    const bool strip_coef = b->ctx[0]->strip_net && b->ctx[0]->strip_head == 2;
+    if (!rc && (b->d.op == A3D_OP_ISO || b->d.op == A3D_OP_VTI) && b->d.scheme != 2 && b->wz_mode < 0 && !strip_coef) {
         auto timed = [&](double* out) -> int {
             double gms[6] = {0, 0, 0, 0, 0, 0};
             double loop = 0.0;
@@ -2426,6 +2491,26 @@ int a3d_batch_set_strip_net(a3d_batch* b, const float* params, int n) {
     drop_graph(b);
     for (a3d_ctx* c : b->ctx) {
         const int rc = a3d_set_strip_net(c, params, n);
+        if (rc) return rc;
+    }
+    return A3D_OK;
+}
+
+int a3d_batch_set_strip_head(a3d_batch* b, int mode) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    drop_graph(b);
+    for (a3d_ctx* c : b->ctx) {
+        const int rc = a3d_set_strip_head(c, mode);
+        if (rc) return rc;
+    }
+    return A3D_OK;
+}
+
+int a3d_batch_set_strip_period(a3d_batch* b, int steps) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    drop_graph(b);
+    for (a3d_ctx* c : b->ctx) {
+        const int rc = a3d_set_strip_period(c, steps);
         if (rc) return rc;
     }
     return A3D_OK;

@@ -55,7 +55,15 @@ struct Scales {
 struct Pml {
     int cx1, cx2, cy1, cy2, cz1, cz2;   // velocity sweep (half-cell)
     int ax1, ax2, ay1, ay2, az1, az2;   // stress sweep
+    // half the damping step dt*d of each index (0 outside the PML), velocity and stress
+    // profiles: the learned strip's coefficient head recomputes c1 = 1/(1 + m h),
+    // c2 = c1 (1 - m h) per cell from them with the cell's multiplier m
+    int hvx, hvy, hvz, hsx, hsy, hsz;
 };
+
+// Learned strip, coefficient head: the damping multiplier m of a cell is
+// STRIP_MMAX * sigmoid(y_0), so m = 1 (the plain ramp) at y_0 = -ln 2.
+constexpr float STRIP_MMAX = 3.0f;
 
 // Transient point source of the 1st-order operators: added to P (and Q) on
 // every load of the velocity sweep, never stored (the stress sweep rewrites
@@ -84,6 +92,7 @@ struct VelArgs {
     Src s;
     Scales sc;
     int wz;                    // 1 (ISO, VTI): w is updated by the stress sweep instead
+    const float* mfac;         // learned strip, coefficient head: damping multiplier per padded point, or null
 };
 
 struct StressArgs {
@@ -112,6 +121,7 @@ struct StressArgs {
     Src s;                     // wz: this step's source, added to P / Q as the velocity sweep reads them
     const void* m4;            // VTI, float model: (vp, rho, eps, delta) interleaved per point, or null;
                                // TTI, float model: (cos, sin tilt, cos, sin azimuth); ISO: (vp, rho)
+    const float* mfac;         // learned strip, coefficient head: damping multiplier per padded point, or null
 };
 
 // One-pass VTI step (FP32, scheme-1 arithmetic): old fields in, new fields out
@@ -163,11 +173,14 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 // wz (ISO, VTI): w is updated by the stress sweep (all windows of a batch share it; the
 // stress sweep then takes the per-step sources srcs[it * nwin + w] too, and its windows
 // run one z segment per tile column).
+// sm (VTI, FP32, w in the velocity sweep): the windows carry the learned strip's damping
+// multipliers (mfac)
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
-                            const Src* srcs, int it, bool wz, hipStream_t st);
+                            const Src* srcs, int it, bool wz, hipStream_t st, bool sm = false);
 // dirs: the windows' stress args have unsplit == 2 (all windows of a batch share it)
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st);
+                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st,
+                               bool sm = false);
 
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.
@@ -232,6 +245,7 @@ struct StripArgs {
     float cdt[3];   // dt/dx, dt/dy, dt/dz
     int dirsplit;   // the stress sweep keeps split fields only for a cell's damped directions
     int normal;     // head mode 1: damp the normal-axis split components and velocity only
+    float* mfac;    // head mode 2: the network writes each strip cell's damping multiplier here
 };
 
 // delta[k] = the damping factors (g_P, g_Q) of every strip cell (after the stress sweep);
@@ -247,6 +261,9 @@ hipError_t launch_strip_sample(const StripArgs& a, const float* Pt, const float*
                                int* count, hipStream_t st);
 // Copies a teacher field onto the student grid's updated region (index shift off).
 hipError_t launch_strip_copy(const float* src, const Geo& gs, float* dst, const Geo& gd, int off, hipStream_t st);
+
+// a[0..n) = v.
+hipError_t launch_fill(float* a, size_t n, float v, hipStream_t st);
 
 // Sets *flag to 1 if any of the n values is not finite.
 hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st);

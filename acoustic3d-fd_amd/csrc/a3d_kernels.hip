@@ -196,6 +196,15 @@ __device__ __forceinline__ C damp3(C v, double fy, double fx, double fz) {
 // own branch). Every lane loads: indices outside the grid are clamped to its
 // edge (entries no updated point reads), and lanes without an x-halo column
 // re-read their own point.
+// Split-PML coefficients of a learned-strip cell from half its damping step h = dt d / 2
+// and the network's multiplier m: c1 = 1 / (1 + m h), c2 = c1 (1 - m h) (h = 0: 1, 1).
+template <class C>
+__device__ __forceinline__ void strip_coef(C m, C h, C& c1, C& c2) {
+    const C t = m * h;
+    c1 = (C)1 / ((C)1 + t);
+    c2 = c1 * ((C)1 - t);
+}
+
 template <int TY, bool HX, bool HY, class C>
 struct Halo {
     static_assert((2 * MM * TX) % (TX * TY) == 0, "the halo rows split evenly over the workgroup");
@@ -242,7 +251,9 @@ __device__ __forceinline__ void store_halo(C (*s)[TX + 2 * MM], const Halo<TY, H
 // --------------------------------------------------------------------------
 // WZ (ISO, VTI): the stress sweep makes w (from its own column's P or Q), so this
 // sweep updates u and v only and reads neither the z window nor w.
-template <class PR, int OP, int TY, bool WZ = false>
+// SM (VTI, FP32, w in this sweep): learned strip with the coefficient head, whose
+// cells take their damping coefficients from their multiplier in a.mfac.
+template <class PR, int OP, int TY, bool WZ = false, bool SM = false>
 __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using M = typename PR::M;
@@ -303,9 +314,22 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const C cx1 = T[a.c.cx1 + xs], cx2 = T[a.c.cx2 + xs];
     const C cy1 = T[a.c.cy1 + yl], cy2 = T[a.c.cy2 + yl];
     const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
+    static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
+    C hx = 0, hy = 0;
+    if constexpr (SM) {
+        hx = T[a.c.hvx + xs];
+        hy = T[a.c.hvy + yl];
+    }
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
+        C hz = 0, mc;
+        bool scell = false;
+        if constexpr (SM) {
+            hz = T[a.c.hvz + z];
+            scell = hx != (C)0 || hy != (C)0 || hz != (C)0;
+            if (scell) mc = (C)bld<NT>(a.mfac, zo, colc);
+        }
         if constexpr (P_Z) wP[7] = ldP(zo + 4 * g.pl, col);
         if constexpr (HAS_Q) wQ[7] = ldQ(zo + 4 * g.pl, col);
         C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
@@ -371,24 +395,32 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
             }
-            stw(U, zo, col, cx2 * u0 - cx1 * dtx * xx, sc.inv[F_U]);
-            stw(V, zo, col, cy2 * v0 - cy1 * dty * yy, sc.inv[F_V]);
-            if constexpr (!WZ) stw(Wv, zo, col, cz2 * w0 - cz1 * dtz * zz, sc.inv[F_W]);
+            C kx1 = cx1, kx2 = cx2, ky1 = cy1, ky2 = cy2, kz1 = cz1, kz2 = cz2;
+            if constexpr (SM) {
+                if (scell) {
+                    strip_coef<C>(mc, hx, kx1, kx2);
+                    strip_coef<C>(mc, hy, ky1, ky2);
+                    strip_coef<C>(mc, hz, kz1, kz2);
+                }
+            }
+            stw(U, zo, col, kx2 * u0 - kx1 * dtx * xx, sc.inv[F_U]);
+            stw(V, zo, col, ky2 * v0 - ky1 * dty * yy, sc.inv[F_V]);
+            if constexpr (!WZ) stw(Wv, zo, col, kz2 * w0 - kz1 * dtz * zz, sc.inv[F_W]);
         }
     }
 }
 
-template <class PR, int OP, int TY, bool WZ = false>
+template <class PR, int OP, int TY, bool WZ = false, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_vel(const VelArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
-    vel_body<PR, OP, TY, WZ>(a, a.s, tix, tiy, seg);
+    vel_body<PR, OP, TY, WZ, SM>(a, a.s, tix, tiy, seg);
 }
 
 // Batched velocity sweep: the tiles of all windows of a batch in one launch; per-window
 // arguments in device memory, per-step sources srcs[it * nwin + w]. WZ (ISO, VTI): the
 // stress sweep updates w.
-template <class PR, int OP, int TY, bool WZ = false>
+template <class PR, int OP, int TY, bool WZ = false, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict__ wins, const int* __restrict__ base,
                                                       int nwin, Launch lb, const Src* __restrict__ srcs, int it) {
     const int t = remap_tile(lb);
@@ -397,7 +429,7 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
     const VelArgs a = wins[w];
     int tix, tiy, seg;
     split_tile(a.l, t - base[w], tix, tiy, seg);
-    vel_body<PR, OP, TY, WZ>(a, srcs[(size_t)it * nwin + w], tix, tiy, seg);
+    vel_body<PR, OP, TY, WZ, SM>(a, srcs[(size_t)it * nwin + w], tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -407,7 +439,9 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
 // P (ISO) or Q (VTI) -- the velocity sweep's arithmetic, bit for bit -- one plane
 // ahead of where the backward window needs it. The column reads F = P or Q at
 // planes z..z+7 before writing plane z, so no other tile's values are involved.
-template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false>
+// SM (VTI, FP32, w in the velocity sweep): learned strip with the coefficient head, whose
+// cells take their damping coefficients from their multiplier in a.mfac.
+template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false, bool SM = false>
 __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using SP = typename PR::SP;
@@ -534,6 +568,12 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     // undamped remainder, which takes the undamped directions' updates without damping.
     constexpr bool dirs = DIRS;
     const bool dX = ax1 != (C)1, dY = ay1 != (C)1;
+    static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
+    C hx = 0, hy = 0;
+    if constexpr (SM) {
+        hx = T[a.c.hsx + xs];
+        hy = T[a.c.hsy + yl];
+    }
 
     for (int z = zb; z < ze; ++z) {
         const int zo = z * g.pl;
@@ -623,8 +663,9 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         // One-sided branches with distinct destinations, left unset where not loaded:
         // an if/else writing the same registers, or zeroing them first, makes the wave
         // wait for earlier loads into those registers.
-        C f0, f1, f2, f3, f4, f5, p0, q0;
+        C f0, f1, f2, f3, f4, f5, p0, q0, mc;
         if (valid && split) {
+            if constexpr (SM) mc = (C)bld<NT>(a.mfac, zo, col);
             if (sx) f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
             if (sy) f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
             if (sz) f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
@@ -765,12 +806,18 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     }
                     C npx, npy, npz, nqx, nqy, nqz;
                     if constexpr (VTI) {
-                        npx = ax2 * px0 - ax1 * r0 * c0 * c0 * e2 * dtx * xx;
-                        npy = ay2 * py0 - ay1 * r0 * c0 * c0 * e2 * dty * yy;
-                        npz = az2 * pz0 - az1 * r0 * c0 * c0 * s2 * dtz * zz;
-                        nqx = ax2 * qx0 - ax1 * r0 * c0 * c0 * s2 * dtx * xx;
-                        nqy = ay2 * qy0 - ay1 * r0 * c0 * c0 * s2 * dty * yy;
-                        nqz = az2 * qz0 - az1 * r0 * c0 * c0 * dtz * zz;
+                        C bx1 = ax1, bx2 = ax2, by1 = ay1, by2 = ay2, bz1 = az1, bz2 = az2;
+                        if constexpr (SM) {
+                            strip_coef<C>(mc, hx, bx1, bx2);
+                            strip_coef<C>(mc, hy, by1, by2);
+                            strip_coef<C>(mc, (C)T[a.c.hsz + z], bz1, bz2);
+                        }
+                        npx = bx2 * px0 - bx1 * r0 * c0 * c0 * e2 * dtx * xx;
+                        npy = by2 * py0 - by1 * r0 * c0 * c0 * e2 * dty * yy;
+                        npz = bz2 * pz0 - bz1 * r0 * c0 * c0 * s2 * dtz * zz;
+                        nqx = bx2 * qx0 - bx1 * r0 * c0 * c0 * s2 * dtx * xx;
+                        nqy = by2 * qy0 - by1 * r0 * c0 * c0 * s2 * dty * yy;
+                        nqz = bz2 * qz0 - bz1 * r0 * c0 * c0 * dtz * zz;
                     } else {
                         npx = ax2 * px0 - ax1 * c0 * c0 * e2 * dtx * xx;
                         npy = ay2 * py0 - ay1 * c0 * c0 * e2 * dty * yy;
@@ -805,17 +852,17 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     }
 }
 
-template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false>
+template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
-    stress_body<PR, OP, TY, WZ, DIRS>(a, a.rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, WZ, DIRS, SM>(a, a.rec, tix, tiy, seg);
 }
 
 // Batched stress sweep; a window's record plane of step it is rec + it * nx * ny. WZ
 // (ISO, VTI; one z segment per tile column): this sweep also updates w, with the
 // step's source of the window (srcs[it * nwin + w], as the velocity sweep adds it).
-template <class PR, int OP, int TY, bool DIRS = false, bool WZ = false>
+template <class PR, int OP, int TY, bool DIRS = false, bool WZ = false, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __restrict__ wins,
                                                          const int* __restrict__ base, int nwin, Launch lb, int it,
                                                          const Src* __restrict__ srcs) {
@@ -827,7 +874,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __res
     int tix, tiy, seg;
     split_tile(a.l, t - base[w], tix, tiy, seg);
     float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
-    stress_body<PR, OP, TY, WZ, DIRS>(a, rec, tix, tiy, seg);
+    stress_body<PR, OP, TY, WZ, DIRS, SM>(a, rec, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -1008,6 +1055,10 @@ __global__ void k_pack2(const float* __restrict__ vp, const float* __restrict__ 
         out[i] = f32x2{vp[i], rho[i]};
 }
 
+__global__ void k_fill(float* __restrict__ a, size_t n, float v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
 __global__ void k_check_finite(const float* __restrict__ a, size_t n, int* flag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1036,25 +1087,25 @@ __global__ void k_probe_read(const float4* __restrict__ a, size_t n, float* out)
 }
 
 // ---- launch dispatch -------------------------------------------------------
-template <class PR, int OP, bool WZ>
+template <class PR, int OP, bool WZ, bool SM = false>
 hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_vel<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_vel<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_vel<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_vel<PR, OP, 2, WZ, SM>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_vel<PR, OP, 4, WZ, SM>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_vel<PR, OP, 8, WZ, SM>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool WZ, bool DIRS>
+template <class PR, int OP, bool WZ, bool DIRS, bool SM = false>
 hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2, WZ, DIRS>), grid, dim3(TX, 2), 0, st, a); break;
-        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4, WZ, DIRS>), grid, dim3(TX, 4), 0, st, a); break;
-        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8, WZ, DIRS>), grid, dim3(TX, 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stress<PR, OP, 2, WZ, DIRS, SM>), grid, dim3(TX, 2), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((k_stress<PR, OP, 4, WZ, DIRS, SM>), grid, dim3(TX, 4), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_stress<PR, OP, 8, WZ, DIRS, SM>), grid, dim3(TX, 8), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1072,27 +1123,27 @@ hipError_t step2_ty(int ty, const Step2Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool WZ>
+template <class PR, int OP, bool WZ, bool SM = false>
 hipError_t vel_batch_ty(int ty, const VelArgs* w, const int* base, int nwin, const Launch& lb, const Src* srcs, int it,
                         hipStream_t st) {
     const dim3 grid(lb.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_vel_batch<PR, OP, 2, WZ>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, srcs, it); break;
-        case 4: hipLaunchKernelGGL((k_vel_batch<PR, OP, 4, WZ>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, srcs, it); break;
-        case 8: hipLaunchKernelGGL((k_vel_batch<PR, OP, 8, WZ>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 2: hipLaunchKernelGGL((k_vel_batch<PR, OP, 2, WZ, SM>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 4: hipLaunchKernelGGL((k_vel_batch<PR, OP, 4, WZ, SM>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, srcs, it); break;
+        case 8: hipLaunchKernelGGL((k_vel_batch<PR, OP, 8, WZ, SM>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, srcs, it); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool DIRS, bool WZ>
+template <class PR, int OP, bool DIRS, bool WZ, bool SM = false>
 hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
                            const Src* srcs, hipStream_t st) {
     const dim3 grid(lb.grid);
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2, DIRS, WZ>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it, srcs); break;
-        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4, DIRS, WZ>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it, srcs); break;
-        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8, DIRS, WZ>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it, srcs); break;
+        case 2: hipLaunchKernelGGL((k_stress_batch<PR, OP, 2, DIRS, WZ, SM>), grid, dim3(TX, 2), 0, st, w, base, nwin, lb, it, srcs); break;
+        case 4: hipLaunchKernelGGL((k_stress_batch<PR, OP, 4, DIRS, WZ, SM>), grid, dim3(TX, 4), 0, st, w, base, nwin, lb, it, srcs); break;
+        case 8: hipLaunchKernelGGL((k_stress_batch<PR, OP, 8, DIRS, WZ, SM>), grid, dim3(TX, 8), 0, st, w, base, nwin, lb, it, srcs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1202,6 +1253,10 @@ int occ_prec(int kind, int op, int ty) {
 }  // namespace
 
 hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st) {
+    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32, w in the velocity sweep
+        if (prec != PREC_FP32 || op != 2 || a.wz) return hipErrorInvalidValue;
+        return vel_ty<Fp32, 2, false, true>(ty, a, st);
+    }
     switch (prec) {
         case PREC_FP32: return vel_op<Fp32>(op, ty, a, st);
         case PREC_FP16: return vel_op<Fp16>(op, ty, a, st);
@@ -1211,6 +1266,11 @@ hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st
 }
 
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st) {
+    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32, w in the velocity sweep
+        if (prec != PREC_FP32 || op != 2 || a.wz) return hipErrorInvalidValue;
+        return a.unsplit == 2 ? stress_ty<Fp32, 2, false, true, true>(ty, a, st)
+                              : stress_ty<Fp32, 2, false, false, true>(ty, a, st);
+    }
     switch (prec) {
         case PREC_FP32: return stress_op<Fp32>(op, ty, a, st);
         case PREC_FP16: return stress_op<Fp16>(op, ty, a, st);
@@ -1220,7 +1280,11 @@ hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStrea
 }
 
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
-                            const Src* srcs, int it, bool wz, hipStream_t st) {
+                            const Src* srcs, int it, bool wz, hipStream_t st, bool sm) {
+    if (sm) {
+        if (prec != PREC_FP32 || op != 2 || wz) return hipErrorInvalidValue;
+        return vel_batch_ty<Fp32, 2, false, true>(ty, wins, base, nwin, lb, srcs, it, st);
+    }
     switch (prec) {
         case PREC_FP32: return vel_batch_op<Fp32>(op, ty, wins, base, nwin, lb, srcs, it, wz, st);
         case PREC_FP16: return vel_batch_op<Fp16>(op, ty, wins, base, nwin, lb, srcs, it, wz, st);
@@ -1230,7 +1294,12 @@ hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const
 }
 
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st) {
+                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st, bool sm) {
+    if (sm) {
+        if (prec != PREC_FP32 || op != 2 || wz) return hipErrorInvalidValue;
+        return dirs ? stress_batch_ty<Fp32, 2, true, false, true>(ty, wins, base, nwin, lb, it, srcs, st)
+                    : stress_batch_ty<Fp32, 2, false, false, true>(ty, wins, base, nwin, lb, it, srcs, st);
+    }
     switch (prec) {
         case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
         case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
@@ -1322,6 +1391,12 @@ hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, con
 hipError_t launch_pack2(const float* vp, const float* rho, void* out, size_t n, hipStream_t s) {
     const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_pack2, dim3(grid), dim3(256), 0, s, vp, rho, (f32x2*)out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* a, size_t n, float v, hipStream_t st) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, a, n, v);
     return hipGetLastError();
 }
 

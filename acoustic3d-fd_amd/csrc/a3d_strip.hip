@@ -19,6 +19,10 @@
 // Q at centre, inward, outward, and the outward normal velocity times the
 // impedance rho*vp; model features vp*dt/d_a, epsilon, delta and the depth in
 // the strip.
+// Head mode 2 ("coefficient"): the network's first output gives each strip cell a
+// multiplier m = STRIP_MMAX * sigmoid(y_0) of its PML damping d; the velocity and
+// stress sweeps recompute the cell's split-PML coefficients from m d inside their
+// update, so nothing is applied after the sweeps (m = 1: the plain ramp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,6 +79,10 @@ __global__ __launch_bounds__(SB) void k_strip_infer(const StripArgs a, const flo
     float x[LB_NIN], h1[LB_NH], h2[LB_NH], y[LB_NOUT], g[LB_NOUT], pred[LB_NOUT];
     (void)strip_features(a, a.ids[k], a.meta[k], x);
     mlp_forward(sp, x, h1, h2, y);
+    if (a.mfac) {   // coefficient head: the cell's damping multiplier
+        a.mfac[a.ids[k]] = STRIP_MMAX / (1.0f + expf(-y[0]));
+        return;
+    }
     strip_head(y, x, g, pred);
     delta[k] = make_float2(g[0], g[1]);
 }
@@ -206,9 +214,13 @@ __global__ __launch_bounds__(SB) void k_strip_infer_mfma(const StripArgs a, cons
             }
         }
         if (k < a.n) {
-            float gg[LB_NOUT], pred[LB_NOUT];
-            strip_head(y, x, gg, pred);
-            delta[k] = make_float2(gg[0], gg[1]);
+            if (a.mfac) {   // coefficient head: the cell's damping multiplier
+                a.mfac[a.ids[k]] = STRIP_MMAX / (1.0f + expf(-y[0]));
+            } else {
+                float gg[LB_NOUT], pred[LB_NOUT];
+                strip_head(y, x, gg, pred);
+                delta[k] = make_float2(gg[0], gg[1]);
+            }
         }
     }
 }

@@ -270,6 +270,10 @@ int a3d_batch_set_w_sweep(a3d_batch* b, int mode);
  * with them in mode 2); every mode gives bitwise the records of single runs. */
 int a3d_batch_set_strip_net(a3d_batch* b, const float* params, int n);
 int a3d_batch_set_strip_profile(a3d_batch* b, float amp);
+/* The strip's head mode and network period on every window (as a3d_set_strip_head and
+ * a3d_set_strip_period for one context). */
+int a3d_batch_set_strip_head(a3d_batch* b, int mode);
+int a3d_batch_set_strip_period(a3d_batch* b, int steps);
 int a3d_batch_get_zseg(const a3d_batch* b, int* zseg_vel, int* zseg_stress);
 
 /* ----------------------------------------------------------------------- */
@@ -301,6 +305,15 @@ int a3d_set_strip_profile(a3d_ctx* ctx, float amp);
  * scaled by 1 - g -- a local increase of the PML's own damping along a. Cells whose
  * stress PML factor along a is 1 are left alone. */
 int a3d_set_strip_head(a3d_ctx* ctx, int mode);
+/* Head mode 2 ("coefficient"): the network's first output y_0 gives each strip cell the
+ * multiplier m = 3 sigmoid(y_0) in (0, 3) of its PML damping (m = 1 at y_0 = -ln 2: the
+ * plain ramp). The velocity and stress sweeps recompute the cell's split-PML coefficients
+ * c1 = 1 / (1 + m h), c2 = c1 (1 - m h) from m and the profile's half damping step h =
+ * dt d / 2 inside their update; nothing is applied after the sweeps. The network runs after
+ * the stress sweep of every `steps`-th time step (a3d_set_strip_period, default 1) and the
+ * multipliers hold until it runs again (1 before its first run). w stays in the velocity
+ * sweep. */
+int a3d_set_strip_period(a3d_ctx* ctx, int steps);
 /* Number of strip cells of the context's grid (0 before a network was set). */
 int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
 /* Teacher-forced training samples: runs shot *desc on `teacher` (same interior,

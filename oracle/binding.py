@@ -93,6 +93,8 @@ def _declare(lib):
     lib.oracle_round_half.argtypes = [c_float]
     lib.oracle_round_half.restype = c_float
     lib.oracle_set_strip_profile.restype = c_int
+    lib.oracle_set_strip_period.argtypes = [c_int]
+    lib.oracle_set_strip_period.restype = c_int
     lib.oracle_set_tti_fault.argtypes = [c_int]
     lib.oracle_set_tti_fault.restype = c_int
     return lib
@@ -195,7 +197,8 @@ def time_steps(shot, nsteps, vel=None, rho=None, eps=None, delta=None, theta=Non
 
 def set_strip_head(mode):
     """0: the strip's damping factors scale every field of a cell; 1: the split components
-    along the normal axis and the normal velocity."""
+    along the normal axis and the normal velocity; 2: the coefficient head (the network's
+    multiplier of each strip cell's PML damping, inside the update)."""
     if load().oracle_set_strip_head(int(mode)) != 0:
         raise ValueError("bad strip head mode")
 
@@ -206,3 +209,9 @@ def set_tti_fault(code):
     operand (oracle.cpp lists them); used only by tests that must show they catch one."""
     if load().oracle_set_tti_fault(int(code)) != 0:
         raise ValueError(f"bad TTI fault code {code}")
+
+
+def set_strip_period(steps):
+    """Coefficient head: the network runs after the stress update of every `steps`-th step."""
+    if load().oracle_set_strip_period(int(steps)) != 0:
+        raise ValueError("bad strip period")

@@ -409,7 +409,10 @@ constexpr int kW1 = 0, kB1 = kW1 + kNh * kNin, kW2 = kB1 + kNh, kB2 = kW2 + kNh 
 std::vector<float> g_strip;   /* empty: plain PML */
 double g_strip_amp = 10.0;    /* PML amplitude of the strip's 6-cell ramp */
 bool g_strip_half = true;     /* network operands rounded to binary16 (matrix-core evaluation) */
-int g_strip_head = 0;         /* 0: damping of every field; 1: normal-axis split components and velocity */
+int g_strip_head = 0;         /* 0: damping of every field; 1: normal-axis split components and velocity;
+                                 2: multiplier of the PML damping inside the update (coefficient head) */
+int g_strip_period = 1;       /* head 2: the network runs after the stress update of every period-th step */
+constexpr float kMmax = 3.0f; /* head 2: m = kMmax * sigmoid(y_0) */
 
 /* Planted mistake in the TTI rotation, for tests that must show they catch one (0: none).
  * 1..8 flip the sign of velocity-sweep term k-1, 9..16 of stress-sweep term k-9, in the
@@ -477,9 +480,13 @@ void strip_net(const float* p, const float* x, float* y) {
 }
 
 /* Oracle layout: z fastest, id = iz + nnz*ix + pl*iy. */
+/* Head 2 (mfac != NULL): each strip cell's damping multiplier m = kMmax * sigmoid(y_0) is
+ * written to mfac instead; the update loops rebuild the cell's split-PML coefficients from
+ * it (pml_coef). */
 void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
                    float* u, float* v, float* w, float* px, float* py, float* pz, float* qx,
-                   float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del) {
+                   float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del,
+                   float* mfac = nullptr) {
     const long pl = (long)nnz * nnx;
     const long stride[3] = {nnz, pl, 1};   /* x, y, z */
     const int n3[3] = {nnx, nny, nnz};
@@ -519,6 +526,10 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
                 x[14] = (float)best * (1.0f / kStripCells);
                 x[15] = 0.f;
                 strip_net(g_strip.data(), x, y);
+                if (mfac) {
+                    mfac[id] = kMmax / (1.0f + std::exp(-y[0]));
+                    continue;
+                }
                 /* stress PML factor along the normal differs from 1 (the ramp's tables) */
                 const bool nd = pp[axis] < nb || pp[axis] > n3[axis] - nb;
                 rows[iy].push_back(Cell{id, kGmax / (1.0f + std::exp(-y[0])), kGmax / (1.0f + std::exp(-y[1])), axis,
@@ -554,6 +565,34 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
             v[c.id] *= fp;
             w[c.id] *= fq;
         }
+}
+
+/* Half the damping step of each index on initial_coffe's velocity (half-cell) and stress
+ * nodes, h = (dt*d0) r^2 / 2 with r as there (0 inside the interior), in float: the
+ * coefficient head's per-cell coefficients are c1 = 1/(1 + m h), c2 = c1 (1 - m h). */
+void pml_half(float dt, float d0, int n, int npml, int width, float* hv, float* hs) {
+    const float dtd0 = dt * d0;
+    for (int id = 0; id < n + 2 * npml; ++id) {
+        double r, ra;
+        if (id < npml) {
+            r = (npml - 0.5 - id) / width;
+            ra = ((npml - id) * 1.0) / width;
+        } else if (id < npml + n) {
+            hv[id] = hs[id] = 0.0f;
+            continue;
+        } else {
+            r = (0.5 + id - n - npml) / width;
+            ra = ((id - n - npml) * 1.0) / width;
+        }
+        hv[id] = (float)((dtd0 * (r * r)) / 2.0);
+        hs[id] = (float)((dtd0 * (ra * ra)) / 2.0);
+    }
+}
+
+inline void pml_coef(float m, float h, float& c1, float& c2) {
+    const float t = m * h;
+    c1 = 1.0f / (1.0f + t);
+    c2 = c1 * (1.0f - t);
 }
 
 template <class T>
@@ -608,6 +647,15 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
     initial_coffe<T>(dt, d0, nz, npml, cz1.data(), cz2.data(), az1.data(), az2.data(), width);
 
     const T dtx = dt / (T)p.dx, dty = dt / (T)p.dy, dtz = dt / (T)p.dz;   /* iso:109-111 */
+    /* learned strip, coefficient head: half damping steps per index and the multipliers */
+    const bool strip2 = strip && g_strip_head == 2;
+    std::vector<float> hvx(nnx), hsx(nnx), hvy(nny), hsy(nny), hvz(nnz), hsz(nnz), mfac;
+    if (strip2) {
+        pml_half((float)dt, (float)d0, nx, npml, width, hvx.data(), hsx.data());
+        pml_half((float)dt, (float)d0, ny, npml, width, hvy.data(), hsy.data());
+        pml_half((float)dt, (float)d0, nz, npml, width, hvz.data(), hsz.data());
+        mfac.assign(N, 1.0f);
+    }
     /* signs of the TTI rotation terms: all +1 unless a test plants a mistake */
     T vsgn[8], ssgn[8];
     for (int k = 0; k < 8; ++k) {
@@ -672,9 +720,18 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                         yy = f[3] * sp * Px + f[4] * cp * Py;                                /* tti:102 */
                         zz = f[5] * st * cp * Qx - f[6] * st * sp * Qy + f[7] * ct * Qz;     /* tti:103 */
                     }
-                    u[id] = cx2[ix] * u[id] - cx1[ix] * dtx * xx;
-                    v[id] = cy2[iy] * v[id] - cy1[iy] * dty * yy;
-                    w[id] = cz2[iz] * w[id] - cz1[iz] * dtz * zz;
+                    T kx1 = cx1[ix], kx2 = cx2[ix], ky1 = cy1[iy], ky2 = cy2[iy], kz1 = cz1[iz], kz2 = cz2[iz];
+                    if constexpr (std::is_same<T, float>::value) {
+                        if (strip2 && (hvx[ix] != 0.0f || hvy[iy] != 0.0f || hvz[iz] != 0.0f)) {
+                            const float m = mfac[id];
+                            pml_coef(m, hvx[ix], kx1, kx2);
+                            pml_coef(m, hvy[iy], ky1, ky2);
+                            pml_coef(m, hvz[iz], kz1, kz2);
+                        }
+                    }
+                    u[id] = kx2 * u[id] - kx1 * dtx * xx;
+                    v[id] = ky2 * v[id] - ky1 * dty * yy;
+                    w[id] = kz2 * w[id] - kz1 * dtz * zz;
                 }
         /* update_stress (iso:145-190, vti:131-203, tti:115-201) */
 #pragma omp parallel for schedule(static)
@@ -716,12 +773,21 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                         xx = dbwd<T>(u.data(), id, nnz);
                         zz = dbwd<T>(w.data(), id, 1);
                         const T r = rho[id], c = vp[id];
-                        px[id] = ax2[ix] * px[id] - ax1[ix] * r * c * c * (1 + 2 * ee) * dtx * xx;
-                        py[id] = ay2[iy] * py[id] - ay1[iy] * r * c * c * (1 + 2 * ee) * dty * yy;
-                        pz[id] = az2[iz] * pz[id] - az1[iz] * r * c * c * fsqrt<T>(1 + 2 * dd) * dtz * zz;
-                        qx[id] = ax2[ix] * qx[id] - ax1[ix] * r * c * c * fsqrt<T>(1 + 2 * dd) * dtx * xx;
-                        qy[id] = ay2[iy] * qy[id] - ay1[iy] * r * c * c * fsqrt<T>(1 + 2 * dd) * dty * yy;
-                        qz[id] = az2[iz] * qz[id] - az1[iz] * r * c * c * dtz * zz;
+                        T bx1 = ax1[ix], bx2 = ax2[ix], by1 = ay1[iy], by2 = ay2[iy], bz1 = az1[iz], bz2 = az2[iz];
+                        if constexpr (std::is_same<T, float>::value) {
+                            if (strip2 && (hsx[ix] != 0.0f || hsy[iy] != 0.0f || hsz[iz] != 0.0f)) {
+                                const float m = mfac[id];
+                                pml_coef(m, hsx[ix], bx1, bx2);
+                                pml_coef(m, hsy[iy], by1, by2);
+                                pml_coef(m, hsz[iz], bz1, bz2);
+                            }
+                        }
+                        px[id] = bx2 * px[id] - bx1 * r * c * c * (1 + 2 * ee) * dtx * xx;
+                        py[id] = by2 * py[id] - by1 * r * c * c * (1 + 2 * ee) * dty * yy;
+                        pz[id] = bz2 * pz[id] - bz1 * r * c * c * fsqrt<T>(1 + 2 * dd) * dtz * zz;
+                        qx[id] = bx2 * qx[id] - bx1 * r * c * c * fsqrt<T>(1 + 2 * dd) * dtx * xx;
+                        qy[id] = by2 * qy[id] - by1 * r * c * c * fsqrt<T>(1 + 2 * dd) * dty * yy;
+                        qz[id] = bz2 * qz[id] - bz1 * r * c * c * dtz * zz;
                     } else {
                         T uy = dbwd<T>(u.data(), id, pl), ux = dbwd<T>(u.data(), id, nnz), uz = dbwd<T>(u.data(), id, 1);
                         T vy = dbwd<T>(v.data(), id, pl), vx = dbwd<T>(v.data(), id, nnz), vz = dbwd<T>(v.data(), id, 1);
@@ -746,10 +812,10 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                     Qp[id] = qx[id] + qy[id] + qz[id];
                 }
         if constexpr (std::is_same<T, float>::value) {
-            if (vti && !g_strip.empty())
+            if (vti && !g_strip.empty() && (!strip2 || it % g_strip_period == 0))
                 strip_correct(nnx, nny, nnz, p.dt, p.dx, p.dy, p.dz, Pp, Qp, u.data(), v.data(), w.data(),
                               px.data(), py.data(), pz.data(), qx.data(), qy.data(), qz.data(), vp.data(),
-                              rho.data(), eps.data(), del.data());
+                              rho.data(), eps.data(), del.data(), strip2 ? mfac.data() : nullptr);
         }
         /* shot_record (iso:294-304): interior z = rec_z (reference: 0) */
         if (shot) {
@@ -861,8 +927,14 @@ int oracle_set_strip_half(int on) {
 float oracle_round_half(float f) { return round_half(f); }
 
 int oracle_set_strip_head(int mode) {
-    if (mode < 0 || mode > 1) return -1;
+    if (mode < 0 || mode > 2) return -1;
     g_strip_head = mode;
+    return 0;
+}
+
+int oracle_set_strip_period(int steps) {
+    if (steps < 1) return -1;
+    g_strip_period = steps;
     return 0;
 }
 

@@ -66,8 +66,14 @@ float oracle_round_half(float f);
 int oracle_set_strip_profile(float amp);
 /* What the damping factors act on: 0 (default) every field of the cell, 1 the split
  * components along the cell's normal axis and the normal velocity (cells whose stress
- * PML factor along the normal is 1 are left alone). */
+ * PML factor along the normal is 1 are left alone), 2 the coefficient head (below). */
 int oracle_set_strip_head(int mode);
+/* Head 2 ("coefficient"): the network's first output y_0 gives each strip cell the
+ * multiplier m = 3 sigmoid(y_0) of its PML damping; the update rebuilds the cell's split-PML
+ * coefficients c1 = 1/(1 + m h), c2 = c1 (1 - m h) from the profile's half damping step h.
+ * The network runs after the stress update of every `steps`-th step (default 1); the
+ * multipliers start at 1. Returns -1 for a bad value. */
+int oracle_set_strip_period(int steps);
 
 /* Plants a mistake in the TTI rotation so a test can show that it catches it
  * (0 = none, the default; 1..8 flip a velocity-sweep term's sign, 9..16 a

@@ -153,6 +153,49 @@ def test_strip_matches_oracle(oracle, scheme, mfma, head):
     assert e_shot <= 1e-5 and e_snap <= 1e-5
 
 
+@pytest.mark.parametrize("scheme,mfma,period", [(0, 1, 1), (3, 1, 1), (1, 1, 2), (0, 0, 1), (3, 1, 4)])
+def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
+    """Head mode 2: the network's multiplier of each strip cell's PML damping, read by the
+    velocity and stress sweeps inside their update (no pass after the sweeps), against the
+    oracle's restatement; also with the network run every `period` steps, and the time loop
+    captured in hipGraphs bitwise equal to host launches."""
+    from a3dfd import Modeller
+    m = _small()
+    nt = 220
+    d = _desc(30, 28, 26, nt, 10, snap_interval=55, scheme=scheme)
+    net = _params(6)
+    with Modeller(d) as mdl:
+        mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+        mdl.set_strip_precision(mfma)
+        mdl.set_strip_head(2)
+        mdl.set_strip_period(period)
+        mdl.set_strip_net(net)
+        shot, snaps, _ = mdl.run(nsnap_max=8)
+        mdl.set_graph(32)
+        gshot, gsnaps, _ = mdl.run(nsnap_max=8)
+    plain, _, _ = _run(m, d)
+    os_ = oracle.make_shot("vti", 30, 28, 26, 10.0, 10.0, 10.0, 4, 25, 3, nt, 1e-3, 25.0, nb=10, snap_interval=55,
+                           mute_tt=0)
+    oracle.set_strip_net(net)
+    oracle.set_strip_half(mfma)
+    oracle.set_strip_head(2)
+    oracle.set_strip_period(period)
+    try:
+        rshot, rsnaps, _ = oracle.run(os_, 32, vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"], nsnap_max=8)
+    finally:
+        oracle.set_strip_net(None)
+        oracle.set_strip_half(1)
+        oracle.set_strip_head(0)
+        oracle.set_strip_period(1)
+    e_shot, e_snap = rel_l2(shot, rshot), rel_l2(snaps, rsnaps)
+    effect = rel_l2(plain, rshot)
+    print(f"[gpu-strip] coefficient head, scheme {scheme} mfma {mfma} period {period}: shot rel L2 {e_shot:.2e}, "
+          f"snapshots {e_snap:.2e}; network effect on the record {effect:.2e}")
+    assert effect > 1e-4          # the multipliers matter
+    assert e_shot <= 1e-5 and e_snap <= 1e-5
+    assert np.array_equal(gshot, shot) and np.array_equal(gsnaps, snaps)
+
+
 def test_collect_samples():
     from a3dfd import Modeller, strip_collect
     m = _small(seed=12, nx=24, ny=22, nz=20)

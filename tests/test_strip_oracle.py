@@ -7,6 +7,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import rel_l2
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -292,3 +294,52 @@ def test_oracle_normal_head_damps_only_the_normal_pml_components(oracle):
         b, n = float(before[va][idx]), float(f[va][idx])
         assert abs(n - b * (1 - g)) <= 3e-3 * g * abs(b) + 1e-6 * abs(b), (va, idx)
     assert n_damped > 0.8 * int(is_strip.sum())
+
+
+def coef_const_params(mult):
+    """Coefficient-head network whose multiplier is the constant mult everywhere: zero weights,
+    output-0 bias logit(mult / 3)."""
+    p = np.zeros(mlp.NPAR, np.float32)
+    r = mult / 3.0
+    p[-2] = np.float32(np.log(r / (1.0 - r)))
+    return p
+
+
+@pytest.mark.parametrize("mult", [0.5, 2.0])
+def test_oracle_coefficient_head_scales_the_ramp(oracle, mult):
+    """Head 2 ("coefficient") with a constant multiplier m is the strip's PML ramp with its
+    damping scaled by m in both the velocity and the stress update: it equals the plain strip
+    (shut damping head) at ramp amplitude 10 m within FP32 rounding of the coefficients, and
+    differs from the amplitude-10 ramp. A period of 3 steps between network runs changes
+    nothing for a constant network (bitwise)."""
+    from a3dfd import models, strip
+    m = models.small_model("vti", 18, 16, 16, seed=9)
+    kw = dict(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
+    shot = oracle.make_shot("vti", 18, 16, 16, 10.0, 10.0, 10.0, 9, 8, 3, 150, 1e-3, 25.0, nb=10,
+                            snap_interval=75, snap_skip0=1, mute_tt=0)
+    p = coef_const_params(mult)
+    b = np.float32(p[-2])
+    mf = float(np.float32(3.0) / (np.float32(1.0) + np.exp(-b)))     # the multiplier as the oracle computes it
+    runs = {}
+    try:
+        oracle.set_strip_net(strip.closed_params())
+        for amp in (10.0, 10.0 * mf):
+            oracle.set_strip_profile(amp)
+            runs[amp] = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
+        oracle.set_strip_profile(10.0)
+        oracle.set_strip_head(2)
+        oracle.set_strip_net(p)
+        head2 = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
+        oracle.set_strip_period(3)
+        head2p = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
+    finally:
+        oracle.set_strip_period(1)
+        oracle.set_strip_head(0)
+        oracle.set_strip_profile(10.0)
+        oracle.set_strip_net(None)
+    (a, sa), (c, sc) = runs[10.0 * mf], runs[10.0]
+    g, sg = head2
+    assert np.abs(g).max() > 0
+    assert rel_l2(g, a) < 1e-5 and rel_l2(sg, sa) < 1e-5
+    assert rel_l2(g, c) > 100 * rel_l2(g, a) and rel_l2(sg, sc) > 100 * rel_l2(sg, sa)
+    assert np.array_equal(head2p[0], g) and np.array_equal(head2p[1], sg)
