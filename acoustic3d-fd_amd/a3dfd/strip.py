@@ -44,7 +44,8 @@ def _set(mdl, m):
     mdl.set_model(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
 
 
-def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, amp=None, device=0, log=None):
+def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, amp=None, device=0, log=None,
+            head=0, sources=None):
     """Samples over seeded models (one source each): X (N, 16), T (N, 2), S (N,). Samples whose
     local amplitude s is below 1e-4 of the largest s of their shot are dropped. horizon > 1:
     the strip grid runs `horizon` steps on its own (with network `params`) between
@@ -55,7 +56,7 @@ def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, am
     try:
         for k, seed in enumerate(seeds):
             m = models.strip_model(seed, n)
-            src = models.strip_source(seed, n)
+            src = models.strip_source(seed, n) if sources is None else sources[k]
             dt_ = _desc(m, src, nt, TEACHER_NB)
             if teacher is None:
                 teacher = Modeller(dt_, device)
@@ -63,6 +64,7 @@ def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, am
                 # builds the strip cell list; the network runs between synchronisations
                 if amp is not None:
                     student.set_strip_profile(amp)
+                student.set_strip_head(head)
                 student.set_strip_net(closed_params() if params is None else params)
             _set(teacher, m)
             _set(student, m)
@@ -192,7 +194,8 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
         if boundary in ("strip", "strip0"):
             if amp is not None:
                 mdl.set_strip_profile(amp)
-            mdl.set_strip_head(head)
+            # strip0: the plain ramp (a damping head shut), whatever head the network uses
+            mdl.set_strip_head(head if boundary == "strip" else 0)
             mdl.set_strip_net(params if boundary == "strip" else closed_params())
         return mdl.run(nsnap_max=nt // snap_every)
 
@@ -289,3 +292,184 @@ def choose(rows, threshold_db=THRESHOLD_DB):
     if not rows or any("strip" not in r for r in rows):
         return "pml"
     return "strip" if max(r["strip"]["reflected_db"] for r in rows) <= threshold_db else "pml"
+
+
+# ---------------------------------------------------------------------------
+# Coefficient head (head mode 2): the network's first output y_0 sets each strip cell's
+# multiplier m = COEF_MMAX sigmoid(y_0) of its PML damping, inside the sweeps' update.
+# ---------------------------------------------------------------------------
+COEF_MMAX = 3.0
+COEF_HEAD = 2
+O_W1, O_B1, O_W2, O_B2, O_W3, O_B3 = 0, 1024, 1088, 5184, 5248, 5376
+COEF_CLOSED_BIAS = float(np.log(1.0 / (COEF_MMAX - 1.0)))   # m = 1: the plain ramp
+COURANT_STEPS = (0.14, 0.18, 0.22, 0.26, 0.30, 0.34)       # vp dt / d of the study models: 0.12-0.36
+RATIO_STEPS = (0.5, 0.7, 0.85, 1.0, 1.2, 1.5, 2.0)         # vp over the ramp's d0 sample: 1/3 - 3
+N_DESIGNED = 6 + len(COURANT_STEPS) + len(RATIO_STEPS)
+
+
+def coef_hidden(base, steep=60.0):
+    """Hidden layers of the coefficient head: units 0-5 are steps in the strip depth (input 14,
+    +1 at depth >= j + 1), units 6-11 steps in the local Courant number vp dt / d (input 11),
+    units 12-18 steps in the velocity over the ramp's d0 sample (input 15, 8 / vp scale), all
+    passed on by layer 2 as tanh(step / 2); the other units are those of `base` (a network
+    trained on the strip's wavefields), cut off from the designed ones. The output layer is
+    left to the caller."""
+    p = np.array(base, np.float32)
+    w1 = p[O_W1:O_B1].reshape(64, 16)
+    b1 = p[O_B1:O_W2]
+    w2 = p[O_W2:O_B2].reshape(64, 64)
+    b2 = p[O_B2:O_W3]
+    w1[:, 15] = 0.0                      # input 15 is new to the base network's units
+    for j in range(N_DESIGNED):
+        w1[j] = 0.0
+        w2[j] = 0.0
+        b2[j] = 0.0
+        w2[j, j] = 0.5
+    for j in range(6):
+        w1[j, 14] = steep
+        b1[j] = -steep * (j + 0.5) / 6.0
+        w1[6 + j, 11] = steep
+        b1[6 + j] = -steep * COURANT_STEPS[j]
+    for j, r in enumerate(RATIO_STEPS):
+        k = 12 + j
+        w1[k, 15] = 8.0 / r
+        b1[k] = -8.0
+    w2[N_DESIGNED:, :N_DESIGNED] = 0.0
+    return p
+
+
+def coef_params(theta, hidden):
+    """Coefficient-head network: hidden layers from `hidden`, output 0 with weights theta[:64]
+    and bias theta[64] (output 1 unused: zero)."""
+    p = np.array(hidden, np.float32)
+    p[O_W3:O_B3] = 0.0
+    p[O_W3:O_W3 + 64] = np.asarray(theta[:64], np.float32)
+    p[O_B3] = np.float32(theta[64])
+    p[O_B3 + 1] = 0.0
+    return p
+
+
+def coef_closed_theta():
+    th = np.zeros(65, np.float64)
+    th[64] = COEF_CLOSED_BIAS
+    return th
+
+
+def model_sources(seed, n):
+    """The 1-4 Ricker sources (10-30 Hz, anywhere in the volume) of seeded study model `seed`."""
+    rng = np.random.default_rng(seed + 17)
+    k = int(rng.integers(1, 5))
+    return [models.strip_source(seed * 8 + j, n) for j in range(k)]
+
+
+class CoefCase:
+    """One (model, source) shot of the coefficient head's training: the 40-cell-PML teacher's
+    interior snapshots (FP32) and a strip-grid context (head 2) to run candidate networks."""
+
+    def __init__(self, seed, src_index, n=128, nt=600, snap_every=30, amp=10.0, period=1, scheme=3, device=0):
+        self.m = models.strip_model(seed, n)
+        srcs = model_sources(seed, n)
+        self.src = srcs[src_index % len(srcs)]
+        self.nt, self.every = nt, snap_every
+        d = _desc(self.m, self.src, nt, TEACHER_NB, snap_interval=snap_every, snap_skip0=1, scheme=scheme)
+        with Modeller(d, device) as t:
+            _set(t, self.m)
+            _, self.ref, _ = t.run(shot=False, nsnap_max=nt // snap_every)
+        d = _desc(self.m, self.src, nt, STRIP_NB, snap_interval=snap_every, snap_skip0=1, scheme=scheme)
+        self.mdl = Modeller(d, device)
+        _set(self.mdl, self.m)
+        self.mdl.set_strip_profile(amp)
+        self.mdl.set_strip_head(COEF_HEAD)
+        self.mdl.set_strip_period(period)
+
+    def db(self, params):
+        """Reflected energy (dB) of the strip run with network `params` against the teacher."""
+        self.mdl.set_strip_net(params)
+        try:
+            _, snaps, _ = self.mdl.run(shot=False, nsnap_max=self.nt // self.every)
+        except A3DError:
+            return 0.0   # unstable: as bad as no absorption
+        return float(energy_db(snaps, self.ref))
+
+    def close(self):
+        self.mdl.close()
+
+
+def coef_pool(seeds, n=128, nt=600, snap_every=30, amp=10.0, period=1, device=0):
+    """Training cases: for each seed, one of its model's 1-4 sources (the k-th seed of a model
+    takes source k), with the teacher's snapshots computed once."""
+    visits, pool = {}, []
+    for s in seeds:
+        visits[s] = visits.get(s, -1) + 1
+        pool.append(CoefCase(s, visits[s], n=n, nt=nt, snap_every=snap_every, amp=amp, period=period, device=device))
+    return pool
+
+
+def train_coef(pool, hidden, iters=100, active=None, batch=None, a=0.02, c=0.1, alpha=0.602, gamma=0.101,
+               npert=2, seed=0, theta0=None, log=None):
+    """Fits the coefficient head's output layer against the teacher's interior by
+    simultaneous-perturbation stochastic approximation (SPSA): each iteration averages `npert`
+    two-sided estimates (J(theta + c_k D) - J(theta - c_k D)) / (2 c_k) D over the active
+    coordinates (D random +-1), on a minibatch of `batch` cases of `pool` (all when None),
+    and steps theta -= a_k times the estimate, with a_k = a / (k + 1 + A)^alpha (A = iters /
+    10) and c_k = c / (k + 1)^gamma. J = mean reflected energy in dB against the teacher.
+    active: indices of theta (64 output weights, then the bias) to fit (default all).
+    Returns (theta, history of (iteration, mean J over the two-sided evaluations))."""
+    rng = np.random.default_rng(seed)
+    theta = coef_closed_theta() if theta0 is None else np.array(theta0, np.float64)
+    act = np.arange(theta.size) if active is None else np.asarray(active)
+    A = max(1, iters // 10)
+    hist = []
+    for k in range(iters):
+        ak = a / (k + 1 + A) ** alpha
+        ck = c / (k + 1) ** gamma
+        cases = pool if batch is None else [pool[i] for i in rng.choice(len(pool), batch, replace=False)]
+        g = np.zeros(theta.size)
+        js = []
+        for _ in range(npert):
+            d = np.zeros(theta.size)
+            d[act] = rng.choice((-1.0, 1.0), size=act.size)
+            jp = float(np.mean([cs.db(coef_params(theta + ck * d, hidden)) for cs in cases]))
+            jm = float(np.mean([cs.db(coef_params(theta - ck * d, hidden)) for cs in cases]))
+            g += (jp - jm) / (2.0 * ck) * d / npert
+            js += [jp, jm]
+        theta -= ak * g
+        hist.append((k, float(np.mean(js))))
+        if log:
+            log(f"coef train: iteration {k}: J {np.mean(js):.3f} dB, |step| {ak * np.abs(g).max():.4f}, "
+                f"bias {theta[64]:.3f}")
+    return theta, hist
+
+
+COEF_SCAN = (0.25, 0.35, 0.5, 0.7, 0.85, 1.0, 1.3)
+
+
+def coef_label(case, hidden=None, mults=COEF_SCAN):
+    """The best constant damping multiplier of a (model, source) case: the reflected energy
+    against the teacher for every multiplier in `mults`, refined by a parabola in log m through
+    the best grid point and its neighbours. Returns (m*, {m: dB})."""
+    hid = np.zeros(NPARAMS, np.float32) if hidden is None else hidden
+    tab = {}
+    for mult in mults:
+        th = np.zeros(65)
+        th[64] = np.log((mult / COEF_MMAX) / (1.0 - mult / COEF_MMAX))
+        tab[mult] = round(case.db(coef_params(th, hid)), 3)
+    ms = list(mults)
+    k = int(np.argmin([tab[x] for x in ms]))
+    best = ms[k]
+    if 0 < k < len(ms) - 1:
+        x = np.log([ms[k - 1], ms[k], ms[k + 1]])
+        y = np.array([tab[ms[k - 1]], tab[ms[k]], tab[ms[k + 1]]])
+        a2, a1, _ = np.polyfit(x, y, 2)
+        if a2 > 0:
+            best = float(np.clip(np.exp(-a1 / (2 * a2)), ms[k - 1], ms[k + 1]))
+    return float(best), tab
+
+
+def coef_targets(X, mstar):
+    """Regression targets that make the damping head's factor g_0 = 0.25 sigmoid(y_0) equal
+    m* / 12, i.e. the coefficient head's multiplier 3 sigmoid(y_0) = m* (the training kernels
+    fit the head's prediction (-x_0 g_0, -x_7 g_1)); output 1 is unused and pulled to 0."""
+    T = np.zeros((len(X), 2), np.float32)
+    T[:, 0] = -X[:, 0] * np.float32(mstar / (4.0 * COEF_MMAX))
+    return T

@@ -703,6 +703,7 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     a.dirsplit = d->scheme == 3;
     a.normal = c->strip_head == 1;
     a.mfac = c->strip_head == 2 ? c->strip_mfac : nullptr;
+    a.vref = c->vel_d0;
     return a;
 }
 
@@ -1587,7 +1588,8 @@ int a3d_strip_collect(a3d_ctx* t, a3d_ctx* s, const a3d_shot_desc* d, int every,
         return fail(A3D_EINVAL, "strip sampling: same interior, teacher boundary wider than the student's");
     if (!t->model_set || !s->model_set) return fail(A3D_ESTATE, "model not set");
     if (!s->strip_ids) return fail(A3D_ESTATE, "student strip cells not built (set a strip network first)");
-    if (s->strip_head == 2) return fail(A3D_EINVAL, "teacher-forced samples are taken for the damping heads (0, 1)");
+    if (s->strip_head == 2 && horizon > 1)
+        return fail(A3D_EINVAL, "coefficient head: samples are taken teacher-forced only (horizon <= 1)");
     int rc = check_desc_shot(t, d);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(t->device));

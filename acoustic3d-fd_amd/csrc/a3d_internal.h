@@ -246,6 +246,7 @@ struct StripArgs {
     int dirsplit;   // the stress sweep keeps split fields only for a cell's damped directions
     int normal;     // head mode 1: damp the normal-axis split components and velocity only
     float* mfac;    // head mode 2: the network writes each strip cell's damping multiplier here
+    float vref;     // head mode 2: the velocity sample the ramp's d0 is computed from (input 15 = vp / vref)
 };
 
 // delta[k] = the damping factors (g_P, g_Q) of every strip cell (after the stress sweep);

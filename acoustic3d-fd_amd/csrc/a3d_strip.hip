@@ -18,7 +18,8 @@
 // face): P at the 7-point star (centre, inward, outward, tangential pairs),
 // Q at centre, inward, outward, and the outward normal velocity times the
 // impedance rho*vp; model features vp*dt/d_a, epsilon, delta and the depth in
-// the strip.
+// the strip; for the coefficient head also vp over the velocity sample the ramp's
+// strength d0 is computed from (the reference's get_d0 sample).
 // Head mode 2 ("coefficient"): the network's first output gives each strip cell a
 // multiplier m = STRIP_MMAX * sigmoid(y_0) of its PML damping d; the velocity and
 // stress sweeps recompute the cell's split-PML coefficients from m d inside their
@@ -65,7 +66,7 @@ __device__ __forceinline__ float strip_features(const StripArgs& a, int id, int 
     x[12] = a.eps[id];
     x[13] = a.del[id];
     x[14] = (float)dist * (1.0f / STRIP_CELLS);
-    x[15] = 0.f;
+    x[15] = a.mfac ? a.vp[id] / a.vref : 0.f;   // coefficient head: velocity over the ramp's d0 sample
     return s;
 }
 

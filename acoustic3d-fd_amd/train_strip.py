@@ -53,10 +53,17 @@ def main():
     ap.add_argument("--adaptive", action="store_true",
                     help="normal head (PML-like damping) on a weaker ramp: each training model's best uniform "
                          "extra damping (scanned against its extended-domain run) is the network's target")
+    ap.add_argument("--coef", action="store_true",
+                    help="coefficient head: each (model, source) case's best constant multiplier of the strip's "
+                         "PML damping (scanned against the 40-cell-PML teacher's interior) labels that case's "
+                         "strip-cell samples; the network learns the multiplier from the local state")
+    ap.add_argument("--period", type=int, default=1, help="coefficient head: network every k steps")
     ap.add_argument("--out", default=strip.NET_PATH)
     args = ap.parse_args()
     if args.adaptive:
         return adaptive(args)
+    if args.coef:
+        return coef(args)
 
     seeds = [10000 + k for k in range(args.models)]
     t0 = time.time()
@@ -125,6 +132,51 @@ def adaptive(args):
                "loss_last_epoch": hist[-1], "judge": rows, "choice": choice}
     strip.save(params, args.out, amp=np.float32(args.amp), head=np.int32(1), loss=np.asarray(hist),
                summary=json.dumps(summary))
+    print(json.dumps(summary), flush=True)
+
+
+def coef(args):
+    """Coefficient head (mode 2). Cases: every source (1-4) of every training model (seeds
+    10000 + k). Per case, the best constant damping multiplier m* (strip.coef_label, against
+    the case's 40-cell-PML teacher run) labels the case's teacher-forced strip-cell samples
+    (features as the head reads them); the network is trained on the GPU (a3d_mlp kernels) to
+    give m* from the local state, and judged on held-out models against the plain strip (m = 1)
+    and the 20-cell PML, all against extended-domain FP64 runs."""
+    from a3dfd import models
+    seeds = [10000 + k for k in range(args.models)]
+    Xs, Ts, labels = [], [], []
+    t0 = time.time()
+    for seed in seeds:
+        for j, src in enumerate(strip.model_sources(seed, args.n)):
+            case = strip.CoefCase(seed, j, n=args.n, nt=args.nt, snap_every=30, amp=args.amp)
+            try:
+                mstar, tab = strip.coef_label(case)
+            finally:
+                case.close()
+            X, _, _ = strip.collect([seed], n=args.n, nt=args.nt, every=args.every, keep=args.keep, amp=args.amp,
+                                    head=strip.COEF_HEAD, sources=[src], params=strip.coef_params(
+                                        strip.coef_closed_theta(), np.zeros(strip.NPARAMS, np.float32)))
+            Xs.append(X)
+            Ts.append(strip.coef_targets(X, mstar))
+            labels.append({"seed": seed, "source": j, "fm": round(src["fm"], 2), "m_star": round(mstar, 4),
+                           "db": {str(k): v for k, v in tab.items()}, "samples": len(X)})
+        log(f"train_strip: model {seed}: m* {[r['m_star'] for r in labels if r['seed'] == seed]} "
+            f"({sum(len(x) for x in Xs)} samples, {time.time() - t0:.1f} s)")
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    mbar = float(np.mean([r["m_star"] for r in labels]))
+    r = mbar / strip.COEF_MMAX
+    init = strip.init_params(0, out_bias=float(np.log(r / (1 - r))))
+    params, hist = strip.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, init=init)
+    log(f"train_strip: {len(X)} samples, mean m* {mbar:.3f}, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    rows = strip.judge([90000 + k for k in range(args.heldout)], params, n=args.n, nt=args.judge_nt, amp=args.amp,
+                       head=strip.COEF_HEAD, log=log)
+    choice = strip.choose(rows)
+    summary = {"mode": "coefficient", "head": strip.COEF_HEAD, "samples": int(len(X)), "models": args.models,
+               "cases": len(labels), "n": args.n, "nt": args.nt, "amp": args.amp, "period": args.period,
+               "labels": labels, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "judge": rows,
+               "choice": choice}
+    strip.save(params, args.out, amp=np.float32(args.amp), head=np.int32(strip.COEF_HEAD),
+               period=np.int32(args.period), loss=np.asarray(hist), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
 
 

@@ -323,7 +323,8 @@ int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
  * network's strip corrections included. At steps start, start+every, ... a fraction
  * `keep` of the strip cells yields, after the student's stress sweep, features
  * X[16], target T[2] = (teacher P, Q - student P, Q) / s and scale S = s.
- * *count: samples written (<= cap). */
+ * *count: samples written (<= cap). A student with the coefficient head (mode 2) is sampled
+ * teacher-forced only (horizon <= 1), its input 15 filled as the head reads it. */
 int a3d_strip_collect(a3d_ctx* teacher, a3d_ctx* student, const a3d_shot_desc* desc, int every, int start,
                       int horizon, float keep, unsigned seed, float* X, float* T, float* S, int cap, int* count);
 
