@@ -15,9 +15,10 @@ every stress sweep (the library's a3d_set_strip_net). This module:
   (P_run - P_extended) over the interior and all snapshot steps, over the norm
   of P_extended, in dB; the extended run has the model padded by `margin`
   cells per face and a 40-cell PML, computed in FP64 on the GPU.
-- `choose`: the fallback rule: the learned strip is used only when its
-  reflected energy on the held-out models is at most -40 dB, otherwise the
-  plain 20-cell PML.
+- `choose`: the fallback rule: the learned strip is used only when, on every
+  held-out model, its reflected energy is at most -40 dB, its shot record is
+  within 1e-3 relative L2 of the extended FP64 run and it runs faster than the
+  20-cell PML; otherwise the plain 20-cell PML.
 """
 import os
 import time
@@ -32,6 +33,7 @@ NET_PATH = os.path.join(HERE, "strip_net.npz")
 TEACHER_NB = 40
 PML_NB = 20
 THRESHOLD_DB = -40.0
+MAX_TRACE_REL_L2 = 1e-3   # shot record against the extended-domain FP64 run
 NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 2 * 64 + 2   # W1 b1 W2 b2 W3 b3
 
 
@@ -162,6 +164,14 @@ def load_head(path=NET_PATH):
         return int(f["head"]) if "head" in f else 0
 
 
+def load_period(path=NET_PATH):
+    """The network period (steps between runs) recorded with the committed network (1 when absent)."""
+    if not os.path.exists(path):
+        return 1
+    with np.load(path, allow_pickle=False) as f:
+        return int(f["period"]) if "period" in f else 1
+
+
 def extended(m, src, margin):
     """Model and source of the extended-domain run: every face padded by `margin` cells
     (edge replication, as the boundary padding itself does)."""
@@ -172,12 +182,15 @@ def extended(m, src, margin):
     return me, se
 
 
-def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None, scheme=0, device=0, head=0):
+def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None, scheme=0, device=0, head=0,
+             period=1):
     """One shot under a boundary: 'strip' (learned, needs params), 'strip0' (the 6-cell strip
     with its damping head shut: its own PML only), 'pml' (20-cell PML), 'extended' (FP64, model
     padded by margin, 40-cell PML). amp: the strip's PML amplitude (None: the library default).
     scheme: the stress update of the non-extended runs (0 reference, 1 split fields in PML cells only).
-    head: what the damping factors act on (0 every field, 1 the normal-axis PML components).
+    head: what the network does (0 damp every field, 1 damp the normal-axis PML components,
+    2 scale each strip cell's PML damping inside the sweeps); period: steps between network
+    runs (head 2).
     Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original interior."""
     if boundary == "extended":
         me, se = extended(m, src, margin)
@@ -196,6 +209,7 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
                 mdl.set_strip_profile(amp)
             # strip0: the plain ramp (a damping head shut), whatever head the network uses
             mdl.set_strip_head(head if boundary == "strip" else 0)
+            mdl.set_strip_period(period)
             mdl.set_strip_net(params if boundary == "strip" else closed_params())
         return mdl.run(nsnap_max=nt // snap_every)
 
@@ -287,11 +301,20 @@ def tune_amp(seeds, amps=(4.0, 5.0, 6.0, 7.0, 8.5, 10.0), n=128, nt=800, snap_ev
     return min(worst, key=worst.get), worst
 
 
-def choose(rows, threshold_db=THRESHOLD_DB):
-    """'strip' when every held-out model's learned-strip reflection is <= threshold, else 'pml'."""
-    if not rows or any("strip" not in r for r in rows):
+def choose(rows, threshold_db=THRESHOLD_DB, max_rel_l2=MAX_TRACE_REL_L2):
+    """The fallback rule: 'strip' only when, on every held-out model, the learned strip's
+    reflected energy is <= threshold_db, its shot record is within max_rel_l2 (relative L2)
+    of the extended-domain FP64 run, and its time loop is shorter than the 20-cell PML's;
+    otherwise 'pml'. Rows need "strip" and "pml" entries (strip.judge's rows)."""
+    if not rows or any("strip" not in r or "pml" not in r for r in rows):
         return "pml"
-    return "strip" if max(r["strip"]["reflected_db"] for r in rows) <= threshold_db else "pml"
+    for r in rows:
+        s, p = r["strip"], r["pml"]
+        if not (s["reflected_db"] <= threshold_db and s["shot_rel_l2"] <= max_rel_l2):
+            return "pml"
+        if s.get("loop_ms") is None or p.get("loop_ms") is None or not s["loop_ms"] < p["loop_ms"]:
+            return "pml"
+    return "strip"
 
 
 # ---------------------------------------------------------------------------

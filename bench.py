@@ -236,6 +236,88 @@ def batched(args):
     print(json.dumps(line), flush=True)
 
 
+def learned_boundary(scheme=3, nt=2000, n=256, seed=95000):
+    """The learned-boundary half of the metric, measured in the bench run: the VTI 256^3
+    configuration (d = 10 m, dt = 0.8 ms, nt 2000, seeded study model and source) as per-shot
+    runs with the 20-cell PML, with the 6-cell strip without network (plain ramp) and with the
+    committed learned strip (its head mode and network period), all under the same stress
+    scheme. shots/hour from each run's time loop (best of two, no snapshots); reflected
+    energy and shot-record error against an extended-domain FP64 run of the same shot
+    (snapshots every 100 steps)."""
+    from a3dfd import STRIP_NB, models, strip
+    t0 = time.time()
+    m = models.strip_model(seed, n)
+    src = models.strip_source(seed, n)
+    params, amp = strip.load_params()
+    head, period = strip.load_head(), strip.load_period()
+    every = 100
+    ref_shot, ref_snaps, _ = strip.run_case(m, src, nt, every, "extended", margin=64)
+    out = {}
+    for name, b in (("pml20", "pml"), ("strip_plain", "strip0"), ("strip_learned", "strip")):
+        kw = dict(amp=amp, scheme=scheme, head=head, period=period)
+        shot, snaps, _ = strip.run_case(m, src, nt, every, b, params, **kw)
+        ms = min(strip.run_case(m, src, nt, nt + 1, b, params, **kw)[2] for _ in range(2))
+        out[name] = {"shots_per_hour": round(3600e3 / ms, 1), "loop_ms": round(ms, 1),
+                     "reflected_db": round(float(strip.energy_db(snaps, ref_snaps)), 2),
+                     "shot_rel_l2_vs_fp64": round(strip.rel_l2(shot, ref_shot), 6)}
+    judged = lambda k: {"reflected_db": out[k]["reflected_db"], "shot_rel_l2": out[k]["shot_rel_l2_vs_fp64"],   # noqa
+                        "loop_ms": out[k]["loop_ms"]}
+    res = {"workload": f"vti_{n}^3_nt{nt}", "scheme": scheme, "network": {"head": head, "period": period,
+                                                                           "ramp_amplitude": amp},
+           "runs": out,
+           "speedup_learned_vs_pml20": round(out["strip_learned"]["shots_per_hour"] / out["pml20"]["shots_per_hour"], 3),
+           # updated points per step, 20-cell PML over the 6-cell strip: the ratio a boundary that costs
+           # nothing per step could reach
+           "volume_bound": round(((n + 2 * 20 - 8) / (n + 2 * STRIP_NB - 8)) ** 3, 3),
+           "gain_db_over_plain_strip": round(out["strip_plain"]["reflected_db"] - out["strip_learned"]["reflected_db"], 2),
+           "choice": strip.choose([{"strip": judged("strip_learned"), "pml": judged("pml20")}]),
+           "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"}
+    log(f"bench: learned boundary {time.time() - t0:.1f} s: {res}")
+    return res
+
+
+def strip_coef(args):
+    """The learned strip's coefficient head (network file --strip-coef, head 2) on the VTI 256^3
+    study model (seed 95000, nt --strip-nt), scheme 3: loop time (best of two runs without
+    snapshots) and reflected energy / shot error against an extended-domain FP64 run, for the
+    network run every k steps (--strip-periods), the same head with a closed network (m = 1),
+    the plain strip and the 20-cell PML; then the held-out 64^3 model of the gain test."""
+    from a3dfd import models, strip
+    with np.load(args.strip_coef, allow_pickle=False) as f:
+        net = f["params"].astype(np.float32)
+    n, nt, seed = args.strip_n, args.strip_nt, 95000
+    m = models.strip_model(seed, n)
+    src = models.strip_source(seed, n)
+    ref_shot, ref_snaps, _ = strip.run_case(m, src, nt, 100, "extended", margin=64)
+    rows = {}
+
+    def one(name, b, params=None, head=0, period=1):
+        kw = dict(amp=10.0, scheme=3, head=head, period=period)
+        shot, snaps, _ = strip.run_case(m, src, nt, 100, b, params, **kw)
+        ms = min(strip.run_case(m, src, nt, nt + 1, b, params, **kw)[2] for _ in range(2))
+        rows[name] = {"loop_ms": round(ms, 1), "reflected_db": round(float(strip.energy_db(snaps, ref_snaps)), 2),
+                      "shot_rel_l2_vs_fp64": round(strip.rel_l2(shot, ref_shot), 5)}
+        log(f"bench: {name}: {rows[name]}")
+
+    one("pml20", "pml")
+    one("strip_plain_damping_head_shut", "strip0")
+    for k in [int(x) for x in args.strip_periods.split(",")]:
+        one(f"coefficient_head_period{k}", "strip", net, head=strip.COEF_HEAD, period=k)
+    closed = strip.coef_params(strip.coef_closed_theta(), np.zeros(strip.NPARAMS, np.float32))
+    one("coefficient_head_closed_period8", "strip", closed, head=strip.COEF_HEAD, period=8)
+    n2, nt2, every, s2 = 64, 400, 20, 90100
+    m2, src2 = models.strip_model(s2, n2), models.strip_source(s2, n2)
+    _, ref2, _ = strip.run_case(m2, src2, nt2, every, "extended", margin=32)
+    held = {"plain_strip_db": round(float(strip.energy_db(strip.run_case(m2, src2, nt2, every, "strip0", amp=10.0)[1],
+                                                           ref2)), 2)}
+    for k in [int(x) for x in args.strip_periods.split(",")]:
+        sn = strip.run_case(m2, src2, nt2, every, "strip", net, amp=10.0, head=strip.COEF_HEAD, period=k)[1]
+        held[f"coefficient_head_period{k}_db"] = round(float(strip.energy_db(sn, ref2)), 2)
+    print(json.dumps({"workload": f"vti_{n}^3_nt{nt}", "network": os.path.basename(args.strip_coef), "rows": rows,
+                      "held_out_64^3_seed90100_nt400": held,
+                      "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"}), flush=True)
+
+
 def strip_bench(args):
     """Learned absorbing strip on the 256^3 VTI study model (d = 10 m, dt = 0.8 ms, nt = 2000):
     shots/hour of per-shot runs with the 20-cell PML against the 6-cell learned strip, each under
@@ -266,7 +348,9 @@ def strip_bench(args):
                      "shot_rel_l2_vs_fp64": strip.rel_l2(shot, ref_shot)}
         log(f"bench: {name}: {out[name]}")
     base = out["pml20"]["shots_per_hour"]
-    choice = strip.choose([{"strip": out["strip_learned"]}])
+    judged = lambda k: {"reflected_db": out[k]["reflected_db"], "shot_rel_l2": out[k]["shot_rel_l2_vs_fp64"],   # noqa
+                        "loop_ms": out[k]["loop_ms"]}
+    choice = strip.choose([{"strip": judged("strip_learned_scheme3"), "pml": judged("pml20_scheme3")}])
     sph = lambda k: out[k]["shots_per_hour"]   # noqa: E731
     same = {str(k): round(sph(f"strip_learned{sfx}") / sph(f"pml20{sfx}"), 3)
             for k, sfx in ((0, ""), (1, "_scheme1"), (3, "_scheme3"))}
@@ -479,6 +563,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
     ap.add_argument("--nt", type=int, default=3000, help="time steps per shot")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-strip", action="store_true",
+                    help="skip the learned-boundary block (strip vs 20-cell PML at VTI 256^3) of the default line")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the accuracy check (the shot against the FP64 instance and scheme 0) after the timing")
     ap.add_argument("--time-stride", type=int, default=8,
@@ -514,6 +600,9 @@ def main():
                     help="with --strip-profiles: 0 damping of every field, 1 of the normal-axis PML components")
     ap.add_argument("--strip-amp", type=float, default=10.0,
                     help="with --strip-profiles: PML amplitude of the ramp under the profiles")
+    ap.add_argument("--strip-coef", type=str, default="",
+                    help="coefficient-head study of the given network file (speed and reflection vs network period)")
+    ap.add_argument("--strip-periods", type=str, default="1,4,8,16")
     ap.add_argument("--strip-n", type=int, default=256)
     ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()
@@ -535,6 +624,8 @@ def main():
         return batched(args)
     if args.strip:
         return strip_bench(args)
+    if args.strip_coef:
+        return strip_coef(args)
     if args.strip_profiles:
         return strip_profiles(args)
     if args.batched_strip:
@@ -582,6 +673,12 @@ def main():
     if args.nt == 3000 and args.storage != 2 and not args.no_check:
         acc = accuracy_check(cfg, mdl, desc)
     mdl.close()
+    lb = None
+    if not args.no_strip:
+        try:
+            lb = learned_boundary()
+        except Exception as e:   # informative; never fail the headline line on it
+            log(f"bench: learned boundary failed: {e}")
 
     total_loop_s = sum(loop_ms) / 1e3
     gpts = npts * args.nt * args.steps / total_loop_s / 1e9
@@ -653,6 +750,7 @@ def main():
                     # fields everywhere, w in the velocity sweep): the bandwidth it would need for this rate
                     "whole_step_reference_formulation": ref_form},
         "cpu_baseline": cpu,
+        "learned_boundary": lb,
     }
     print(json.dumps(line), flush=True)
 
