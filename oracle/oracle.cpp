@@ -486,7 +486,7 @@ void strip_net(const float* p, const float* x, float* y) {
 void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, float dz, float* P, float* Q,
                    float* u, float* v, float* w, float* px, float* py, float* pz, float* qx,
                    float* qy, float* qz, const float* vp, const float* rho, const float* eps, const float* del,
-                   float* mfac = nullptr) {
+                   float* mfac = nullptr, float vref = 0.0f) {
     const long pl = (long)nnz * nnx;
     const long stride[3] = {nnz, pl, 1};   /* x, y, z */
     const int n3[3] = {nnx, nny, nnz};
@@ -524,7 +524,7 @@ void strip_correct(int nnx, int nny, int nnz, float dt, float dx, float dy, floa
                 x[12] = eps[id];
                 x[13] = del[id];
                 x[14] = (float)best * (1.0f / kStripCells);
-                x[15] = 0.f;
+                x[15] = mfac ? vp[id] / vref : 0.f;   /* head 2: velocity over the d0 sample */
                 strip_net(g_strip.data(), x, y);
                 if (mfac) {
                     mfac[id] = kMmax / (1.0f + std::exp(-y[0]));
@@ -815,7 +815,8 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
             if (vti && !g_strip.empty() && (!strip2 || it % g_strip_period == 0))
                 strip_correct(nnx, nny, nnz, p.dt, p.dx, p.dy, p.dz, Pp, Qp, u.data(), v.data(), w.data(),
                               px.data(), py.data(), pz.data(), qx.data(), qy.data(), qz.data(), vp.data(),
-                              rho.data(), eps.data(), del.data(), strip2 ? mfac.data() : nullptr);
+                              rho.data(), eps.data(), del.data(), strip2 ? mfac.data() : nullptr,
+                              (float)vp[(long)nny * nnx * nnz / 2]);
         }
         /* shot_record (iso:294-304): interior z = rec_z (reference: 0) */
         if (shot) {
