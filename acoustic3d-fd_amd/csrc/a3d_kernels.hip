@@ -196,6 +196,13 @@ __device__ __forceinline__ C damp3(C v, double fy, double fx, double fz) {
 // own branch). Every lane loads: indices outside the grid are clamped to its
 // edge (entries no updated point reads), and lanes without an x-halo column
 // re-read their own point.
+// Table entry at a wave-uniform index (a per-plane factor): the tables are constant while
+// a sweep runs, so the read goes through the scalar data cache into scalar registers.
+template <class C>
+__device__ __forceinline__ C ztab(const C* T, int i) {
+    return ((const __attribute__((address_space(4))) C*)T)[i];
+}
+
 // Split-PML coefficients of a learned-strip cell from half its damping step h = dt d / 2
 // and the network's multiplier m: c1 = 1 / (1 + m h), c2 = c1 (1 - m h) (h = 0: 1, 1).
 template <class C>
@@ -312,13 +319,15 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     }
     const int xs = x < g.nnx ? x : 0;
     const C cx1 = T[a.c.cx1 + xs], cx2 = T[a.c.cx2 + xs];
-    const C cy1 = T[a.c.cy1 + yl], cy2 = T[a.c.cy2 + yl];
+    // a wave is one tile row (TX lanes): its y factors are wave-uniform
+    const int yw = __builtin_amdgcn_readfirstlane(yl);
+    const C cy1 = ztab(T, a.c.cy1 + yw), cy2 = ztab(T, a.c.cy2 + yw);
     const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
     static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
     C hx = 0, hy = 0;
     if constexpr (SM) {
         hx = T[a.c.hvx + xs];
-        hy = T[a.c.hvy + yl];
+        hy = ztab(T, a.c.hvy + yw);
     }
 
     for (int z = zb; z < ze; ++z) {
@@ -326,7 +335,7 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         C hz = 0, mc;
         bool scell = false;
         if constexpr (SM) {
-            hz = T[a.c.hvz + z];
+            hz = ztab(T, a.c.hvz + z);
             scell = hx != (C)0 || hy != (C)0 || hz != (C)0;
             if (scell) mc = (C)bld<NT>(a.mfac, zo, colc);
         }
@@ -357,7 +366,7 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         load_halo(hP, g, zo, x0, y0, tx, ty, yl, ldP);
         Halo<TY, Q_XY, Q_XY, C> hQ;
         if constexpr (Q_XY) load_halo(hQ, g, zo, x0, y0, tx, ty, yl, ldQ);
-        const C cz1 = T[a.c.cz1 + z], cz2 = T[a.c.cz2 + z];
+        const C cz1 = ztab(T, a.c.cz1 + z), cz2 = ztab(T, a.c.cz2 + z);
         __syncthreads();
         sP[ty + MM][tx + MM] = pc;
         store_halo(sP, hP, tx, ty);
@@ -513,13 +522,20 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const double fa0 = ISO ? a.s.p0 : a.s.q0, fa1 = ISO ? a.s.p1 : a.s.q1;
     // lanes of the source columns add this step's source to F as the velocity sweep does
     const bool srccol = WZ && ((a.s.s0 >= 0 && a.s.s0 % g.pl == col) || (a.s.s1 >= 0 && a.s.s1 % g.pl == col));
+    // waves holding a source column (wave-uniform: the others skip the per-lane test)
+    const bool srcw = WZ && __builtin_amdgcn_ballot_w64(srccol) != 0;
     // new w at plane p (velocity sweep arithmetic) from Fj(j) = F at plane p-3+j
     auto wcalc = [&](int p, auto Fj, C wo, C r, C cz1, C cz2) -> C {
-        C zz = dst8<C>([&](int j) {
-            C v = Fj(j);
-            if (srccol) v = inj<C>(v, (p - 3 + j) * g.pl + col, a.s.s0, fa0, a.s.s1, fa1);
-            return v;
-        });
+        C zz;
+        if (srcw) {
+            zz = dst8<C>([&](int j) {
+                C v = Fj(j);
+                if (srccol) v = inj<C>(v, (p - 3 + j) * g.pl + col, a.s.s0, fa0, a.s.s1, fa1);
+                return v;
+            });
+        } else {
+            zz = dst8<C>([&](int j) { return Fj(j); });
+        }
         zz /= r;
         return cz2 * wo - cz1 * dtz * zz;
     };
@@ -534,7 +550,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             const C wo = ldW(p * g.pl, col);
             if (p >= MM && p < g.nnz - MM) {
                 const C r = (C)bld(rho, p * g.pl, colc);
-                const C wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, T[a.c.cz1 + p], T[a.c.cz2 + p]);
+                const C wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, ztab(T, a.c.cz1 + p), ztab(T, a.c.cz2 + p));
                 if (valid && p >= zb) stw((S*)Wv, p * g.pl, col, wn, sc.inv[F_W]);
                 wW[k] = stored(Wv, wn, sc.inv[F_W], sc.s[F_W]);   // as the stress sweep would read it
             } else {
@@ -561,7 +577,9 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     }
     const int xs = x < g.nnx ? x : 0;
     const C ax1 = T[a.c.ax1 + xs], ax2 = T[a.c.ax2 + xs];
-    const C ay1 = T[a.c.ay1 + yl], ay2 = T[a.c.ay2 + yl];
+    // A wave is one tile row (TX lanes): the row's factors and conditions are wave-uniform.
+    const int yw = __builtin_amdgcn_readfirstlane(yl);
+    const C ay1 = ztab(T, a.c.ay1 + yw), ay2 = ztab(T, a.c.ay2 + yw);
     const bool recx = rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
     // Directional split (DIRS, unsplit == 2): a PML cell keeps the split fields of its
     // damped directions only (factor != 1); P minus those fields (Q likewise) is the
@@ -572,114 +590,169 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     C hx = 0, hy = 0;
     if constexpr (SM) {
         hx = T[a.c.hsx + xs];
-        hy = T[a.c.hsy + yl];
+        hy = ztab(T, a.c.hsy + yw);
     }
 
-    for (int z = zb; z < ze; ++z) {
+    // Unsplit scheme: outside the PML (all stress factors 1) P and Q are updated
+    // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
+    // updates in exact arithmetic; the split fields are read and written only in
+    // PML cells. split: the split update; rem: the remainder needs the old P (Q);
+    // sx, sy, sz: the split fields kept.
+    struct Mode {
+        bool split, dZ, rem, sx, sy, sz;
+    };
+    auto mode = [&](int z) {
+        Mode m{false, false, false, true, true, true};
+        if constexpr (DIRS) {
+            m.dZ = !(z >= a.zi0 && z < a.zi1);
+            m.split = dX || dY || m.dZ;
+            m.rem = !(dX && dY && m.dZ);
+            m.sx = dX;
+            m.sy = dY;
+            m.sz = m.dZ;
+        } else {
+            m.split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && z >= a.zi0 && z < a.zi1);
+        }
+        return m;
+    };
+    // Everything plane z reads from memory (the z-window leads at planes z+3 and z+7).
+    // Members a variant does not use are never touched.
+    struct Plane {
+        C Fl, wl, ul, vl, r3, czw1, czw2, az1, az2;
+        f32x4 m3, m, t;
+        f32x2 mm;
+        C c0, r0, e0, d0, ct, st, cp, sp, uc, vc;
+        C f0, f1, f2, f3, f4, f5, p0, q0, mc;
+        Halo<TY, true, TTI, C> hU;
+        Halo<TY, TTI, true, C> hV;
+        Halo<TY, TTI, TTI, C> hW;
+    };
+    auto load = [&](int z, Plane& L) {
+        const int zo = z * g.pl;
+        if constexpr (WZ) {
+            L.Fl = ldw<NT>(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
+            L.wl = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);
+            if constexpr (PACKED) L.m3 = bld4<NT>(m4, zo + 3 * g.pl, colc);
+            else L.r3 = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
+            L.czw1 = ztab(T, a.c.cz1 + z + 3);
+            L.czw2 = ztab(T, a.c.cz2 + z + 3);
+        } else if constexpr (!TTI) {
+            L.wl = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);   // TTI reads w's x-y halo too
+        } else {
+            L.wl = ldW(zo + 3 * g.pl, col);
+            L.ul = ldU(zo + 3 * g.pl, col);
+            L.vl = ldV(zo + 3 * g.pl, col);
+        }
+        if constexpr (PACKED) {
+            if constexpr (!WZ) L.m = bld4<NT>(m4, zo, colc);
+        } else if constexpr (PACK2) {
+            L.mm = bld<NT>(m2, zo, colc);
+        } else {
+            L.c0 = (C)bld<NT>(vp, zo, colc);
+            if constexpr (!WZ && !TTI) L.r0 = (C)bld<NT>(rho, zo, colc);
+            if constexpr (!ISO) {
+                L.e0 = (C)bld<NT>(eps, zo, colc);
+                L.d0 = (C)bld<NT>(del, zo, colc);
+            }
+        }
+        if constexpr (PTRIG) {
+            L.t = bld4<NT>(m4, zo, colc);
+        } else if constexpr (TTI) {
+            L.ct = (C)bld<NT>(Ct, zo, colc);
+            L.st = (C)bld<NT>(St, zo, colc);
+            L.cp = (C)bld<NT>(Cp, zo, colc);
+            L.sp = (C)bld<NT>(Sp, zo, colc);
+        }
+        if constexpr (!TTI) {
+            L.uc = ldU(zo, col);
+            L.vc = ldV(zo, col);
+        }
+        load_halo(L.hU, g, zo, x0, y0, tx, ty, yl, ldU);
+        load_halo(L.hV, g, zo, x0, y0, tx, ty, yl, ldV);
+        if constexpr (TTI) load_halo(L.hW, g, zo, x0, y0, tx, ty, yl, ldW);
+        L.az1 = ztab(T, a.c.az1 + z);
+        L.az2 = ztab(T, a.c.az2 + z);
+        // One-sided branches with distinct destinations, left unset where not loaded:
+        // an if/else writing the same registers, or zeroing them first, makes the wave
+        // wait for earlier loads into those registers.
+        const Mode md = mode(z);
+        if (valid && md.split) {
+            if constexpr (SM) L.mc = (C)bld<NT>(a.mfac, zo, col);
+            if (md.sx) L.f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
+            if (md.sy) L.f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
+            if (md.sz) L.f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
+            if constexpr (!ISO) {
+                if (md.sx) L.f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
+                if (md.sy) L.f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
+                if (md.sz) L.f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
+            }
+        }
+        if (valid && (!md.split || md.rem)) {
+            // WZ: the old value of F at plane z is fF[0]
+            if constexpr (!(WZ && ISO)) L.p0 = ldw<NT>((const S*)Po, zo, col, sc.s[F_P]);
+            if constexpr (!ISO && !WZ) L.q0 = ldw<NT>((const S*)Qo, zo, col, sc.s[F_Q]);
+        }
+    };
+
+    auto body = [&](int z, const Plane& L) {
         const int zo = z * g.pl;
         C wo = 0, czw1 = 0, czw2 = 0;   // WZ: old w, z velocity factors at plane z+3
         if constexpr (WZ) {
-            fF[7] = ldw<NT>(Fo, min(z + 7, g.nnz - 1) * g.pl, col, sF);
-            wo = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);
-            if constexpr (PACKED) mq[3] = bld4<NT>(m4, zo + 3 * g.pl, colc);
-            else fr[3] = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
-            czw1 = T[a.c.cz1 + z + 3];
-            czw2 = T[a.c.cz2 + z + 3];
-        } else if constexpr (!TTI) {
-            wW[7] = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);   // TTI reads w's x-y halo too
+            fF[7] = L.Fl;
+            wo = L.wl;
+            if constexpr (PACKED) mq[3] = L.m3;
+            else fr[3] = L.r3;
+            czw1 = L.czw1;
+            czw2 = L.czw2;
         } else {
-            wW[7] = ldW(zo + 3 * g.pl, col);
+            wW[7] = L.wl;
         }
         if constexpr (TTI) {
-            wU[7] = ldU(zo + 3 * g.pl, col);
-            wV[7] = ldV(zo + 3 * g.pl, col);
+            wU[7] = L.ul;
+            wV[7] = L.vl;
         }
         C c0 = 0, r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
         if constexpr (PACKED) {
-            f32x4 m;
-            if constexpr (WZ) m = mq[0];
-            else m = bld4<NT>(m4, zo, colc);
+            const f32x4 m = WZ ? mq[0] : L.m;
             c0 = (C)m.x;
             r0 = (C)m.y;
             e0 = (C)m.z;
             d0 = (C)m.w;
         } else if constexpr (PACK2) {
-            const f32x2 m = bld<NT>(m2, zo, colc);
-            c0 = (C)m.x;
-            r0 = (C)m.y;
+            c0 = (C)L.mm.x;
+            r0 = (C)L.mm.y;
         } else {
-            c0 = (C)bld<NT>(vp, zo, colc);
+            c0 = L.c0;
             if constexpr (WZ) r0 = fr[0];
-            else if constexpr (!TTI) r0 = (C)bld<NT>(rho, zo, colc);
+            else if constexpr (!TTI) r0 = L.r0;
             if constexpr (!ISO) {
-                e0 = (C)bld<NT>(eps, zo, colc);
-                d0 = (C)bld<NT>(del, zo, colc);
+                e0 = L.e0;
+                d0 = L.d0;
             }
         }
         if constexpr (PTRIG) {
-            const f32x4 t4 = bld4<NT>(m4, zo, colc);
-            ct = t4.x;
-            st = t4.y;
-            cp = t4.z;
-            sp = t4.w;
+            ct = L.t.x;
+            st = L.t.y;
+            cp = L.t.z;
+            sp = L.t.w;
         } else if constexpr (TTI) {
-            ct = (C)bld<NT>(Ct, zo, colc);
-            st = (C)bld<NT>(St, zo, colc);
-            cp = (C)bld<NT>(Cp, zo, colc);
-            sp = (C)bld<NT>(Sp, zo, colc);
+            ct = L.ct;
+            st = L.st;
+            cp = L.cp;
+            sp = L.sp;
         }
         C uc, vc;
         if constexpr (TTI) {
             uc = wU[4];
             vc = wV[4];
         } else {
-            uc = ldU(zo, col);
-            vc = ldV(zo, col);
+            uc = L.uc;
+            vc = L.vc;
         }
-        Halo<TY, true, TTI, C> hU;
-        Halo<TY, TTI, true, C> hV;
-        Halo<TY, TTI, TTI, C> hW;
-        load_halo(hU, g, zo, x0, y0, tx, ty, yl, ldU);
-        load_halo(hV, g, zo, x0, y0, tx, ty, yl, ldV);
-        if constexpr (TTI) load_halo(hW, g, zo, x0, y0, tx, ty, yl, ldW);
-        const C az1 = T[a.c.az1 + z], az2 = T[a.c.az2 + z];
-        // Unsplit scheme: outside the PML (all stress factors 1) P and Q are updated
-        // directly, P -= K (dP_x + dP_y + dP_z), which equals the sum of the split
-        // updates in exact arithmetic; the split fields are read and written only in
-        // PML cells. f0..f5: the split fields px..qz, or P (and Q) where unsplit.
-        // split: the split update; rem: the remainder needs the old P (Q); sx, sy, sz: the
-        // split fields kept
-        bool split, dZ = false, rem = false, sx = true, sy = true, sz = true;
-        if constexpr (DIRS) {
-            dZ = !(z >= a.zi0 && z < a.zi1);
-            split = dX || dY || dZ;
-            rem = !(dX && dY && dZ);
-            sx = dX;
-            sy = dY;
-            sz = dZ;
-        } else {
-            split = !a.unsplit || !(ax1 == (C)1 && ay1 == (C)1 && z >= a.zi0 && z < a.zi1);
-        }
-        // One-sided branches with distinct destinations, left unset where not loaded:
-        // an if/else writing the same registers, or zeroing them first, makes the wave
-        // wait for earlier loads into those registers.
-        C f0, f1, f2, f3, f4, f5, p0, q0, mc;
-        if (valid && split) {
-            if constexpr (SM) mc = (C)bld<NT>(a.mfac, zo, col);
-            if (sx) f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
-            if (sy) f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
-            if (sz) f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
-            if constexpr (!ISO) {
-                if (sx) f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
-                if (sy) f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
-                if (sz) f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
-            }
-        }
-        if (valid && (!split || rem)) {
-            // WZ: the old value of F at plane z is fF[0]
-            if constexpr (!(WZ && ISO)) p0 = ldw<NT>((const S*)Po, zo, col, sc.s[F_P]);
-            if constexpr (!ISO && !WZ) q0 = ldw<NT>((const S*)Qo, zo, col, sc.s[F_Q]);
-        }
+        const C az1 = L.az1, az2 = L.az2;
+        const Mode md = mode(z);
+        const bool split = md.split, dZ = md.dZ, rem = md.rem, sx = md.sx, sy = md.sy, sz = md.sz;
+        C f0 = L.f0, f1 = L.f1, f2 = L.f2, f3 = L.f3, f4 = L.f4, f5 = L.f5, p0 = L.p0, q0 = L.q0, mc = L.mc;
         if constexpr (WZ) {
             if constexpr (ISO) p0 = fF[0];
             else q0 = fF[0];
@@ -687,11 +760,11 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         __syncthreads();
         sU[ty + MM][tx + MM] = uc;
         sV[ty + MM][tx + MM] = vc;
-        store_halo(sU, hU, tx, ty);
-        store_halo(sV, hV, tx, ty);
+        store_halo(sU, L.hU, tx, ty);
+        store_halo(sV, L.hV, tx, ty);
         if constexpr (TTI) {
             sW[ty + MM][tx + MM] = wW[4];
-            store_halo(sW, hW, tx, ty);
+            store_halo(sW, L.hW, tx, ty);
         }
         __syncthreads();
         // WZ: new w at plane z+3 completes the backward window
@@ -755,7 +828,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             } else {
                 // SV taper (reference vti:154-175, tti:139-160)
                 C ee = e0, dd = d0;
-                if (a.sv) {
+                if (a.sv && abs(z - a.szp) <= 15) {   // r2 <= 225 needs |z - szp| <= 15
                     const int rx = x - a.sxp, ry = y - a.syp, rz = z - a.szp;
                     const int r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 <= 225) {
@@ -810,7 +883,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                         if constexpr (SM) {
                             strip_coef<C>(mc, hx, bx1, bx2);
                             strip_coef<C>(mc, hy, by1, by2);
-                            strip_coef<C>(mc, (C)T[a.c.hsz + z], bz1, bz2);
+                            strip_coef<C>(mc, ztab(T, a.c.hsz + z), bz1, bz2);
                         }
                         npx = bx2 * px0 - bx1 * r0 * c0 * c0 * e2 * dtx * xx;
                         npy = by2 * py0 - by1 * r0 * c0 * c0 * e2 * dty * yy;
@@ -847,8 +920,14 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 stw(Qo, zo, col, Qn, sc.inv[F_Q]);
             }
             stw(Po, zo, col, Pn, sc.inv[F_P]);
-            if (recx && z == a.zrec) rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
+            if (z == a.zrec && recx) rec[(y - a.nb) * a.nx + (x - a.nb)] = (float)Pn;
         }
+    };
+
+    for (int z = zb; z < ze; ++z) {
+        Plane L;
+        load(z, L);
+        body(z, L);
     }
 }
 
