@@ -7,7 +7,8 @@ fixtures tests/golden/make_fullsize_golden.py wrote from the pinned CPU oracle:
 
 The FP32 reference arithmetic (scheme 0) must be bitwise equal to the oracle on every
 sampled receiver, the last receiver plane and the sampled snapshots; the faster schemes
-within the 1e-5 bar; binary16 storage and the FP64 instance against the FP64 oracle."""
+within the 1e-5 bar; binary16 storage and the FP64 instance against the FP64 oracle (the
+FP64 instance bitwise after the fixture's rounding to float)."""
 import ast
 import hashlib
 import os
@@ -25,7 +26,13 @@ FILES = {"cfg1": "cfg1_2nd_100_nt500.npz", "cfg2": "cfg2_iso_256_nt2000.npz",
          "cfg3": "cfg3_vti_400x400x300_nt3000.npz"}
 
 
+_CACHE = {}
+
+
 def _fixture(name):
+    """(meta, model, arrays) of a fixture; the model is generated once per test session."""
+    if name in _CACHE:
+        return _CACHE[name]
     from a3dfd import models
     with np.load(os.path.join(HERE, FILES[name]), allow_pickle=False) as f:
         data = {k: f[k] for k in f.files}
@@ -34,6 +41,8 @@ def _fixture(name):
     for k in MODEL_KEYS[name]:
         digest = hashlib.sha256(np.ascontiguousarray(c[k]).tobytes()).hexdigest()
         assert digest == str(data[f"sha256_{k}"]), f"{name}: the model generator's {k} changed"
+    _CACHE.clear()   # one configuration's model in memory at a time
+    _CACHE[name] = (meta, c, data)
     return meta, c, data
 
 
@@ -87,3 +96,51 @@ def test_cfg2_full_length_with_mute(scheme):
     # the record is muted before the 2000 m/s first arrival plus the wavelet and 60 samples:
     # the first samples of every sampled trace are zero
     assert np.all(got[0][:, :, :20] == 0.0)
+
+
+# VTI 400x400x300, nt 3000: the benchmarked shot. The reference arithmetic (scheme 0) is
+# bitwise equal to the FP32 oracle over the whole record and both snapshots. The benchmark
+# path (stress scheme 3, autotuned launch shapes and w placement) meets the 1e-5 bar on the
+# record and the snapshots. The receiver plane of the last step alone carries 1/165 of the
+# record's peak amplitude, so its relative error is rounding-dominated: the FP32 oracle is
+# itself 4.7e-5 from the FP64 oracle there; the GPU plane must stay within 3x that distance.
+@pytest.mark.parametrize("scheme", [0, 3])
+def test_cfg3_full_length(scheme):
+    meta, c, data = _fixture("cfg3")
+    got = _run("cfg3", meta, c, nsnap=2, scheme=scheme)
+    tr, pl, sn = got
+    e32 = _compare(f"cfg3 VTI scheme {scheme}", got, data, 32, scheme == 0, 1.0)
+    e64 = _compare(f"cfg3 VTI scheme {scheme}", got, data, 64, False, 1.0)
+    own = rel_l2(data["plane32"], data["plane64"])   # the FP32 oracle's own distance on the last plane
+    print(f"[gpu-fullnt] cfg3 scheme {scheme}: record {e32[0]:.2e}, snapshots {max(e32[2:]):.2e} vs FP32 oracle; "
+          f"last plane {e64[1]:.2e} vs FP64 oracle (FP32 oracle: {own:.2e})", flush=True)
+    assert e32[0] <= 1e-5 and max(e32[2:]) <= 1e-5, e32
+    assert e64[1] <= 3 * own, (e64[1], own)
+    assert e64[0] <= 1e-4 and max(e64[2:]) <= 1e-4, e64
+
+
+# FP64 instance against the FP64 oracle: bitwise equal after the fixture's rounding to float.
+def test_cfg3_full_length_fp64_instance():
+    meta, c, data = _fixture("cfg3")
+    got = _run("cfg3", meta, c, nsnap=2, storage=2)
+    got = tuple(None if g is None else g.astype(np.float32) for g in got)
+    _compare("cfg3 VTI FP64 instance", got, data, 64, True, 0.0)
+
+
+# binary16 wavefield storage (per-field power-of-two scales) against the FP64 oracle, at
+# twice the errors measured on MI355X: sampled record 7.1e-3, snapshots 5.5e-3 (it 1000) and
+# 1.9e-2 (it 2000), last receiver plane 0.235. The last plane's amplitude is 1/165 of the
+# record's peak, far below the field scale, where binary16's 11-bit significand leaves
+# little relative precision; the record and snapshots carry the accuracy statement.
+FP16_BOUNDS = {"traces": 1.4e-2, "plane": 0.47, "snaps": (1.1e-2, 3.8e-2)}
+
+
+def test_cfg3_full_length_fp16_storage():
+    meta, c, data = _fixture("cfg3")
+    tr, pl, sn = _run("cfg3", meta, c, nsnap=2, storage=1)
+    et, ep = rel_l2(tr, data["traces64"]), rel_l2(pl, data["plane64"])
+    es = [rel_l2(sn[k], data["snaps64"][k]) for k in range(len(sn))]
+    print(f"[gpu-fullnt] cfg3 FP16 storage vs FP64 oracle: traces {et:.2e}, last plane {ep:.2e}, "
+          f"snapshots {', '.join(f'{e:.2e}' for e in es)}", flush=True)
+    assert et <= FP16_BOUNDS["traces"] and ep <= FP16_BOUNDS["plane"], (et, ep)
+    assert all(e <= b for e, b in zip(es, FP16_BOUNDS["snaps"])), es
