@@ -168,6 +168,8 @@ def load(path=None):
     lib.a3d_run_static.restype = c_int
     lib.a3d_probe_bandwidth.argtypes = [c_int, ctypes.c_size_t, c_int, POINTER(c_double), POINTER(c_double)]
     lib.a3d_probe_bandwidth.restype = c_int
+    lib.a3d_probe_stress_pattern.argtypes = [c_int, c_int, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]
+    lib.a3d_probe_stress_pattern.restype = c_int
     lib.a3d_last_error.restype = c_char_p
     lib.a3d_version.restype = c_char_p
     for name in ("a3d_create", "a3d_destroy", "a3d_set_model", "a3d_run", "a3d_set_kernel_timing",
@@ -184,6 +186,18 @@ def probe_bandwidth(device=0, nbytes=2 << 30, reps=10):
     c, r = c_double(), c_double()
     _check(lib, lib.a3d_probe_bandwidth(device, nbytes, reps, ctypes.byref(c), ctypes.byref(r)))
     return c.value, r.value
+
+
+STRESS_PATTERNS = ("stream_mix_every_lane", "interior_in_place_with_leads", "plus_z_pml", "plus_y_pml",
+                   "plus_x_pml_whole_sweep")
+
+
+def probe_stress_pattern(nnx, nny, nnz, device=0, reps=20):
+    """{pattern: (ms per launch, algorithmic GB/s)} of the stress sweep's replayed access pattern."""
+    lib = load()
+    ms, nb = (c_double * 5)(), (c_double * 5)()
+    _check(lib, lib.a3d_probe_stress_pattern(device, nnx, nny, nnz, reps, ms, nb))
+    return {k: (ms[i], nb[i] / (ms[i] * 1e-3) / 1e9) for i, k in enumerate(STRESS_PATTERNS)}
 
 
 class A3DError(RuntimeError):

@@ -580,6 +580,8 @@ def main():
                     help="0: split PML fields everywhere (bitwise reference arithmetic); 1: split only in PML cells; "
                          "2: scheme 1 in one pass per step (bitwise equal to 1); 3: split fields only for the "
                          "damped directions of PML cells")
+    ap.add_argument("--probe-stress", action="store_true",
+                    help="time the VTI stress sweep's memory access pattern replayed without arithmetic")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run the calibration streams (2 GiB float4 copy and read) and print GB/s")
     ap.add_argument("--batched", action="store_true",
@@ -618,6 +620,15 @@ def main():
         c, r = probe_bandwidth(0, 2 << 30, 10)
         print(json.dumps({"probe": "float4 grid 1024x256, 2 GiB buffers, 10 reps",
                           "copy_GBs_read_plus_write": round(c, 1), "read_only_GBs": round(r, 1)}), flush=True)
+        return
+
+    if args.probe_stress:
+        from a3dfd._lib import probe_stress_pattern
+        res = probe_stress_pattern(440, 440, 340)
+        print(json.dumps({"probe": "VTI stress sweep access pattern replayed without arithmetic, padded "
+                          "440x440x340 (400x400x300 + 20-cell PML + 4-cell ring), 64x4 tiles, 20 launches each",
+                          "patterns": {k: {"ms": round(v[0], 4), "algorithmic_GBs": round(v[1], 1)}
+                                       for k, v in res.items()}}), flush=True)
         return
 
     if args.batched:

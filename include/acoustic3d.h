@@ -378,6 +378,14 @@ int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* u
  * for the FETCH_SIZE / WRITE_SIZE byte counters. */
 int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs);
 
+/* Measurement support: the VTI stress sweep's memory access pattern replayed without
+ * its arithmetic on a padded nnx x nny x nnz grid (20-cell PMLs), `reps` launches per
+ * pattern on `device`. ms[0..4] and bytes[0..4] (algorithmic bytes per launch) for the
+ * patterns: 0 its six read and three write streams over every lane (no leads, separate
+ * outputs); 1 the sweep's interior pattern (in place, the z+3 / z+7 leads); 2 plus the
+ * z-PML split pairs; 3 plus the y-PML pairs; 4 plus the x-PML pairs (the whole pattern). */
+int a3d_probe_stress_pattern(int device, int nnx, int nny, int nnz, int reps, double* ms, double* bytes);
+
 const char* a3d_last_error(void);
 const char* a3d_version(void);
 
