@@ -330,13 +330,20 @@ int check_desc_geometry(const a3d_shot_desc* d) {
     return A3D_OK;
 }
 
+// the batched stress sweeps' split mode of a stress scheme (3: directional split, 4: one
+// field per damped direction)
+int dirs_mode(int scheme) { return scheme == 4 ? 2 : scheme == 3 ? 1 : 0; }
+
 int check_desc_shot(const a3d_ctx* c, const a3d_shot_desc* d) {
     if (!same_geometry(c->d, *d)) return fail(A3D_EINVAL, "shot geometry differs from the context's");
     if (d->nt < 1) return fail(A3D_EINVAL, "nt must be positive");
     if (!(d->dt > 0) || !(d->fm > 0)) return fail(A3D_EINVAL, "dt and fm must be positive");
     if (d->rec_z < 0 || d->rec_z >= d->nz) return fail(A3D_EINVAL, "rec_z %d outside [0,%d)", d->rec_z, d->nz);
     if (d->snap_interval < 0) return fail(A3D_EINVAL, "snap_interval must be >= 0");
-    if (d->scheme < 0 || d->scheme > 3) return fail(A3D_EINVAL, "scheme %d not in 0..3", d->scheme);
+    if (d->scheme < 0 || d->scheme > 4) return fail(A3D_EINVAL, "scheme %d not in 0..4", d->scheme);
+    if (d->scheme == 4 && c->strip_net && c->strip_head != 2)
+        return fail(A3D_EINVAL, "scheme 4 keeps one field per damped direction; the strip's damping heads "
+                                "scale the p and q split fields apart (use scheme 3 or the coefficient head)");
     if (d->scheme == 2 && (d->op != A3D_OP_VTI || c->prec != PREC_FP32 || c->strip_net))
         return fail(A3D_EINVAL, "scheme 2 (one-pass step) is built for the VTI operator, FP32 storage, PML boundary");
     if (d->op == A3D_OP_2ND) {
@@ -613,7 +620,7 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.nx = nx;
     sa.ny = ny;
     sa.sc = sc;
-    sa.unsplit = d->scheme == 3 ? 2 : (d->scheme != 0 || prec == PREC_FP16) ? 1 : 0;
+    sa.unsplit = d->scheme == 4 ? 3 : d->scheme == 3 ? 2 : (d->scheme != 0 || prec == PREC_FP16) ? 1 : 0;
     sa.zi0 = zi0;
     sa.zi1 = zi1;
     va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
@@ -700,7 +707,7 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
     a.cdt[0] = d->dt / d->dx;
     a.cdt[1] = d->dt / d->dy;
     a.cdt[2] = d->dt / d->dz;
-    a.dirsplit = d->scheme == 3;
+    a.dirsplit = d->scheme >= 3;
     a.normal = c->strip_head == 1;
     a.mfac = c->strip_head == 2 ? c->strip_mfac : nullptr;
     a.vref = c->vel_d0;
@@ -1763,12 +1770,13 @@ a3d_shot_desc base_desc(int op, int nx, int ny, int nz, float dx, float dy, floa
     a3d_shot_desc d{};
     // A3D_SCHEME=1 selects the unsplit-interior stress update for the legacy entry points,
     // A3D_SCHEME=2 the one-pass step for VTI (scheme 1 for ISO/TTI), A3D_SCHEME=3 split
-    // fields for the damped directions only; default 0: the reference's split fields
-    // everywhere, bit for bit.
+    // fields for the damped directions only, A3D_SCHEME=4 one field per damped direction;
+    // default 0: the reference's split fields everywhere, bit for bit.
     if (const char* e = getenv("A3D_SCHEME")) {
         if (op != A3D_OP_2ND && (e[0] == '1' || e[0] == '2')) d.scheme = 1;
         if (op == A3D_OP_VTI && e[0] == '2') d.scheme = 2;
         if (op != A3D_OP_2ND && e[0] == '3') d.scheme = 3;
+        if (op != A3D_OP_2ND && e[0] == '4') d.scheme = 4;
     }
     d.op = op;
     d.nx = nx;
@@ -2104,7 +2112,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const size_t e = 2 * (gv.size() + k);
             if (timed) HIP_TRY(hipEventRecord(gev[e], b->st));
             HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
-                                        (int)g.wins.size(), g.lb, it, b->d.scheme == 3, wz, g.src, b->st, sm));
+                                        (int)g.wins.size(), g.lb, it, dirs_mode(b->d.scheme), wz, g.src, b->st, sm));
             if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
         }
         for (size_t k = 0; k < swin.size(); ++k) {
@@ -2645,7 +2653,7 @@ extern "C" int a3d_run_static(const a3d_shot_desc* desc, int device, const float
     if ((op == A3D_OP_VTI || op == A3D_OP_TTI) && (!eps || !del)) return fail(A3D_EINVAL, "epsilon and delta required");
     if (op == A3D_OP_TTI && (!theta || !phi)) return fail(A3D_EINVAL, "theta and phi required");
     if (desc->storage != 0) return fail(A3D_EINVAL, "the allocation-free shot runs FP32 storage only");
-    if (desc->scheme == 2) return fail(A3D_EINVAL, "the allocation-free shot runs schemes 0, 1 and 3");
+    if (desc->scheme == 2) return fail(A3D_EINVAL, "the allocation-free shot runs schemes 0, 1, 3 and 4");
     if (desc->snap_interval != 0) return fail(A3D_EINVAL, "the allocation-free shot takes no snapshots");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));

@@ -114,7 +114,9 @@ struct StressArgs {
     float* rec;                // receiver plane of this step ([ny][nx]) or nullptr
     int zrec, nb, nx, ny;
     int unsplit;               // 1: P, Q updated directly where all stress PML factors are 1;
-                               // 2: also split fields only for the damped directions of PML cells
+                               // 2: also split fields only for the damped directions of PML cells;
+                               // 3: as 2 with one field per damped direction (VTI, TTI: the
+                               //    split p and q fields are multiples of it)
     int zi0, zi1;              // planes whose z stress factor is exactly 1: zi0 <= z < zi1
     Scales sc;
     int wz;                    // 1 (ISO, VTI; one z segment): this sweep also updates w
@@ -177,9 +179,10 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
 // multipliers (mfac)
 hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const int* base, int nwin, const Launch& lb,
                             const Src* srcs, int it, bool wz, hipStream_t st, bool sm = false);
-// dirs: the windows' stress args have unsplit == 2 (all windows of a batch share it)
+// dirs: the windows' stress args have unsplit == 2 (dirs 1) or 3 (dirs 2), else 0 (all
+// windows of a batch share it)
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st,
+                               const Launch& lb, int it, int dirs, bool wz, const Src* srcs, hipStream_t st,
                                bool sm = false);
 
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or

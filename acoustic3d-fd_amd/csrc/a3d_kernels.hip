@@ -450,7 +450,7 @@ __global__ __launch_bounds__(TX* TY) void k_vel_batch(const VelArgs* __restrict_
 // planes z..z+7 before writing plane z, so no other tile's values are involved.
 // SM (VTI, FP32, w in the velocity sweep): learned strip with the coefficient head, whose
 // cells take their damping coefficients from their multiplier in a.mfac.
-template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false, bool SM = false>
+template <class PR, int OP, int TY, bool WZ = false, int DIRS = 0, bool SM = false>
 __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using SP = typename PR::SP;
@@ -581,10 +581,16 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     const int yw = __builtin_amdgcn_readfirstlane(yl);
     const C ay1 = ztab(T, a.c.ay1 + yw), ay2 = ztab(T, a.c.ay2 + yw);
     const bool recx = rec && valid && x >= a.nb && x < a.nb + a.nx && y >= a.nb && y < a.nb + a.ny;
-    // Directional split (DIRS, unsplit == 2): a PML cell keeps the split fields of its
+    // Directional split (DIRS 1, unsplit == 2): a PML cell keeps the split fields of its
     // damped directions only (factor != 1); P minus those fields (Q likewise) is the
     // undamped remainder, which takes the undamped directions' updates without damping.
-    constexpr bool dirs = DIRS;
+    // DIRS 2 (unsplit == 3; VTI, TTI): one field per damped direction. The reference's
+    // p_d and q_d obey the same recurrence (same PML factors, both start at 0) and differ
+    // only by a per-point constant factor of their source terms, so both are multiples of
+    // f_d, the split field without anisotropy factor (f_d <- b2 f_d - b1 K dt_d D_d):
+    // p_x,y = e2 f_x,y, p_z = s2 f_z, q_x,y = s2 f_x,y, q_z = f_z (e2 = 1+2eps, s2 = sqrt(1+2delta)).
+    constexpr bool dirs = DIRS != 0;
+    constexpr bool ONE = DIRS == 2 && !ISO;
     const bool dX = ax1 != (C)1, dY = ay1 != (C)1;
     static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
     C hx = 0, hy = 0;
@@ -681,7 +687,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             if (md.sx) L.f0 = ldw<NT>(px, zo, col, sc.s[F_PX]);
             if (md.sy) L.f1 = ldw<NT>(py, zo, col, sc.s[F_PY]);
             if (md.sz) L.f2 = ldw<NT>(pz, zo, col, sc.s[F_PZ]);
-            if constexpr (!ISO) {
+            if constexpr (!ISO && !ONE) {
                 if (md.sx) L.f3 = ldw<NT>(qx, zo, col, sc.s[F_QX]);
                 if (md.sy) L.f4 = ldw<NT>(qy, zo, col, sc.s[F_QY]);
                 if (md.sz) L.f5 = ldw<NT>(qz, zo, col, sc.s[F_QZ]);
@@ -862,7 +868,30 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                 }
                 const C e2 = 1 + 2 * ee;
                 const C s2 = csqrt(1 + 2 * dd);
-                if (split) {
+                if (ONE && split) {
+                    const C fx0 = dX ? f0 : (C)0, fy0 = dY ? f1 : (C)0, fz0 = dZ ? f2 : (C)0;
+                    C rP = 0, rQ = 0;
+                    if (rem) {
+                        rP = p0 - e2 * (fx0 + fy0) - s2 * fz0;
+                        rQ = q0 - s2 * (fx0 + fy0) - fz0;
+                    }
+                    C bx1 = ax1, bx2 = ax2, by1 = ay1, by2 = ay2, bz1 = az1, bz2 = az2;
+                    if constexpr (SM) {
+                        strip_coef<C>(mc, hx, bx1, bx2);
+                        strip_coef<C>(mc, hy, by1, by2);
+                        strip_coef<C>(mc, ztab(T, a.c.hsz + z), bz1, bz2);
+                    }
+                    const C K = VTI ? r0 * c0 * c0 : c0 * c0;
+                    const C nfx = bx2 * fx0 - bx1 * K * dtx * xx;
+                    const C nfy = by2 * fy0 - by1 * K * dty * yy;
+                    const C nfz = bz2 * fz0 - bz1 * K * dtz * zz;
+                    if (sx) stw(px, zo, col, nfx, sc.inv[F_PX]);
+                    if (sy) stw(py, zo, col, nfy, sc.inv[F_PY]);
+                    if (sz) stw(pz, zo, col, nfz, sc.inv[F_PZ]);
+                    const C hxy = nfx + nfy;
+                    Pn = e2 * hxy + s2 * nfz + rP;
+                    Qn = s2 * hxy + nfz + rQ;
+                } else if (split) {
                     C px0 = f0, py0 = f1, pz0 = f2, qx0 = f3, qy0 = f4, qz0 = f5, rP = 0, rQ = 0;
                     if (dirs) {
                         // undamped directions: no split field (factors 1, old value 0)
@@ -931,7 +960,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     }
 }
 
-template <class PR, int OP, int TY, bool WZ = false, bool DIRS = false, bool SM = false>
+template <class PR, int OP, int TY, bool WZ = false, int DIRS = 0, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
     int tix, tiy, seg;
     if (!decode_tile(a.l, tix, tiy, seg)) return;
@@ -941,7 +970,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress(const StressArgs a) {
 // Batched stress sweep; a window's record plane of step it is rec + it * nx * ny. WZ
 // (ISO, VTI; one z segment per tile column): this sweep also updates w, with the
 // step's source of the window (srcs[it * nwin + w], as the velocity sweep adds it).
-template <class PR, int OP, int TY, bool DIRS = false, bool WZ = false, bool SM = false>
+template <class PR, int OP, int TY, int DIRS = 0, bool WZ = false, bool SM = false>
 __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __restrict__ wins,
                                                          const int* __restrict__ base, int nwin, Launch lb, int it,
                                                          const Src* __restrict__ srcs) {
@@ -1178,7 +1207,7 @@ hipError_t vel_ty(int ty, const VelArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool WZ, bool DIRS, bool SM = false>
+template <class PR, int OP, bool WZ, int DIRS, bool SM = false>
 hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
     switch (ty) {
@@ -1215,7 +1244,7 @@ hipError_t vel_batch_ty(int ty, const VelArgs* w, const int* base, int nwin, con
     return hipGetLastError();
 }
 
-template <class PR, int OP, bool DIRS, bool WZ, bool SM = false>
+template <class PR, int OP, int DIRS, bool WZ, bool SM = false>
 hipError_t stress_batch_ty(int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
                            const Src* srcs, hipStream_t st) {
     const dim3 grid(lb.grid);
@@ -1243,24 +1272,29 @@ hipError_t vel_batch_op(int op, int ty, const VelArgs* w, const int* base, int n
 
 template <class PR>
 hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base, int nwin, const Launch& lb, int it,
-                           bool dirs, bool wz, const Src* srcs, hipStream_t st) {
+                           int dirs, bool wz, const Src* srcs, hipStream_t st) {
+    // dirs: 0 split everywhere / unsplit interior, 1 directional split, 2 one field per
+    // damped direction (ISO has no q fields: 2 is 1 there)
     switch (op) {
         case 1:
             if (wz)
-                return dirs ? stress_batch_ty<PR, 1, true, true>(ty, w, base, nwin, lb, it, srcs, st)
-                            : stress_batch_ty<PR, 1, false, true>(ty, w, base, nwin, lb, it, srcs, st);
-            return dirs ? stress_batch_ty<PR, 1, true, false>(ty, w, base, nwin, lb, it, srcs, st)
-                        : stress_batch_ty<PR, 1, false, false>(ty, w, base, nwin, lb, it, srcs, st);
+                return dirs ? stress_batch_ty<PR, 1, 1, true>(ty, w, base, nwin, lb, it, srcs, st)
+                            : stress_batch_ty<PR, 1, 0, true>(ty, w, base, nwin, lb, it, srcs, st);
+            return dirs ? stress_batch_ty<PR, 1, 1, false>(ty, w, base, nwin, lb, it, srcs, st)
+                        : stress_batch_ty<PR, 1, 0, false>(ty, w, base, nwin, lb, it, srcs, st);
         case 2:
             if (wz)
-                return dirs ? stress_batch_ty<PR, 2, true, true>(ty, w, base, nwin, lb, it, srcs, st)
-                            : stress_batch_ty<PR, 2, false, true>(ty, w, base, nwin, lb, it, srcs, st);
-            return dirs ? stress_batch_ty<PR, 2, true, false>(ty, w, base, nwin, lb, it, srcs, st)
-                        : stress_batch_ty<PR, 2, false, false>(ty, w, base, nwin, lb, it, srcs, st);
+                return dirs == 2 ? stress_batch_ty<PR, 2, 2, true>(ty, w, base, nwin, lb, it, srcs, st)
+                     : dirs      ? stress_batch_ty<PR, 2, 1, true>(ty, w, base, nwin, lb, it, srcs, st)
+                                 : stress_batch_ty<PR, 2, 0, true>(ty, w, base, nwin, lb, it, srcs, st);
+            return dirs == 2 ? stress_batch_ty<PR, 2, 2, false>(ty, w, base, nwin, lb, it, srcs, st)
+                 : dirs      ? stress_batch_ty<PR, 2, 1, false>(ty, w, base, nwin, lb, it, srcs, st)
+                             : stress_batch_ty<PR, 2, 0, false>(ty, w, base, nwin, lb, it, srcs, st);
         case 3:
             if (wz) return hipErrorInvalidValue;
-            return dirs ? stress_batch_ty<PR, 3, true, false>(ty, w, base, nwin, lb, it, srcs, st)
-                        : stress_batch_ty<PR, 3, false, false>(ty, w, base, nwin, lb, it, srcs, st);
+            return dirs == 2 ? stress_batch_ty<PR, 3, 2, false>(ty, w, base, nwin, lb, it, srcs, st)
+                 : dirs      ? stress_batch_ty<PR, 3, 1, false>(ty, w, base, nwin, lb, it, srcs, st)
+                             : stress_batch_ty<PR, 3, 0, false>(ty, w, base, nwin, lb, it, srcs, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1277,16 +1311,21 @@ hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
 
 template <class PR>
 hipError_t stress_op(int op, int ty, const StressArgs& a, hipStream_t st) {
+    // unsplit: 0, 1 -> DIRS 0; 2 -> DIRS 1 (directional split); 3 -> DIRS 2 (one field per
+    // damped direction; ISO has no q fields, so 3 is 2 there)
     switch (op) {
         case 1:
-            if (a.unsplit == 2) return a.wz ? stress_ty<PR, 1, true, true>(ty, a, st) : stress_ty<PR, 1, false, true>(ty, a, st);
-            return a.wz ? stress_ty<PR, 1, true, false>(ty, a, st) : stress_ty<PR, 1, false, false>(ty, a, st);
+            if (a.unsplit >= 2) return a.wz ? stress_ty<PR, 1, true, 1>(ty, a, st) : stress_ty<PR, 1, false, 1>(ty, a, st);
+            return a.wz ? stress_ty<PR, 1, true, 0>(ty, a, st) : stress_ty<PR, 1, false, 0>(ty, a, st);
         case 2:
-            if (a.unsplit == 2) return a.wz ? stress_ty<PR, 2, true, true>(ty, a, st) : stress_ty<PR, 2, false, true>(ty, a, st);
-            return a.wz ? stress_ty<PR, 2, true, false>(ty, a, st) : stress_ty<PR, 2, false, false>(ty, a, st);
+            if (a.unsplit == 3) return a.wz ? stress_ty<PR, 2, true, 2>(ty, a, st) : stress_ty<PR, 2, false, 2>(ty, a, st);
+            if (a.unsplit == 2) return a.wz ? stress_ty<PR, 2, true, 1>(ty, a, st) : stress_ty<PR, 2, false, 1>(ty, a, st);
+            return a.wz ? stress_ty<PR, 2, true, 0>(ty, a, st) : stress_ty<PR, 2, false, 0>(ty, a, st);
         case 3:
             if (a.wz) return hipErrorInvalidValue;
-            return a.unsplit == 2 ? stress_ty<PR, 3, false, true>(ty, a, st) : stress_ty<PR, 3, false, false>(ty, a, st);
+            return a.unsplit == 3   ? stress_ty<PR, 3, false, 2>(ty, a, st)
+                   : a.unsplit == 2 ? stress_ty<PR, 3, false, 1>(ty, a, st)
+                                    : stress_ty<PR, 3, false, 0>(ty, a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1347,8 +1386,9 @@ hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st) {
     if (a.mfac) {   // learned strip, coefficient head: VTI, FP32, w in the velocity sweep
         if (prec != PREC_FP32 || op != 2 || a.wz) return hipErrorInvalidValue;
-        return a.unsplit == 2 ? stress_ty<Fp32, 2, false, true, true>(ty, a, st)
-                              : stress_ty<Fp32, 2, false, false, true>(ty, a, st);
+        return a.unsplit == 3   ? stress_ty<Fp32, 2, false, 2, true>(ty, a, st)
+               : a.unsplit == 2 ? stress_ty<Fp32, 2, false, 1, true>(ty, a, st)
+                                : stress_ty<Fp32, 2, false, 0, true>(ty, a, st);
     }
     switch (prec) {
         case PREC_FP32: return stress_op<Fp32>(op, ty, a, st);
@@ -1373,11 +1413,12 @@ hipError_t launch_vel_batch(int prec, int op, int ty, const VelArgs* wins, const
 }
 
 hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins, const int* base, int nwin,
-                               const Launch& lb, int it, bool dirs, bool wz, const Src* srcs, hipStream_t st, bool sm) {
+                               const Launch& lb, int it, int dirs, bool wz, const Src* srcs, hipStream_t st, bool sm) {
     if (sm) {
         if (prec != PREC_FP32 || op != 2 || wz) return hipErrorInvalidValue;
-        return dirs ? stress_batch_ty<Fp32, 2, true, false, true>(ty, wins, base, nwin, lb, it, srcs, st)
-                    : stress_batch_ty<Fp32, 2, false, false, true>(ty, wins, base, nwin, lb, it, srcs, st);
+        return dirs == 2 ? stress_batch_ty<Fp32, 2, 2, false, true>(ty, wins, base, nwin, lb, it, srcs, st)
+             : dirs      ? stress_batch_ty<Fp32, 2, 1, false, true>(ty, wins, base, nwin, lb, it, srcs, st)
+                         : stress_batch_ty<Fp32, 2, 0, false, true>(ty, wins, base, nwin, lb, it, srcs, st);
     }
     switch (prec) {
         case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);

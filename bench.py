@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Benchmark: 1st-order VTI pseudo-acoustic shot, 400 x 400 x 300 cells,
 dx = 12.5 m, dt = 1 ms, nt = 3000, 20-cell split PML, FP32, on one MI355X.
-Default stress update: scheme 3 (split PML fields kept only for the damped
-directions of PML cells, within 1e-5 relative L2 of the reference arithmetic:
-4.3e-7 on this shot's record); --scheme 1 keeps all six in every PML cell,
---scheme 0 runs the reference's split-everywhere arithmetic bit for bit.
+Default stress update: scheme 4 (one PML field per damped direction of a PML
+cell, of which the reference's split p and q fields are per-point multiples;
+within 1e-5 relative L2 of the reference arithmetic: 4.1e-7 on this shot's
+record); --scheme 3 keeps the (p, q) pair per damped direction, --scheme 1 all
+six in every PML cell, --scheme 0 runs the reference's split-everywhere
+arithmetic bit for bit.
 
 A "step" is one full shot (nt time steps of the velocity and stress sweeps)
 over the synthetic seeded model, with the model and wavefields resident in
@@ -48,7 +50,8 @@ def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
     one plane ahead (also the unsplit cell's Q), the old w, and writes w; otherwise as above.
     scheme 2 (one pass, FP32): (0, R P,Q,u,v,w, 4 model; W P,Q,u,v,w; + R/W 6 split in PML cells).
     scheme 3: the unsplit cell plus R/W of 2 split fields per damped direction; a cell damped in
-    all three directions is the split cell (it reads no old P)."""
+    all three directions is the split cell (it reads no old P). scheme 4: as 3 with one field
+    per damped direction (R/W 1 field; the all-damped cell R/W 3 fields)."""
     ws = {0: 4, 1: 2, 2: 8}[storage]
     ms = 8 if storage == 2 else 4
     ss = max(ws, 4)                     # split PML fields are FP32 under binary16 storage
@@ -64,6 +67,8 @@ def algorithmic_bytes(cfg, storage, scheme, npts, npml=20, wz=0):
     f = inner / npts
     if scheme == 3:
         return a_vel, dir_split_bytes(cfg, npml, unsplit, split, 4 * ss)
+    if scheme == 4:
+        return a_vel, dir_split_bytes(cfg, npml, unsplit, split - 6 * ss, 2 * ss)
     if scheme == 2:
         one = 10 * ws + 4 * ms
         return 0, f * one + (1 - f) * (one + 12 * ss)
@@ -131,7 +136,8 @@ def precision_study(cfg):
     nz = cfg["nz"]
     out = {}
     for name, storage, scheme in (("fp64", 2, 0), ("fp32", 0, 0), ("fp32_unsplit", 0, 1), ("fp32_dirsplit", 0, 3),
-                                  ("fp16", 1, 1), ("fp16_dirsplit", 1, 3)):
+                                  ("fp32_onefield", 0, 4), ("fp16", 1, 1), ("fp16_dirsplit", 1, 3),
+                                  ("fp16_onefield", 1, 4)):
         d = make_desc("vti", cfg["nx"], cfg["ny"], nz, cfg["dx"], cfg["dy"], cfg["dz"], cfg["sx"], cfg["sy"],
                       cfg["sz"], cfg["nt"], cfg["dt"], cfg["fm"], storage=storage, scheme=scheme,
                       snap_interval=1000, snap_skip0=1)
@@ -145,7 +151,7 @@ def precision_study(cfg):
     ref = out["fp64"]
     res = {"workload": "vti_400x400x300_nt%d" % cfg["nt"], "reference": "GPU FP64 instance"}
     rl = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64)))  # noqa
-    for name in ("fp32", "fp32_unsplit", "fp32_dirsplit", "fp16", "fp16_dirsplit"):
+    for name in ("fp32", "fp32_unsplit", "fp32_dirsplit", "fp32_onefield", "fp16", "fp16_dirsplit", "fp16_onefield"):
         o = out[name]
         res[name] = {"shot_rel_l2_vs_fp64": rl(o["shot"], ref["shot"]),
                      "snapshot_it1000_rel_l2_vs_fp64": rl(o["snap"], ref["snap"]),
@@ -154,9 +160,10 @@ def precision_study(cfg):
     a, b = out["fp32_unsplit"], out["fp32"]
     res["fp32_unsplit_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
                                    "snapshot_it2000": rl(a["snap2"], b["snap2"])}
-    a = out["fp32_dirsplit"]
-    res["fp32_dirsplit_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
-                                    "snapshot_it2000": rl(a["snap2"], b["snap2"])}
+    for name in ("fp32_dirsplit", "fp32_onefield"):
+        a = out[name]
+        res[name + "_vs_fp32"] = {"shot_rel_l2": rl(a["shot"], b["shot"]), "snapshot_it1000": rl(a["snap"], b["snap"]),
+                                  "snapshot_it2000": rl(a["snap2"], b["snap2"])}
     res["fp64"] = {"loop_ms": round(ref["ms"], 1), "Gpt_s": round(ref["gpts"], 2)}
     print(json.dumps(res), flush=True)
 
@@ -495,10 +502,10 @@ def operators(args):
     c2 = models.cfg2()
     runs += [("iso", c2, dict(v_mute=2000.0), s, c2["nt"]) for s in (0, 1, 3)]
     c3 = models.cfg3()
-    runs += [("vti", c3, {}, s, c3["nt"]) for s in (0, 1, 3)]
+    runs += [("vti", c3, {}, s, c3["nt"]) for s in (0, 1, 3, 4)]
     c4 = models.cfg4_model()
     c4.update(sx=301, sy=301, sz=1)
-    runs += [("tti", c4, {}, s, 500) for s in (0, 1, 3)]
+    runs += [("tti", c4, {}, s, 500) for s in (0, 1, 3, 4)]
     if args.ops:
         runs = [r for r in runs if r[0] in args.ops.split(",")]
     for op, c, kw, scheme, nt in runs:
@@ -528,11 +535,15 @@ def operators(args):
         inner = (c["nx"] + 1) * (c["ny"] + 1) * (c["nz"] + 1) if op != "2nd" else npts
         f = min(1.0, inner / npts)
         a1 = a_split if scheme == 0 else f * a_unsplit + (1 - f) * a_split
-        if scheme == 3:
+        if scheme in (3, 4):
             # old P (Q) reads a split cell skips: those the sweep does not take from its w window
             drop = {"iso": 0 if launch["w_sweep"] else 4, "vti": 4 if launch["w_sweep"] else 8, "tti": 8}[op]
             per_dir = (a_split - a_unsplit + drop) / 3
-            a1 = dir_split_bytes(c, nbb, a_unsplit, a_split, per_dir)
+            a_all = a_split
+            if scheme == 4 and op != "iso":   # one field per damped direction instead of the (p, q) pair
+                per_dir -= 8
+                a_all -= 24
+            a1 = dir_split_bytes(c, nbb, a_unsplit, a_all, per_dir)
         k0 = kms[0] / max(kn[0], 1)
         k1 = kms[1] / max(kn[1], 1) if kn[1] else 0.0
         g0 = a0 * npts / (k0 * 1e-3) / 1e9
@@ -576,10 +587,10 @@ def main():
     ap.add_argument("--precision-study", action="store_true",
                     help="run the shot in FP32, FP16-storage and FP64 and print the errors vs FP64")
     ap.add_argument("--storage", type=int, default=0, help="0 FP32 (default), 1 FP16 wavefield storage, 2 FP64")
-    ap.add_argument("--scheme", type=int, default=3,
+    ap.add_argument("--scheme", type=int, default=4,
                     help="0: split PML fields everywhere (bitwise reference arithmetic); 1: split only in PML cells; "
                          "2: scheme 1 in one pass per step (bitwise equal to 1); 3: split fields only for the "
-                         "damped directions of PML cells")
+                         "damped directions of PML cells; 4: one field per damped direction")
     ap.add_argument("--probe-stress", action="store_true",
                     help="time the VTI stress sweep's memory access pattern replayed without arithmetic")
     ap.add_argument("--probe-copy", action="store_true",
@@ -746,7 +757,10 @@ def main():
                                  "scheme 0 (bitwise the FP32 oracle) measured below",
                               2: "scheme 1 in one pass: bitwise equal to scheme 1",
                               3: "split PML fields for damped directions only: same equations, other rounding; "
-                                 "error vs scheme 0 (bitwise the FP32 oracle) measured below"}[args.scheme],
+                                 "error vs scheme 0 (bitwise the FP32 oracle) measured below",
+                              4: "one PML field per damped direction (the reference's split p and q fields are "
+                                 "per-point multiples of it): same equations, other rounding; error vs scheme 0 "
+                                 "(bitwise the FP32 oracle) measured below"}[args.scheme],
                    "errors_measured_in_this_run": acc},
         "roofline": {"bound": "hbm", "kernel": main_kernel, "achieved": round(ach_str, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_str / HBM_PEAK_GBS, 4),

@@ -132,8 +132,15 @@ typedef struct a3d_shot_desc {
                               in the interior); 3: scheme 1, and a PML cell keeps the split
                               fields of its damped directions only (P, Q minus them carry the
                               undamped directions' updates): 64 instead of 92 B/point in the
-                              faces' PML cells, within 1e-5 relative L2 of scheme 0. Storage 1
-                              (binary16) uses scheme 1 or 3. */
+                              faces' PML cells, within 1e-5 relative L2 of scheme 0; 4 (VTI,
+                              TTI; ISO runs it as 3): scheme 3 with one field f per damped
+                              direction instead of the pair (p, q): the reference's split p and q
+                              of a direction obey one recurrence and differ only by the per-point
+                              anisotropy factors of their source terms, so p_x,y = (1+2eps) f,
+                              q_x,y = sqrt(1+2delta) f, p_z = sqrt(1+2delta) f, q_z = f (56
+                              instead of 64 B/point in the faces' PML cells; not with the
+                              learned strip's damping heads). Storage 1 (binary16) uses scheme 1,
+                              3 or 4. */
 } a3d_shot_desc;
 
 typedef struct a3d_ctx a3d_ctx;
@@ -353,7 +360,7 @@ int a3d_mlp_train(a3d_mlp* m, const float* X, const float* T, int n, int epochs,
 /* Size of the library's static scratch array, held by every device the library runs
  * the entry point below on, until the process ends. */
 #define A3D_STATIC_SCRATCH_MIB 256
-/* One shot of *desc, as a3d_run with FP32 storage (storage 0), stress scheme 0, 1 or 3,
+/* One shot of *desc, as a3d_run with FP32 storage (storage 0), stress scheme 0, 1, 3 or 4,
  * no snapshots, the default launch shape and w in the velocity sweep, calling no
  * hipMalloc/hipFree or other allocating routine: the padded model, wavefields, tables
  * and receiver record live in the static scratch array (A3D_STATIC_SCRATCH_MIB MiB per

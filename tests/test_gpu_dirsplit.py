@@ -1,9 +1,10 @@
-"""GPU: stress scheme 3, split PML fields kept only for the damped directions of a PML cell.
-P (Q) minus the kept split fields is the undamped remainder, which takes the undamped
-directions' updates; in the exact arithmetic this is the reference's split update, so the
-bar is the 1e-5 relative L2 of the FP32 oracle (scheme 0 arithmetic) on records and
-snapshots. The update of a cell does not depend on the launch, the sweep that makes w or
-the z chunks, so those must give the same bits."""
+"""GPU: stress scheme 3, split PML fields kept only for the damped directions of a PML cell,
+and scheme 4, one field per damped direction (VTI, TTI: the reference's p and q split fields
+of a direction are per-point multiples of it). P (Q) minus the kept split fields is the
+undamped remainder, which takes the undamped directions' updates; in the exact arithmetic
+this is the reference's split update, so the bar is the 1e-5 relative L2 of the FP32 oracle
+(scheme 0 arithmetic) on records and snapshots. The update of a cell does not depend on the
+launch, the sweep that makes w or the z chunks, so those must give the same bits."""
 import numpy as np
 import pytest
 
@@ -32,46 +33,55 @@ def _gpu(op, args, nb, launch=None, wz=None, zchunk=0, nsnap=2, **kw):
     return m, out, snaps
 
 
+@pytest.mark.parametrize("scheme", [3, 4])
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])
-def test_dir_split_within_parity_bar(oracle, op):
+def test_dir_split_within_parity_bar(oracle, op, scheme):
     """FP32 against the FP32 oracle; a multi-tile grid whose x-, y- and z-faces, edges and
     corners of the PML share tiles with interior cells."""
     nx, ny, nz, nb = 70, 41, 30, 10
     args = (op, nx, ny, nz, D, D, D, 36, 21, 4, 160, 1e-3, 25.0)
-    m, g, gs = _gpu(op, args, nb, scheme=3, snap_interval=159)
+    m, g, gs = _gpu(op, args, nb, scheme=scheme, snap_interval=159)
     _, g1, _ = _gpu(op, args, nb, scheme=1, snap_interval=159)
     r32, s32, _ = oracle.run(oracle.make_shot(*args, nb=nb, snap_interval=159), 32, nsnap_max=2, **m)
     e, es = rel_l2(g, r32), rel_l2(gs, s32)
-    print(f"[gpu-dirsplit] {op}: scheme 3 vs FP32 oracle {e:.2e} (snapshot {es:.2e}); "
+    print(f"[gpu-dirsplit] {op}: scheme {scheme} vs FP32 oracle {e:.2e} (snapshot {es:.2e}); "
           f"scheme 1 {rel_l2(g1, r32):.2e}")
     assert np.abs(g).max() > 0
     assert e <= 1e-5 and es <= 1e-5
     assert not np.array_equal(g, g1)   # the remainder path is really taken
+    if scheme == 4:
+        _, g3, s3 = _gpu(op, args, nb, scheme=3, snap_interval=159)
+        if op == "iso":   # no q fields: scheme 4 is scheme 3
+            assert np.array_equal(g, g3) and np.array_equal(gs, s3)
+        else:             # one field per direction: other rounding than the pair
+            assert not np.array_equal(g, g3)
 
 
+@pytest.mark.parametrize("scheme", [3, 4])
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])
-def test_dir_split_fp64(oracle, op):
+def test_dir_split_fp64(oracle, op, scheme):
     """FP64 storage and arithmetic against the FP64 oracle: agreement at the FP32 output rounding."""
     nx, ny, nz, nb = 40, 30, 28, 10
     args = (op, nx, ny, nz, D, D, D, 20, 15, 4, 150, 1e-3, 25.0)
-    m, g, _ = _gpu(op, args, nb, scheme=3, storage=2)
+    m, g, _ = _gpu(op, args, nb, scheme=scheme, storage=2)
     r64, _, _ = oracle.run(oracle.make_shot(*args, nb=nb), 64, **m)
     e = rel_l2(g, r64)
-    print(f"[gpu-dirsplit] {op}: FP64 scheme 3 vs FP64 oracle {e:.2e}")
+    print(f"[gpu-dirsplit] {op}: FP64 scheme {scheme} vs FP64 oracle {e:.2e}")
     assert e <= 1e-7
 
 
+@pytest.mark.parametrize("scheme", [3, 4])
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])
-def test_dir_split_fp16(oracle, op):
+def test_dir_split_fp16(oracle, op, scheme):
     """binary16 wavefield storage (split fields FP32): the error against the FP64 oracle stays
     within the bounds of the binary16 scheme-1 run."""
     nx, ny, nz, nb = 24, 20, 22, 10
     args = (op, nx, ny, nz, D, D, D, 12, 10, 3, 150, 1e-3, 25.0)
-    m, g3, s3 = _gpu(op, args, nb, scheme=3, storage=1, snap_interval=149, seed=13)
+    m, g3, s3 = _gpu(op, args, nb, scheme=scheme, storage=1, snap_interval=149, seed=13)
     _, g1, _ = _gpu(op, args, nb, scheme=1, storage=1, snap_interval=149, seed=13)
     r64, s64, _ = oracle.run(oracle.make_shot(*args, nb=nb, snap_interval=149), 64, nsnap_max=2, **m)
     e3, e1, es = rel_l2(g3, r64), rel_l2(g1, r64), rel_l2(s3, s64)
-    print(f"[gpu-dirsplit] {op}: FP16 scheme 3 vs FP64 {e3:.2e} (snapshot {es:.2e}); scheme 1 {e1:.2e}")
+    print(f"[gpu-dirsplit] {op}: FP16 scheme {scheme} vs FP64 {e3:.2e} (snapshot {es:.2e}); scheme 1 {e1:.2e}")
     assert np.isfinite(g3).all()
     # about twice the errors measured on MI355X (shot 3.3e-3 / 2.7e-3 / 3.1e-3, snapshot
     # 4.5e-2 / 4.5e-2 / 3.8e-2 for ISO / VTI / TTI)
@@ -79,15 +89,16 @@ def test_dir_split_fp16(oracle, op):
     assert e3 <= shot_max and es <= snap_max
 
 
+@pytest.mark.parametrize("scheme", [3, 4])
 @pytest.mark.parametrize("op", ["iso", "vti", "tti"])
-def test_dir_split_launch_sweep_and_chunk_invariance(op):
+def test_dir_split_launch_sweep_and_chunk_invariance(op, scheme):
     """Tile rows, y tiles, z segments, w made by either sweep (ISO, VTI) and z chunks: the same bits."""
     nx, ny, nz, nb = 45, 33, 31, 10
     args = (op, nx, ny, nz, D, D, D, 22, 16, 5, 130, 1e-3, 25.0)
     cases = [((8, 1, 8, 1), 0, 0), ((2, 5, 2, 5), 0, 0), ((4, 3, 4, 3), 0, 11)]
     if op != "tti":
         cases += [((4, 2, 2, 1), 1, 0), ((8, 1, 8, 1), 1, 0)]
-    outs = [_gpu(op, args, nb, launch=L, wz=wz, zchunk=zc, scheme=3, snap_interval=65, nsnap=3)[1:]
+    outs = [_gpu(op, args, nb, launch=L, wz=wz, zchunk=zc, scheme=scheme, snap_interval=65, nsnap=3)[1:]
             for L, wz, zc in cases]
     a, sa = outs[0]
     assert np.abs(a).max() > 0
@@ -96,27 +107,29 @@ def test_dir_split_launch_sweep_and_chunk_invariance(op):
 
 
 def test_dir_split_larger_grid_against_reference_arithmetic():
-    """A 180x160x140 VTI shot (PML 20, 700 steps): scheme 3 against scheme 0 (the reference's
-    arithmetic, bitwise equal to the FP32 oracle) within the parity bar."""
+    """A 180x160x140 VTI shot (PML 20, 700 steps): schemes 3 and 4 against scheme 0 (the
+    reference's arithmetic, bitwise equal to the FP32 oracle) within the parity bar."""
     args = ("vti", 180, 160, 140, 12.5, 12.5, 12.5, 90, 80, 10, 700, 1e-3, 20.0)
     _, g0, s0 = _gpu("vti", args, 20, scheme=0, snap_interval=350, seed=5)
-    _, g3, s3 = _gpu("vti", args, 20, scheme=3, snap_interval=350, seed=5)
-    e, es = rel_l2(g3, g0), rel_l2(s3, s0)
-    print(f"[gpu-dirsplit] 180x160x140: scheme 3 vs scheme 0 {e:.2e} (snapshot {es:.2e})")
     assert np.abs(g0).max() > 0
-    assert e <= 1e-5 and es <= 1e-5
+    for scheme in (3, 4):
+        _, g3, s3 = _gpu("vti", args, 20, scheme=scheme, snap_interval=350, seed=5)
+        e, es = rel_l2(g3, g0), rel_l2(s3, s0)
+        print(f"[gpu-dirsplit] 180x160x140: scheme {scheme} vs scheme 0 {e:.2e} (snapshot {es:.2e})")
+        assert e <= 1e-5 and es <= 1e-5
 
 
+@pytest.mark.parametrize("scheme", [3, 4])
 @pytest.mark.parametrize("op", ["vti", "tti"])
-def test_dir_split_batched_windows(oracle, op):
-    """Batched windows with scheme 3: batched launches and the captured graph equal the
+def test_dir_split_batched_windows(oracle, op, scheme):
+    """Batched windows with schemes 3 and 4: batched launches and the captured graph equal the
     sequential runs, and each window is within the parity bar of the oracle on its sub-model."""
     from a3dfd import Batch, Window, make_desc
     from a3dfd.models import small_model
     NX, NY, NZ = 64, 52, 30
     m = small_model(op, NX, NY, NZ, seed=37)
     wins = [(0, 0, 0, 30, 28, 26, 12, 10, 2), (20, 10, 0, 44, 40, 30, 20, 21, 2), (40, 30, 2, 24, 22, 20, 9, 8, 3)]
-    desc = make_desc(op, NX, NY, NZ, D, D, D, 1, 1, 1, 70, 1e-3, 25.0, nb=8, scheme=3)
+    desc = make_desc(op, NX, NY, NZ, D, D, D, 1, 1, 1, 70, 1e-3, 25.0, nb=8, scheme=scheme)
     print(f"\n[gpu-dirsplit] batched {op}", flush=True)
     with Batch(desc, [Window(*w) for w in wins]) as b:
         b.set_model(**m)

@@ -153,7 +153,7 @@ def test_strip_matches_oracle(oracle, scheme, mfma, head):
     assert e_shot <= 1e-5 and e_snap <= 1e-5
 
 
-@pytest.mark.parametrize("scheme,mfma,period", [(0, 1, 1), (3, 1, 1), (1, 1, 2), (0, 0, 1), (3, 1, 4)])
+@pytest.mark.parametrize("scheme,mfma,period", [(0, 1, 1), (3, 1, 1), (1, 1, 2), (0, 0, 1), (3, 1, 4), (4, 1, 1)])
 def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
     """Head mode 2: the network's multiplier of each strip cell's PML damping, read by the
     velocity and stress sweeps inside their update (no pass after the sweeps), against the
