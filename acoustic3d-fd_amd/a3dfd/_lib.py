@@ -121,6 +121,7 @@ def load(path=None):
     lib.a3d_batch_run.argtypes = [vp, c_int, POINTER(fp), POINTER(c_double)]
     lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
     lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
+    lib.a3d_batch_set_persist.argtypes = [vp, c_int, c_int, c_int]
     lib.a3d_batch_set_zseg.argtypes = [vp, c_int, c_int]
     lib.a3d_batch_set_strip_net.argtypes = [vp, fp, c_int]
     lib.a3d_batch_set_w_sweep.argtypes = [vp, c_int]
@@ -462,7 +463,8 @@ class Batch:
         return n.value
 
     def run(self, mode=2, shots=True, which=None):
-        """mode 0: sequential plain runs; 1: batched; 2: batched, one hipGraph. Returns (shots, loop_ms)."""
+        """mode 0: sequential plain runs; 1: batched; 2: batched, one hipGraph; 3: persistent kernel
+        (device-side time loop, launches captured in one hipGraph). Returns (shots, loop_ms)."""
         nw = len(self.windows)
         outs = [None] * nw
         ptrs = (POINTER(c_float) * nw)()
@@ -476,6 +478,11 @@ class Batch:
 
     def set_graph_chunk(self, steps):
         _check(self.lib, self.lib.a3d_batch_set_graph_chunk(self._h, steps))
+
+    def set_persist(self, tile_rows=4, steps=100, group=1):
+        """Mode 3: tile rows of the persistent kernel (4 or 8), time steps per launch and
+        consecutive tiles per ticket."""
+        _check(self.lib, self.lib.a3d_batch_set_persist(self._h, int(tile_rows), int(steps), int(group)))
 
     def info(self):
         nw = len(self.windows)

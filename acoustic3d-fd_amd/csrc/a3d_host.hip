@@ -487,10 +487,10 @@ Scales field_scales(const a3d_ctx* c, const a3d_shot_desc* d) {
 }
 
 // The stress sweep can update w (from its own column's old P or Q, one z segment)
-// for ISO and VTI in the two-sweep step.
+// for ISO and VTI in the two-sweep step (with the learned strip's coefficient head too:
+// a strip cell's w takes the velocity sweep's multiplied z coefficients).
 bool wz_eligible(const a3d_ctx* c, const a3d_shot_desc* d) {
-    return (d->op == A3D_OP_ISO || d->op == A3D_OP_VTI) && d->scheme != 2 && c->zchunk == 0 &&
-           !(c->strip_net && c->strip_head == 2);
+    return (d->op == A3D_OP_ISO || d->op == A3D_OP_VTI) && d->scheme != 2 && c->zchunk == 0;
 }
 bool wz_active(const a3d_ctx* c, const a3d_shot_desc* d) {
     return wz_eligible(c, d) && (c->wz_mode == 1 || (c->wz_mode < 0 && c->wz_tuned));
@@ -1964,6 +1964,9 @@ struct a3d_batch {
     std::vector<hipGraphExec_t> exec;
     int graph_chunk = 0;                  // 0: one graph for the whole loop
     long long graph_launches = 0;
+    int persist_ty = 4;                   // mode 3: tile rows of the persistent kernel (4 or 8)
+    int persist_steps = 100;              // mode 3: time steps per persistent launch
+    int persist_group = 1;                // mode 3: consecutive tiles per ticket
 };
 
 namespace {
@@ -2037,6 +2040,208 @@ void drop_graph(a3d_batch* b) {
     b->graph.clear();
 }
 
+// Per-window sentinel, record transpose (+ mute) and copy to the caller's shots.
+int batch_outputs(a3d_batch* b, float** shots) {
+    const int nw = (int)b->ctx.size();
+    const int nt = b->d.nt;
+    const int op = b->d.op;
+    int rc = A3D_OK;
+    int unstable = 0;
+    for (int w = 0; w < nw && !rc; ++w) {
+        a3d_ctx* c = b->ctx[w];
+        const a3d_shot_desc* d = &b->wd[w];
+        const size_t n = (size_t)d->nx * d->ny * nt;
+        HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->st));
+        HIP_TRY(launch_check_finite(c->rec, n, c->flag, c->st));
+        int hflag = 0;
+        HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(hipStreamSynchronize(c->st));
+        unstable |= hflag;
+        if (shots && shots[w]) {
+            if ((rc = ensure(&c->shot_dev, &c->shot_cap, n))) return rc;
+            MuteArgs m{};
+            m.mute = d->mute;
+            m.dx = d->dx;
+            m.dy = d->dy;
+            m.dz = d->dz;
+            m.dt = d->dt;
+            m.fm = d->fm;
+            m.sx = d->sx;
+            m.sy = d->sy;
+            m.sz = d->sz;
+            m.tt = d->mute_tt;
+            m.vcut = d->v_mute > 0 ? d->v_mute
+                                   : (op == A3D_OP_ISO ? c->vel_first : c->vel_first * sqrtf(1 + 2 * c->eps_first));
+            HIP_TRY(launch_shot_transpose(c->rec, nt, d->nx, d->ny, m, c->shot_dev, c->st));
+            HIP_TRY(hipMemcpyAsync(shots[w], c->shot_dev, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+            HIP_TRY(hipStreamSynchronize(c->st));
+        }
+    }
+    if (!rc && unstable) return fail(A3D_EUNSTABLE, "non-finite values in a window's shot record");
+    return rc;
+}
+
+
+// Mode 3: the time loop as persistent launches of persist_steps steps each (k_persist:
+// resident grid, ticket counters, grid barriers), captured in one hipGraph and launched
+// once (with A3D_PROGRESS set: launched one by one, a line after each). FP32 storage,
+// PML boundary. Every window's results equal its plain run bit for bit: each tile is
+// updated by the same code as in the other modes, and the barriers order the sweeps.
+int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>& sa, const std::vector<Src>& src_all,
+                 bool wz, double* loop_ms) {
+    const int nw = (int)b->ctx.size(), nt = b->d.nt, op = b->d.op, ty = b->persist_ty;
+    const int dirs = dirs_mode(b->d.scheme);
+    if (b->ctx[0]->prec != PREC_FP32) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs FP32 storage");
+    if (b->d.scheme == 2) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs stress schemes 0, 1, 3 and 4");
+    for (auto* c : b->ctx)
+        if (c->strip_net) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs the PML boundary");
+    const int per_cu = persist_blocks_per_cu(op, ty, dirs, wz);
+    if (per_cu < 1) return fail(A3D_EINVAL, "no persistent kernel for operator %d with %d tile rows", op, ty);
+    const unsigned nblocks = (unsigned)(per_cu * b->ctx[0]->ncu) & ~7u;
+    if (nblocks < 8) return fail(A3D_EINVAL, "persistent grid too small (%d workgroups per CU)", per_cu);
+    // launches for the persistent kernel's tile rows; ticket order: the windows with the
+    // longest tile columns first; each window's tiles in 8 contiguous bands, one per XCD
+    for (int w = 0; w < nw; ++w) {
+        va[w].l = make_launch(va[w].g, ty, b->zs_vel);
+        sa[w].l = make_launch(sa[w].g, ty, wz ? 1 : b->zs_str);
+    }
+    std::vector<int> vord(nw), sord(nw);
+    for (int w = 0; w < nw; ++w) vord[w] = sord[w] = w;
+    std::stable_sort(vord.begin(), vord.end(), [&](int a, int c) { return va[a].l.zlen > va[c].l.zlen; });
+    std::stable_sort(sord.begin(), sord.end(), [&](int a, int c) { return sa[a].l.zlen > sa[c].l.zlen; });
+    // per band: prefix sums of ticket counts (a ticket covers `group` consecutive tiles of
+    // one window) and each window's tile range in the band
+    const int group = b->persist_group, n1 = nw + 1;
+    std::vector<int> vb(8 * n1), sb(8 * n1), vl(16 * n1), sl(16 * n1);
+    auto bands = [&](const std::vector<int>& ord, auto ntiles, std::vector<int>& base, std::vector<int>& lo) {
+        for (int x = 0; x < 8; ++x) {
+            base[x * n1] = 0;
+            for (int j = 0; j < nw; ++j) {
+                const int n = ntiles(ord[j]), per = (n + 7) / 8;
+                const int l0 = std::min(x * per, n), l1 = std::min((x + 1) * per, n);
+                lo[x * n1 + j] = l0;
+                lo[8 * n1 + x * n1 + j] = l1;
+                base[x * n1 + j + 1] = base[x * n1 + j] + (l1 - l0 + group - 1) / group;
+            }
+        }
+    };
+    bands(vord, [&](int w) { return va[w].l.ntiles; }, vb, vl);
+    bands(sord, [&](int w) { return sa[w].l.ntiles; }, sb, sl);
+    const int steps = std::max(1, std::min(b->persist_steps, nt));
+    const size_t nsync = (size_t)PERSIST_LINE * (1 + 2 * (size_t)steps * 8);
+    // one device block for everything the kernel reads besides the windows' own buffers
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const size_t o_va = 0, o_sa = o_va + al(nw * sizeof(VelArgs)), o_vb = o_sa + al(nw * sizeof(StressArgs)),
+                 o_sb = o_vb + al(vb.size() * sizeof(int)), o_vl = o_sb + al(sb.size() * sizeof(int)),
+                 o_sl = o_vl + al(vl.size() * sizeof(int)), o_vw = o_sl + al(sl.size() * sizeof(int)),
+                 o_sw = o_vw + al(nw * sizeof(int)), o_src = o_sw + al(nw * sizeof(int)),
+                 o_sync = o_src + al(src_all.size() * sizeof(Src)), o_err = o_sync + al(nsync * sizeof(unsigned)),
+                 total = o_err + al(sizeof(int));
+    unsigned char* mem = nullptr;
+    if (hipMalloc(&mem, total) != hipSuccess) return fail(A3D_ENOMEM, "persistent batch argument allocation failed");
+    auto done = [&](int code) {
+        (void)hipFree(mem);
+        return code;
+    };
+#define P_TRY(x)                                                                                    \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return done(fail(A3D_EHIP, "%s: %s", #x, hipGetErrorString(e_)));    \
+    } while (0)
+    P_TRY(hipMemcpy(mem + o_va, va.data(), nw * sizeof(VelArgs), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_sa, sa.data(), nw * sizeof(StressArgs), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_vb, vb.data(), vb.size() * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_sb, sb.data(), sb.size() * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_vl, vl.data(), vl.size() * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_sl, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_vw, vord.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_sw, sord.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_src, src_all.data(), src_all.size() * sizeof(Src), hipMemcpyHostToDevice));
+    P_TRY(hipMemset(mem + o_err, 0, sizeof(int)));
+    int khz = 0;
+    P_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    PersistArgs p{};
+    p.va = (const VelArgs*)(mem + o_va);
+    p.sa = (const StressArgs*)(mem + o_sa);
+    p.vbase = (const int*)(mem + o_vb);
+    p.sbase = (const int*)(mem + o_sb);
+    p.vlo = (const int*)(mem + o_vl);
+    p.slo = (const int*)(mem + o_sl);
+    p.vwin = (const int*)(mem + o_vw);
+    p.swin = (const int*)(mem + o_sw);
+    p.srcs = (const Src*)(mem + o_src);
+    p.sync = (unsigned*)(mem + o_sync);
+    p.err = (int*)(mem + o_err);
+    p.nwin = nw;
+    p.group = group;
+    p.nblocks = nblocks;
+    // a barrier waits at most 4 s (a sweep over every window takes milliseconds)
+    p.wait_ticks = 4000LL * std::max(khz, 1);
+    const bool progress = getenv("A3D_PROGRESS") != nullptr;
+    auto enqueue = [&](int i0) -> hipError_t {
+        hipError_t e = launch_zero_u32(p.sync, nsync, b->st);
+        if (e == hipSuccess) e = launch_persist(op, ty, dirs, wz, p, i0, std::min(nt, i0 + steps), b->st);
+        return e;
+    };
+    // the launches captured in one hipGraph; with A3D_PROGRESS, one graph per launch,
+    // each followed by a synchronisation and a progress line
+    std::vector<hipGraphExec_t> execs;
+    auto capture = [&](int i0, int i1) -> hipError_t {
+        hipError_t e = hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (int i = i0; i < i1 && e == hipSuccess; i += steps) e = enqueue(i);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(b->st, &g);
+        if (e == hipSuccess) e = e2;
+        hipGraphExec_t x = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (x) execs.push_back(x);
+        return e;
+    };
+    auto run_all = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (progress) {
+            int tv = 0, ts = 0;
+            for (int x = 0; x < 8; ++x) {
+                tv += vb[x * n1 + nw];
+                ts += sb[x * n1 + nw];
+            }
+            fprintf(stderr,
+                    "a3d: persistent grid of %u workgroups (%d per CU), %d tile rows, %d steps per launch, "
+                    "%d tiles per ticket, %d + %d tickets per step\n",
+                    nblocks, per_cu, ty, steps, group, tv, ts);
+            for (int i0 = 0; i0 < nt && e == hipSuccess; i0 += steps) e = capture(i0, std::min(nt, i0 + steps));
+        } else {
+            e = capture(0, nt);
+        }
+        if (e == hipSuccess) e = hipEventRecord(b->ev0, b->st);
+        for (size_t k = 0; k < execs.size() && e == hipSuccess; ++k) {
+            e = hipGraphLaunch(execs[k], b->st);
+            ++b->graph_launches;
+            if (progress && e == hipSuccess) {
+                e = hipStreamSynchronize(b->st);
+                fprintf(stderr, "a3d: persistent launch %zu/%zu done\n", k + 1, execs.size());
+            }
+        }
+        if (e == hipSuccess) e = hipEventRecord(b->ev1, b->st);
+        if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+        else (void)hipStreamSynchronize(b->st);   // nothing of this run may still be in flight
+        return e;
+    };
+    const hipError_t erun = run_all();
+    for (auto x : execs) (void)hipGraphExecDestroy(x);
+    P_TRY(erun);
+    float ms = 0.f;
+    P_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (loop_ms) *loop_ms = ms;
+    int herr = 0;
+    P_TRY(hipMemcpy(&herr, p.err, sizeof(int), hipMemcpyDeviceToHost));
+#undef P_TRY
+    if (herr) return done(fail(A3D_EHIP, "persistent batch: a grid barrier wait ran out (grid of %u workgroups)", nblocks));
+    return done(A3D_OK);
+}
+
 template <class T>
 int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* group_ms) {
     const int nw = (int)b->ctx.size();
@@ -2063,6 +2268,10 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     const bool wz = (op == A3D_OP_ISO || op == A3D_OP_VTI) && b->d.scheme != 2 && !sm &&
                     (b->wz_mode == 1 || (b->wz_mode < 0 && b->wz_tuned));
     for (int w = 0; w < nw; ++w) va[w].wz = sa[w].wz = wz ? 1 : 0;
+    if (mode == 3) {
+        if ((rc = persist_loop(b, va, sa, src_all, wz, loop_ms))) return rc;
+        return batch_outputs(b, shots);
+    }
     const bool replay = mode == 2 && !b->exec.empty();
     const bool progress = getenv("A3D_PROGRESS") != nullptr;
     std::vector<BatchGroup> gv_local, gs_local;
@@ -2179,40 +2388,8 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if (loop_ms) *loop_ms = ms;
     rc = cleanup(A3D_OK);
-    // outputs: per-window sentinel, transpose (+ mute) and copy
-    int unstable = 0;
-    for (int w = 0; w < nw && !rc; ++w) {
-        a3d_ctx* c = b->ctx[w];
-        const a3d_shot_desc* d = &b->wd[w];
-        const size_t n = (size_t)d->nx * d->ny * nt;
-        HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->st));
-        HIP_TRY(launch_check_finite(c->rec, n, c->flag, c->st));
-        int hflag = 0;
-        HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
-        HIP_TRY(hipStreamSynchronize(c->st));
-        unstable |= hflag;
-        if (shots && shots[w]) {
-            if ((rc = ensure(&c->shot_dev, &c->shot_cap, n))) return rc;
-            MuteArgs m{};
-            m.mute = d->mute;
-            m.dx = d->dx;
-            m.dy = d->dy;
-            m.dz = d->dz;
-            m.dt = d->dt;
-            m.fm = d->fm;
-            m.sx = d->sx;
-            m.sy = d->sy;
-            m.sz = d->sz;
-            m.tt = d->mute_tt;
-            m.vcut = d->v_mute > 0 ? d->v_mute
-                                   : (op == A3D_OP_ISO ? c->vel_first : c->vel_first * sqrtf(1 + 2 * c->eps_first));
-            HIP_TRY(launch_shot_transpose(c->rec, nt, d->nx, d->ny, m, c->shot_dev, c->st));
-            HIP_TRY(hipMemcpyAsync(shots[w], c->shot_dev, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
-            HIP_TRY(hipStreamSynchronize(c->st));
-        }
-    }
-    if (!rc && unstable) return fail(A3D_EUNSTABLE, "non-finite values in a window's shot record");
-    return rc;
+    if (rc) return rc;
+    return batch_outputs(b, shots);
 }
 
 int batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms, double* group_ms) {
@@ -2330,7 +2507,7 @@ int a3d_batch_set_model(a3d_batch* b, const float* vel, const float* rho, const 
 int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms) {
     if (!b) return fail(A3D_EINVAL, "null batch");
     if (!b->model_set) return fail(A3D_ESTATE, "model not set");
-    if (mode < 0 || mode > 2) return fail(A3D_EINVAL, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(A3D_EINVAL, "mode must be 0, 1, 2 or 3");
     HIP_TRY(hipSetDevice(b->device));
     if (mode == 0) {
         // baseline: the windows one after the other as plain single-shot runs
@@ -2347,6 +2524,17 @@ int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms) {
         return A3D_OK;
     }
     return batch_run(b, mode, shots, loop_ms, nullptr);
+}
+
+int a3d_batch_set_persist(a3d_batch* b, int tile_rows, int steps, int group) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    if (tile_rows != 4 && tile_rows != 8) return fail(A3D_EINVAL, "persistent tile rows must be 4 or 8");
+    if (steps < 1) return fail(A3D_EINVAL, "steps per persistent launch must be positive");
+    if (group < 1 || group > 64) return fail(A3D_EINVAL, "tiles per ticket must be in 1..64");
+    b->persist_ty = tile_rows;
+    b->persist_steps = steps;
+    b->persist_group = group;
+    return A3D_OK;
 }
 
 int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses) {

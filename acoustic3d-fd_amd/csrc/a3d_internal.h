@@ -185,6 +185,38 @@ hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins,
                                const Launch& lb, int it, int dirs, bool wz, const Src* srcs, hipStream_t st,
                                bool sm = false);
 
+// Persistent batched time loop: one launch runs steps it0 .. it1-1 of every window. A
+// resident grid (nblocks workgroups, a multiple of 8, all held by the GPU at once) takes
+// each sweep's tiles from ticket counters and meets at a grid barrier after each sweep.
+// Every window's tiles are cut into 8 contiguous bands, band x for the workgroups with
+// blockIdx % 8 == x (the hardware deals workgroup b to XCD b % 8), so each XCD gets an
+// eighth of every window and neighbouring tiles share its L2; each XCD has its own
+// counter per sweep and step.
+constexpr int PERSIST_LINE = 16;   // unsigned counters per 64-byte line
+struct PersistArgs {
+    const VelArgs* va;       // per-window sweep arguments (device), launches for this TY
+    const StressArgs* sa;
+    const int* vbase;        // [8][nwin + 1]: per XCD band, prefix sums of the windows' ticket
+    const int* sbase;        //   counts in ticket order (a ticket covers `group` consecutive tiles)
+    const int* vlo;          // [8][nwin + 1]: per XCD band, the window's first tile in the band
+    const int* slo;          //   ([x][nwin]: unused) and, at [8][...] + 8 * (nwin + 1), its end
+    const int* vwin;         // window index of each ticket-order slot
+    const int* swin;
+    const Src* srcs;         // per-step sources [nt][nwin] by window index
+    unsigned* sync;          // [0] barrier arrivals; the tickets of step it0 + k, sweep s and
+                             // band x at PERSIST_LINE * (1 + (2k + s) * 8 + x) (zeroed before
+                             // each launch)
+    int* err;                // raised when a barrier wait ran out
+    int nwin, group;
+    unsigned nblocks;
+    long long wait_ticks;    // barrier wait limit in wall-clock ticks
+};
+// FP32 storage; ty 4 or 8; dirs as launch_stress_batch; wz (ISO, VTI).
+hipError_t launch_persist(int op, int ty, int dirs, bool wz, const PersistArgs& p, int it0, int it1, hipStream_t st);
+// Resident workgroups per CU of the persistent kernel instance (0 if it does not exist).
+int persist_blocks_per_cu(int op, int ty, int dirs, bool wz);
+hipError_t launch_zero_u32(unsigned* a, size_t n, hipStream_t st);
+
 // Resident workgroups per CU of a sweep kernel: kind 0 = velocity sweep (or
 // the 2nd-order step for op 0), kind 1 = stress sweep.
 int max_blocks_per_cu(int prec, int kind, int op, int ty);

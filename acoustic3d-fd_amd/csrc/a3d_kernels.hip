@@ -323,7 +323,7 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const int yw = __builtin_amdgcn_readfirstlane(yl);
     const C cy1 = ztab(T, a.c.cy1 + yw), cy2 = ztab(T, a.c.cy2 + yw);
     const C dtx = (C)a.dtx, dty = (C)a.dty, dtz = (C)a.dtz;
-    static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
+    static_assert(!SM || VTI, "the coefficient head runs the VTI operator");
     C hx = 0, hy = 0;
     if constexpr (SM) {
         hx = T[a.c.hvx + xs];
@@ -335,7 +335,8 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         C hz = 0, mc;
         bool scell = false;
         if constexpr (SM) {
-            hz = ztab(T, a.c.hvz + z);
+            // WZ: u and v only, so only the x and y ramps matter here
+            if constexpr (!WZ) hz = ztab(T, a.c.hvz + z);
             scell = hx != (C)0 || hy != (C)0 || hz != (C)0;
             if (scell) mc = (C)bld<NT>(a.mfac, zo, colc);
         }
@@ -550,7 +551,13 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             const C wo = ldW(p * g.pl, col);
             if (p >= MM && p < g.nnz - MM) {
                 const C r = (C)bld(rho, p * g.pl, colc);
-                const C wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, ztab(T, a.c.cz1 + p), ztab(T, a.c.cz2 + p));
+                C c1 = ztab(T, a.c.cz1 + p), c2 = ztab(T, a.c.cz2 + p);
+                if constexpr (SM) {
+                    // the velocity sweep's coefficients of a strip cell (its z ramp)
+                    const C hzp = ztab(T, a.c.hvz + p);
+                    if (hzp != (C)0) strip_coef<C>((C)bld(a.mfac, p * g.pl, colc), hzp, c1, c2);
+                }
+                const C wn = wcalc(p, [&](int j) { return fw[k + j]; }, wo, r, c1, c2);
                 if (valid && p >= zb) stw((S*)Wv, p * g.pl, col, wn, sc.inv[F_W]);
                 wW[k] = stored(Wv, wn, sc.inv[F_W], sc.s[F_W]);   // as the stress sweep would read it
             } else {
@@ -592,7 +599,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     constexpr bool dirs = DIRS != 0;
     constexpr bool ONE = DIRS == 2 && !ISO;
     const bool dX = ax1 != (C)1, dY = ay1 != (C)1;
-    static_assert(!SM || (VTI && !WZ), "the coefficient head runs VTI with w in the velocity sweep");
+    static_assert(!SM || VTI, "the coefficient head runs the VTI operator");
     C hx = 0, hy = 0;
     if constexpr (SM) {
         hx = T[a.c.hsx + xs];
@@ -628,7 +635,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         f32x4 m3, m, t;
         f32x2 mm;
         C c0, r0, e0, d0, ct, st, cp, sp, uc, vc;
-        C f0, f1, f2, f3, f4, f5, p0, q0, mc;
+        C f0, f1, f2, f3, f4, f5, p0, q0, mc, mw;
         Halo<TY, true, TTI, C> hU;
         Halo<TY, TTI, true, C> hV;
         Halo<TY, TTI, TTI, C> hW;
@@ -642,6 +649,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             else L.r3 = (C)bld<NT>(rho, zo + 3 * g.pl, colc);
             L.czw1 = ztab(T, a.c.cz1 + z + 3);
             L.czw2 = ztab(T, a.c.cz2 + z + 3);
+            // strip cell of w's plane z+3 (its z ramp): its multiplier (wave-uniform test)
+            if constexpr (SM)
+                if (ztab(T, a.c.hvz + min(z + 3, g.nnz - 1)) != (C)0)
+                    L.mw = (C)bld<NT>(a.mfac, min(z + 3, g.nnz - 1) * g.pl, colc);
         } else if constexpr (!TTI) {
             L.wl = ldw<NT>(Wv, zo + 3 * g.pl, col, sc.s[F_W]);   // TTI reads w's x-y halo too
         } else {
@@ -710,6 +721,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             else fr[3] = L.r3;
             czw1 = L.czw1;
             czw2 = L.czw2;
+            if constexpr (SM) {
+                const C hzw = ztab(T, a.c.hvz + min(z + 3, g.nnz - 1));
+                if (hzw != (C)0) strip_coef<C>(L.mw, hzw, czw1, czw2);
+            }
         } else {
             wW[7] = L.wl;
         }
@@ -988,6 +1003,119 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __res
 // --------------------------------------------------------------------------
 // 2nd-order step with sponge, dipole and receivers folded in
 // --------------------------------------------------------------------------
+// --------------------------------------------------------------------------
+// Persistent batched time loop
+// --------------------------------------------------------------------------
+// One launch runs steps it0 .. it1-1 of every window of a batch. The grid is
+// resident (no more workgroups than the GPU holds at once); each sweep's tiles
+// are handed out by ticket counters, one per XCD band (an eighth of every
+// window, in ticket order: windows with the longest tile columns first, so the
+// last tiles taken are short ones), and the grid meets at a barrier after each
+// sweep (a stress tile reads its neighbours' velocities, a velocity tile their
+// stresses). Every barrier wait is bounded: past p.wait_ticks the workgroup
+// raises *p.err, and a workgroup that sees *p.err set leaves at its next
+// barrier, so the launch always ends.
+
+// Next ticket of counter c for the whole workgroup. The value is wave-uniform
+// (readfirstlane), so the window search and the window's arguments that follow
+// are scalar loads held in scalar registers, as in the launched sweeps.
+__device__ __forceinline__ int next_ticket(unsigned* c) {
+    __shared__ int t;
+    __syncthreads();   // every thread is past the previous tile (and its ticket)
+    if (threadIdx.x == 0 && threadIdx.y == 0) t = (int)atomicAdd(c, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
+// Grid barrier (arrival counter p.sync[0], never reset within a launch: the k-th
+// barrier waits for k * nblocks arrivals). Release: each thread's stores are made
+// visible at device scope before its workgroup arrives; acquire: each thread
+// fences after the wait, so later loads see the other workgroups' stores.
+// Returns false when a wait ran out (here or in another workgroup).
+__device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned& target) {
+    __shared__ int ok;
+    __threadfence();
+    __syncthreads();
+    target += p.nblocks;
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        int good = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (good) {
+            atomicAdd(p.sync, 1u);
+            const long long t0 = (long long)wall_clock64();
+            int spins = 0;   // second bound, independent of the clock: about 2^22 sleeps
+            while (__hip_atomic_load(p.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    good = 0;
+                    break;
+                }
+                if ((long long)wall_clock64() - t0 > p.wait_ticks || ++spins > (1 << 22)) {
+                    atomicExch(p.err, 1);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        ok = good;
+    }
+    __syncthreads();
+    __threadfence();
+    return ok != 0;
+}
+
+template <class PR, int OP, int TY, int DIRS, bool WZ>
+__global__ __launch_bounds__(TX* TY) void k_persist(const PersistArgs p, int it0, int it1) {
+    unsigned target = 0;
+    const int x = (int)(blockIdx.x & 7u);
+    const int n1 = p.nwin + 1;
+    const int* vb = p.vbase + x * n1;
+    const int* sb = p.sbase + x * n1;
+    const int* vlo = p.vlo + x * n1;
+    const int* slo = p.slo + x * n1;
+    const int* vhi = vlo + 8 * n1;
+    const int* shi = slo + 8 * n1;
+    const int ntv = vb[p.nwin], nts = sb[p.nwin];
+    for (int it = it0; it < it1; ++it) {
+        const int k = it - it0;
+        for (;;) {
+            const int t = next_ticket(p.sync + PERSIST_LINE * (1 + (2 * k) * 8 + x));
+            if (t >= ntv) break;
+            const int j = find_window(vb, p.nwin, t);
+            const int w = p.vwin[j];
+            const VelArgs a = p.va[w];
+            const Src s0 = p.srcs[(size_t)it * p.nwin + w];
+            const int t0 = vlo[j] + (t - vb[j]) * p.group, t1 = min(t0 + p.group, vhi[j]);
+            for (int tt = t0; tt < t1; ++tt) {
+                int tix, tiy, seg;
+                split_tile(a.l, tt, tix, tiy, seg);
+                vel_body<PR, OP, TY, WZ, false>(a, s0, tix, tiy, seg);
+            }
+        }
+        if (!grid_barrier(p, target)) return;
+        for (;;) {
+            const int t = next_ticket(p.sync + PERSIST_LINE * (1 + (2 * k + 1) * 8 + x));
+            if (t >= nts) break;
+            const int j = find_window(sb, p.nwin, t);
+            const int w = p.swin[j];
+            StressArgs a = p.sa[w];
+            if constexpr (WZ) a.s = p.srcs[(size_t)it * p.nwin + w];
+            float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
+            const int t0 = slo[j] + (t - sb[j]) * p.group, t1 = min(t0 + p.group, shi[j]);
+            for (int tt = t0; tt < t1; ++tt) {
+                int tix, tiy, seg;
+                split_tile(a.l, tt, tix, tiy, seg);
+                stress_body<PR, OP, TY, WZ, DIRS, false>(a, rec, tix, tiy, seg);
+            }
+        }
+        if (!grid_barrier(p, target)) return;
+    }
+}
+
+__global__ void k_zero_u32(unsigned* __restrict__ a, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
 template <class PR, int TY>
 __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     using S = typename PR::S;
@@ -1371,9 +1499,9 @@ int occ_prec(int kind, int op, int ty) {
 }  // namespace
 
 hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st) {
-    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32, w in the velocity sweep
-        if (prec != PREC_FP32 || op != 2 || a.wz) return hipErrorInvalidValue;
-        return vel_ty<Fp32, 2, false, true>(ty, a, st);
+    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32
+        if (prec != PREC_FP32 || op != 2) return hipErrorInvalidValue;
+        return a.wz ? vel_ty<Fp32, 2, true, true>(ty, a, st) : vel_ty<Fp32, 2, false, true>(ty, a, st);
     }
     switch (prec) {
         case PREC_FP32: return vel_op<Fp32>(op, ty, a, st);
@@ -1384,8 +1512,12 @@ hipError_t launch_vel(int prec, int op, int ty, const VelArgs& a, hipStream_t st
 }
 
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st) {
-    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32, w in the velocity sweep
-        if (prec != PREC_FP32 || op != 2 || a.wz) return hipErrorInvalidValue;
+    if (a.mfac) {   // learned strip, coefficient head: VTI, FP32
+        if (prec != PREC_FP32 || op != 2) return hipErrorInvalidValue;
+        if (a.wz)
+            return a.unsplit == 3   ? stress_ty<Fp32, 2, true, 2, true>(ty, a, st)
+                   : a.unsplit == 2 ? stress_ty<Fp32, 2, true, 1, true>(ty, a, st)
+                                    : stress_ty<Fp32, 2, true, 0, true>(ty, a, st);
         return a.unsplit == 3   ? stress_ty<Fp32, 2, false, 2, true>(ty, a, st)
                : a.unsplit == 2 ? stress_ty<Fp32, 2, false, 1, true>(ty, a, st)
                                 : stress_ty<Fp32, 2, false, 0, true>(ty, a, st);
@@ -1435,6 +1567,61 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st) {
         case PREC_FP64: return step2_ty<Fp64>(ty, a, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+namespace {
+// The persistent kernel instance of (op, ty, dirs, wz), FP32 storage; nullptr if none.
+typedef void (*PersistKern)(PersistArgs, int, int);
+template <int TY>
+PersistKern persist_kern_ty(int op, int dirs, bool wz) {
+    switch (op) {
+        case 1:
+            if (wz) return dirs ? k_persist<Fp32, 1, TY, 1, true> : k_persist<Fp32, 1, TY, 0, true>;
+            return dirs ? k_persist<Fp32, 1, TY, 1, false> : k_persist<Fp32, 1, TY, 0, false>;
+        case 2:
+            if (wz)
+                return dirs == 2 ? k_persist<Fp32, 2, TY, 2, true>
+                       : dirs    ? k_persist<Fp32, 2, TY, 1, true>
+                                 : k_persist<Fp32, 2, TY, 0, true>;
+            return dirs == 2 ? k_persist<Fp32, 2, TY, 2, false>
+                   : dirs    ? k_persist<Fp32, 2, TY, 1, false>
+                             : k_persist<Fp32, 2, TY, 0, false>;
+        case 3:
+            if (wz) return nullptr;
+            return dirs == 2 ? k_persist<Fp32, 3, TY, 2, false>
+                   : dirs    ? k_persist<Fp32, 3, TY, 1, false>
+                             : k_persist<Fp32, 3, TY, 0, false>;
+        default: return nullptr;
+    }
+}
+PersistKern persist_kern(int op, int ty, int dirs, bool wz) {
+    switch (ty) {
+        case 4: return persist_kern_ty<4>(op, dirs, wz);
+        case 8: return persist_kern_ty<8>(op, dirs, wz);
+        default: return nullptr;
+    }
+}
+}  // namespace
+
+int persist_blocks_per_cu(int op, int ty, int dirs, bool wz) {
+    const PersistKern k = persist_kern(op, ty, dirs, wz);
+    if (!k) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TX * ty, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+hipError_t launch_persist(int op, int ty, int dirs, bool wz, const PersistArgs& p, int it0, int it1, hipStream_t st) {
+    const PersistKern k = persist_kern(op, ty, dirs, wz);
+    if (!k || p.nblocks == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(TX, ty), 0, st, p, it0, it1);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned* a, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n);
+    return hipGetLastError();
 }
 
 int max_blocks_per_cu(int prec, int kind, int op, int ty) {

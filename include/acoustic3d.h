@@ -250,10 +250,20 @@ int a3d_batch_tune(a3d_batch* b, int steps, int* nclasses);
 /* mode 0: the windows one after another as plain single-shot runs (baseline);
  * mode 1: all windows advanced together, one batched launch per sweep and
  * tile-row group per step; mode 2: mode 1 with the whole time loop captured in
- * one hipGraph and launched once. shots: nwin host pointers ([ny][nx][nt] of each
- * window) or NULL; entries may be NULL. *loop_ms: time-loop duration. Results
- * are bitwise identical in all modes. */
+ * one hipGraph and launched once; mode 3 (FP32 storage, PML boundary): a
+ * persistent kernel runs the time loop on the device -- a resident grid takes
+ * the tiles of every window from a ticket counter per sweep (windows with the
+ * longest tile columns first) and meets at a grid barrier after each sweep --
+ * one launch per a3d_batch_set_persist steps, the launches captured in one
+ * hipGraph (each barrier wait is bounded; a wait that runs out fails the run
+ * with A3D_EHIP). shots: nwin host pointers ([ny][nx][nt] of each window) or
+ * NULL; entries may be NULL. *loop_ms: time-loop duration. Results are bitwise
+ * identical in all modes. */
 int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms);
+/* Mode 3: tile rows of the persistent kernel (4, the default, or 8; the z segments
+ * per sweep are those of a3d_batch_set_zseg / a3d_batch_tune), time steps per
+ * persistent launch (default 100) and consecutive tiles per ticket (1..64, default 1). */
+int a3d_batch_set_persist(a3d_batch* b, int tile_rows, int steps, int group);
 /* Mode 2 captures the time loop in graphs of `steps` time steps each (0, the
  * default: one graph for the whole loop); with the environment variable
  * A3D_PROGRESS set, a line is printed to stderr after each graph completes. */

@@ -158,7 +158,8 @@ def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
     """Head mode 2: the network's multiplier of each strip cell's PML damping, read by the
     velocity and stress sweeps inside their update (no pass after the sweeps), against the
     oracle's restatement; also with the network run every `period` steps, and the time loop
-    captured in hipGraphs bitwise equal to host launches."""
+    captured in hipGraphs bitwise equal to host launches; with w updated by the stress sweep
+    (a strip cell's w taking the velocity sweep's multiplied coefficients) bitwise equal too."""
     from a3dfd import Modeller
     m = _small()
     nt = 220
@@ -173,6 +174,11 @@ def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
         shot, snaps, _ = mdl.run(nsnap_max=8)
         mdl.set_graph(32)
         gshot, gsnaps, _ = mdl.run(nsnap_max=8)
+        mdl.set_graph(0)
+        mdl.set_w_sweep(1)
+        wshot, wsnaps, _ = mdl.run(nsnap_max=8)
+        mdl.set_launch(8, 3, 2, 1)
+        wshot2, _, _ = mdl.run(nsnap_max=8)
     plain, _, _ = _run(m, d)
     os_ = oracle.make_shot("vti", 30, 28, 26, 10.0, 10.0, 10.0, 4, 25, 3, nt, 1e-3, 25.0, nb=10, snap_interval=55,
                            mute_tt=0)
@@ -194,6 +200,7 @@ def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
     assert effect > 1e-4          # the multipliers matter
     assert e_shot <= 1e-5 and e_snap <= 1e-5
     assert np.array_equal(gshot, shot) and np.array_equal(gsnaps, snaps)
+    assert np.array_equal(wshot, shot) and np.array_equal(wsnaps, snaps) and np.array_equal(wshot2, shot)
 
 
 def test_collect_samples():
