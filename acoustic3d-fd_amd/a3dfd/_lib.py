@@ -154,6 +154,7 @@ def load(path=None):
     lib.a3d_mlp_create.argtypes = [c_int, POINTER(vp)]
     lib.a3d_mlp_destroy.argtypes = [vp]
     lib.a3d_mlp_set_params.argtypes = [vp, fp, c_int]
+    lib.a3d_mlp_set_head.argtypes = [vp, c_int]
     lib.a3d_mlp_get_params.argtypes = [vp, fp, c_int]
     lib.a3d_mlp_predict.argtypes = [vp, fp, c_int, fp]
     lib.a3d_mlp_loss_grad.argtypes = [vp, fp, fp, c_int, POINTER(c_double), fp]
@@ -394,6 +395,11 @@ class Mlp:
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32).ravel()
         _check(self.lib, self.lib.a3d_mlp_set_params(self._h, _fptr(p), p.size))
+
+    def set_head(self, head):
+        """What predict / loss_grad / train fit: 0 the damping head's change, 2 the coefficient
+        head's multiplier (targets (m, anything), output 1 without loss)."""
+        _check(self.lib, self.lib.a3d_mlp_set_head(self._h, int(head)))
 
     def params(self):
         p = np.empty(self.nparams, np.float32)

@@ -29,7 +29,10 @@ from . import models
 from ._lib import STRIP_NB, A3DError, Mlp, Modeller, make_desc, strip_collect
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-NET_PATH = os.path.join(HERE, "strip_net.npz")
+# the committed trained network: the coefficient head (mode 2); the damping-head network
+# trained by teacher forcing is kept for the head studies
+NET_PATH = os.path.join(HERE, "strip_coef_net.npz")
+DAMP_NET_PATH = os.path.join(HERE, "strip_net.npz")
 TEACHER_NB = 40
 PML_NB = 20
 THRESHOLD_DB = -40.0
@@ -85,10 +88,12 @@ def collect(seeds, n=128, nt=600, every=3, keep=0.02, horizon=1, params=None, am
     return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ss)
 
 
-def train(X, T, epochs=30, batch=4096, lr=2e-3, lr_end=2e-5, seed=0, init=None, device=0):
-    """Trains the network on the GPU; returns (params, loss per epoch)."""
+def train(X, T, epochs=30, batch=4096, lr=2e-3, lr_end=2e-5, seed=0, init=None, device=0, head=0):
+    """Trains the network on the GPU; returns (params, loss per epoch). head 2: the targets are
+    the coefficient head's multipliers (T[:, 0]), fitted directly."""
     with Mlp(device) as net:
         net.set_params(init if init is not None else init_params(seed))
+        net.set_head(head)
         hist = net.train(X, T, epochs, batch=batch, lr=lr, lr_end=lr_end, seed=seed)
         return net.params(), hist
 
@@ -207,10 +212,20 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
         if boundary in ("strip", "strip0"):
             if amp is not None:
                 mdl.set_strip_profile(amp)
-            # strip0: the plain ramp (a damping head shut), whatever head the network uses
-            mdl.set_strip_head(head if boundary == "strip" else 0)
-            mdl.set_strip_period(period)
-            mdl.set_strip_net(params if boundary == "strip" else closed_params())
+            if boundary == "strip":
+                mdl.set_strip_head(head)
+                mdl.set_strip_period(period)
+                mdl.set_strip_net(params)
+            elif head == COEF_HEAD:
+                # strip0 beside a coefficient head: the plain ramp through the same sweeps, every
+                # multiplier 1 (the closed head, run once)
+                mdl.set_strip_head(COEF_HEAD)
+                mdl.set_strip_period(0)
+                mdl.set_strip_net(coef_params(coef_closed_theta(), np.zeros(NPARAMS, np.float32)))
+            else:
+                # strip0: the plain ramp (a damping head shut)
+                mdl.set_strip_head(0)
+                mdl.set_strip_net(closed_params())
         return mdl.run(nsnap_max=nt // snap_every)
 
 
@@ -254,7 +269,7 @@ def adaptive_targets(X, gstar):
 
 
 def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, device=0, log=None, head=0,
-          extra_amps=()):
+          extra_amps=(), period=1, scheme=0):
     """Reflected energy (dB) and shot-record rel. L2 against the extended FP64 run for the
     learned strip, the strip without network and the 20-cell PML, on held-out models; also
     the strip without network at each ramp amplitude in extra_amps (key strip0_amp<a>)."""
@@ -271,7 +286,8 @@ def judge(seeds, params, n=128, nt=800, snap_every=20, margin=64, amp=None, devi
             if b == "strip" and params is None:
                 continue
             try:
-                shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, amp=a, device=device, head=head)
+                shot, snaps, ms = run_case(m, src, nt, snap_every, b, params, amp=a, device=device, head=head,
+                                           period=period, scheme=scheme)
             except A3DError as e:   # an unstable boundary is judged as failed
                 row[key] = {"reflected_db": float("inf"), "shot_rel_l2": float("inf"), "loop_ms": None, "error": str(e)}
                 continue
@@ -487,6 +503,97 @@ def coef_label(case, hidden=None, mults=COEF_SCAN):
         if a2 > 0:
             best = float(np.clip(np.exp(-a1 / (2 * a2)), ms[k - 1], ms[k + 1]))
     return float(best), tab
+
+
+def _layer2(params, x):
+    """Layer-2 activations of network `params` for input x (FP64, numpy)."""
+    p = np.asarray(params, np.float64)
+    w1, b1 = p[O_W1:O_B1].reshape(64, 16), p[O_B1:O_W2]
+    w2, b2 = p[O_W2:O_B2].reshape(64, 64), p[O_B2:O_W3]
+    return np.tanh(w2 @ np.tanh(w1 @ x + b1) + b2)
+
+
+def depth_coef_params(mult):
+    """Coefficient-head network whose multiplier at strip depth d = 1..6 (1 = outermost) is
+    mult[d - 1], whatever the local state: the depth units of coef_hidden (steps at d >= j + 1)
+    with output weights solving y(d) = logit(mult[d - 1] / COEF_MMAX) for d = 1..6 (minimum
+    norm; the Courant and velocity-ratio units get no weight)."""
+    mult = np.asarray(mult, np.float64)
+    if mult.shape != (6,) or np.any(mult <= 0) or np.any(mult >= COEF_MMAX):
+        raise ValueError(f"six multipliers in (0, {COEF_MMAX}) expected")
+    hidden = coef_hidden(np.zeros(NPARAMS, np.float32))
+    rows = []
+    for d in range(1, 7):
+        x = np.zeros(16)
+        x[14] = d / 6.0
+        rows.append(np.append(_layer2(hidden, x)[:6], 1.0))
+    r = mult / COEF_MMAX
+    sol, *_ = np.linalg.lstsq(np.array(rows), np.log(r / (1.0 - r)), rcond=None)
+    th = np.zeros(65)
+    th[:6], th[64] = sol[:6], sol[6]
+    return coef_params(th, hidden)
+
+
+def search_depth_profile(pool, start=0.7, steps=(0.4, 0.2, 0.1), sweeps=2, log=None):
+    """Per-depth multipliers of the coefficient head found by coordinate search on the mean
+    reflected energy of the training cases in `pool` (CoefCase, judged against their teachers):
+    from the constant `start`, each depth's log multiplier is moved by +-step while that lowers
+    the mean dB by more than 0.02, for each step size in turn. Returns (multipliers, mean dB,
+    history of (stage, mean dB))."""
+    def J(logm):
+        p = depth_coef_params(np.exp(logm))
+        v = float(np.mean([c.db(p) for c in pool]))
+        if log:
+            log(f"depth profile: candidate {np.exp(logm).round(3).tolist()}: {v:.3f} dB")
+        return v
+
+    cur = np.log(np.full(6, float(start)))
+    best = J(cur)
+    hist = [("start", best)]
+    for step in steps:
+        for _ in range(sweeps):
+            improved = False
+            for k in range(6):
+                for sgn in (1.0, -1.0):
+                    trial = cur.copy()
+                    trial[k] += sgn * step
+                    if not 0.05 < np.exp(trial[k]) < 0.95 * COEF_MMAX:
+                        continue
+                    v = J(trial)
+                    if v < best - 0.02:
+                        best, cur, improved = v, trial, True
+                        break
+            hist.append((f"step {step}", best))
+            if not improved:
+                break
+    return np.exp(cur), best, hist
+
+
+def depth_targets(X, mult, quiet=0.05, seed=0):
+    """Samples and targets for training the coefficient head on its multiplier (strip.train with
+    head=2): each sample's target is the multiplier of its strip depth (input 14 = depth / 6).
+    A fraction `quiet` of the samples is added again with a zero wavefield (inputs 0-10), the
+    state of a cell the wave has not reached, whose multiplier holds when the wave arrives
+    (the network runs every few steps). Returns (X, T)."""
+    rng = np.random.default_rng(seed)
+    q = X[rng.choice(len(X), int(quiet * len(X)), replace=False)].copy()
+    q[:, :11] = 0.0
+    X = np.concatenate([X, q])
+    d = np.clip(np.rint(X[:, 14] * 6.0).astype(int), 1, 6)
+    T = np.zeros((len(X), 2), np.float32)
+    T[:, 0] = np.asarray(mult, np.float32)[d - 1]
+    return X, T
+
+
+def pooled_multiplier(tabs, grid=np.exp(np.linspace(np.log(0.25), np.log(1.3), 200))):
+    """The constant multiplier with the lowest reflected energy averaged over cases, from each
+    case's scan {m: dB} interpolated linearly in log m."""
+    tot = np.zeros(grid.size)
+    for tab in tabs:
+        ms = np.array(sorted(float(k) for k in tab))
+        db = np.array([tab[k] for k in sorted(tab, key=float)], np.float64)
+        tot += np.interp(np.log(grid), np.log(ms), db)
+    return float(grid[int(np.argmin(tot))])
 
 
 def coef_targets(X, mstar):

@@ -259,6 +259,7 @@ struct a3d_ctx {
                                 // 2: multiplier of the PML damping inside the sweeps (coefficient head)
     float* strip_mfac = nullptr;   // head 2: damping multiplier per padded point (1 outside the strip)
     int strip_period = 1;          // head 2: the network runs after the stress sweep of every period-th step
+                                   // (0: after the first step's only)
     int zchunk = 0;             // 1st-order two-sweep step in z chunks of this many planes (0: whole grid)
     // which sweep updates w (ISO, VTI): requested mode (-1 tuner, 0 velocity sweep,
     // 1 stress sweep) and the tuner's choice for mode -1
@@ -719,7 +720,8 @@ StripArgs strip_args(const a3d_ctx* c, const a3d_shot_desc* d) {
 // the cells' damping multipliers, read by the next steps' sweeps, every strip_period steps.
 int strip_step_on(a3d_ctx* c, const StripArgs& a, hipStream_t st, int it) {
     if (c->strip_head == 2) {
-        if (it % c->strip_period == 0) HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
+        if (c->strip_period > 0 ? it % c->strip_period == 0 : it == 0)
+            HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
         return A3D_OK;
     }
     HIP_TRY(launch_strip_infer(a, c->strip_net, c->strip_delta, c->strip_mfma, st));
@@ -1567,7 +1569,7 @@ int a3d_set_strip_head(a3d_ctx* c, int mode) {
 }
 
 int a3d_set_strip_period(a3d_ctx* c, int steps) {
-    if (!c || steps < 1) return fail(A3D_EINVAL, "strip network period must be >= 1 step");
+    if (!c || steps < 0) return fail(A3D_EINVAL, "strip network period must be >= 0 steps");
     c->strip_period = steps;
     return A3D_OK;
 }

@@ -10,6 +10,7 @@
 // the cell's P (o = 0) and Q (o = 1) removed this step. The trained quantity is the
 // change it makes in units of the local amplitude s: pred = (-x[0] g_0, -x[7] g_1),
 // x[0] = P/s and x[7] = Q/s being the cell's own values among the features.
+// The coefficient head (training mode 2) is trained on its multiplier instead (coef_head).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -65,6 +66,16 @@ __device__ __forceinline__ void strip_head(const float (&y)[LB_NOUT], const floa
         g[o] = STRIP_GMAX / (1.0f + expf(-y[o]));
         pred[o] = -x[LB_HEAD_IN[o]] * g[o];
     }
+}
+
+// Training target of the coefficient head (training mode 2): the multiplier itself,
+// pred = (m, 0) with m = STRIP_MMAX sigmoid(y_0) for every sample (whatever the local
+// amplitude); output 1 is unused and carries no loss. dm/dy_0 = m (1 - m / STRIP_MMAX).
+__device__ __forceinline__ void coef_head(const float (&y)[LB_NOUT], float (&pred)[LB_NOUT], float& dm) {
+    const float m = STRIP_MMAX / (1.0f + expf(-y[0]));
+    pred[0] = m;
+    pred[1] = 0.f;
+    dm = m * (1.f - m * (1.f / STRIP_MMAX));
 }
 
 }  // namespace a3d

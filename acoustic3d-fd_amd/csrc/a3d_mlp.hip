@@ -42,7 +42,7 @@ __device__ __forceinline__ void stage_params(float* sp, const float* __restrict_
 __global__ __launch_bounds__(MB) void k_mlp_fwd(const float* __restrict__ p, const float* __restrict__ X,
                                                 const float* __restrict__ T, const int* __restrict__ idx, int nb,
                                                 float inv_n, float* __restrict__ H1, float* __restrict__ H2,
-                                                float* __restrict__ DY, float* __restrict__ loss_part) {
+                                                float* __restrict__ DY, float* __restrict__ loss_part, int head) {
     __shared__ float sp[LB_NPAR];
     __shared__ float red[MB / 64];
     stage_params(sp, p);
@@ -54,19 +54,29 @@ __global__ __launch_bounds__(MB) void k_mlp_fwd(const float* __restrict__ p, con
 #pragma unroll
         for (int i = 0; i < LB_NIN; ++i) x[i] = X[(size_t)r * LB_NIN + i];
         mlp_forward(sp, x, h1, h2, y);
-        float g[LB_NOUT], pred[LB_NOUT];
-        strip_head(y, x, g, pred);
 #pragma unroll
         for (int i = 0; i < LB_NH; ++i) {
             H1[(size_t)s * LB_NH + i] = h1[i];
             H2[(size_t)s * LB_NH + i] = h2[i];
         }
+        if (head == 2) {
+            float pred[LB_NOUT], dm;
+            coef_head(y, pred, dm);
+            const float e = pred[0] - T[(size_t)r * LB_NOUT];
+            err = e * e;
+            DY[(size_t)s * LB_NOUT] = 2.f * e * inv_n * dm;
+            DY[(size_t)s * LB_NOUT + 1] = 0.f;
+        } else {
+            float g[LB_NOUT], pred[LB_NOUT];
+            strip_head(y, x, g, pred);
 #pragma unroll
-        for (int o = 0; o < LB_NOUT; ++o) {
-            const float e = pred[o] - T[(size_t)r * LB_NOUT + o];
-            err += e * e;
-            // d pred / d y = -x_k * g * (1 - g / GMAX)
-            DY[(size_t)s * LB_NOUT + o] = 2.f * e * inv_n * (-x[LB_HEAD_IN[o]] * g[o] * (1.f - g[o] * (1.f / STRIP_GMAX)));
+            for (int o = 0; o < LB_NOUT; ++o) {
+                const float e = pred[o] - T[(size_t)r * LB_NOUT + o];
+                err += e * e;
+                // d pred / d y = -x_k * g * (1 - g / GMAX)
+                DY[(size_t)s * LB_NOUT + o] =
+                    2.f * e * inv_n * (-x[LB_HEAD_IN[o]] * g[o] * (1.f - g[o] * (1.f / STRIP_GMAX)));
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) err += __shfl_xor(err, o);
@@ -174,7 +184,7 @@ __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, f
 }
 
 __global__ __launch_bounds__(MB) void k_mlp_predict(const float* __restrict__ p, const float* __restrict__ X, int n,
-                                                    float* __restrict__ Y) {
+                                                    float* __restrict__ Y, int head) {
     __shared__ float sp[LB_NPAR];
     stage_params(sp, p);
     const int s = blockIdx.x * MB + threadIdx.x;
@@ -183,8 +193,9 @@ __global__ __launch_bounds__(MB) void k_mlp_predict(const float* __restrict__ p,
 #pragma unroll
     for (int i = 0; i < LB_NIN; ++i) x[i] = X[(size_t)s * LB_NIN + i];
     mlp_forward(sp, x, h1, h2, y);
-    float g[LB_NOUT], pred[LB_NOUT];
-    strip_head(y, x, g, pred);
+    float g[LB_NOUT], pred[LB_NOUT], dm;
+    if (head == 2) coef_head(y, pred, dm);
+    else strip_head(y, x, g, pred);
 #pragma unroll
     for (int o = 0; o < LB_NOUT; ++o) Y[(size_t)s * LB_NOUT + o] = pred[o];
 }
@@ -199,6 +210,7 @@ struct a3d_mlp {
     float* m = nullptr;      // Adam moments
     float* v = nullptr;
     long long adam_t = 0;
+    int head = 0;            // training target: 0 the damping head's change, 2 the coefficient head's multiplier
     // per-minibatch work buffers
     int cap = 0;
     float *H1 = nullptr, *H2 = nullptr, *D1 = nullptr, *D2 = nullptr, *DY = nullptr, *LP = nullptr;
@@ -237,7 +249,7 @@ int ensure_work(a3d_mlp* m, int nb) {
 // squared-error sum is left in m->LP (one value per block).
 int loss_grad(a3d_mlp* m, const float* X, const float* T, const int* idx, int nb, float inv_n) {
     const int nblk = (nb + MB - 1) / MB;
-    k_mlp_fwd<<<nblk, MB, 0, m->st>>>(m->p, X, T, idx, nb, inv_n, m->H1, m->H2, m->DY, m->LP);
+    k_mlp_fwd<<<nblk, MB, 0, m->st>>>(m->p, X, T, idx, nb, inv_n, m->H1, m->H2, m->DY, m->LP, m->head);
     MLP_TRY(hipGetLastError());
     k_mlp_bwd<<<nblk, MB, 0, m->st>>>(m->p, m->H1, m->H2, m->DY, nb, m->D1, m->D2);
     MLP_TRY(hipGetLastError());
@@ -296,6 +308,12 @@ int a3d_mlp_set_params(a3d_mlp* m, const float* params, int n) {
     return A3D_OK;
 }
 
+int a3d_mlp_set_head(a3d_mlp* m, int head) {
+    if (!m || (head != 0 && head != 2)) return report_error(A3D_EINVAL, "a3d_mlp_set_head", "head must be 0 or 2");
+    m->head = head;
+    return A3D_OK;
+}
+
 int a3d_mlp_get_params(a3d_mlp* m, float* params, int n) {
     if (!m || !params || n != LB_NPAR) return report_error(A3D_EINVAL, "a3d_mlp_get_params", "bad argument");
     MLP_TRY(hipSetDevice(m->device));
@@ -316,7 +334,7 @@ int a3d_mlp_predict(a3d_mlp* m, const float* X, int n, float* Y) {
     int rc = A3D_OK;
     if (hipMemcpy(dx, X, (size_t)n * LB_NIN * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = A3D_EHIP;
     if (!rc) {
-        k_mlp_predict<<<(n + MB - 1) / MB, MB, 0, m->st>>>(m->p, dx, n, dy);
+        k_mlp_predict<<<(n + MB - 1) / MB, MB, 0, m->st>>>(m->p, dx, n, dy, m->head);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->st) != hipSuccess ||
             hipMemcpy(Y, dy, (size_t)n * LB_NOUT * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = A3D_EHIP;

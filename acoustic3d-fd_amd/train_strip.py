@@ -57,11 +57,29 @@ def main():
                     help="coefficient head: each (model, source) case's best constant multiplier of the strip's "
                          "PML damping (scanned against the 40-cell-PML teacher's interior) labels that case's "
                          "strip-cell samples; the network learns the multiplier from the local state")
+    ap.add_argument("--coef-profile", action="store_true",
+                    help="coefficient head, depth profile: per-depth multipliers found by coordinate search on "
+                         "the training cases' reflected energy (against their 40-cell-PML teachers), then the "
+                         "network trained on the GPU on teacher-forced samples labelled with their depth's "
+                         "multiplier")
     ap.add_argument("--period", type=int, default=1, help="coefficient head: network every k steps")
-    ap.add_argument("--out", default=strip.NET_PATH)
+    ap.add_argument("--labels-from", default="",
+                    help="coefficient head: take the (model, source) labels m* and their scans from the summary "
+                         "of an earlier run's network file instead of scanning again")
+    ap.add_argument("--bias", choices=("mean", "pooled"), default="mean",
+                    help="coefficient head: output bias at start: the mean label, or the constant multiplier "
+                         "with the lowest reflected energy pooled over the labelled cases (from their scans)")
+    ap.add_argument("--scheme", type=int, default=3, help="stress scheme of the strip runs")
+    ap.add_argument("--out", default=None,
+                    help="network file (default: the committed coefficient-head network for --coef, the "
+                         "damping-head network otherwise)")
     args = ap.parse_args()
+    if args.out is None:
+        args.out = strip.NET_PATH if args.coef or args.coef_profile else strip.DAMP_NET_PATH
     if args.adaptive:
         return adaptive(args)
+    if args.coef_profile:
+        return coef_profile(args)
     if args.coef:
         return coef(args)
 
@@ -142,17 +160,24 @@ def coef(args):
     (features as the head reads them); the network is trained on the GPU (a3d_mlp kernels) to
     give m* from the local state, and judged on held-out models against the plain strip (m = 1)
     and the 20-cell PML, all against extended-domain FP64 runs."""
-    from a3dfd import models
     seeds = [10000 + k for k in range(args.models)]
     Xs, Ts, labels = [], [], []
+    known = {}
+    if args.labels_from:
+        with np.load(args.labels_from, allow_pickle=False) as f:
+            for r in json.loads(str(f["summary"]))["labels"]:
+                known[(r["seed"], r["source"])] = (r["m_star"], {float(k): v for k, v in r["db"].items()})
     t0 = time.time()
     for seed in seeds:
         for j, src in enumerate(strip.model_sources(seed, args.n)):
-            case = strip.CoefCase(seed, j, n=args.n, nt=args.nt, snap_every=30, amp=args.amp)
-            try:
-                mstar, tab = strip.coef_label(case)
-            finally:
-                case.close()
+            if (seed, j) in known:
+                mstar, tab = known[(seed, j)]
+            else:
+                case = strip.CoefCase(seed, j, n=args.n, nt=args.nt, snap_every=30, amp=args.amp, scheme=args.scheme)
+                try:
+                    mstar, tab = strip.coef_label(case)
+                finally:
+                    case.close()
             X, _, _ = strip.collect([seed], n=args.n, nt=args.nt, every=args.every, keep=args.keep, amp=args.amp,
                                     head=strip.COEF_HEAD, sources=[src], params=strip.coef_params(
                                         strip.coef_closed_theta(), np.zeros(strip.NPARAMS, np.float32)))
@@ -164,19 +189,83 @@ def coef(args):
             f"({sum(len(x) for x in Xs)} samples, {time.time() - t0:.1f} s)")
     X, T = np.concatenate(Xs), np.concatenate(Ts)
     mbar = float(np.mean([r["m_star"] for r in labels]))
-    r = mbar / strip.COEF_MMAX
+    m0 = mbar if args.bias == "mean" else strip.pooled_multiplier([r["db"] for r in labels])
+    r = m0 / strip.COEF_MMAX
     init = strip.init_params(0, out_bias=float(np.log(r / (1 - r))))
     params, hist = strip.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, init=init)
-    log(f"train_strip: {len(X)} samples, mean m* {mbar:.3f}, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    log(f"train_strip: {len(X)} samples, mean m* {mbar:.3f}, start bias m {m0:.3f}, "
+        f"loss {hist[0]:.3e} -> {hist[-1]:.3e}")
     rows = strip.judge([90000 + k for k in range(args.heldout)], params, n=args.n, nt=args.judge_nt, amp=args.amp,
-                       head=strip.COEF_HEAD, log=log)
+                       head=strip.COEF_HEAD, log=log, period=args.period, scheme=args.scheme)
     choice = strip.choose(rows)
+    gains = [round(r["strip0"]["reflected_db"] - r["strip"]["reflected_db"], 2) for r in rows]
+    log(f"train_strip: held-out gain over the closed head {gains} dB, mean {np.mean(gains):.2f} dB")
     summary = {"mode": "coefficient", "head": strip.COEF_HEAD, "samples": int(len(X)), "models": args.models,
                "cases": len(labels), "n": args.n, "nt": args.nt, "amp": args.amp, "period": args.period,
-               "labels": labels, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1], "judge": rows,
+               "scheme": args.scheme, "epochs": args.epochs, "lr": args.lr, "bias": args.bias,
+               "start_multiplier": m0, "labels": labels, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+               "judge": rows, "heldout_gain_db": gains, "heldout_mean_gain_db": float(np.mean(gains)),
                "choice": choice}
     strip.save(params, args.out, amp=np.float32(args.amp), head=np.int32(strip.COEF_HEAD),
                period=np.int32(args.period), loss=np.asarray(hist), summary=json.dumps(summary))
+    print(json.dumps(summary), flush=True)
+
+
+def coef_profile(args):
+    """Coefficient head (mode 2) with a learned depth profile. (1) Training cases: every source
+    (1-4, 10-30 Hz) of training models 10000 + k, each with its 40-cell-PML teacher run; (2) the
+    per-depth multipliers of the strip's PML damping that minimise the cases' mean reflected
+    energy, by coordinate search (strip.search_depth_profile); (3) teacher-forced samples of
+    the same models (and a share of them with the wavefield zeroed, the state ahead of the
+    wave) labelled with their depth's multiplier, on which the network is trained with the GPU
+    training kernels on the multiplier itself (training mode 2); (4) the judge on held-out models (the trained network, the
+    exact profile and the closed head m = 1, all against extended-domain FP64 runs)."""
+    seeds = [10000 + k for k in range(args.models)]
+    t0 = time.time()
+    pool, srcs = [], []
+    for seed in seeds:
+        for j, src in enumerate(strip.model_sources(seed, args.n)):
+            pool.append(strip.CoefCase(seed, j, n=args.n, nt=args.nt, snap_every=30, amp=args.amp,
+                                       period=0, scheme=args.scheme))
+    log(f"train_strip: {len(pool)} training cases from {len(seeds)} models ({time.time() - t0:.1f} s)")
+    try:
+        closed_db = float(np.mean([c.db(strip.depth_coef_params(np.ones(6))) for c in pool]))
+        mult, best, hist = strip.search_depth_profile(pool, log=log)
+    finally:
+        for c in pool:
+            c.close()
+    log(f"train_strip: depth profile {mult.round(3).tolist()} (1 = outermost): {best:.3f} dB against "
+        f"{closed_db:.3f} dB closed ({time.time() - t0:.1f} s)")
+    Xs = []
+    for seed in seeds:
+        X, _, _ = strip.collect([seed], n=args.n, nt=args.nt, every=args.every, keep=args.keep, amp=args.amp,
+                                head=strip.COEF_HEAD, params=strip.depth_coef_params(np.ones(6)))
+        Xs.append(X)
+    X, T = strip.depth_targets(np.concatenate(Xs), mult)
+    m0 = float(np.mean(mult))
+    r = m0 / strip.COEF_MMAX
+    init = strip.init_params(0, out_bias=float(np.log(r / (1 - r))))
+    params, loss = strip.train(X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, init=init,
+                               head=strip.COEF_HEAD)
+    log(f"train_strip: {len(X)} samples, loss {loss[0]:.3e} -> {loss[-1]:.3e} ({time.time() - t0:.1f} s)")
+    held = [90000 + k for k in range(args.heldout)]
+    rows = strip.judge(held, params, n=args.n, nt=args.judge_nt, amp=args.amp, head=strip.COEF_HEAD, log=log,
+                       period=args.period, scheme=args.scheme)
+    exact = strip.judge(held, strip.depth_coef_params(mult), n=args.n, nt=args.judge_nt, amp=args.amp,
+                        head=strip.COEF_HEAD, period=0, scheme=args.scheme)
+    gains = [round(r["strip0"]["reflected_db"] - r["strip"]["reflected_db"], 2) for r in rows]
+    gains_exact = [round(r["strip0"]["reflected_db"] - r["strip"]["reflected_db"], 2) for r in exact]
+    log(f"train_strip: held-out gain over the closed head {gains} dB (mean {np.mean(gains):.2f}); "
+        f"exact profile {np.mean(gains_exact):.2f} dB")
+    summary = {"mode": "coefficient, learned depth profile", "head": strip.COEF_HEAD, "models": args.models,
+               "cases": int(sum(len(strip.model_sources(s, args.n)) for s in seeds)), "n": args.n, "nt": args.nt,
+               "amp": args.amp, "period": args.period, "scheme": args.scheme, "epochs": args.epochs, "lr": args.lr,
+               "depth_multipliers": [round(float(v), 4) for v in mult], "train_db": best, "train_closed_db": closed_db,
+               "search": hist, "samples": int(len(X)), "loss_first_epoch": loss[0], "loss_last_epoch": loss[-1],
+               "judge": rows, "heldout_gain_db": gains, "heldout_mean_gain_db": float(np.mean(gains)),
+               "heldout_gain_db_exact_profile": gains_exact, "choice": strip.choose(rows)}
+    strip.save(params, args.out, amp=np.float32(args.amp), head=np.int32(strip.COEF_HEAD),
+               period=np.int32(args.period), loss=np.asarray(loss), summary=json.dumps(summary))
     print(json.dumps(summary), flush=True)
 
 

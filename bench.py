@@ -255,11 +255,11 @@ def batched(args):
     print(json.dumps(line), flush=True)
 
 
-def learned_boundary(scheme=3, nt=2000, n=256, seed=95000):
+def learned_boundary(scheme=4, nt=2000, n=256, seed=95000):
     """The learned-boundary half of the metric, measured in the bench run: the VTI 256^3
     configuration (d = 10 m, dt = 0.8 ms, nt 2000, seeded study model and source) as per-shot
-    runs with the 20-cell PML, with the 6-cell strip without network (plain ramp) and with the
-    committed learned strip (its head mode and network period), all under the same stress
+    runs with the 20-cell PML, with the 6-cell strip with its head closed (plain ramp) and with
+    the committed learned strip (its head mode and network period), all under the same stress
     scheme. shots/hour from each run's time loop (best of two, no snapshots); reflected
     energy and shot-record error against an extended-domain FP64 run of the same shot
     (snapshots every 100 steps)."""
@@ -311,7 +311,7 @@ def strip_coef(args):
     rows = {}
 
     def one(name, b, params=None, head=0, period=1):
-        kw = dict(amp=10.0, scheme=3, head=head, period=period)
+        kw = dict(amp=10.0, scheme=args.scheme, head=head, period=period)
         shot, snaps, _ = strip.run_case(m, src, nt, 100, b, params, **kw)
         ms = min(strip.run_case(m, src, nt, nt + 1, b, params, **kw)[2] for _ in range(2))
         rows[name] = {"loop_ms": round(ms, 1), "reflected_db": round(float(strip.energy_db(snaps, ref_snaps)), 2),

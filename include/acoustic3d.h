@@ -327,9 +327,10 @@ int a3d_set_strip_head(a3d_ctx* ctx, int mode);
  * plain ramp). The velocity and stress sweeps recompute the cell's split-PML coefficients
  * c1 = 1 / (1 + m h), c2 = c1 (1 - m h) from m and the profile's half damping step h =
  * dt d / 2 inside their update; nothing is applied after the sweeps. The network runs after
- * the stress sweep of every `steps`-th time step (a3d_set_strip_period, default 1) and the
- * multipliers hold until it runs again (1 before its first run). w stays in the velocity
- * sweep. */
+ * the stress sweep of every `steps`-th time step (a3d_set_strip_period, default 1; 0: after
+ * the first step's only, for a network whose multipliers depend on the model and the strip
+ * depth alone) and the multipliers hold until it runs again (1 before its first run). w may
+ * be updated by either sweep (a3d_set_w_sweep). */
 int a3d_set_strip_period(a3d_ctx* ctx, int steps);
 /* Number of strip cells of the context's grid (0 before a network was set). */
 int a3d_strip_info(const a3d_ctx* ctx, int* ncells);
@@ -352,8 +353,12 @@ int a3d_mlp_create(int device, a3d_mlp** out);
 int a3d_mlp_destroy(a3d_mlp* m);
 int a3d_mlp_set_params(a3d_mlp* m, const float* params, int n);
 int a3d_mlp_get_params(a3d_mlp* m, float* params, int n);
+/* What predict, loss_grad and train fit: 0 (default) the damping head's change below;
+ * 2 the coefficient head's multiplier, Y = (m, 0) with m = 3 sigmoid(y_0) for every sample
+ * (output 1 unused, no loss on it). */
+int a3d_mlp_set_head(a3d_mlp* m, int head);
 /* Y[n][2]: the damping head's change of the cell's P/s and Q/s,
- * (-X[.][0] g_P, -X[.][7] g_Q), for features X[n][16]. */
+ * (-X[.][0] g_P, -X[.][7] g_Q), for features X[n][16] (head 0; see a3d_mlp_set_head). */
 int a3d_mlp_predict(a3d_mlp* m, const float* X, int n, float* Y);
 /* Mean over samples of the summed squared error of Y against T and its gradient (FP32). */
 int a3d_mlp_loss_grad(a3d_mlp* m, const float* X, const float* T, int n, double* loss, float* grad);

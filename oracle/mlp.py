@@ -64,18 +64,42 @@ def forward(p, X):
     return -xs * g, (X, h1, h2, g, xs)
 
 
-def loss(p, X, T):
+MMAX = 3.0
+
+
+def forward_coef(p, X):
+    """Coefficient-head training target: pred = (m, 0), m = MMAX * sigmoid(y_0) (output 1 unused)."""
+    y, (X, h1, h2) = raw(p, X)
+    m = MMAX / (1.0 + np.exp(-y[:, 0]))
+    return np.stack([m, np.zeros_like(m)], axis=1), (X, h1, h2, m)
+
+
+def loss(p, X, T, head=0):
+    """Mean over samples of the summed squared error; head 2: of output 0 (the multiplier) only."""
+    T = np.asarray(T, np.float64)
+    if head == 2:
+        pred, _ = forward_coef(p, X)
+        return float(np.sum((pred[:, 0] - T[:, 0]) ** 2) / len(pred))
     pred, _ = forward(p, X)
-    return float(np.sum((pred - np.asarray(T, np.float64)) ** 2) / len(pred))
+    return float(np.sum((pred - T) ** 2) / len(pred))
 
 
-def loss_grad(p, X, T):
-    """Loss and its gradient (flat, FP64) by backpropagation."""
+def loss_grad(p, X, T, head=0):
+    """Loss and its gradient (flat, FP64) by backpropagation (head as in `loss`)."""
     w = unpack(p)
-    pred, (X, h1, h2, g, xs) = forward(p, X)
-    n = len(pred)
-    e = pred - np.asarray(T, np.float64)
-    dy = 2.0 * e / n * (-xs * g * (1.0 - g / GMAX))
+    T = np.asarray(T, np.float64)
+    if head == 2:
+        pred, (X, h1, h2, m) = forward_coef(p, X)
+        n = len(pred)
+        e = np.zeros_like(pred)
+        e[:, 0] = pred[:, 0] - T[:, 0]
+        dy = np.zeros_like(pred)
+        dy[:, 0] = 2.0 * e[:, 0] / n * m * (1.0 - m / MMAX)
+    else:
+        pred, (X, h1, h2, g, xs) = forward(p, X)
+        n = len(pred)
+        e = pred - T
+        dy = 2.0 * e / n * (-xs * g * (1.0 - g / GMAX))
     grad = {"W3": dy.T @ h2, "b3": dy.sum(0)}
     d2 = (dy @ w["W3"]) * (1.0 - h2 ** 2)
     grad["W2"] = d2.T @ h1

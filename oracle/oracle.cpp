@@ -411,7 +411,8 @@ double g_strip_amp = 10.0;    /* PML amplitude of the strip's 6-cell ramp */
 bool g_strip_half = true;     /* network operands rounded to binary16 (matrix-core evaluation) */
 int g_strip_head = 0;         /* 0: damping of every field; 1: normal-axis split components and velocity;
                                  2: multiplier of the PML damping inside the update (coefficient head) */
-int g_strip_period = 1;       /* head 2: the network runs after the stress update of every period-th step */
+int g_strip_period = 1;       /* head 2: the network runs after the stress update of every period-th step
+                                 (0: after the first step's only) */
 constexpr float kMmax = 3.0f; /* head 2: m = kMmax * sigmoid(y_0) */
 
 /* Planted mistake in the TTI rotation, for tests that must show they catch one (0: none).
@@ -812,7 +813,8 @@ int run_1st(const OracleShot& p, const float* vel, const float* rho_in, const fl
                     Qp[id] = qx[id] + qy[id] + qz[id];
                 }
         if constexpr (std::is_same<T, float>::value) {
-            if (vti && !g_strip.empty() && (!strip2 || it % g_strip_period == 0))
+            if (vti && !g_strip.empty() &&
+                (!strip2 || (g_strip_period > 0 ? it % g_strip_period == 0 : it == 0)))
                 strip_correct(nnx, nny, nnz, p.dt, p.dx, p.dy, p.dz, Pp, Qp, u.data(), v.data(), w.data(),
                               px.data(), py.data(), pz.data(), qx.data(), qy.data(), qz.data(), vp.data(),
                               rho.data(), eps.data(), del.data(), strip2 ? mfac.data() : nullptr,
@@ -934,7 +936,7 @@ int oracle_set_strip_head(int mode) {
 }
 
 int oracle_set_strip_period(int steps) {
-    if (steps < 1) return -1;
+    if (steps < 0) return -1;
     g_strip_period = steps;
     return 0;
 }

@@ -66,6 +66,27 @@ def test_gradient_matches_fp64(n):
         k += m
 
 
+def test_coefficient_head_gradient_matches_fp64():
+    """Training mode 2 (the coefficient head's multiplier as the target): GPU loss, gradient and
+    prediction against the FP64 network."""
+    from a3dfd import Mlp
+    rng = np.random.default_rng(12)
+    X = rng.normal(0, 1, (4096, 16)).astype(np.float32)
+    T = rng.uniform(0.2, 2.5, (4096, 2)).astype(np.float32)
+    p = _params(7)
+    with Mlp() as net:
+        net.set_params(p)
+        net.set_head(2)
+        loss, g = net.loss_grad(X, T)
+        y = net.predict(X)
+    rloss, rg = mlp.loss_grad(p, X, T, head=2)
+    ref, _ = mlp.forward_coef(p, X)
+    print(f"[gpu-strip] coefficient head: loss {loss:.6f} vs {rloss:.6f}, gradient rel L2 {rel_l2(g, rg):.2e}")
+    assert abs(loss - rloss) <= 1e-5 * abs(rloss)
+    assert rel_l2(g, rg) <= 1e-4
+    assert rel_l2(y, ref) <= 1e-5 and not np.any(y[:, 1])
+
+
 def test_training_fits_a_teacher_network():
     from a3dfd import Mlp
     rng = np.random.default_rng(3)
@@ -153,7 +174,8 @@ def test_strip_matches_oracle(oracle, scheme, mfma, head):
     assert e_shot <= 1e-5 and e_snap <= 1e-5
 
 
-@pytest.mark.parametrize("scheme,mfma,period", [(0, 1, 1), (3, 1, 1), (1, 1, 2), (0, 0, 1), (3, 1, 4), (4, 1, 1)])
+@pytest.mark.parametrize("scheme,mfma,period", [(0, 1, 1), (3, 1, 1), (1, 1, 2), (0, 0, 1), (3, 1, 4), (4, 1, 1),
+                                                (4, 1, 0)])
 def test_coefficient_head_matches_oracle(oracle, scheme, mfma, period):
     """Head mode 2: the network's multiplier of each strip cell's PML damping, read by the
     velocity and stress sweeps inside their update (no pass after the sweeps), against the
@@ -227,23 +249,30 @@ def test_collect_samples():
 
 
 def test_trained_network_gain_over_closed_head():
-    """What the committed trained network gains over the same 6-cell strip with its damping
-    head shut, on a held-out model, judged against an extended-domain FP64 run. The network
-    trained by teacher forcing keeps its head almost shut, so the gain is about 0 dB; the test
-    prints it and requires that the network does not make the strip worse (by more than
-    0.1 dB) and that both stay far below the reflection of an undamped truncation."""
+    """What the committed trained network (coefficient head, run once per shot) gains over the
+    same 6-cell strip with its head closed (every multiplier 1: the plain ramp), on the first
+    four held-out 128^3 models (seeds 90000-90003, one source each, nt 800), judged against
+    extended-domain FP64 runs. Measured on MI355X: -0.57, +4.34, +2.85, +3.91 dB (mean +2.6;
+    +1.6 dB over twelve held-out models). The test asks for a mean gain of at least 1.5 dB and
+    no model more than 1.5 dB worse."""
     from a3dfd import models, strip
     params, amp = strip.load_params()
-    assert params is not None, "trained network missing (acoustic3d-fd_amd/a3dfd/strip_net.npz)"
-    n, nt, every, seed = 64, 400, 20, 90100
-    m = models.strip_model(seed, n)
-    src = models.strip_source(seed, n)
-    _, ref, _ = strip.run_case(m, src, nt, every, "extended", margin=32)
-    db = {}
-    for b in ("strip0", "strip"):
-        _, snaps, _ = strip.run_case(m, src, nt, every, b, params, amp=amp, head=strip.load_head())
-        db[b] = float(strip.energy_db(snaps, ref))
-    print(f"\n[gpu-strip] held-out 64^3: strip without network {db['strip0']:.2f} dB, learned {db['strip']:.2f} dB, "
-          f"gain {db['strip0'] - db['strip']:+.2f} dB")
-    assert db["strip"] <= db["strip0"] + 0.1
-    assert db["strip0"] < -20.0
+    assert params is not None, "trained network missing (acoustic3d-fd_amd/a3dfd/strip_coef_net.npz)"
+    head, period = strip.load_head(), strip.load_period()
+    assert head == strip.COEF_HEAD
+    n, nt, every = 128, 800, 20
+    gains = []
+    for seed in (90000, 90001, 90002, 90003):
+        m = models.strip_model(seed, n)
+        src = models.strip_source(seed, n)
+        _, ref, _ = strip.run_case(m, src, nt, every, "extended", margin=64)
+        db = {}
+        for b in ("strip0", "strip"):
+            _, snaps, _ = strip.run_case(m, src, nt, every, b, params, amp=amp, head=head, period=period, scheme=4)
+            db[b] = float(strip.energy_db(snaps, ref))
+        gains.append(db["strip0"] - db["strip"])
+        print(f"\n[gpu-strip] held-out {seed}: closed head {db['strip0']:.2f} dB, learned {db['strip']:.2f} dB, "
+              f"gain {gains[-1]:+.2f} dB")
+        assert db["strip0"] < -20.0
+    print(f"[gpu-strip] mean gain {np.mean(gains):+.2f} dB")
+    assert np.mean(gains) >= 1.5 and min(gains) >= -1.5

@@ -43,6 +43,47 @@ def test_fp64_gradient_matches_central_differences():
         assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
+def test_fp64_coefficient_head_gradient_matches_central_differences():
+    """Training mode 2 (the coefficient head's multiplier m = 3 sigmoid(y_0) as the target):
+    the hand-written gradient against central differences; output 1 carries no loss."""
+    rng = np.random.default_rng(4)
+    p = mlp.init(2, scale=1.0).astype(np.float64)
+    p[mlp.NPAR - 200:] += rng.normal(0, 0.3, 200)
+    X = rng.normal(0, 1, (40, mlp.NIN))
+    T = rng.uniform(0.2, 2.5, (40, mlp.NOUT))
+    _, g = mlp.loss_grad(p, X, T, head=2)
+    idx = np.concatenate([rng.choice(mlp.NPAR, 60, replace=False), np.arange(mlp.NPAR - 130, mlp.NPAR, 7)])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(mlp.NPAR)
+        e[i] = h
+        fd = (mlp.loss(p + e, X, T, head=2) - mlp.loss(p - e, X, T, head=2)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+    w3_1 = slice(mlp.NPAR - 2 - 64, mlp.NPAR - 2)   # W3 row 1 and b3[1]: no loss through output 1
+    assert not np.any(g[w3_1]) and g[-1] == 0.0
+    # the multiplier of a network whose output 0 is the constant b: 3 sigmoid(b)
+    q = np.zeros(mlp.NPAR)
+    q[-2] = 0.4
+    pred, _ = mlp.forward_coef(q, X)
+    np.testing.assert_allclose(pred[:, 0], 3.0 / (1.0 + np.exp(-0.4)), rtol=1e-15)
+
+
+def test_depth_profile_network_gives_its_multipliers():
+    """strip.depth_coef_params: the FP64 network gives each strip depth its multiplier, whatever
+    the wavefield and model inputs."""
+    from a3dfd import strip
+    mult = np.array([0.9, 0.75, 0.6, 0.5, 0.8, 1.4])
+    p = strip.depth_coef_params(mult)
+    rng = np.random.default_rng(6)
+    X = rng.uniform(-1, 1, (600, mlp.NIN))
+    d = rng.integers(1, 7, 600)
+    X[:, 14] = d / 6.0
+    X[:, 11] = rng.uniform(0.12, 0.36, 600)
+    X[:, 15] = rng.uniform(0.4, 2.5, 600)
+    pred, _ = mlp.forward_coef(p, X)
+    np.testing.assert_allclose(pred[:, 0], mult[d - 1], rtol=2e-4)
+
+
 def _single_path(k, a=0.7, b=1.3, c=-0.9, c2=0.4, bias=(0.05, -0.02)):
     """Network whose raw outputs depend on input k only: y = (c, c2) * tanh(b * tanh(a * x_k)) + bias."""
     w = {n: np.zeros(s) for n, s in mlp.SHAPES}
@@ -310,8 +351,8 @@ def test_oracle_coefficient_head_scales_the_ramp(oracle, mult):
     """Head 2 ("coefficient") with a constant multiplier m is the strip's PML ramp with its
     damping scaled by m in both the velocity and the stress update: it equals the plain strip
     (shut damping head) at ramp amplitude 10 m within FP32 rounding of the coefficients, and
-    differs from the amplitude-10 ramp. A period of 3 steps between network runs changes
-    nothing for a constant network (bitwise)."""
+    differs from the amplitude-10 ramp. A period of 3 steps between network runs, or one run
+    after the first step (period 0), changes nothing for a constant network (bitwise)."""
     from a3dfd import models, strip
     m = models.small_model("vti", 18, 16, 16, seed=9)
     kw = dict(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
@@ -332,6 +373,8 @@ def test_oracle_coefficient_head_scales_the_ramp(oracle, mult):
         head2 = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
         oracle.set_strip_period(3)
         head2p = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
+        oracle.set_strip_period(0)          # the network after the first step only
+        head2o = oracle.run(shot, 32, nsnap_max=2, **kw)[:2]
     finally:
         oracle.set_strip_period(1)
         oracle.set_strip_head(0)
@@ -343,3 +386,4 @@ def test_oracle_coefficient_head_scales_the_ramp(oracle, mult):
     assert rel_l2(g, a) < 1e-5 and rel_l2(sg, sa) < 1e-5
     assert rel_l2(g, c) > 100 * rel_l2(g, a) and rel_l2(sg, sc) > 100 * rel_l2(sg, sa)
     assert np.array_equal(head2p[0], g) and np.array_equal(head2p[1], sg)
+    assert np.array_equal(head2o[0], g) and np.array_equal(head2o[1], sg)
