@@ -195,7 +195,7 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
     scheme: the stress update of the non-extended runs (0 reference, 1 split fields in PML cells only).
     head: what the network does (0 damp every field, 1 damp the normal-axis PML components,
     2 scale each strip cell's PML damping inside the sweeps); period: steps between network
-    runs (head 2).
+    runs (head 2; 0: once, after the first step, with the network evaluated in FP32).
     Returns (shot (ny, nx, nt), snapshots (k, ny, nx, nz), loop_ms) on the original interior."""
     if boundary == "extended":
         me, se = extended(m, src, margin)
@@ -215,6 +215,8 @@ def run_case(m, src, nt, snap_every, boundary, params=None, margin=64, amp=None,
             if boundary == "strip":
                 mdl.set_strip_head(head)
                 mdl.set_strip_period(period)
+                if head == COEF_HEAD and period == 0:
+                    mdl.set_strip_precision(0)   # run once per shot: FP32 evaluation costs nothing
                 mdl.set_strip_net(params)
             elif head == COEF_HEAD:
                 # strip0 beside a coefficient head: the plain ramp through the same sweeps, every
@@ -420,6 +422,8 @@ class CoefCase:
         self.mdl.set_strip_profile(amp)
         self.mdl.set_strip_head(COEF_HEAD)
         self.mdl.set_strip_period(period)
+        if period == 0:
+            self.mdl.set_strip_precision(0)
 
     def db(self, params):
         """Reflected energy (dB) of the strip run with network `params` against the teacher."""
