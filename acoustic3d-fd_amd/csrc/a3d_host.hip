@@ -625,6 +625,7 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     sa.zi0 = zi0;
     sa.zi1 = zi1;
     va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
+    va.zrev = va.wz;   // no z window in the velocity sweep: its planes walked downward
     if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
     // the stress sweep's (and the one-pass step's) interleaved model (VTI, float model),
     // built once per model
@@ -2269,7 +2270,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
     const bool sm = b->ctx[0]->strip_net && b->ctx[0]->strip_head == 2;
     const bool wz = (op == A3D_OP_ISO || op == A3D_OP_VTI) && b->d.scheme != 2 && !sm &&
                     (b->wz_mode == 1 || (b->wz_mode < 0 && b->wz_tuned));
-    for (int w = 0; w < nw; ++w) va[w].wz = sa[w].wz = wz ? 1 : 0;
+    for (int w = 0; w < nw; ++w) va[w].wz = va[w].zrev = sa[w].wz = wz ? 1 : 0;
     if (mode == 3) {
         if ((rc = persist_loop(b, va, sa, src_all, wz, loop_ms))) return rc;
         return batch_outputs(b, shots);

@@ -92,6 +92,7 @@ struct VelArgs {
     Src s;
     Scales sc;
     int wz;                    // 1 (ISO, VTI): w is updated by the stress sweep instead
+    int zrev;                  // with wz (no z window in this sweep): planes walked downward
     const float* mfac;         // learned strip, coefficient head: damping multiplier per padded point, or null
 };
 

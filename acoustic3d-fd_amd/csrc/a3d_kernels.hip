@@ -330,7 +330,13 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         hy = ztab(T, a.c.hvy + yw);
     }
 
-    for (int z = zb; z < ze; ++z) {
+    // Without z windows (w made by the stress sweep: no z difference here) the planes are
+    // walked downward (a.zrev): the sweep then starts where the previous stress sweep ended,
+    // whose last planes may still be in the last-level cache, and ends where the next
+    // stress sweep starts (VTI 400x400x300: +1% per step, bit-identical).
+    const bool down = !P_Z && !HAS_Q && a.zrev;
+    for (int i = 0; i < ze - zb; ++i) {
+        const int z = down ? ze - 1 - i : zb + i;
         const int zo = z * g.pl;
         C hz = 0, mc;
         bool scell = false;
