@@ -295,6 +295,40 @@ def learned_boundary(scheme=4, nt=2000, n=256, seed=95000):
     return res
 
 
+def strip_heldout(args):
+    """The committed learned strip on --strip-heldout held-out study models (seeds 90000 + k,
+    --strip-n cubed, nt --strip-nt, one source each, scheme --scheme): reflected energy of the
+    20-cell PML, the strip with its head closed and the learned strip against extended-domain
+    FP64 runs, loop times; mean gain over the closed head with its standard error, and the
+    share of models at or below -40 dB."""
+    from a3dfd import strip
+    params, amp = strip.load_params()
+    head, period = strip.load_head(), strip.load_period()
+    n, nt = args.strip_n, args.strip_nt
+    every = max(1, nt // 40)
+    seeds = [90000 + k for k in range(args.strip_heldout)]
+    rows = strip.judge(seeds, params, n=n, nt=nt, snap_every=every, amp=amp, head=head, period=period,
+                       scheme=args.scheme, log=log)
+    g = np.array([r["strip0"]["reflected_db"] - r["strip"]["reflected_db"] for r in rows])
+    db = {k: np.array([r[k]["reflected_db"] for r in rows]) for k in ("pml", "strip0", "strip")}
+    ms = {k: np.array([r[k]["loop_ms"] for r in rows]) for k in ("pml", "strip0", "strip")}
+    out = {"workload": f"vti_{n}^3_nt{nt}", "models": len(rows), "seeds": [seeds[0], seeds[-1]], "scheme": args.scheme,
+           "network": {"head": head, "period": period, "ramp_amplitude": amp},
+           "gain_db_over_closed_head": {"mean": round(float(g.mean()), 3),
+                                        "stderr": round(float(g.std(ddof=1) / np.sqrt(len(g))), 3) if len(g) > 1 else None,
+                                        "median": round(float(np.median(g)), 3), "min": round(float(g.min()), 2),
+                                        "max": round(float(g.max()), 2), "share_positive": round(float((g > 0).mean()), 3)},
+           "mean_reflected_db": {k: round(float(v.mean()), 2) for k, v in db.items()},
+           "share_at_or_below_-40dB": {k: round(float((v <= -40.0).mean()), 3) for k, v in db.items()},
+           "loop_ms_mean": {k: round(float(v.mean()), 1) for k, v in ms.items()},
+           "speedup_learned_vs_pml20": round(float(ms["pml"].mean() / ms["strip"].mean()), 3),
+           "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64",
+           "rows": [{"seed": r["seed"], "fm": r["fm"], "src": r["src"], "pml_db": r["pml"]["reflected_db"],
+                     "closed_db": r["strip0"]["reflected_db"], "learned_db": r["strip"]["reflected_db"]}
+                    for r in rows]}
+    print(json.dumps(out), flush=True)
+
+
 def strip_coef(args):
     """The learned strip's coefficient head (network file --strip-coef, head 2) on the VTI 256^3
     study model (seed 95000, nt --strip-nt), scheme 3: loop time (best of two runs without
@@ -636,6 +670,8 @@ def main():
     ap.add_argument("--strip-coef", type=str, default="",
                     help="coefficient-head study of the given network file (speed and reflection vs network period)")
     ap.add_argument("--strip-periods", type=str, default="1,4,8,16")
+    ap.add_argument("--strip-heldout", type=int, default=0,
+                    help="judge the committed learned strip on this many held-out study models (--strip-n, --strip-nt)")
     ap.add_argument("--strip-n", type=int, default=256)
     ap.add_argument("--strip-nt", type=int, default=2000)
     args = ap.parse_args()
@@ -670,6 +706,8 @@ def main():
         return strip_coef(args)
     if args.strip_profiles:
         return strip_profiles(args)
+    if args.strip_heldout:
+        return strip_heldout(args)
     if args.batched_strip:
         return batched_strip(args)
     if args.operators:
