@@ -61,12 +61,12 @@ KEYS = {"2nd": ("vel",), "iso": ("vel", "rho"), "vti": ("vel", "rho", "eps", "de
         "tti": ("vel", "eps", "delta", "theta", "phi")}
 # (operator, grid, source, nt, extra descriptor fields): the second case of each pair is smaller
 CASES = {
-    "vti": [((40, 36, 30), (20, 18, 4), 200, dict(nb=10, scheme=3, mute=1, mute_tt=0)),
+    "vti": [((40, 36, 30), (20, 18, 4), 200, dict(nb=10, scheme=4, mute=1, mute_tt=0)),
             ((24, 20, 22), (12, 10, 3), 120, dict(nb=10, scheme=0))],
     "2nd": [((48, 40, 36), (24, 20, 5), 160, dict(nb=20, rec_z=5)),
             ((24, 20, 22), (12, 10, 3), 100, dict(nb=8))],
-    "tti": [((36, 30, 28), (18, 15, 4), 140, dict(nb=12, scheme=1)),
-            ((24, 20, 22), (12, 10, 3), 90, dict(nb=12, scheme=3))],
+    "tti": [((36, 30, 28), (18, 15, 4), 140, dict(nb=12, scheme=3)),
+            ((24, 20, 22), (12, 10, 3), 90, dict(nb=12, scheme=4))],
 }
 
 
