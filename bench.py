@@ -428,10 +428,10 @@ def strip_bench(args):
 def batched_strip(args):
     """Shots/hour of the learned-strip batched form against 20-cell-PML per-shot runs: --nwin
     shots (default 8) on the 256^3 VTI study model (d = 10 m, dt = 0.8 ms, nt = --strip-nt),
-    sources at seeded positions, every window the full model. Runs (stress scheme 3 unless
-    noted): the strip batched with the time loop in hipGraphs (mode 2) and as per-shot runs
-    (mode 0); the 20-cell PML as per-shot runs under schemes 3 and 0 (reference formulation)
-    and batched in graphs."""
+    sources at seeded positions, every window the full model. Runs (stress scheme --scheme
+    unless noted): the strip with the committed network (its head and period) batched with
+    the time loop in hipGraphs (mode 2) and as per-shot runs (mode 0); the 20-cell PML as
+    per-shot runs under --scheme and scheme 0 (reference formulation) and batched in graphs."""
     from a3dfd import STRIP_NB, Batch, Window, make_desc, models, strip
     os.environ.setdefault("A3D_PROGRESS", "1")
     n, nt = args.strip_n, args.strip_nt
@@ -441,11 +441,9 @@ def batched_strip(args):
     srcs = [models.strip_source(95000 + k, n) for k in range(nshot)]
     wins = [Window(0, 0, 0, n, n, n, s["sx"], s["sy"], s["sz"]) for s in srcs]
     params, amp = strip.load_params()
-    head = strip.load_head()
+    head, period = strip.load_head(), strip.load_period()
     if params is None:
-        params = strip.closed_params()
-    if head != 0:
-        raise SystemExit("bench.py --batched-strip: batches run the strip's default head mode (0)")
+        params, head, period = strip.closed_params(), 0, 1
     model = dict(vel=m["vel"], rho=m["rho"], eps=m["eps"], delta=m["delta"])
     res = {}
 
@@ -454,25 +452,26 @@ def batched_strip(args):
         with Batch(d, wins) as b:
             b.set_model(**model)
             if net is not None:
-                b.set_strip_net(net, amp=amp)
+                b.set_strip_net(net, amp=amp, head=head, period=period)
             b.tune()
             b.set_graph_chunk(250)
             _, ms = b.run(mode=mode, shots=False)
         res[name] = {"loop_s": round(ms / 1e3, 2), "shots_per_hour": round(nshot * 3600e3 / ms, 1)}
         log(f"bench: {name}: {res[name]}")
 
-    run("strip_batched_graph", STRIP_NB, 3, 2, params)
-    run("strip_per_shot", STRIP_NB, 3, 0, params)
-    run("pml20_per_shot", 20, 3, 0, None)
+    sch = args.scheme
+    run("strip_batched_graph", STRIP_NB, sch, 2, params)
+    run("strip_per_shot", STRIP_NB, sch, 0, params)
+    run("pml20_per_shot", 20, sch, 0, None)
     run("pml20_per_shot_scheme0", 20, 0, 0, None)
-    run("pml20_batched_graph", 20, 3, 2, None)
+    run("pml20_batched_graph", 20, sch, 2, None)
     sph = lambda k: res[k]["shots_per_hour"]   # noqa: E731
     line = {
         "metric": "shots/hour, learned-strip batched form vs 20-cell-PML per-shot runs (VTI 256^3)",
         "value": sph("strip_batched_graph"), "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic seeded VTI study model, sources at seeded positions",
         "config": {"workload": f"vti_{n}^3_nt{nt}_{nshot}shots_strip_batched", "strip_amp": amp,
-                   "fm": round(fm, 2), "scheme": 3},
+                   "fm": round(fm, 2), "scheme": args.scheme, "network": {"head": head, "period": period}},
         "runs": res,
         "speedup_vs_pml20_per_shot": round(sph("strip_batched_graph") / sph("pml20_per_shot"), 3),
         "speedup_vs_pml20_per_shot_reference_formulation": round(sph("strip_batched_graph")
