@@ -2098,7 +2098,10 @@ int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>
     if (b->d.scheme == 2) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs stress schemes 0, 1, 3 and 4");
     for (auto* c : b->ctx)
         if (c->strip_net) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs the PML boundary");
-    const int per_cu = persist_blocks_per_cu(op, ty, dirs, wz);
+    // At most 16 waves per CU (a workgroup is ty waves): the most seen resident together
+    // on MI355X. The occupancy query said 7 workgroups of 4 waves for a 69-VGPR ISO
+    // instance, and a grid of that size did not meet at its first barrier.
+    const int per_cu = std::min(persist_blocks_per_cu(op, ty, dirs, wz), 16 / ty);
     if (per_cu < 1) return fail(A3D_EINVAL, "no persistent kernel for operator %d with %d tile rows", op, ty);
     const unsigned nblocks = (unsigned)(per_cu * b->ctx[0]->ncu) & ~7u;
     if (nblocks < 8) return fail(A3D_EINVAL, "persistent grid too small (%d workgroups per CU)", per_cu);
@@ -2133,13 +2136,12 @@ int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>
     const int steps = std::max(1, std::min(b->persist_steps, nt));
     const size_t nsync = (size_t)PERSIST_LINE * (1 + 2 * (size_t)steps * 8);
     // one device block for everything the kernel reads besides the windows' own buffers
+    std::vector<int> tabs;
+    for (const std::vector<int>* v : {&vb, &sb, &vl, &sl, &vord, &sord}) tabs.insert(tabs.end(), v->begin(), v->end());
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    const size_t o_va = 0, o_sa = o_va + al(nw * sizeof(VelArgs)), o_vb = o_sa + al(nw * sizeof(StressArgs)),
-                 o_sb = o_vb + al(vb.size() * sizeof(int)), o_vl = o_sb + al(sb.size() * sizeof(int)),
-                 o_sl = o_vl + al(vl.size() * sizeof(int)), o_vw = o_sl + al(sl.size() * sizeof(int)),
-                 o_sw = o_vw + al(nw * sizeof(int)), o_src = o_sw + al(nw * sizeof(int)),
-                 o_sync = o_src + al(src_all.size() * sizeof(Src)), o_err = o_sync + al(nsync * sizeof(unsigned)),
-                 total = o_err + al(sizeof(int));
+    const size_t o_va = 0, o_sa = o_va + al(nw * sizeof(VelArgs)), o_tab = o_sa + al(nw * sizeof(StressArgs)),
+                 o_src = o_tab + al(tabs.size() * sizeof(int)), o_sync = o_src + al(src_all.size() * sizeof(Src)),
+                 o_err = o_sync + al(nsync * sizeof(unsigned)), total = o_err + al(sizeof(int));
     unsigned char* mem = nullptr;
     if (hipMalloc(&mem, total) != hipSuccess) return fail(A3D_ENOMEM, "persistent batch argument allocation failed");
     auto done = [&](int code) {
@@ -2153,12 +2155,7 @@ int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>
     } while (0)
     P_TRY(hipMemcpy(mem + o_va, va.data(), nw * sizeof(VelArgs), hipMemcpyHostToDevice));
     P_TRY(hipMemcpy(mem + o_sa, sa.data(), nw * sizeof(StressArgs), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_vb, vb.data(), vb.size() * sizeof(int), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_sb, sb.data(), sb.size() * sizeof(int), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_vl, vl.data(), vl.size() * sizeof(int), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_sl, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_vw, vord.data(), nw * sizeof(int), hipMemcpyHostToDevice));
-    P_TRY(hipMemcpy(mem + o_sw, sord.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+    P_TRY(hipMemcpy(mem + o_tab, tabs.data(), tabs.size() * sizeof(int), hipMemcpyHostToDevice));
     P_TRY(hipMemcpy(mem + o_src, src_all.data(), src_all.size() * sizeof(Src), hipMemcpyHostToDevice));
     P_TRY(hipMemset(mem + o_err, 0, sizeof(int)));
     int khz = 0;
@@ -2166,12 +2163,7 @@ int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>
     PersistArgs p{};
     p.va = (const VelArgs*)(mem + o_va);
     p.sa = (const StressArgs*)(mem + o_sa);
-    p.vbase = (const int*)(mem + o_vb);
-    p.sbase = (const int*)(mem + o_sb);
-    p.vlo = (const int*)(mem + o_vl);
-    p.slo = (const int*)(mem + o_sl);
-    p.vwin = (const int*)(mem + o_vw);
-    p.swin = (const int*)(mem + o_sw);
+    p.tabs = (const int*)(mem + o_tab);
     p.srcs = (const Src*)(mem + o_src);
     p.sync = (unsigned*)(mem + o_sync);
     p.err = (int*)(mem + o_err);

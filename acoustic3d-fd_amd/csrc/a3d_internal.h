@@ -197,12 +197,11 @@ constexpr int PERSIST_LINE = 16;   // unsigned counters per 64-byte line
 struct PersistArgs {
     const VelArgs* va;       // per-window sweep arguments (device), launches for this TY
     const StressArgs* sa;
-    const int* vbase;        // [8][nwin + 1]: per XCD band, prefix sums of the windows' ticket
-    const int* sbase;        //   counts in ticket order (a ticket covers `group` consecutive tiles)
-    const int* vlo;          // [8][nwin + 1]: per XCD band, the window's first tile in the band
-    const int* slo;          //   ([x][nwin]: unused) and, at [8][...] + 8 * (nwin + 1), its end
-    const int* vwin;         // window index of each ticket-order slot
-    const int* swin;
+    const int* tabs;         // per XCD band x: vbase[8][nwin + 1], sbase[8][nwin + 1] (prefix sums of
+                             // the windows' ticket counts in ticket order; a ticket covers `group`
+                             // consecutive tiles), vlo[16][nwin + 1], slo[16][nwin + 1] (rows 0-7:
+                             // each window's first tile in band x, rows 8-15: its end), then
+                             // vwin[nwin], swin[nwin] (the window of each ticket-order slot)
     const Src* srcs;         // per-step sources [nt][nwin] by window index
     unsigned* sync;          // [0] barrier arrivals; the tickets of step it0 + k, sweep s and
                              // band x at PERSIST_LINE * (1 + (2k + s) * 8 + x) (zeroed before

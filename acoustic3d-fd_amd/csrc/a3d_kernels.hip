@@ -1018,8 +1018,8 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const StressArgs* __res
 // window, in ticket order: windows with the longest tile columns first, so the
 // last tiles taken are short ones), and the grid meets at a barrier after each
 // sweep (a stress tile reads its neighbours' velocities, a velocity tile their
-// stresses). Every barrier wait is bounded: past p.wait_ticks the workgroup
-// raises *p.err, and a workgroup that sees *p.err set leaves at its next
+// stresses). Every barrier wait is bounded: past wait_ticks the workgroup
+// raises *err, and a workgroup that sees *err set leaves at its next
 // barrier, so the launch always ends.
 
 // Next ticket of counter c for the whole workgroup. The value is wave-uniform
@@ -1033,29 +1033,30 @@ __device__ __forceinline__ int next_ticket(unsigned* c) {
     return __builtin_amdgcn_readfirstlane(t);
 }
 
-// Grid barrier (arrival counter p.sync[0], never reset within a launch: the k-th
+// Grid barrier (arrival counter sync[0], never reset within a launch: the k-th
 // barrier waits for k * nblocks arrivals). Release: each thread's stores are made
 // visible at device scope before its workgroup arrives; acquire: each thread
 // fences after the wait, so later loads see the other workgroups' stores.
 // Returns false when a wait ran out (here or in another workgroup).
-__device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned& target) {
+__device__ __forceinline__ bool grid_barrier(unsigned* sync, int* err, unsigned nblocks, long long wait_ticks,
+                                             unsigned& target) {
     __shared__ int ok;
     __threadfence();
     __syncthreads();
-    target += p.nblocks;
+    target += nblocks;
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        int good = __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        int good = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
         if (good) {
-            atomicAdd(p.sync, 1u);
+            atomicAdd(sync, 1u);
             const long long t0 = (long long)wall_clock64();
             int spins = 0;   // second bound, independent of the clock: about 2^22 sleeps
-            while (__hip_atomic_load(p.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                if (__hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                     good = 0;
                     break;
                 }
-                if ((long long)wall_clock64() - t0 > p.wait_ticks || ++spins > (1 << 22)) {
-                    atomicExch(p.err, 1);
+                if ((long long)wall_clock64() - t0 > wait_ticks || ++spins > (1 << 22)) {
+                    atomicExch(err, 1);
                     good = 0;
                     break;
                 }
@@ -1069,51 +1070,59 @@ __device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned& tar
     return ok != 0;
 }
 
+// The window arguments and tables come in as separate __restrict__ kernel parameters, not
+// as fields of one argument struct: the compiler may then take them as unwritten by the
+// kernel and load them with scalar loads after the sweeps' stores (through a struct field
+// they become vector loads held in vector registers: 125-134 instead of 97-105 VGPRs for
+// TTI).
 template <class PR, int OP, int TY, int DIRS, bool WZ>
-__global__ __launch_bounds__(TX* TY) void k_persist(const PersistArgs p, int it0, int it1) {
+__global__ __launch_bounds__(TX* TY) void k_persist(const VelArgs* __restrict__ va, const StressArgs* __restrict__ sa,
+                                                   const int* __restrict__ tabs, const Src* __restrict__ srcs,
+                                                   unsigned* sync, int* err, int nwin, int group, unsigned nblocks,
+                                                   long long wait_ticks, int it0, int it1) {
     unsigned target = 0;
     const int x = (int)(blockIdx.x & 7u);
-    const int n1 = p.nwin + 1;
-    const int* vb = p.vbase + x * n1;
-    const int* sb = p.sbase + x * n1;
-    const int* vlo = p.vlo + x * n1;
-    const int* slo = p.slo + x * n1;
-    const int* vhi = vlo + 8 * n1;
-    const int* shi = slo + 8 * n1;
-    const int ntv = vb[p.nwin], nts = sb[p.nwin];
+    const int n1 = nwin + 1;
+    // tabs: vbase[8][n1], sbase[8][n1], vlo[16][n1], slo[16][n1], vwin[nwin], swin[nwin]
+    const int* vb = tabs + x * n1;
+    const int* sb = tabs + 8 * n1 + x * n1;
+    const int* vlo = tabs + 16 * n1 + x * n1;
+    const int* slo = tabs + 32 * n1 + x * n1;
+    const int* vwin = tabs + 48 * n1;
+    const int* swin = vwin + nwin;
     for (int it = it0; it < it1; ++it) {
         const int k = it - it0;
         for (;;) {
-            const int t = next_ticket(p.sync + PERSIST_LINE * (1 + (2 * k) * 8 + x));
-            if (t >= ntv) break;
-            const int j = find_window(vb, p.nwin, t);
-            const int w = p.vwin[j];
-            const VelArgs a = p.va[w];
-            const Src s0 = p.srcs[(size_t)it * p.nwin + w];
-            const int t0 = vlo[j] + (t - vb[j]) * p.group, t1 = min(t0 + p.group, vhi[j]);
+            const int t = next_ticket(sync + PERSIST_LINE * (1 + (2 * k) * 8 + x));
+            if (t >= vb[nwin]) break;
+            const int j = find_window(vb, nwin, t);
+            const int w = vwin[j];
+            const VelArgs a = va[w];
+            const Src s0 = srcs[(size_t)it * nwin + w];
+            const int t0 = vlo[j] + (t - vb[j]) * group, t1 = min(t0 + group, vlo[8 * n1 + j]);
             for (int tt = t0; tt < t1; ++tt) {
                 int tix, tiy, seg;
                 split_tile(a.l, tt, tix, tiy, seg);
                 vel_body<PR, OP, TY, WZ, false>(a, s0, tix, tiy, seg);
             }
         }
-        if (!grid_barrier(p, target)) return;
+        if (!grid_barrier(sync, err, nblocks, wait_ticks, target)) return;
         for (;;) {
-            const int t = next_ticket(p.sync + PERSIST_LINE * (1 + (2 * k + 1) * 8 + x));
-            if (t >= nts) break;
-            const int j = find_window(sb, p.nwin, t);
-            const int w = p.swin[j];
-            StressArgs a = p.sa[w];
-            if constexpr (WZ) a.s = p.srcs[(size_t)it * p.nwin + w];
+            const int t = next_ticket(sync + PERSIST_LINE * (1 + (2 * k + 1) * 8 + x));
+            if (t >= sb[nwin]) break;
+            const int j = find_window(sb, nwin, t);
+            const int w = swin[j];
+            StressArgs a = sa[w];
+            if constexpr (WZ) a.s = srcs[(size_t)it * nwin + w];
             float* rec = a.rec ? a.rec + (size_t)it * a.nx * a.ny : nullptr;
-            const int t0 = slo[j] + (t - sb[j]) * p.group, t1 = min(t0 + p.group, shi[j]);
+            const int t0 = slo[j] + (t - sb[j]) * group, t1 = min(t0 + group, slo[8 * n1 + j]);
             for (int tt = t0; tt < t1; ++tt) {
                 int tix, tiy, seg;
                 split_tile(a.l, tt, tix, tiy, seg);
                 stress_body<PR, OP, TY, WZ, DIRS, false>(a, rec, tix, tiy, seg);
             }
         }
-        if (!grid_barrier(p, target)) return;
+        if (!grid_barrier(sync, err, nblocks, wait_ticks, target)) return;
     }
 }
 
@@ -1577,7 +1586,8 @@ hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st) {
 
 namespace {
 // The persistent kernel instance of (op, ty, dirs, wz), FP32 storage; nullptr if none.
-typedef void (*PersistKern)(PersistArgs, int, int);
+typedef void (*PersistKern)(const VelArgs*, const StressArgs*, const int*, const Src*, unsigned*, int*, int, int,
+                            unsigned, long long, int, int);
 template <int TY>
 PersistKern persist_kern_ty(int op, int dirs, bool wz) {
     switch (op) {
@@ -1620,7 +1630,8 @@ int persist_blocks_per_cu(int op, int ty, int dirs, bool wz) {
 hipError_t launch_persist(int op, int ty, int dirs, bool wz, const PersistArgs& p, int it0, int it1, hipStream_t st) {
     const PersistKern k = persist_kern(op, ty, dirs, wz);
     if (!k || p.nblocks == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(TX, ty), 0, st, p, it0, it1);
+    hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(TX, ty), 0, st, p.va, p.sa, p.tabs, p.srcs, p.sync, p.err, p.nwin,
+                       p.group, p.nblocks, p.wait_ticks, it0, it1);
     return hipGetLastError();
 }
 
