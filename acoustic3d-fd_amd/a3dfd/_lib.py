@@ -122,6 +122,7 @@ def load(path=None):
     lib.a3d_batch_info.argtypes = [vp, POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]
     lib.a3d_batch_set_graph_chunk.argtypes = [vp, c_int]
     lib.a3d_batch_set_persist.argtypes = [vp, c_int, c_int, c_int]
+    lib.a3d_batch_set_tti_angles.argtypes = [vp, c_int]
     lib.a3d_batch_set_zseg.argtypes = [vp, c_int, c_int]
     lib.a3d_batch_set_strip_net.argtypes = [vp, fp, c_int]
     lib.a3d_batch_set_w_sweep.argtypes = [vp, c_int]
@@ -148,6 +149,7 @@ def load(path=None):
     lib.a3d_set_graph.argtypes = [vp, c_int]
     lib.a3d_set_graph.restype = c_int
     lib.a3d_set_w_sweep.argtypes = [vp, c_int]
+    lib.a3d_set_tti_angles.argtypes = [vp, c_int]
     lib.a3d_get_w_sweep.argtypes = [vp, POINTER(ShotDesc), POINTER(c_int)]
     lib.a3d_strip_collect.argtypes = [vp, vp, POINTER(ShotDesc), c_int, c_int, c_int, c_float, ctypes.c_uint, fp, fp,
                                       fp, c_int, POINTER(c_int)]
@@ -172,6 +174,8 @@ def load(path=None):
     lib.a3d_probe_bandwidth.restype = c_int
     lib.a3d_probe_stress_pattern.argtypes = [c_int, c_int, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]
     lib.a3d_probe_stress_pattern.restype = c_int
+    lib.a3d_probe_rot_sincos.argtypes = [c_int, fp, fp, fp, c_int]
+    lib.a3d_probe_rot_sincos.restype = c_int
     lib.a3d_last_error.restype = c_char_p
     lib.a3d_version.restype = c_char_p
     for name in ("a3d_create", "a3d_destroy", "a3d_set_model", "a3d_run", "a3d_set_kernel_timing",
@@ -200,6 +204,16 @@ def probe_stress_pattern(nnx, nny, nnz, device=0, reps=20):
     ms, nb = (c_double * 5)(), (c_double * 5)()
     _check(lib, lib.a3d_probe_stress_pattern(device, nnx, nny, nnz, reps, ms, nb))
     return {k: (ms[i], nb[i] / (ms[i] * 1e-3) / 1e9) for i, k in enumerate(STRESS_PATTERNS)}
+
+
+def probe_rot_sincos(t, device=0):
+    """(sin t, cos t) as the TTI sweeps evaluate them from the angles (float32 arrays)."""
+    lib = load()
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    s, c = np.empty_like(t), np.empty_like(t)
+    f = lambda a: a.ctypes.data_as(POINTER(c_float))
+    _check(lib, lib.a3d_probe_rot_sincos(device, f(t), f(s), f(c), int(t.size)))
+    return s, c
 
 
 class A3DError(RuntimeError):
@@ -285,6 +299,12 @@ class Modeller:
     def set_graph(self, steps):
         """Time loop as hipGraphs of at most `steps` steps (0: host launches)."""
         _check(self.lib, self.lib.a3d_set_graph(self._h, int(steps)))
+
+    def set_tti_angles(self, on):
+        """TTI (FP32/FP16 storage): the sweeps read the tilt and azimuth (8 bytes per point)
+        and evaluate the rotation terms in FP32 (1), or read the terms built once per model
+        (0, default; bitwise the oracle's arithmetic)."""
+        _check(self.lib, self.lib.a3d_set_tti_angles(self._h, int(on)))
 
     def set_w_sweep(self, mode):
         """Which sweep updates w (ISO, VTI): 0 velocity sweep, 1 stress sweep (bit-identical,
@@ -489,6 +509,11 @@ class Batch:
         """Mode 3: tile rows of the persistent kernel (4 or 8), time steps per launch and
         consecutive tiles per ticket."""
         _check(self.lib, self.lib.a3d_batch_set_persist(self._h, int(tile_rows), int(steps), int(group)))
+
+    def set_tti_angles(self, on):
+        """TTI: every window's sweeps evaluate the rotation terms from the angles (1) or
+        read them (0, default)."""
+        _check(self.lib, self.lib.a3d_batch_set_tti_angles(self._h, int(on)))
 
     def info(self):
         nw = len(self.windows)

@@ -239,6 +239,9 @@ struct a3d_ctx {
     int* flag = nullptr;
     bool model_set = false;
     bool m4_ok = false;         // V_M4 holds the current model
+    int tti_angles = 0;         // TTI (float model): 1 the sweeps evaluate the rotation terms from
+                                // the angles (8 bytes per point instead of 16), 0 read them
+    bool m4_ang = false;        // V_M4 holds the angles (TTI, tti_angles) rather than the terms
     float vel_first = 0.f, eps_first = 0.f, vel_d0 = 0.f, vel_raw1 = 0.f;
     float zmin = 1.0f;          // minimum acoustic impedance rho*v (binary16 velocity-field scale)
     // launch shape per sweep: [0] 2nd-order step / velocity sweep, [1] stress sweep
@@ -294,7 +297,8 @@ Launch make_launch(const Geo& g, int ty, int zseg, int z0 = MM, int z1 = -1) {
 int single_wave_zseg(const a3d_ctx* c, int kind, int ty) {
     const Launch l1 = make_launch(c->g, ty, 1);
     const int per_plane = l1.ntx * l1.nty;
-    const int cap = std::max(1, max_blocks_per_cu(c->prec, kind, c->d.op, ty)) * c->ncu;
+    const int kop = c->d.op == A3D_OP_TTI && c->tti_angles && c->prec != PREC_FP64 ? 4 : c->d.op;
+    const int cap = std::max(1, max_blocks_per_cu(c->prec, kind, kop, ty)) * c->ncu;
     const int nzi = c->g.nnz - 2 * MM;
     int nseg = std::max(1, cap / std::max(per_plane, 1));
     nseg = std::min(nseg, std::max(1, nzi / 16));
@@ -334,6 +338,12 @@ int check_desc_geometry(const a3d_shot_desc* d) {
 // the batched stress sweeps' split mode of a stress scheme (3: directional split, 4: one
 // field per damped direction)
 int dirs_mode(int scheme) { return scheme == 4 ? 2 : scheme == 3 ? 1 : 0; }
+
+// Kernel op code of a context's sweeps: the operator, or 4 for TTI whose sweeps evaluate
+// the rotation terms from the angles (float model).
+int kernel_op(const a3d_ctx* c) {
+    return c->d.op == A3D_OP_TTI && c->tti_angles && c->prec != PREC_FP64 ? 4 : c->d.op;
+}
 
 int check_desc_shot(const a3d_ctx* c, const a3d_shot_desc* d) {
     if (!same_geometry(c->d, *d)) return fail(A3D_EINVAL, "shot geometry differs from the context's");
@@ -638,8 +648,13 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
             c->vol[V_M4] = nullptr;
             return fail(A3D_ENOMEM, "interleaved model: allocation failed");
         }
-        if (!c->m4_ok) {
-            if (op == A3D_OP_VTI)
+        const bool ang = kernel_op(c) == 4;
+        if (!c->m4_ok || c->m4_ang != ang) {
+            if (ang)
+                HIP_TRY(launch_pack_angles((const float*)c->vol[V_CT], (const float*)c->vol[V_ST],
+                                           (const float*)c->vol[V_CP], (const float*)c->vol[V_SP], c->vol[V_M4],
+                                           c->nvox, c->st));
+            else if (op == A3D_OP_VTI)
                 HIP_TRY(launch_pack4((const float*)c->vol[V_VP], (const float*)c->vol[V_RHO],
                                      (const float*)c->vol[V_EPS], (const float*)c->vol[V_DEL], c->vol[V_M4], c->nvox,
                                      c->st));
@@ -651,6 +666,7 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
                                      (const float*)c->vol[V_CP], (const float*)c->vol[V_SP], c->vol[V_M4], c->nvox,
                                      c->st));
             c->m4_ok = true;
+            c->m4_ang = ang;
         }
         sa.m4 = c->vol[V_M4];
         if (op == A3D_OP_TTI) va.t4 = c->vol[V_M4];
@@ -832,7 +848,7 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
         StressArgs sa;
         std::vector<Src> srcs;
         if ((rc = prepare_first<T>(c, d, sc, va, sa, srcs))) return rc;
-        const int op = d->op;
+        const int kop = kernel_op(c);
         const bool strip = c->strip_net != nullptr;
         const StripArgs stp = strip_args(c, d);
         if (d->scheme == 2) {
@@ -933,22 +949,22 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
                     if (k < nch) {
                         vch[k].s = srcs[it];
                         timing_begin(c, 0);
-                        HIP_TRY(launch_vel(prec, op, c->ty[0], vch[k], c->st));
+                        HIP_TRY(launch_vel(prec, kop, c->ty[0], vch[k], c->st));
                         timing_end(c);
                     }
                     if (k >= 1) {
                         sch[k - 1].rec = sa.rec;
                         timing_begin(c, 1);
-                        HIP_TRY(launch_stress(prec, op, c->ty[1], sch[k - 1], c->st));
+                        HIP_TRY(launch_stress(prec, kop, c->ty[1], sch[k - 1], c->st));
                         timing_end(c);
                     }
                 }
             } else {
                 timing_begin(c, 0);
-                HIP_TRY(launch_vel(prec, op, c->ty[0], va, c->st));
+                HIP_TRY(launch_vel(prec, kop, c->ty[0], va, c->st));
                 timing_end(c);
                 timing_begin(c, 1);
-                HIP_TRY(launch_stress(prec, op, c->ty[1], sa, c->st));
+                HIP_TRY(launch_stress(prec, kop, c->ty[1], sa, c->st));
                 timing_end(c);
             }
             if (strip && (rc = strip_step(c, stp, it))) return rc;
@@ -1038,7 +1054,7 @@ std::vector<std::pair<TuneKey, TuneVal>> g_tune_cache;
 std::mutex g_tune_mu;
 
 int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
-    const TuneKey key{c->device, c->d.op, c->prec, d->scheme * 4 + (c->wz_mode + 1), c->zchunk,
+    const TuneKey key{c->device, kernel_op(c), c->prec, d->scheme * 4 + (c->wz_mode + 1), c->zchunk,
                       c->g.nnx, c->g.nny, c->g.nnz};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1541,6 +1557,15 @@ int a3d_set_strip_net(a3d_ctx* c, const float* params, int n) {
         return fail(A3D_ENOMEM, "strip network allocation failed");
     }
     HIP_TRY(hipMemcpy(c->strip_net, params, LB_NPAR * sizeof(float), hipMemcpyHostToDevice));
+    return A3D_OK;
+}
+
+int a3d_set_tti_angles(a3d_ctx* c, int on) {
+    if (!c) return fail(A3D_EINVAL, "null context");
+    if (on != 0 && on != 1) return fail(A3D_EINVAL, "TTI angle mode must be 0 or 1");
+    if (on && (c->d.op != A3D_OP_TTI || c->prec == PREC_FP64))
+        return fail(A3D_EINVAL, "rotation terms from the angles: TTI with FP32 or FP16 storage only");
+    c->tti_angles = on;
     return A3D_OK;
 }
 
@@ -2092,7 +2117,7 @@ int batch_outputs(a3d_batch* b, float** shots) {
 // updated by the same code as in the other modes, and the barriers order the sweeps.
 int persist_loop(a3d_batch* b, std::vector<VelArgs>& va, std::vector<StressArgs>& sa, const std::vector<Src>& src_all,
                  bool wz, double* loop_ms) {
-    const int nw = (int)b->ctx.size(), nt = b->d.nt, op = b->d.op, ty = b->persist_ty;
+    const int nw = (int)b->ctx.size(), nt = b->d.nt, op = kernel_op(b->ctx[0]), ty = b->persist_ty;
     const int dirs = dirs_mode(b->d.scheme);
     if (b->ctx[0]->prec != PREC_FP32) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs FP32 storage");
     if (b->d.scheme == 2) return fail(A3D_EINVAL, "the persistent batch (mode 3) runs stress schemes 0, 1, 3 and 4");
@@ -2307,7 +2332,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
         for (size_t k = 0; k < gv.size(); ++k) {
             const BatchGroup& g = gv[k];
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k], b->st));
-            HIP_TRY(launch_vel_batch(b->ctx[0]->prec, op, g.ty, (const VelArgs*)g.args, g.base, (int)g.wins.size(),
+            HIP_TRY(launch_vel_batch(b->ctx[0]->prec, kernel_op(b->ctx[0]), g.ty, (const VelArgs*)g.args, g.base, (int)g.wins.size(),
                                      g.lb, g.src, it, wz, b->st, sm));
             if (timed) HIP_TRY(hipEventRecord(gev[2 * k + 1], b->st));
         }
@@ -2315,7 +2340,7 @@ int batch_run_t(a3d_batch* b, int mode, float** shots, double* loop_ms, double* 
             const BatchGroup& g = gs[k];
             const size_t e = 2 * (gv.size() + k);
             if (timed) HIP_TRY(hipEventRecord(gev[e], b->st));
-            HIP_TRY(launch_stress_batch(b->ctx[0]->prec, op, g.ty, (const StressArgs*)g.args, g.base,
+            HIP_TRY(launch_stress_batch(b->ctx[0]->prec, kernel_op(b->ctx[0]), g.ty, (const StressArgs*)g.args, g.base,
                                         (int)g.wins.size(), g.lb, it, dirs_mode(b->d.scheme), wz, g.src, b->st, sm));
             if (timed) HIP_TRY(hipEventRecord(gev[e + 1], b->st));
         }
@@ -2529,6 +2554,14 @@ int a3d_batch_set_persist(a3d_batch* b, int tile_rows, int steps, int group) {
     b->persist_ty = tile_rows;
     b->persist_steps = steps;
     b->persist_group = group;
+    return A3D_OK;
+}
+
+int a3d_batch_set_tti_angles(a3d_batch* b, int on) {
+    if (!b) return fail(A3D_EINVAL, "null batch");
+    for (auto* c : b->ctx)
+        if (int rc = a3d_set_tti_angles(c, on)) return rc;
+    drop_graph(b);   // a kept graph holds the other kernels
     return A3D_OK;
 }
 

@@ -238,6 +238,10 @@ hipError_t launch_pad(const float* src, int nx, int ny, int nz, int off, const G
 // each (M)cos((double)angle) with M float or double (model_double).
 hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int model_double, hipStream_t s);
 // (vp, rho, eps, delta) of a float model interleaved into one float4 per point.
+// TTI: tilt and azimuth per point as one float2, atan2 of the float rotation terms in
+// double (the angles to within the terms' rounding), for sweeps that evaluate the terms.
+hipError_t launch_pack_angles(const float* ct, const float* st, const float* cp, const float* sp, void* out, size_t n,
+                              hipStream_t s);
 hipError_t launch_pack4(const float* vp, const float* rho, const float* eps, const float* del, void* out, size_t n,
                         hipStream_t s);
 // (vp, rho) of a float model interleaved into one float2 per point.

@@ -265,7 +265,8 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     using S = typename PR::S;
     using M = typename PR::M;
     using C = typename PR::C;
-    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
+    // OP 4: TTI with the rotation terms evaluated here from the angles (float model)
+    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3 || OP == 4, ANG = OP == 4;
     static_assert(!(WZ && TTI), "TTI's w needs x-y differences of Q");
     constexpr bool P_Z = !VTI && !WZ;   // P differentiated along z (ISO: all axes; TTI: all axes)
     constexpr bool HAS_Q = !ISO && !WZ;
@@ -298,9 +299,12 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
     const M* St = (const M*)a.st;
     const M* Cp = (const M*)a.cp;
     const M* Sp = (const M*)a.sp;
-    // TTI with a float model: the four rotation terms as one 16-byte load per point
-    constexpr bool PTRIG = TTI && sizeof(M) == 4;
+    // TTI with a float model: the four rotation terms as one 16-byte load per point, or
+    // (ANG) the tilt and azimuth as one 8-byte load, their sines and cosines computed here
+    static_assert(!ANG || sizeof(M) == 4, "angles in the sweep: float model");
+    constexpr bool PTRIG = TTI && sizeof(M) == 4 && !ANG;
     const f32x4* T4 = (const f32x4*)a.t4;
+    const f32x2* A2 = (const f32x2*)a.t4;
     const C* T = (const C*)a.tab;
 
     auto ldP = [&](int zo, int o) {
@@ -349,6 +353,7 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
         if constexpr (P_Z) wP[7] = ldP(zo + 4 * g.pl, col);
         if constexpr (HAS_Q) wQ[7] = ldQ(zo + 4 * g.pl, col);
         C r0 = 1, ct = 0, st = 0, cp = 0, sp = 0;
+        f32x2 an;
         const C u0 = ldw<NT>(U, zo, colc, sc.s[F_U]);
         const C v0 = ldw<NT>(V, zo, colc, sc.s[F_V]);
         C w0 = 0;
@@ -360,6 +365,8 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
             st = t4.y;
             cp = t4.z;
             sp = t4.w;
+        } else if constexpr (ANG) {
+            an = bld<NT>(A2, zo, colc);
         } else if constexpr (TTI) {
             ct = (C)bld<NT>(Ct, zo, colc);
             st = (C)bld<NT>(St, zo, colc);
@@ -407,6 +414,10 @@ __device__ __forceinline__ void vel_body(const VelArgs& a, const Src S0, int tix
                 const C Px = dst8<C>([&](int k) { return sP[r][c - 3 + k]; });
                 const C Qy = dst8<C>([&](int k) { return sQ[r - 3 + k][c]; });
                 const C Qx = dst8<C>([&](int k) { return sQ[r][c - 3 + k]; });
+                if constexpr (ANG) {
+                    rot_sincos(an.x, st, ct);
+                    rot_sincos(an.y, sp, cp);
+                }
                 xx = ct * cp * Px - ct * sp * Py - st * Pz;
                 yy = sp * Px + cp * Py;
                 zz = st * cp * Qx - st * sp * Qy + ct * Qz;
@@ -463,8 +474,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     using SP = typename PR::SP;
     using M = typename PR::M;
     using C = typename PR::C;
-    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3;
+    // OP 4: TTI with the rotation terms evaluated here from the angles (float model)
+    constexpr bool ISO = OP == 1, VTI = OP == 2, TTI = OP == 3 || OP == 4, ANG = OP == 4;
     static_assert(!(WZ && TTI), "TTI's w needs x-y differences of Q");
+    static_assert(!ANG || sizeof(M) == 4, "angles in the sweep: float model");
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
     __shared__ C sU[H][W];
     __shared__ C sV[H][W];
@@ -517,9 +530,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     // (one stream instead of four; the host always provides it); WZ keeps the
     // planes z..z+3 in mq (fr is the density FIFO otherwise)
     constexpr bool PACKED = VTI && sizeof(M) == 4;
-    // TTI with a float model: the four rotation terms as one 16-byte load per point;
+    // TTI with a float model: the four rotation terms as one 16-byte load per point, or
+    // (ANG) the tilt and azimuth as one 8-byte load, their sines and cosines computed here;
     // ISO with a float model and w in the velocity sweep: (vp, rho) as one 8-byte load
-    constexpr bool PTRIG = TTI && sizeof(M) == 4;
+    constexpr bool PTRIG = TTI && sizeof(M) == 4 && !ANG;
     constexpr bool PACK2 = ISO && sizeof(M) == 4 && !WZ;
     const f32x4* m4 = (const f32x4*)a.m4;
     const f32x2* m2 = (const f32x2*)a.m4;
@@ -639,7 +653,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
     struct Plane {
         C Fl, wl, ul, vl, r3, czw1, czw2, az1, az2;
         f32x4 m3, m, t;
-        f32x2 mm;
+        f32x2 mm, an;
         C c0, r0, e0, d0, ct, st, cp, sp, uc, vc;
         C f0, f1, f2, f3, f4, f5, p0, q0, mc, mw;
         Halo<TY, true, TTI, C> hU;
@@ -680,6 +694,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
         }
         if constexpr (PTRIG) {
             L.t = bld4<NT>(m4, zo, colc);
+        } else if constexpr (ANG) {
+            L.an = bld<NT>(m2, zo, colc);
         } else if constexpr (TTI) {
             L.ct = (C)bld<NT>(Ct, zo, colc);
             L.st = (C)bld<NT>(St, zo, colc);
@@ -739,6 +755,7 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             wV[7] = L.vl;
         }
         C c0 = 0, r0 = 0, e0 = 0, d0 = 0, ct = 0, st = 0, cp = 0, sp = 0;
+        f32x2 an;
         if constexpr (PACKED) {
             const f32x4 m = WZ ? mq[0] : L.m;
             c0 = (C)m.x;
@@ -762,6 +779,8 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
             st = L.t.y;
             cp = L.t.z;
             sp = L.t.w;
+        } else if constexpr (ANG) {
+            an = L.an;
         } else if constexpr (TTI) {
             ct = L.ct;
             st = L.st;
@@ -883,6 +902,10 @@ __device__ __forceinline__ void stress_body(const StressArgs& a, float* rec, int
                     const C wy = dst8<C>([&](int k) { return sW[r - 4 + k][c]; });
                     const C wx = dst8<C>([&](int k) { return sW[r][c - 4 + k]; });
                     const C wz = Wz;
+                    if constexpr (ANG) {
+                        rot_sincos(an.x, st, ct);
+                        rot_sincos(an.y, sp, cp);
+                    }
                     xx = ct * cp * ux - ct * sp * uy - st * uz;
                     yy = sp * vx + cp * vy;
                     zz = st * cp * wx - st * sp * wy + ct * wz;
@@ -1294,6 +1317,13 @@ __global__ void k_trig(M* __restrict__ th, M* __restrict__ st, M* __restrict__ p
     }
 }
 
+__global__ void k_pack_angles(const float* __restrict__ ct, const float* __restrict__ st,
+                              const float* __restrict__ cp, const float* __restrict__ sp, f32x2* __restrict__ out,
+                              size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f32x2{(float)atan2((double)st[i], (double)ct[i]), (float)atan2((double)sp[i], (double)cp[i])};
+}
+
 __global__ void k_pack4(const float* __restrict__ vp, const float* __restrict__ rho, const float* __restrict__ eps,
                         const float* __restrict__ del, f32x4* __restrict__ out, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1409,6 +1439,10 @@ hipError_t vel_batch_op(int op, int ty, const VelArgs* w, const int* base, int n
         case 2: return wz ? vel_batch_ty<PR, 2, true>(ty, w, base, nwin, lb, srcs, it, st)
                           : vel_batch_ty<PR, 2, false>(ty, w, base, nwin, lb, srcs, it, st);
         case 3: return wz ? hipErrorInvalidValue : vel_batch_ty<PR, 3, false>(ty, w, base, nwin, lb, srcs, it, st);
+        case 4:
+            if constexpr (sizeof(typename PR::M) == 4)
+                return wz ? hipErrorInvalidValue : vel_batch_ty<PR, 4, false>(ty, w, base, nwin, lb, srcs, it, st);
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -1438,6 +1472,14 @@ hipError_t stress_batch_op(int op, int ty, const StressArgs* w, const int* base,
             return dirs == 2 ? stress_batch_ty<PR, 3, 2, false>(ty, w, base, nwin, lb, it, srcs, st)
                  : dirs      ? stress_batch_ty<PR, 3, 1, false>(ty, w, base, nwin, lb, it, srcs, st)
                              : stress_batch_ty<PR, 3, 0, false>(ty, w, base, nwin, lb, it, srcs, st);
+        case 4:
+            if constexpr (sizeof(typename PR::M) == 4) {
+                if (wz) return hipErrorInvalidValue;
+                return dirs == 2 ? stress_batch_ty<PR, 4, 2, false>(ty, w, base, nwin, lb, it, srcs, st)
+                     : dirs      ? stress_batch_ty<PR, 4, 1, false>(ty, w, base, nwin, lb, it, srcs, st)
+                                 : stress_batch_ty<PR, 4, 0, false>(ty, w, base, nwin, lb, it, srcs, st);
+            }
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -1448,6 +1490,9 @@ hipError_t vel_op(int op, int ty, const VelArgs& a, hipStream_t st) {
         case 1: return a.wz ? vel_ty<PR, 1, true>(ty, a, st) : vel_ty<PR, 1, false>(ty, a, st);
         case 2: return a.wz ? vel_ty<PR, 2, true>(ty, a, st) : vel_ty<PR, 2, false>(ty, a, st);
         case 3: return a.wz ? hipErrorInvalidValue : vel_ty<PR, 3, false>(ty, a, st);
+        case 4:
+            if constexpr (sizeof(typename PR::M) == 4) return a.wz ? hipErrorInvalidValue : vel_ty<PR, 4, false>(ty, a, st);
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -1469,6 +1514,14 @@ hipError_t stress_op(int op, int ty, const StressArgs& a, hipStream_t st) {
             return a.unsplit == 3   ? stress_ty<PR, 3, false, 2>(ty, a, st)
                    : a.unsplit == 2 ? stress_ty<PR, 3, false, 1>(ty, a, st)
                                     : stress_ty<PR, 3, false, 0>(ty, a, st);
+        case 4:
+            if constexpr (sizeof(typename PR::M) == 4) {
+                if (a.wz) return hipErrorInvalidValue;
+                return a.unsplit == 3   ? stress_ty<PR, 4, false, 2>(ty, a, st)
+                       : a.unsplit == 2 ? stress_ty<PR, 4, false, 1>(ty, a, st)
+                                        : stress_ty<PR, 4, false, 0>(ty, a, st);
+            }
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -1507,6 +1560,9 @@ int occ_prec(int kind, int op, int ty) {
             }
         case 1: return occ_op<PR, 1>(kind, ty);
         case 2: return occ_op<PR, 2>(kind, ty);
+        case 4:
+            if constexpr (sizeof(typename PR::M) == 4) return occ_op<PR, 4>(kind, ty);
+            return occ_op<PR, 3>(kind, ty);
         default: return occ_op<PR, 3>(kind, ty);
     }
 }
@@ -1607,6 +1663,11 @@ PersistKern persist_kern_ty(int op, int dirs, bool wz) {
             return dirs == 2 ? k_persist<Fp32, 3, TY, 2, false>
                    : dirs    ? k_persist<Fp32, 3, TY, 1, false>
                              : k_persist<Fp32, 3, TY, 0, false>;
+        case 4:
+            if (wz) return nullptr;
+            return dirs == 2 ? k_persist<Fp32, 4, TY, 2, false>
+                   : dirs    ? k_persist<Fp32, 4, TY, 1, false>
+                             : k_persist<Fp32, 4, TY, 0, false>;
         default: return nullptr;
     }
 }
@@ -1702,6 +1763,13 @@ hipError_t launch_trig(void* th, void* st, void* ph, void* sp, size_t n, int mod
         k_trig<double><<<grid, 256, 0, s>>>((double*)th, (double*)st, (double*)ph, (double*)sp, n);
     else
         k_trig<float><<<grid, 256, 0, s>>>((float*)th, (float*)st, (float*)ph, (float*)sp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_angles(const float* ct, const float* st, const float* cp, const float* sp, void* out, size_t n,
+                              hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_pack_angles, dim3(grid), dim3(256), 0, s, ct, st, cp, sp, (f32x2*)out, n);
     return hipGetLastError();
 }
 

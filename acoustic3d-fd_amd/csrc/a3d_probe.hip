@@ -103,10 +103,33 @@ hipError_t launch_probe_pattern(int pat, const ProbeArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+__global__ void k_probe_sincos(const float* __restrict__ t, float* __restrict__ s, float* __restrict__ c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rot_sincos(t[i], s[i], c[i]);
+}
+
 }  // namespace
 }  // namespace a3d
 
 using namespace a3d;
+
+extern "C" int a3d_probe_rot_sincos(int device, const float* t, float* s, float* c, int n) {
+    if (!t || !s || !c || n < 1) return report_error(A3D_EINVAL, "a3d_probe_rot_sincos", "bad argument");
+    if (hipSetDevice(device) != hipSuccess) return report_error(A3D_EHIP, "a3d_probe_rot_sincos", "hipSetDevice");
+    float* d = nullptr;
+    if (hipMalloc(&d, 3 * (size_t)n * sizeof(float)) != hipSuccess)
+        return report_error(A3D_ENOMEM, "a3d_probe_rot_sincos", "allocation failed");
+    hipError_t e = hipMemcpy(d, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_probe_sincos, dim3((n + 255) / 256), dim3(256), 0, 0, d, d + n, d + 2 * (size_t)n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(s, d + n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c, d + 2 * (size_t)n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return report_error(A3D_EHIP, "a3d_probe_rot_sincos", hipGetErrorString(e));
+    return A3D_OK;
+}
 
 extern "C" int a3d_probe_stress_pattern(int device, int nnx, int nny, int nnz, int reps, double* ms,
                                         double* bytes) {

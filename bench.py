@@ -217,6 +217,7 @@ def batched(args):
     res = {}
     with Batch(desc, [Window(*w) for w in wins]) as b:
         b.set_model(vel=cfg["vel"], eps=cfg["eps"], delta=cfg["delta"], theta=cfg["theta"], phi=cfg["phi"])
+        b.set_tti_angles(args.tti_angles)
         t1 = time.time()
         ncls = b.tune()
         info = b.info()
@@ -240,6 +241,7 @@ def batched(args):
         "value": None, "unit": "shots/h", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic seeded TTI model (layered + noise, smooth eps/delta/tilt/azimuth)",
         "config": {"workload": f"tti_600x600x300_{len(wins)}win_nt{nt}", "npml": 40, "nt": nt, "scheme": args.scheme,
+                   "tti_angles": args.tti_angles,
                    "windows": "uniform in [160..320]^2 x [200..300] cells, seed 4",
                    "updated_points_per_step": info["updated_points"], "size_classes": ncls,
                    "z_segments": list(info["zseg"])},
@@ -551,15 +553,26 @@ def operators(args):
     c4 = models.cfg4_model()
     c4.update(sx=301, sy=301, sz=1)
     runs += [("tti", c4, {}, s, 500) for s in (0, 1, 3, 4)]
+    runs.append(("tti", c4, {"tti_angles": 1}, 4, 500))   # rotation terms from the angles in the sweeps
     if args.ops:
         runs = [r for r in runs if r[0] in args.ops.split(",")]
     for op, c, kw, scheme, nt in runs:
         keys = {"2nd": ("vel",), "iso": ("vel", "rho"), "vti": ("vel", "rho", "eps", "delta"),
                 "tti": ("vel", "eps", "delta", "theta", "phi")}[op]
+        kw = dict(kw)
+        angles = kw.pop("tti_angles", 0)
         d = make_desc(op, c["nx"], c["ny"], c["nz"], c["dx"], c["dy"], c["dz"], c["sx"], c["sy"], c["sz"], nt,
                       c["dt"], c["fm"], scheme=scheme, **kw)
         with Modeller(d) as mdl:
             mdl.set_model(**{k: c[k] for k in keys})
+            err_terms = None
+            if angles:
+                # the record against the run that reads the terms (same scheme), then the angles
+                ref_rec, _, _ = mdl.run()
+                mdl.set_tti_angles(1)
+                rec, _, _ = mdl.run()
+                err_terms = float(np.sqrt(((rec.astype(np.float64) - ref_rec) ** 2).sum()
+                                          / max((ref_rec.astype(np.float64) ** 2).sum(), 1e-300)))
             mdl.run(shot=False)                     # warm-up (and autotune)
             mdl.set_kernel_timing(True, stride=8)
             mdl.kernel_times(reset=True)
@@ -589,6 +602,9 @@ def operators(args):
                 per_dir -= 8
                 a_all -= 24
             a1 = dir_split_bytes(c, nbb, a_unsplit, a_all, per_dir)
+        if angles:   # the two angles (8 B) instead of the four rotation terms (16 B) per sweep
+            a0 -= 8
+            a1 -= 8
         k0 = kms[0] / max(kn[0], 1)
         k1 = kms[1] / max(kn[1], 1) if kn[1] else 0.0
         g0 = a0 * npts / (k0 * 1e-3) / 1e9
@@ -605,6 +621,9 @@ def operators(args):
                                                                        "frac": round(g0 / HBM_PEAK_GBS, 4)}},
                 "whole_step_GBs": round((a0 + a1) * gpts, 1),
                 "whole_step_frac": round((a0 + a1) * gpts / HBM_PEAK_GBS, 4)}
+        if angles:
+            # 8 bytes of angles instead of the 16-byte rotation terms in each sweep
+            line["tti_angles"] = {"record_rel_l2_vs_terms_read": err_terms, "bytes_per_point_saved_per_sweep": 8}
         if op != "2nd":
             line["kernels"]["k_stress"] = {"avg_ms": round(k1, 4), "bytes_per_point": round(a1, 2),
                                            "GBs": round(g1, 1), "frac": round(g1 / HBM_PEAK_GBS, 4)}
@@ -645,6 +664,9 @@ def main():
     ap.add_argument("--nwin", type=int, default=64, help="windows of the batched run")
     ap.add_argument("--batch-modes", type=str, default="0,2,3",
                     help="with --batched: modes run (0 sequential, 2 batched graphs, 3 persistent kernel)")
+    ap.add_argument("--tti-angles", type=int, default=1, choices=(0, 1),
+                    help="with --batched: the TTI sweeps evaluate the rotation terms from the angles (1) or read "
+                         "them (0)")
     ap.add_argument("--persist-ty", type=int, default=4, choices=(4, 8),
                     help="with --batched: tile rows of the persistent kernel")
     ap.add_argument("--persist-group", type=int, default=1,

@@ -195,6 +195,14 @@ int a3d_get_launch(a3d_ctx* ctx, int* out6);
  * autotuner's choice (the velocity sweep when the tuner does not run).
  * a3d_get_w_sweep reports the mode a shot of *desc runs with. */
 int a3d_set_w_sweep(a3d_ctx* ctx, int mode);
+
+/* TTI rotation terms (no counterpart in the reference, whose TTI kernels recompute
+ * cos/sin of the tilt and azimuth per point in double, Toa_gpu_3dtti_fd_1orderfunciton.cu:
+ * 101-103,184-186): on = 0 (default) the sweeps read the four terms, (float)cos((double)
+ * angle) built once per model (16 bytes per point per sweep, results bitwise the
+ * oracle's); on = 1 they read the two angles (8 bytes per point) and evaluate the terms
+ * in FP32 (abs. error <= 7e-8 per term). TTI with FP32 or FP16 storage only. */
+int a3d_set_tti_angles(a3d_ctx* ctx, int on);
 int a3d_get_w_sweep(a3d_ctx* ctx, const a3d_shot_desc* desc, int* mode);
 
 /* Time loop as hipGraphs (no counterpart in the reference, which launches every kernel
@@ -264,6 +272,8 @@ int a3d_batch_run(a3d_batch* b, int mode, float** shots, double* loop_ms);
  * per sweep are those of a3d_batch_set_zseg / a3d_batch_tune), time steps per
  * persistent launch (default 100) and consecutive tiles per ticket (1..64, default 1). */
 int a3d_batch_set_persist(a3d_batch* b, int tile_rows, int steps, int group);
+/* a3d_set_tti_angles for every window of the batch. */
+int a3d_batch_set_tti_angles(a3d_batch* b, int on);
 /* Mode 2 captures the time loop in graphs of `steps` time steps each (0, the
  * default: one graph for the whole loop); with the environment variable
  * A3D_PROGRESS set, a line is printed to stderr after each graph completes. */
@@ -407,6 +417,10 @@ int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, do
  * outputs); 1 the sweep's interior pattern (in place, the z+3 / z+7 leads); 2 plus the
  * z-PML split pairs; 3 plus the y-PML pairs; 4 plus the x-PML pairs (the whole pattern). */
 int a3d_probe_stress_pattern(int device, int nnx, int nny, int nnz, int reps, double* ms, double* bytes);
+
+/* Measurement support: the FP32 sine and cosine the TTI sweeps use for the rotation terms
+ * with a3d_set_tti_angles(ctx, 1), evaluated on `device` for the n angles t (host arrays). */
+int a3d_probe_rot_sincos(int device, const float* t, float* s, float* c, int n);
 
 const char* a3d_last_error(void);
 const char* a3d_version(void);
