@@ -257,7 +257,7 @@ def batched(args):
     print(json.dumps(line), flush=True)
 
 
-def learned_boundary(scheme=4, nt=2000, n=256, seed=95000):
+def learned_boundary(scheme=4, nt=2000, n=256, seed=95000, heldout=12):
     """The learned-boundary half of the metric, measured in the bench run: the VTI 256^3
     configuration (d = 10 m, dt = 0.8 ms, nt 2000, seeded study model and source) as per-shot
     runs with the 20-cell PML, with the 6-cell strip with its head closed (plain ramp) and with
@@ -293,8 +293,40 @@ def learned_boundary(scheme=4, nt=2000, n=256, seed=95000):
            "gain_db_over_plain_strip": round(out["strip_plain"]["reflected_db"] - out["strip_learned"]["reflected_db"], 2),
            "choice": strip.choose([{"strip": judged("strip_learned"), "pml": judged("pml20")}]),
            "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"}
+    # the same comparison on held-out study models (seeds 90000 + k, 128^3, nt 800): the one
+    # model above says little about the network's gain on its own
+    from a3dfd import strip as strip_mod
+    seeds = [90000 + k for k in range(heldout)]
+    rows = strip_mod.judge(seeds, params, n=128, nt=800, snap_every=20, amp=amp, head=head, period=period,
+                           scheme=scheme, log=log) if heldout else []
+    if rows:
+        res["held_out_in_this_run"] = heldout_summary(rows, seeds, 128, 800, scheme, head, period, amp, with_rows=False)
     log(f"bench: learned boundary {time.time() - t0:.1f} s: {res}")
     return res
+
+
+def heldout_summary(rows, seeds, n, nt, scheme, head, period, amp, with_rows=True):
+    """Mean gain of the learned strip over its closed head (with standard error), reflected
+    energies, share of models at or below -40 dB and loop times over strip.judge rows."""
+    g = np.array([r["strip0"]["reflected_db"] - r["strip"]["reflected_db"] for r in rows])
+    db = {k: np.array([r[k]["reflected_db"] for r in rows]) for k in ("pml", "strip0", "strip")}
+    ms = {k: np.array([r[k]["loop_ms"] for r in rows]) for k in ("pml", "strip0", "strip")}
+    out = {"workload": f"vti_{n}^3_nt{nt}", "models": len(rows), "seeds": [seeds[0], seeds[-1]], "scheme": scheme,
+           "network": {"head": head, "period": period, "ramp_amplitude": amp},
+           "gain_db_over_closed_head": {"mean": round(float(g.mean()), 3),
+                                        "stderr": round(float(g.std(ddof=1) / np.sqrt(len(g))), 3) if len(g) > 1 else None,
+                                        "median": round(float(np.median(g)), 3), "min": round(float(g.min()), 2),
+                                        "max": round(float(g.max()), 2), "share_positive": round(float((g > 0).mean()), 3)},
+           "mean_reflected_db": {k: round(float(v.mean()), 2) for k, v in db.items()},
+           "share_at_or_below_-40dB": {k: round(float((v <= -40.0).mean()), 3) for k, v in db.items()},
+           "loop_ms_mean": {k: round(float(v.mean()), 1) for k, v in ms.items()},
+           "speedup_learned_vs_pml20": round(float(ms["pml"].mean() / ms["strip"].mean()), 3),
+           "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"}
+    if with_rows:
+        out["rows"] = [{"seed": r["seed"], "fm": r["fm"], "src": r["src"], "pml_db": r["pml"]["reflected_db"],
+                        "closed_db": r["strip0"]["reflected_db"], "learned_db": r["strip"]["reflected_db"]}
+                       for r in rows]
+    return out
 
 
 def strip_heldout(args):
@@ -311,24 +343,7 @@ def strip_heldout(args):
     seeds = [90000 + k for k in range(args.strip_heldout)]
     rows = strip.judge(seeds, params, n=n, nt=nt, snap_every=every, amp=amp, head=head, period=period,
                        scheme=args.scheme, log=log)
-    g = np.array([r["strip0"]["reflected_db"] - r["strip"]["reflected_db"] for r in rows])
-    db = {k: np.array([r[k]["reflected_db"] for r in rows]) for k in ("pml", "strip0", "strip")}
-    ms = {k: np.array([r[k]["loop_ms"] for r in rows]) for k in ("pml", "strip0", "strip")}
-    out = {"workload": f"vti_{n}^3_nt{nt}", "models": len(rows), "seeds": [seeds[0], seeds[-1]], "scheme": args.scheme,
-           "network": {"head": head, "period": period, "ramp_amplitude": amp},
-           "gain_db_over_closed_head": {"mean": round(float(g.mean()), 3),
-                                        "stderr": round(float(g.std(ddof=1) / np.sqrt(len(g))), 3) if len(g) > 1 else None,
-                                        "median": round(float(np.median(g)), 3), "min": round(float(g.min()), 2),
-                                        "max": round(float(g.max()), 2), "share_positive": round(float((g > 0).mean()), 3)},
-           "mean_reflected_db": {k: round(float(v.mean()), 2) for k, v in db.items()},
-           "share_at_or_below_-40dB": {k: round(float((v <= -40.0).mean()), 3) for k, v in db.items()},
-           "loop_ms_mean": {k: round(float(v.mean()), 1) for k, v in ms.items()},
-           "speedup_learned_vs_pml20": round(float(ms["pml"].mean() / ms["strip"].mean()), 3),
-           "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64",
-           "rows": [{"seed": r["seed"], "fm": r["fm"], "src": r["src"], "pml_db": r["pml"]["reflected_db"],
-                     "closed_db": r["strip0"]["reflected_db"], "learned_db": r["strip"]["reflected_db"]}
-                    for r in rows]}
-    print(json.dumps(out), flush=True)
+    print(json.dumps(heldout_summary(rows, seeds, n, nt, args.scheme, head, period, amp)), flush=True)
 
 
 def strip_coef(args):
