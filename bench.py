@@ -295,9 +295,8 @@ def learned_boundary(scheme=4, nt=2000, n=256, seed=95000, heldout=12):
            "judge": "extended domain (+64 cells per face, 40-cell PML) in FP64"}
     # the same comparison on held-out study models (seeds 90000 + k, 128^3, nt 800): the one
     # model above says little about the network's gain on its own
-    from a3dfd import strip as strip_mod
     seeds = [90000 + k for k in range(heldout)]
-    rows = strip_mod.judge(seeds, params, n=128, nt=800, snap_every=20, amp=amp, head=head, period=period,
+    rows = strip.judge(seeds, params, n=128, nt=800, snap_every=20, amp=amp, head=head, period=period,
                            scheme=scheme, log=log) if heldout else []
     if rows:
         res["held_out_in_this_run"] = heldout_summary(rows, seeds, 128, 800, scheme, head, period, amp, with_rows=False)
