@@ -274,6 +274,32 @@ struct a3d_ctx {
 
 namespace {
 
+// XCD runs of equal tile counts (the last ones shorter when 8 does not divide the count).
+void uniform_bands(Launch& l) {
+    const int per = (l.ntiles + 7) / 8;
+    for (int k = 0; k <= 8; ++k) l.band[k] = std::min(k * per, l.ntiles);
+    l.grid = 8 * per;
+}
+
+// XCD runs of about equal summed cost(t) (contiguous, in tile order); grid = 8 x the
+// longest run.
+template <class F>
+void balance_bands(Launch& l, F cost) {
+    std::vector<double> pre(l.ntiles + 1, 0.0);
+    for (int t = 0; t < l.ntiles; ++t) pre[t + 1] = pre[t] + cost(t);
+    l.band[0] = 0;
+    l.band[8] = l.ntiles;
+    for (int k = 1; k < 8; ++k) {
+        const double target = pre[l.ntiles] * k / 8.0;
+        int t = (int)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+        if (t > 0 && target - pre[t - 1] < pre[t] - target) --t;
+        l.band[k] = std::max(l.band[k - 1], std::min(t, l.ntiles));
+    }
+    int longest = 1;
+    for (int k = 0; k < 8; ++k) longest = std::max(longest, l.band[k + 1] - l.band[k]);
+    l.grid = 8 * longest;
+}
+
 // Tiles of 64 x ty points over the updated x-y range, the planes [z0, z1) (default: all
 // updated planes) cut into zseg segments.
 Launch make_launch(const Geo& g, int ty, int zseg, int z0 = MM, int z1 = -1) {
@@ -286,9 +312,9 @@ Launch make_launch(const Geo& g, int ty, int zseg, int z0 = MM, int z1 = -1) {
     l.zlen = (nzi + nseg - 1) / nseg;
     l.nseg = (nzi + l.zlen - 1) / l.zlen;
     l.ntiles = l.ntx * l.nty * l.nseg;
-    l.grid = ((l.ntiles + 7) / 8) * 8;
     l.zbeg = z0;
     l.zend = z1;
+    uniform_bands(l);
     return l;
 }
 
@@ -637,6 +663,44 @@ int prepare_first(a3d_ctx* c, const a3d_shot_desc* d, const Scales& sc, VelArgs&
     va.wz = sa.wz = wz_active(c, d) ? 1 : 0;
     va.zrev = va.wz;   // no z window in the velocity sweep: its planes walked downward
     if (sa.wz) sa.l = make_launch(g, c->ty[1], 1);
+    // The stress sweep's XCD runs balanced by estimated bytes per tile: a PML cell reads
+    // and writes split fields (scheme 1: all six; schemes 3 and 4: those of its damped
+    // directions), so the tile rows along the y faces weigh more than interior rows and
+    // equal runs would give the XCDs that hold them more bytes. Schemes 0 and 2 move the
+    // same bytes in every cell.
+    if (sa.unsplit != 0 && d->scheme != 2) {
+        const T* axf = t + pm.ax1;
+        const T* ayf = t + pm.ay1;
+        const T* azf = t + pm.az1;
+        auto prefix = [](int n, auto damped) {
+            std::vector<int> p(n + 1, 0);
+            for (int i = 0; i < n; ++i) p[i + 1] = p[i] + (damped(i) ? 1 : 0);
+            return p;
+        };
+        const std::vector<int> px = prefix(g.nnx, [&](int i) { return axf[i] != (T)1; });
+        const std::vector<int> py = prefix(g.nny, [&](int i) { return ayf[i] != (T)1; });
+        const std::vector<int> pz = prefix(g.nnz, [&](int i) { return azf[i] != (T)1; });
+        // bytes per cell by the number of damped directions (VTI, w in the stress sweep)
+        const double w1[4] = {48, 92, 92, 92}, w3[4] = {48, 56, 64, 68};
+        const double* wc = sa.unsplit >= 2 ? w3 : w1;
+        const Launch L = sa.l;
+        const int tyr = c->ty[1];
+        balance_bands(sa.l, [&](int tt) {
+            const int tix = tt % L.ntx, r = tt / L.ntx, tiy = r % L.nty, seg = r / L.nty;
+            const int x0 = std::max(tix * TX, MM), x1 = std::min(tix * TX + TX, g.nnx - MM);
+            const int y0 = std::max(MM + tiy * tyr, MM), y1 = std::min(MM + tiy * tyr + tyr, g.nny - MM);
+            const int z0 = L.zbeg + seg * L.zlen, z1 = std::min(z0 + L.zlen, L.zend);
+            if (x1 <= x0 || y1 <= y0 || z1 <= z0) return 0.0;
+            const int n[3][2] = {{x1 - x0 - (px[x1] - px[x0]), px[x1] - px[x0]},
+                                 {y1 - y0 - (py[y1] - py[y0]), py[y1] - py[y0]},
+                                 {z1 - z0 - (pz[z1] - pz[z0]), pz[z1] - pz[z0]}};
+            double cost = 0;
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b)
+                    for (int e = 0; e < 2; ++e) cost += (double)n[0][a] * n[1][b] * n[2][e] * wc[a + b + e];
+            return cost;
+        });
+    }
     // the stress sweep's (and the one-pass step's) interleaved model (VTI, float model),
     // built once per model
     // TTI (float model): the four rotation terms interleaved, one 16-byte load per point in
@@ -2044,7 +2108,7 @@ int build_groups(a3d_batch* b, const std::vector<A>& args, const std::vector<int
             base.push_back(base.back() + a.l.ntiles);
         }
         g.lb.ntiles = base.back();
-        g.lb.grid = ((g.lb.ntiles + 7) / 8) * 8;
+        uniform_bands(g.lb);
         if (hipMalloc(&g.args, ga.size() * sizeof(A)) != hipSuccess ||
             hipMalloc(&g.base, base.size() * sizeof(int)) != hipSuccess) {
             out.push_back(g);

@@ -35,6 +35,8 @@ struct Geo {
 struct Launch {
     int ntx, nty, nseg, zlen, ntiles, grid;
     int zbeg, zend;
+    int band[9];   // tiles [band[k], band[k+1]) go to XCD k (workgroups with blockIdx % 8 == k),
+                   // in order; grid = 8 x the longest band
 };
 
 // Precision instances of the sweep kernels.

@@ -138,12 +138,14 @@ __device__ __forceinline__ C lap9(F w) {
     return c;
 }
 
-// Workgroup -> tile. The hardware deals workgroup b to XCD b % 8; tile ids
-// are assigned so that XCD k owns the contiguous run [k*per, (k+1)*per).
+// Workgroup -> tile. The hardware deals workgroup b to XCD b % 8; XCD k owns the
+// contiguous run of tiles [l.band[k], l.band[k+1]) (equal runs, or runs of equal
+// estimated bytes), its workgroups in order; a workgroup past its run has no tile.
 // Tile order: x tile fastest, then y tile, then z segment.
 __device__ __forceinline__ int remap_tile(const Launch& l) {
-    const int per = l.grid >> 3;
-    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    const int k = (int)(blockIdx.x & 7u);
+    const int t = l.band[k] + (int)(blockIdx.x >> 3);
+    return t < l.band[k + 1] ? t : l.ntiles;
 }
 
 __device__ __forceinline__ void split_tile(const Launch& l, int t, int& tix, int& tiy, int& seg) {
