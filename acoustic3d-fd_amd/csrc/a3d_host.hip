@@ -1556,8 +1556,10 @@ int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, do
         hipEventCreate(&e1) != hipSuccess)
         rc = fail(A3D_EHIP, "probe stream/event creation failed");
     if (!rc) rc = hipMemsetAsync(a, 0, bytes, st) == hipSuccess ? A3D_OK : fail(A3D_EHIP, "memset failed");
-    if (!rc) rc = run(0, 1024, copy_gbs);
-    if (!rc) rc = run(1, 1024, read_gbs);
+    // the grid-stride copy (the byte counters' calibration dispatch), then the ceiling forms
+    if (!rc) rc = run(0, 1024, nullptr);
+    if (!rc) rc = run(0, 0, copy_gbs);
+    if (!rc) rc = run(1, 0, read_gbs);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);

@@ -267,6 +267,8 @@ hipError_t launch_shot_transpose(const float* rec, int nt, int nx, int ny, const
                                  float* shot, hipStream_t st);
 
 // Calibration streams: float4 copy a -> b, or read-only reduction of a (b: 64 B scratch).
+// grid > 0: the grid-stride float4 copy / read-only sum with that many workgroups of 256;
+// grid 0: the ceiling forms (flat, non-temporal float4 streams)
 hipError_t launch_probe(int read_only, const float* a, float* b, size_t bytes, int grid, hipStream_t st);
 
 // Learned absorbing strip (VTI, FP32): the outer STRIP_CELLS updated cells of

@@ -720,7 +720,9 @@ def main():
     if args.probe_copy:
         from a3dfd._lib import probe_bandwidth
         c, r = probe_bandwidth(0, 2 << 30, 10)
-        print(json.dumps({"probe": "float4 grid 1024x256, 2 GiB buffers, 10 reps",
+        print(json.dumps({"probe": "2 GiB buffers, 10 reps: copy one float4 per thread (256 per workgroup), read four "
+                                   "per thread (1024 per workgroup), non-temporal loads and stores; before them the "
+                                   "grid-stride float4 copy (1024 x 256 threads) the byte counters are calibrated on",
                           "copy_GBs_read_plus_write": round(c, 1), "read_only_GBs": round(r, 1)}), flush=True)
         return
 

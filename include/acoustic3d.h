@@ -404,10 +404,12 @@ int a3d_run_static(const a3d_shot_desc* desc, int device, const float* vel, cons
 /* Padded grid actually updated per step: points = (nnx-8)*(nny-8)*(nnz-8). */
 int a3d_grid_info(const a3d_ctx* ctx, int* nnx, int* nny, int* nnz, long long* updated_points);
 
-/* Measurement support: achievable HBM bandwidth of a plain float4 copy
- * (read+write bytes / time) and of a float4 read-only stream over `bytes`
- * per buffer, `reps` repetitions, on `device`. Also the calibration kernels
- * for the FETCH_SIZE / WRITE_SIZE byte counters. */
+/* Measurement support: achievable HBM bandwidth of a float4 copy (read+write
+ * bytes / time; one float4 per thread, non-temporal) and of a float4 read-only
+ * stream (four per thread, non-temporal) over `bytes` per buffer, `reps`
+ * repetitions, on `device`. It first runs the grid-stride float4 copy
+ * (k_probe_copy, 1024 x 256 threads), the calibration dispatch of the
+ * FETCH_SIZE / WRITE_SIZE byte counters. */
 int a3d_probe_bandwidth(int device, size_t bytes, int reps, double* copy_gbs, double* read_gbs);
 
 /* Measurement support: the VTI stress sweep's memory access pattern replayed without
