@@ -147,6 +147,8 @@ def load(path=None):
     lib.a3d_batch_set_strip_period.restype = c_int
     lib.a3d_set_zchunk.argtypes = [vp, c_int]
     lib.a3d_set_graph.argtypes = [vp, c_int]
+    lib.a3d_run_shots.argtypes = [vp, POINTER(ShotDesc), c_int, POINTER(c_int), fp, POINTER(c_double)]
+    lib.a3d_run_shots.restype = c_int
     lib.a3d_set_graph.restype = c_int
     lib.a3d_set_w_sweep.argtypes = [vp, c_int]
     lib.a3d_set_tti_angles.argtypes = [vp, c_int]
@@ -278,6 +280,18 @@ class Modeller:
         if snaps is not None:
             snaps = snaps[: min(ns.value, nsnap_max)]
         return out, snaps, ms.value
+
+    def run_shots(self, sources, desc=None, shot=True):
+        """2nd order: the shots with sources [(sx, sy, sz), ...] (1..16) stepped together, one
+        launch per time step; returns (records (n, ny, nx, nt) or None, loop_ms)."""
+        d = desc or self.desc
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1, 3))
+        n = src.shape[0]
+        out = np.empty((n, d.ny, d.nx, d.nt), np.float32) if shot else None
+        ms = c_double(0.0)
+        _check(self.lib, self.lib.a3d_run_shots(self._h, ctypes.byref(d), n, src.ctypes.data_as(POINTER(c_int)),
+                                                _fptr(out), ctypes.byref(ms)))
+        return out, ms.value
 
     def set_kernel_timing(self, on=True, stride=1):
         """Time the hot kernels with HIP events; stride N > 1 times every N-th time step."""

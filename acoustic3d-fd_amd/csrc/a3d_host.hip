@@ -270,6 +270,10 @@ struct a3d_ctx {
     int wz_tuned = 0;
     bool out_device = false;    // the shot record is transposed into shot_dev, the caller's device buffer
     int graph_steps = 0;        // > 0: the time loop runs as hipGraphs of at most this many steps
+    void* multi = nullptr;      // a3d_run_shots: the shots' two pressure levels
+    size_t multi_cap = 0;       //   (bytes)
+    float* mrec = nullptr;      // a3d_run_shots: the shots' receiver records
+    size_t mrec_cap = 0;
 };
 
 namespace {
@@ -1084,6 +1088,114 @@ int run_shot_t(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn
     return A3D_OK;
 }
 
+// Several 2nd-order shots on the context's model in one launch per step (k_step2_shots):
+// shot k's source at src[3k..3k+2] (the descriptor's source ignored), records into
+// shots_out + k * ny * nx * nt. No snapshots. Each shot's record equals its own run's.
+template <class T>
+int run_shots_2nd_t(a3d_ctx* c, const a3d_shot_desc* d, int n, const int* src, float* shots_out, double* loop_ms) {
+    const Geo g = c->g;
+    const int prec = c->prec;
+    const int nx = d->nx, ny = d->ny, nz = d->nz, nt = d->nt;
+    const size_t plane_rec = (size_t)nx * ny, rec_n = plane_rec * nt;
+    const size_t wbytes = c->nvox * c->wsize;
+    int rc;
+    if (c->multi_cap < 2 * n * wbytes) {
+        if (c->multi) (void)hipFree(c->multi);
+        c->multi = nullptr;
+        c->multi_cap = 0;
+        if (hipMalloc(&c->multi, 2 * n * wbytes) != hipSuccess) {
+            c->multi = nullptr;
+            return fail(A3D_ENOMEM, "multi-shot levels: allocation of %zu bytes failed", 2 * n * wbytes);
+        }
+        c->multi_cap = 2 * n * wbytes;
+    }
+    if ((rc = ensure(&c->mrec, &c->mrec_cap, rec_n * n))) return rc;
+    if (c->vv_dt != d->dt) {
+        HIP_TRY(launch_pad(c->stage, nx, ny, nz, c->off, g, c->vol[V_VV], prec == PREC_FP64, 1, d->dt, c->st));
+        c->vv_dt = d->dt;
+    }
+    unsigned char* X = (unsigned char*)c->multi;
+    unsigned char* Y = X + n * wbytes;
+    HIP_TRY(hipMemsetAsync(c->multi, 0, 2 * n * wbytes, c->st));
+    std::vector<T> wlt(nt);
+    for (int it = 0; it < nt; ++it) wlt[it] = ricker_value<T>(it, d->fm, d->dt);
+    Step2Args a{};
+    a.g = g;
+    a.l = make_launch(g, c->ty[0], c->zseg[0]);
+    a.l.ntiles *= n;
+    uniform_bands(a.l);
+    a.vv = c->vol[V_VV];
+    a.fx = c->dtab;
+    a.fy = c->dtab + g.nnx;
+    a.fz = c->dtab + g.nnx + g.nny;
+    a.dz2 = (T)(1.0 / ((T)d->dz * (T)d->dz));
+    a.dx2 = (T)(1.0 / ((T)d->dx * (T)d->dx));
+    a.dy2 = (T)(1.0 / ((T)d->dy * (T)d->dy));
+    a.sc = field_scales(c, d);
+    const int off = c->off;
+    a.zrec = off + d->rec_z;
+    a.off = off;
+    a.nx = nx;
+    a.ny = ny;
+    Step2Shots sh{};
+    sh.n = n;
+    sh.stride = wbytes;
+    sh.rstride = rec_n;
+    for (int k = 0; k < n; ++k) {
+        sh.s0[k] = (src[3 * k] + off) + g.ld * (src[3 * k + 1] + off) + g.pl * (src[3 * k + 2] + off);
+        sh.s1[k] = sh.s0[k] + 2 * g.pl;
+    }
+    HIP_TRY(mark(c->ev0, c->st));
+    for (int it = 0; it < nt; ++it) {
+        a.X = X;
+        a.Y = Y;
+        a.ya0 = wlt[it];
+        a.ya1 = -wlt[it];
+        a.xa0 = it > 0 ? (double)wlt[it - 1] : 0.0;
+        a.xa1 = it > 0 ? -(double)wlt[it - 1] : 0.0;
+        a.rec = c->mrec + plane_rec * it;
+        timing_step(c, it);
+        timing_begin(c, 0);
+        HIP_TRY(launch_step2_shots(prec, c->ty[0], a, sh, c->st));
+        timing_end(c);
+        std::swap(X, Y);
+    }
+    HIP_TRY(mark(c->ev1, c->st));
+    float ms = 0.f;
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (loop_ms) *loop_ms = ms;
+    if (c->tm.on) timing_collect(c);
+    HIP_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->st));
+    HIP_TRY(launch_check_finite(c->mrec, rec_n * n, c->flag, c->st));
+    int hflag = 0;
+    HIP_TRY(hipMemcpyAsync(&hflag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    if (shots_out) {
+        if ((rc = ensure(&c->shot_dev, &c->shot_cap, rec_n))) return rc;
+        for (int k = 0; k < n; ++k) {
+            MuteArgs m{};
+            m.mute = d->mute;
+            m.dx = d->dx;
+            m.dy = d->dy;
+            m.dz = d->dz;
+            m.dt = d->dt;
+            m.fm = d->fm;
+            m.sx = src[3 * k];
+            m.sy = src[3 * k + 1];
+            m.sz = src[3 * k + 2];
+            m.tt = d->mute_tt;
+            m.vcut = d->v_mute > 0 ? d->v_mute : c->vel_raw1;
+            HIP_TRY(launch_shot_transpose(c->mrec + k * rec_n, nt, nx, ny, m, c->shot_dev, c->st));
+            HIP_TRY(hipMemcpyAsync(shots_out + k * rec_n, c->shot_dev, rec_n * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->st));
+            HIP_TRY(hipStreamSynchronize(c->st));
+        }
+    }
+    if (hflag) return fail(A3D_EUNSTABLE, "non-finite values in a shot record (unstable time step?)");
+    return A3D_OK;
+}
+
 int run_shot(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, const SnapFn& snapfn, int* nsnap_taken,
              double* loop_ms) {
     int rc = check_desc_shot(c, d);
@@ -1392,7 +1504,7 @@ int a3d_destroy(a3d_ctx* c) {
     if (c->dtab) (void)hipFree(c->dtab);
     if (c->flag) (void)hipFree(c->flag);
     for (void* p : {(void*)c->strip_net, (void*)c->strip_ids, (void*)c->strip_meta, (void*)c->strip_delta,
-                    (void*)c->strip_mfac})
+                    (void*)c->strip_mfac, c->multi, (void*)c->mrec})
         if (p) (void)hipFree(p);
     for (auto e : c->tm.pool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1461,6 +1573,25 @@ int a3d_run(a3d_ctx* c, const a3d_shot_desc* d, float* shot_out, float* snaps_ou
             if (k < nsnap_max) std::memcpy(snaps_out + (size_t)k * vol, h, vol * sizeof(float));
         };
     return run_shot(c, d, shot_out, fn, nsnap_taken, loop_ms);
+}
+
+int a3d_run_shots(a3d_ctx* c, const a3d_shot_desc* d, int nshot, const int* src, float* shots_out, double* loop_ms) {
+    if (!c || !d || !src) return fail(A3D_EINVAL, "null argument");
+    if (d->op != A3D_OP_2ND) return fail(A3D_EINVAL, "several shots in one launch: 2nd-order operator only");
+    if (nshot < 1 || nshot > MAX_SHOTS) return fail(A3D_EINVAL, "shots per run must be in 1..%d, got %d", MAX_SHOTS, nshot);
+    if (d->snap_interval != 0) return fail(A3D_EINVAL, "several shots in one launch take no snapshots");
+    for (int k = 0; k < nshot; ++k) {
+        a3d_shot_desc dk = *d;
+        dk.sx = src[3 * k];
+        dk.sy = src[3 * k + 1];
+        dk.sz = src[3 * k + 2];
+        if (int rc = check_desc_shot(c, &dk)) return rc;
+    }
+    if (!c->model_set) return fail(A3D_ESTATE, "model not set");
+    if (int rc = maybe_autotune(c, d)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->prec == PREC_FP64) return run_shots_2nd_t<double>(c, d, nshot, src, shots_out, loop_ms);
+    return run_shots_2nd_t<float>(c, d, nshot, src, shots_out, loop_ms);
 }
 
 int a3d_set_kernel_timing(a3d_ctx* c, int enable) {

@@ -171,6 +171,18 @@ hipError_t launch_fused_vti(int ty, const FusedArgs& a, hipStream_t st);
 int fused_blocks_per_cu(int ty);
 hipError_t launch_stress(int prec, int op, int ty, const StressArgs& a, hipStream_t st);
 hipError_t launch_step2(int prec, int ty, const Step2Args& a, hipStream_t st);
+// Several shots of the 2nd-order step on one model in one launch: shot k's levels at
+// a.X / a.Y + k * stride bytes, its dipole cells s0[k], s1[k], its receiver plane at
+// a.rec + k * rstride floats. a.l: the launch of one shot with ntiles = n x its tiles
+// (tile t belongs to shot t % n), bands over that count.
+constexpr int MAX_SHOTS = 16;
+struct Step2Shots {
+    int n;
+    int s0[MAX_SHOTS], s1[MAX_SHOTS];
+    size_t stride;             // bytes between two shots' volumes
+    size_t rstride;            // floats between two shots' receiver records
+};
+hipError_t launch_step2_shots(int prec, int ty, const Step2Args& a, const Step2Shots& s, hipStream_t st);
 
 // Batched sweeps over several windows (each an independent padded grid with its own
 // arguments in device memory). base: nwin+1 prefix sums of the windows' tile counts;

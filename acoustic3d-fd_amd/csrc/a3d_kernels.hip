@@ -1156,16 +1156,15 @@ __global__ void k_zero_u32(unsigned* __restrict__ a, size_t n) {
     if (i < n) a[i] = 0u;
 }
 
+// One tile (64 x TY columns, one z segment) of a 2nd-order step.
 template <class PR, int TY>
-__global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
+__device__ __forceinline__ void step2_tile(const Step2Args& a, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using M = typename PR::M;
     using C = typename PR::C;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
     __shared__ C sY[H][W];
 
-    int tix, tiy, seg;
-    if (!decode_tile(a.l, tix, tiy, seg)) return;
     const Geo g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = tix * TX, y0 = MM + tiy * TY;
@@ -1230,6 +1229,33 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
             if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
         }
     }
+}
+
+template <class PR, int TY>
+__global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    step2_tile<PR, TY>(a, tix, tiy, seg);
+}
+
+// Several shots on one model: tile t is spatial tile t / n of shot t % n, so the shots'
+// copies of one tile run side by side on one XCD and share its L2 rows of (v dt)^2 and
+// the sponge tables. Each tile runs the single-shot step's code on its shot's levels:
+// every shot's results equal its own run bit for bit.
+template <class PR, int TY>
+__global__ __launch_bounds__(TX* TY) void k_step2_shots(const Step2Args a, const Step2Shots s) {
+    const int t = remap_tile(a.l);
+    if (t >= a.l.ntiles) return;
+    const int k = t % s.n;
+    int tix, tiy, seg;
+    split_tile(a.l, t / s.n, tix, tiy, seg);
+    Step2Args b = a;
+    b.X = (char*)a.X + k * s.stride;
+    b.Y = (const char*)a.Y + k * s.stride;
+    b.s0 = s.s0[k];
+    b.s1 = s.s1[k];
+    b.rec = a.rec ? a.rec + k * s.rstride : nullptr;
+    step2_tile<PR, TY>(b, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -1651,6 +1677,28 @@ hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins,
         case PREC_FP32: return stress_batch_op<Fp32>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
         case PREC_FP16: return stress_batch_op<Fp16>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
         case PREC_FP64: return stress_batch_op<Fp64>(op, ty, wins, base, nwin, lb, it, dirs, wz, srcs, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class PR>
+hipError_t step2_shots_ty(int ty, const Step2Args& a, const Step2Shots& s, hipStream_t st) {
+    const dim3 grid(a.l.grid);
+    switch (ty) {
+        case 2: hipLaunchKernelGGL((k_step2_shots<PR, 2>), grid, dim3(TX, 2), 0, st, a, s); break;
+        case 4: hipLaunchKernelGGL((k_step2_shots<PR, 4>), grid, dim3(TX, 4), 0, st, a, s); break;
+        case 8: hipLaunchKernelGGL((k_step2_shots<PR, 8>), grid, dim3(TX, 8), 0, st, a, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step2_shots(int prec, int ty, const Step2Args& a, const Step2Shots& s, hipStream_t st) {
+    if (s.n < 1 || s.n > MAX_SHOTS) return hipErrorInvalidValue;
+    switch (prec) {
+        case PREC_FP32: return step2_shots_ty<Fp32>(ty, a, s, st);
+        case PREC_FP16: return step2_shots_ty<Fp16>(ty, a, s, st);
+        case PREC_FP64: return step2_shots_ty<Fp64>(ty, a, s, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -599,6 +599,13 @@ def operators(args):
             mdl.run(shot=False)
             _, _, ms_graph = mdl.run(shot=False)
             mdl.set_graph(0)
+            together = {}
+            if op == "2nd":
+                # several shots on the model stepped together (one launch per step over all of them)
+                for k in (4, 8):
+                    srcs = [(c["sx"] - 20 + 5 * j, c["sy"], c["sz"]) for j in range(k)]
+                    mdl.run_shots(srcs, shot=False)
+                    together[k] = min(mdl.run_shots(srcs, shot=False)[1] for _ in range(2))
             gi = mdl.grid_info()
             launch = mdl.launch_info()
         npts = gi["updated_points"]
@@ -635,6 +642,11 @@ def operators(args):
                                                                        "frac": round(g0 / HBM_PEAK_GBS, 4)}},
                 "whole_step_GBs": round((a0 + a1) * gpts, 1),
                 "whole_step_frac": round((a0 + a1) * gpts / HBM_PEAK_GBS, 4)}
+        if together:
+            line["shots_together"] = {str(k): {"loop_ms": round(v, 2), "Gpt_s": round(k * npts * nt / v / 1e6, 2),
+                                               "GBs": round(a0 * k * npts * nt / v / 1e6, 1),
+                                               "frac": round(a0 * k * npts * nt / v / 1e6 / HBM_PEAK_GBS, 4)}
+                                      for k, v in together.items()}
         if angles:
             # 8 bytes of angles instead of the 16-byte rotation terms in each sweep
             line["tti_angles"] = {"record_rel_l2_vs_terms_read": err_terms, "bytes_per_point_saved_per_sweep": 8}

@@ -212,6 +212,17 @@ int a3d_get_w_sweep(a3d_ctx* ctx, const a3d_shot_desc* desc, int* mode);
  * Not used while per-kernel timing is on. */
 int a3d_set_graph(a3d_ctx* ctx, int steps);
 
+/* Several 2nd-order shots on the context's model at once (no counterpart in the
+ * reference, which runs one shot per cuda_3dfd call; Toa_3dfd_cuda.cu:366-508): nshot
+ * (1..16) shots whose sources are src[3k], src[3k+1], src[3k+2] (sx, sy, sz as in
+ * a3d_shot_desc; the descriptor's own source is not used) step together, one launch per
+ * time step over every shot's tiles, so a small grid fills the GPU. shots_out (host,
+ * may be NULL) receives nshot records of [ny][nx][nt] floats, each equal bit for bit to
+ * a3d_run of that shot. No snapshots (desc->snap_interval must be 0); *loop_ms: the time
+ * loop. Allocates 2 x nshot pressure volumes and the records on the device. */
+int a3d_run_shots(a3d_ctx* ctx, const a3d_shot_desc* desc, int nshot, const int* src, float* shots_out,
+                  double* loop_ms);
+
 /* 1st-order two-sweep step in z chunks of `planes` planes (0: whole-grid sweeps, the
  * default; else >= 8): per step the velocity sweep of chunk k+1 runs before the stress
  * sweep of chunk k, so the velocities a stress chunk reads were written one launch
