@@ -1273,9 +1273,14 @@ int autotune(a3d_ctx* c, const a3d_shot_desc* d, int steps, double* best_ms) {
     const int wz_mode = c->wz_mode;
     for (int m = m0; m <= m1 && !rc; ++m) {
         c->wz_mode = m;
+        // the 2nd-order step also tries segments of 4 and 2 planes (k_step2_blk: every load of
+        // a segment at once), which pay on grids too small for plane-streaming tiles
+        std::vector<int> cand(segs, segs + 9);
+        if (c->d.op == A3D_OP_2ND)
+            for (int zl : {4, 2}) cand.push_back((nzi + zl - 1) / zl);
         for (int ty : tys) {
-            for (int ns : segs) {
-                if (ns > 1 && nzi / ns < 8) continue;
+            for (int ns : cand) {
+                if (ns > 1 && nzi / ns < 8 && !(c->d.op == A3D_OP_2ND && (nzi + ns - 1) / ns <= 4)) continue;
                 for (int k = 0; k < 2; ++k) {
                     c->ty[k] = ty;
                     c->zseg[k] = ns;

@@ -1238,6 +1238,86 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
     step2_tile<PR, TY>(a, tix, tiy, seg);
 }
 
+// 2nd-order step over z segments of at most ZB planes with every load of the segment
+// issued at once (its z column, and each plane's previous level, (v dt)^2 and x-y halo):
+// one memory round trip per workgroup instead of one per plane, for grids too small to
+// keep the GPU busy with plane-streaming tiles. Each point's arithmetic is the streaming
+// kernel's, operation for operation: the same bits.
+template <class PR, int TY, int ZB>
+__global__ __launch_bounds__(TX* TY) void k_step2_blk(const Step2Args a) {
+    using S = typename PR::S;
+    using M = typename PR::M;
+    using C = typename PR::C;
+    constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
+    __shared__ C sY[ZB][H][W];
+
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    const Geo g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = tix * TX, y0 = MM + tiy * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool yin = y < g.nny;
+    const bool valid = y < g.nny - MM && x >= MM && x < g.nnx - MM;
+    const int yl = yin ? y : g.nny - 1;
+    const int col = x + g.ld * yl;
+    const int colc = min(max(x, MM), g.nnx - MM - 1) + g.ld * min(y, g.nny - MM - 1);
+    const int zb = a.l.zbeg + seg * a.l.zlen;
+    const int ze = min(zb + a.l.zlen, a.l.zend);
+    const float sp = a.sc.s[F_P], ip = a.sc.inv[F_P];
+    S* X = (S*)a.X;
+    const S* Y = (const S*)a.Y;
+    const M* VV = (const M*)a.vv;
+    const C dx2 = (C)a.dx2, dy2 = (C)a.dy2, dz2 = (C)a.dz2;
+    auto ldY = [&](int zo, int o) {
+        return inj<C>(ldw(Y, zo, o, sp), zo + o, a.s0, a.ya0, a.s1, a.ya1);
+    };
+
+    C wY[ZB + 8];   // Y(zb-4+k); planes past the grid (a last, shorter segment) clamped, unused
+#pragma unroll
+    for (int k = 0; k < ZB + 8; ++k) wY[k] = ldY(min(zb - 4 + k, g.nnz - 1) * g.pl, col);
+    C xo[ZB], vq[ZB];
+    Halo<TY, true, true, C> hY[ZB];
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) {
+        const int zo = min(zb + j, g.nnz - 1) * g.pl;
+        xo[j] = inj<C>(ldw((const S*)X, zo, colc, sp), zo + colc, a.s0, a.xa0, a.s1, a.xa1);
+        vq[j] = (C)bld(VV, zo, colc);
+        load_halo(hY[j], g, zo, x0, y0, tx, ty, yl, ldY);
+    }
+    const int xs = x < g.nnx ? x : 0;
+    const double fx = a.fx[xs], fy = a.fy[yl];
+    const bool recx = a.rec && valid && x >= a.off && x < a.off + a.nx && y >= a.off && y < a.off + a.ny;
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) {
+        sY[j][ty + MM][tx + MM] = wY[4 + j];
+        store_halo(sY[j], hY[j], tx, ty);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ZB; ++j) {
+        const int z = zb + j;
+        if (z >= ze) break;   // uniform: the segment's end
+        const int zo = z * g.pl;
+        const double fz = a.fz[z];
+        C c1 = lap9<C>([&](int k) { return wY[j + k]; });   // z
+        const C cur = wY[j + 4];
+        if (valid) {
+            const int r = ty + MM, c = tx + MM;
+            C c2 = lap9<C>([&](int k) { return sY[j][r][c - 4 + k]; });   // x
+            C c3 = lap9<C>([&](int k) { return sY[j][r - 4 + k][c]; });   // y
+            c1 *= dz2;
+            c2 *= dx2;
+            c3 *= dy2;
+            const C prev = damp3<C>(xo[j], fy, fx, fz);
+            const C lap = vq[j] * (c1 + c2 + c3);
+            const C nv = (C)(2.0 * (double)cur - (double)prev + (double)lap);
+            stw<0>(X, zo, col, damp3<C>(nv, fy, fx, fz), ip);
+            if (recx && z == a.zrec) a.rec[(y - a.off) * a.nx + (x - a.off)] = (float)damp3<C>(cur, fy, fx, fz);
+        }
+    }
+}
+
 // Several shots on one model: tile t is spatial tile t / n of shot t % n, so the shots'
 // copies of one tile run side by side on one XCD and share its L2 rows of (v dt)^2 and
 // the sponge tables. Each tile runs the single-shot step's code on its shot's levels:
@@ -1445,6 +1525,15 @@ hipError_t stress_ty(int ty, const StressArgs& a, hipStream_t st) {
 template <class PR>
 hipError_t step2_ty(int ty, const Step2Args& a, hipStream_t st) {
     const dim3 grid(a.l.grid);
+    if (a.l.zlen <= 4) {   // short segments: every load of a segment at once
+        switch (ty) {
+            case 2: hipLaunchKernelGGL((k_step2_blk<PR, 2, 4>), grid, dim3(TX, 2), 0, st, a); break;
+            case 4: hipLaunchKernelGGL((k_step2_blk<PR, 4, 4>), grid, dim3(TX, 4), 0, st, a); break;
+            case 8: hipLaunchKernelGGL((k_step2_blk<PR, 8, 4>), grid, dim3(TX, 8), 0, st, a); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (ty) {
         case 2: hipLaunchKernelGGL((k_step2<PR, 2>), grid, dim3(TX, 2), 0, st, a); break;
         case 4: hipLaunchKernelGGL((k_step2<PR, 4>), grid, dim3(TX, 4), 0, st, a); break;
