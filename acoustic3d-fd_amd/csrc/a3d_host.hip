@@ -274,6 +274,8 @@ struct a3d_ctx {
     size_t multi_cap = 0;       //   (bytes)
     float* mrec = nullptr;      // a3d_run_shots: the shots' receiver records
     size_t mrec_cap = 0;
+    int mty = 4, mzseg = 1;     // a3d_run_shots with several shots: tile rows, z segments
+    bool mshape = false;        //   (set by a3d_set_launch or the first multi-shot run's tuning)
 };
 
 namespace {
@@ -1121,7 +1123,8 @@ int run_shots_2nd_t(a3d_ctx* c, const a3d_shot_desc* d, int n, const int* src, f
     for (int it = 0; it < nt; ++it) wlt[it] = ricker_value<T>(it, d->fm, d->dt);
     Step2Args a{};
     a.g = g;
-    a.l = make_launch(g, c->ty[0], c->zseg[0]);
+    const int ty = n > 1 && c->mshape ? c->mty : c->ty[0];
+    a.l = make_launch(g, ty, n > 1 && c->mshape ? c->mzseg : c->zseg[0]);
     a.l.ntiles *= n;
     uniform_bands(a.l);
     a.vv = c->vol[V_VV];
@@ -1156,7 +1159,7 @@ int run_shots_2nd_t(a3d_ctx* c, const a3d_shot_desc* d, int n, const int* src, f
         a.rec = c->mrec + plane_rec * it;
         timing_step(c, it);
         timing_begin(c, 0);
-        HIP_TRY(launch_step2_shots(prec, c->ty[0], a, sh, c->st));
+        HIP_TRY(launch_step2_shots(prec, ty, a, sh, c->st));
         timing_end(c);
         std::swap(X, Y);
     }
@@ -1595,8 +1598,38 @@ int a3d_run_shots(a3d_ctx* c, const a3d_shot_desc* d, int nshot, const int* src,
     if (!c->model_set) return fail(A3D_ESTATE, "model not set");
     if (int rc = maybe_autotune(c, d)) return rc;
     HIP_TRY(hipSetDevice(c->device));
-    if (c->prec == PREC_FP64) return run_shots_2nd_t<double>(c, d, nshot, src, shots_out, loop_ms);
-    return run_shots_2nd_t<float>(c, d, nshot, src, shots_out, loop_ms);
+    auto run = [&](const a3d_shot_desc* dd, float* out, double* ms) {
+        return c->prec == PREC_FP64 ? run_shots_2nd_t<double>(c, dd, nshot, src, out, ms)
+                                    : run_shots_2nd_t<float>(c, dd, nshot, src, out, ms);
+    };
+    if (nshot > 1 && !c->mshape) {
+        // several shots fill the GPU by themselves: their launch shape is timed apart from the
+        // one-shot shape (24 steps per candidate: 4 or 8 tile rows, 12- or 4-plane segments)
+        a3d_shot_desc td = *d;
+        td.nt = std::min(d->nt, 24);
+        const int nzi = c->g.nnz - 2 * MM;
+        double best = 1e30;
+        int bty = c->ty[0], bzs = c->zseg[0];
+        c->mshape = true;
+        for (int ty : {4, 8})
+            for (int zl : {12, 4}) {
+                c->mty = ty;
+                c->mzseg = std::max(1, (nzi + zl - 1) / zl);
+                double ms = 0;
+                if (int rc = run(&td, nullptr, &ms)) {
+                    c->mshape = false;
+                    return rc;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bty = c->mty;
+                    bzs = c->mzseg;
+                }
+            }
+        c->mty = bty;
+        c->mzseg = bzs;
+    }
+    return run(d, shots_out, loop_ms);
 }
 
 int a3d_set_kernel_timing(a3d_ctx* c, int enable) {
@@ -1629,6 +1662,9 @@ int a3d_set_launch(a3d_ctx* c, int ty_vel, int zseg_vel, int ty_stress, int zseg
     c->ty[0] = ty_vel;
     c->ty[1] = ty_stress;
     c->zseg[0] = zseg_vel > 0 ? zseg_vel : single_wave_zseg(c, 0, ty_vel);
+    c->mty = c->ty[0];   // a3d_run_shots takes the given shape too
+    c->mzseg = c->zseg[0];
+    c->mshape = true;
     c->zseg[1] = zseg_stress > 0 ? zseg_stress : single_wave_zseg(c, 1, ty_stress);
     c->tuned = true;   // an explicit shape is never overridden by the autotuner
     return A3D_OK;

@@ -1244,15 +1244,13 @@ __global__ __launch_bounds__(TX* TY) void k_step2(const Step2Args a) {
 // keep the GPU busy with plane-streaming tiles. Each point's arithmetic is the streaming
 // kernel's, operation for operation: the same bits.
 template <class PR, int TY, int ZB>
-__global__ __launch_bounds__(TX* TY) void k_step2_blk(const Step2Args a) {
+__device__ __forceinline__ void step2_blk_tile(const Step2Args& a, int tix, int tiy, int seg) {
     using S = typename PR::S;
     using M = typename PR::M;
     using C = typename PR::C;
     constexpr int H = TY + 2 * MM, W = TX + 2 * MM;
     __shared__ C sY[ZB][H][W];
 
-    int tix, tiy, seg;
-    if (!decode_tile(a.l, tix, tiy, seg)) return;
     const Geo g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = tix * TX, y0 = MM + tiy * TY;
@@ -1318,11 +1316,18 @@ __global__ __launch_bounds__(TX* TY) void k_step2_blk(const Step2Args a) {
     }
 }
 
+template <class PR, int TY, int ZB>
+__global__ __launch_bounds__(TX* TY) void k_step2_blk(const Step2Args a) {
+    int tix, tiy, seg;
+    if (!decode_tile(a.l, tix, tiy, seg)) return;
+    step2_blk_tile<PR, TY, ZB>(a, tix, tiy, seg);
+}
+
 // Several shots on one model: tile t is spatial tile t / n of shot t % n, so the shots'
 // copies of one tile run side by side on one XCD and share its L2 rows of (v dt)^2 and
 // the sponge tables. Each tile runs the single-shot step's code on its shot's levels:
 // every shot's results equal its own run bit for bit.
-template <class PR, int TY>
+template <class PR, int TY, bool BLK>
 __global__ __launch_bounds__(TX* TY) void k_step2_shots(const Step2Args a, const Step2Shots s) {
     const int t = remap_tile(a.l);
     if (t >= a.l.ntiles) return;
@@ -1335,7 +1340,8 @@ __global__ __launch_bounds__(TX* TY) void k_step2_shots(const Step2Args a, const
     b.s0 = s.s0[k];
     b.s1 = s.s1[k];
     b.rec = a.rec ? a.rec + k * s.rstride : nullptr;
-    step2_tile<PR, TY>(b, tix, tiy, seg);
+    if constexpr (BLK) step2_blk_tile<PR, TY, 4>(b, tix, tiy, seg);
+    else step2_tile<PR, TY>(b, tix, tiy, seg);
 }
 
 // --------------------------------------------------------------------------
@@ -1773,10 +1779,19 @@ hipError_t launch_stress_batch(int prec, int op, int ty, const StressArgs* wins,
 template <class PR>
 hipError_t step2_shots_ty(int ty, const Step2Args& a, const Step2Shots& s, hipStream_t st) {
     const dim3 grid(a.l.grid);
+    if (a.l.zlen <= 4) {   // short segments: every load of a segment at once
+        switch (ty) {
+            case 2: hipLaunchKernelGGL((k_step2_shots<PR, 2, true>), grid, dim3(TX, 2), 0, st, a, s); break;
+            case 4: hipLaunchKernelGGL((k_step2_shots<PR, 4, true>), grid, dim3(TX, 4), 0, st, a, s); break;
+            case 8: hipLaunchKernelGGL((k_step2_shots<PR, 8, true>), grid, dim3(TX, 8), 0, st, a, s); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (ty) {
-        case 2: hipLaunchKernelGGL((k_step2_shots<PR, 2>), grid, dim3(TX, 2), 0, st, a, s); break;
-        case 4: hipLaunchKernelGGL((k_step2_shots<PR, 4>), grid, dim3(TX, 4), 0, st, a, s); break;
-        case 8: hipLaunchKernelGGL((k_step2_shots<PR, 8>), grid, dim3(TX, 8), 0, st, a, s); break;
+        case 2: hipLaunchKernelGGL((k_step2_shots<PR, 2, false>), grid, dim3(TX, 2), 0, st, a, s); break;
+        case 4: hipLaunchKernelGGL((k_step2_shots<PR, 4, false>), grid, dim3(TX, 4), 0, st, a, s); break;
+        case 8: hipLaunchKernelGGL((k_step2_shots<PR, 8, false>), grid, dim3(TX, 8), 0, st, a, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -219,7 +219,9 @@ int a3d_set_graph(a3d_ctx* ctx, int steps);
  * time step over every shot's tiles, so a small grid fills the GPU. shots_out (host,
  * may be NULL) receives nshot records of [ny][nx][nt] floats, each equal bit for bit to
  * a3d_run of that shot. No snapshots (desc->snap_interval must be 0); *loop_ms: the time
- * loop. Allocates 2 x nshot pressure volumes and the records on the device. */
+ * loop. Launch shape: a3d_set_launch's if given, else timed on the first run with several
+ * shots (24 steps each of 4 or 8 tile rows, 12- or 4-plane segments) and kept. Allocates
+ * 2 x nshot pressure volumes and the records on the device. */
 int a3d_run_shots(a3d_ctx* ctx, const a3d_shot_desc* desc, int nshot, const int* src, float* shots_out,
                   double* loop_ms);
 

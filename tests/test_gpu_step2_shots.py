@@ -31,7 +31,8 @@ def test_shots_together_equal_single_runs(storage):
     srcs = _sources(5) + [(0, 0, 0), (NX - 1, NY - 1, NZ - 1)]
     with Modeller(_desc(storage)) as mdl:
         mdl.set_model(**m)
-        for launch in ((4, 3, 4, 3), (2, 1, 2, 1), (8, 2, 8, 2)):
+        short = -(-(mdl.grid_info()["nnz"] - 8) // 4)   # segments of <= 4 planes: all loads at once
+        for launch in ((4, 3, 4, 3), (2, 1, 2, 1), (8, 2, 8, 2), (4, short, 4, short)):
             mdl.set_launch(*launch)
             print(f"\n[gpu-step2-shots] storage={storage} launch={launch} shots={len(srcs)}", flush=True)
             recs, _ = mdl.run_shots(srcs)
@@ -39,6 +40,8 @@ def test_shots_together_equal_single_runs(storage):
                 one, _, _ = mdl.run(_desc(storage, sx, sy, sz))
                 assert np.abs(one).max() > 0
                 assert np.array_equal(recs[k], one), (launch, k)
+            solo, _ = mdl.run_shots(srcs[-1:])   # one shot: the launch shape as given
+            assert np.array_equal(solo[0], recs[-1]), launch
 
 
 def test_sixteen_shots_and_one():
