@@ -1596,6 +1596,11 @@ int a3d_run_shots(a3d_ctx* c, const a3d_shot_desc* d, int nshot, const int* src,
         if (int rc = check_desc_shot(c, &dk)) return rc;
     }
     if (!c->model_set) return fail(A3D_ESTATE, "model not set");
+    a3d_shot_desc d0 = *d;   // the one-shot tuner runs the first shot (the descriptor's source is not used)
+    d0.sx = src[0];
+    d0.sy = src[1];
+    d0.sz = src[2];
+    d = &d0;
     if (int rc = maybe_autotune(c, d)) return rc;
     HIP_TRY(hipSetDevice(c->device));
     auto run = [&](const a3d_shot_desc* dd, float* out, double* ms) {
