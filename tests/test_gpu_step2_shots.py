@@ -88,3 +88,20 @@ def test_shots_together_refuse_bad_arguments():
         mdl.set_model(**small_model("vti", 30, 26, 24, seed=3))
         with pytest.raises(A3DError):
             mdl.run_shots([(12, 10, 3)])
+
+
+def test_descriptor_source_not_used():
+    """The shots' sources come from the list; the descriptor's own (here outside the model)
+    is not used, by the run or by the launch-shape tuning of a fresh context."""
+    from a3dfd import Modeller
+    from a3dfd.models import small_model
+    m = small_model("2nd", NX, NY, NZ, seed=69)
+    srcs = _sources(3, seed=11)
+    with Modeller(_desc(sx=-5)) as mdl:
+        mdl.set_model(**m)
+        a, _ = mdl.run_shots(srcs)
+    with Modeller(_desc()) as mdl:
+        mdl.set_model(**m)
+        b, _ = mdl.run_shots(srcs)
+    assert np.abs(a).max() > 0
+    assert np.array_equal(a, b)
