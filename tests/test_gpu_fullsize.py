@@ -4,7 +4,8 @@ oracle on the same seeded models. 2nd order 100^3 with a 20-cell sponge, ISO 256
 (npml 20), VTI 400x400x300 (npml 20) and TTI 600x600x300 (npml 40);
 the shot record and one snapshot. The reference arithmetic (scheme 0) is required to
 be bitwise equal for 2nd order, ISO and VTI and within 1e-5 for TTI; the VTI bench's
-own stress scheme 3 within 1e-5."""
+own stress scheme 3 and TTI with the rotation terms evaluated from the angles (schemes
+0 and 4) within 1e-5."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,12 @@ def test_full_grid_parity(oracle, op):
         with Modeller(make_desc(*args, scheme=scheme, **kw)) as mdl:
             mdl.set_model(**m)
             outs[scheme] = mdl.run(nsnap_max=1)[:2]
+    if op == "tti":   # the rotation terms evaluated from the angles in the sweeps, schemes 0 and 4
+        for scheme in (0, 4):
+            with Modeller(make_desc(*args, scheme=scheme, **kw)) as mdl:
+                mdl.set_model(**m)
+                mdl.set_tti_angles(1)
+                outs[f"{scheme} angles"] = mdl.run(nsnap_max=1)[:2]
     ref, rsn, secs = oracle.run(oracle.make_shot(*args, **kw), 32, nsnap_max=1, **m)
     assert np.abs(ref).max() > 0 and np.abs(rsn).max() > 0
     for scheme, (out, snaps) in outs.items():
